@@ -1,0 +1,295 @@
+"""ctypes binding of libsvm.so, the MI355X spherical vertex-model library.
+
+The library is the product; this module only moves numpy arrays across its C
+ABI (include/svm.h).  Importing it never falls back to anything: if
+libsvm.so is missing or cannot be loaded, `lib()` raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsvm.so")
+
+SVM_OK = 0
+STATUS = {
+    0: "SVM_OK", 1: "SVM_ERR_ARG", 2: "SVM_ERR_HIP", 3: "SVM_ERR_NO_DEVICE",
+    4: "SVM_ERR_RING_OVERFLOW", 5: "SVM_ERR_TOPOLOGY", 6: "SVM_ERR_DELAUNAY",
+    7: "SVM_ERR_STATE", 8: "SVM_ERR_CAPACITY",
+}
+
+# Every symbol include/svm.h declares.
+EXPORTS = [
+    "svm_create", "svm_destroy", "svm_set_stream", "svm_synchronize", "svm_get_num_vertices",
+    "svm_last_error", "svm_build_from_cell_points", "svm_generate_cell_points",
+    "svm_generate_directors", "svm_set_topology", "svm_get_topology", "svm_get_delaunay_triangles",
+    "svm_set_positions", "svm_get_positions", "svm_set_directors", "svm_get_directors",
+    "svm_set_velocities", "svm_get_velocities", "svm_get_forces", "svm_get_cell_positions",
+    "svm_get_area_perimeter", "svm_set_preferred_parameters", "svm_set_preferences",
+    "svm_set_scalar_model_parameter", "svm_set_t1_threshold", "svm_compute_geometry",
+    "svm_compute_forces", "svm_move_particles", "svm_enforce_topology", "svm_get_energies",
+    "svm_get_counters", "svm_set_brownian", "svm_brownian_steps", "svm_set_active",
+    "svm_active_steps", "svm_run_graph", "svm_fire_set_parameters", "svm_fire_minimize",
+    "svm_fire_get_state", "svm_set_profiling", "svm_get_timing",
+]
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [("n_cells", C.c_int32), ("ring_capacity", C.c_int32), ("a0", C.c_double),
+                ("p0", C.c_double), ("kr", C.c_double), ("t1_threshold", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class FireParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("alpha_start", C.c_double), ("dt_max", C.c_double),
+                ("dt_inc", C.c_double), ("dt_dec", C.c_double), ("alpha_dec", C.c_double),
+                ("n_min", C.c_int32), ("max_iterations", C.c_int32), ("force_cutoff", C.c_double),
+                ("alpha_min", C.c_double), ("stop_rms", C.c_int32), ("reserved", C.c_int32)]
+
+
+class FireState(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("n_since_negative_power", C.c_int32),
+                ("converged", C.c_int32), ("reserved", C.c_int32), ("dt", C.c_double),
+                ("alpha", C.c_double), ("force_max", C.c_double), ("power", C.c_double)]
+
+
+_LIB = None
+
+
+def lib():
+    """Load libsvm.so (raises if it is not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run python -m curvedspacevertexmodel_amd.build")
+        L = C.CDLL(LIB_PATH)
+        L.svm_last_error.restype = C.c_char_p
+        L.svm_last_error.argtypes = [C.c_void_p]
+        _LIB = L
+    return _LIB
+
+
+class SVMError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def generate_cell_points(n_cells, seed):
+    out = np.zeros((n_cells, 3), dtype=np.float64)
+    r = lib().svm_generate_cell_points(C.c_int32(n_cells), C.c_uint64(seed), _dp(out))
+    if r:
+        raise SVMError(r, "svm_generate_cell_points")
+    return out
+
+
+def generate_directors(positions, seed):
+    pos = np.ascontiguousarray(positions, dtype=np.float64)
+    out = np.zeros_like(pos)
+    r = lib().svm_generate_directors(C.c_int32(pos.shape[0]), _dp(pos), C.c_uint64(seed), _dp(out))
+    if r:
+        raise SVMError(r, "svm_generate_directors")
+    return out
+
+
+class Context:
+    """A batch of spheres resident on the GPU (one svm_context)."""
+
+    def __init__(self, systems):
+        """systems: list of dicts with n_cells, a0, p0, kr, t1_threshold, seed, ring_capacity."""
+        self.n = len(systems)
+        arr = (SystemDesc * self.n)()
+        for i, s in enumerate(systems):
+            arr[i] = SystemDesc(int(s["n_cells"]), int(s.get("ring_capacity", 0)), float(s.get("a0", 1.0)),
+                                float(s.get("p0", 3.8)), float(s.get("kr", 1.0)),
+                                float(s.get("t1_threshold", 0.1)), int(s.get("seed", 1137 + i)))
+        self.n_cells = [int(s["n_cells"]) for s in systems]
+        self.n_vertices = [2 * n - 4 for n in self.n_cells]
+        h = C.c_void_p()
+        r = lib().svm_create(arr, C.c_int32(self.n), C.byref(h))
+        if r:
+            raise SVMError(r, "svm_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().svm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, r, what):
+        if r:
+            raise SVMError(r, f"{what}: {lib().svm_last_error(self.h).decode()}")
+
+    # ---- topology ----
+    def set_topology(self, sys, vpos, vn, vcn, ring, counts):
+        vpos = np.ascontiguousarray(vpos, np.float64)
+        vn = np.ascontiguousarray(vn, np.int32)
+        vcn = np.ascontiguousarray(vcn, np.int32)
+        ring = np.ascontiguousarray(ring, np.int32)
+        counts = np.ascontiguousarray(counts, np.int32)
+        self._ck(lib().svm_set_topology(self.h, C.c_int32(sys), C.c_int32(vpos.shape[0]), _dp(vpos), _ip(vn),
+                                        _ip(vcn), _ip(ring), C.c_int32(ring.shape[1]), _ip(counts)),
+                 "svm_set_topology")
+
+    def get_topology(self, sys, stride=32):
+        nv, nc = self.n_vertices[sys], self.n_cells[sys]
+        vn = np.zeros((nv, 3), np.int32)
+        vcn = np.zeros((nv, 3), np.int32)
+        ring = np.zeros((nc, stride), np.int32)
+        counts = np.zeros(nc, np.int32)
+        self._ck(lib().svm_get_topology(self.h, C.c_int32(sys), _ip(vn), _ip(vcn), _ip(ring), C.c_int32(stride),
+                                        _ip(counts)), "svm_get_topology")
+        return vn, vcn, ring, counts
+
+    def build_from_cell_points(self, sys, points):
+        pts = np.ascontiguousarray(points, np.float64)
+        self._ck(lib().svm_build_from_cell_points(self.h, C.c_int32(sys), _dp(pts)), "svm_build_from_cell_points")
+
+    def delaunay_triangles(self, sys):
+        out = np.zeros((2 * self.n_cells[sys] - 4, 3), np.int32)
+        self._ck(lib().svm_get_delaunay_triangles(self.h, C.c_int32(sys), _ip(out)), "svm_get_delaunay_triangles")
+        return out
+
+    # ---- state ----
+    def _get3(self, fn, sys, rows):
+        out = np.zeros((rows, 3), np.float64)
+        self._ck(getattr(lib(), fn)(self.h, C.c_int32(sys), _dp(out)), fn)
+        return out
+
+    def _set3(self, fn, sys, a):
+        a = np.ascontiguousarray(a, np.float64)
+        self._ck(getattr(lib(), fn)(self.h, C.c_int32(sys), _dp(a)), fn)
+
+    def positions(self, sys):
+        return self._get3("svm_get_positions", sys, self.n_vertices[sys])
+
+    def set_positions(self, sys, a):
+        self._set3("svm_set_positions", sys, a)
+
+    def directors(self, sys):
+        return self._get3("svm_get_directors", sys, self.n_vertices[sys])
+
+    def set_directors(self, sys, a):
+        self._set3("svm_set_directors", sys, a)
+
+    def velocities(self, sys):
+        return self._get3("svm_get_velocities", sys, self.n_vertices[sys])
+
+    def set_velocities(self, sys, a):
+        self._set3("svm_set_velocities", sys, a)
+
+    def forces(self, sys):
+        return self._get3("svm_get_forces", sys, self.n_vertices[sys])
+
+    def cell_positions(self, sys):
+        return self._get3("svm_get_cell_positions", sys, self.n_cells[sys])
+
+    def area_perimeter(self, sys):
+        out = np.zeros((self.n_cells[sys], 2), np.float64)
+        self._ck(lib().svm_get_area_perimeter(self.h, C.c_int32(sys), _dp(out)), "svm_get_area_perimeter")
+        return out
+
+    def set_preferences(self, sys, prefs):
+        prefs = np.ascontiguousarray(prefs, np.float64)
+        self._ck(lib().svm_set_preferences(self.h, C.c_int32(sys), _dp(prefs)), "svm_set_preferences")
+
+    def set_preferred_parameters(self, sys, a0, p0):
+        self._ck(lib().svm_set_preferred_parameters(self.h, C.c_int32(sys), C.c_double(a0), C.c_double(p0)),
+                 "svm_set_preferred_parameters")
+
+    def set_kr(self, sys, kr):
+        self._ck(lib().svm_set_scalar_model_parameter(self.h, C.c_int32(sys), C.c_double(kr)), "set Kr")
+
+    def set_t1_threshold(self, sys, t):
+        self._ck(lib().svm_set_t1_threshold(self.h, C.c_int32(sys), C.c_double(t)), "set threshold")
+
+    # ---- operations ----
+    def compute_geometry(self):
+        self._ck(lib().svm_compute_geometry(self.h), "svm_compute_geometry")
+
+    def compute_forces(self, precision=0):
+        self._ck(lib().svm_compute_forces(self.h, C.c_int32(precision)), "svm_compute_forces")
+
+    def move_particles(self, sys, disp, scale=1.0):
+        disp = np.ascontiguousarray(disp, np.float64)
+        self._ck(lib().svm_move_particles(self.h, C.c_int32(sys), _dp(disp), C.c_double(scale)),
+                 "svm_move_particles")
+
+    def enforce_topology(self):
+        self._ck(lib().svm_enforce_topology(self.h), "svm_enforce_topology")
+
+    def energies(self):
+        pe = np.zeros(self.n, np.float64)
+        ke = np.zeros(self.n, np.float64)
+        self._ck(lib().svm_get_energies(self.h, _dp(pe), _dp(ke)), "svm_get_energies")
+        return pe, ke
+
+    def counters(self):
+        out = np.zeros((self.n, 3), np.int64)
+        self._ck(lib().svm_get_counters(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "svm_get_counters")
+        return out
+
+    def set_brownian(self, sys, dt=0.01, mu=1.0, T=0.0):
+        self._ck(lib().svm_set_brownian(self.h, C.c_int32(sys), C.c_double(dt), C.c_double(mu), C.c_double(T)),
+                 "svm_set_brownian")
+
+    def set_active(self, sys, dt=0.01, mu=1.0, v0=0.01, tau=1.0):
+        self._ck(lib().svm_set_active(self.h, C.c_int32(sys), C.c_double(dt), C.c_double(mu), C.c_double(v0),
+                                      C.c_double(tau)), "svm_set_active")
+
+    def brownian_steps(self, n):
+        self._ck(lib().svm_brownian_steps(self.h, C.c_int32(n)), "svm_brownian_steps")
+
+    def active_steps(self, n):
+        self._ck(lib().svm_active_steps(self.h, C.c_int32(n)), "svm_active_steps")
+
+    def run_graph(self, updater, steps_per_graph, replays):
+        self._ck(lib().svm_run_graph(self.h, C.c_int32(updater), C.c_int32(steps_per_graph), C.c_int32(replays)),
+                 "svm_run_graph")
+
+    def fire_set_parameters(self, sys, dt=0.01, alpha_start=0.99, dt_max=0.1, dt_inc=1.05, dt_dec=0.95,
+                            alpha_dec=0.9, n_min=5, max_iterations=1000, force_cutoff=1e-7, alpha_min=0.0,
+                            stop_rms=0):
+        p = FireParams(dt, alpha_start, dt_max, dt_inc, dt_dec, alpha_dec, n_min, max_iterations, force_cutoff,
+                       alpha_min, stop_rms, 0)
+        self._ck(lib().svm_fire_set_parameters(self.h, C.c_int32(sys), C.byref(p)), "svm_fire_set_parameters")
+
+    def fire_minimize(self, check_every=50):
+        self._ck(lib().svm_fire_minimize(self.h, C.c_int32(check_every)), "svm_fire_minimize")
+
+    def fire_state(self, sys):
+        st = FireState()
+        self._ck(lib().svm_fire_get_state(self.h, C.c_int32(sys), C.byref(st)), "svm_fire_get_state")
+        return {k: getattr(st, k) for k, _ in FireState._fields_ if k != "reserved"}
+
+    def synchronize(self):
+        self._ck(lib().svm_synchronize(self.h), "svm_synchronize")
+
+    def set_stream(self, stream_ptr):
+        self._ck(lib().svm_set_stream(self.h, C.c_void_p(stream_ptr)), "svm_set_stream")
+
+    def set_profiling(self, on):
+        self._ck(lib().svm_set_profiling(self.h, C.c_int32(1 if on else 0)), "svm_set_profiling")
+
+    def timing(self):
+        n = 32
+        names = (C.c_char_p * n)()
+        ms = np.zeros(n, np.float64)
+        k = C.c_int32(0)
+        self._ck(lib().svm_get_timing(self.h, names, _dp(ms), C.c_int32(n), C.byref(k)), "svm_get_timing")
+        return {names[i].decode(): float(ms[i]) for i in range(k.value)}

@@ -1,0 +1,849 @@
+// svm_capi.hip -- host side of the C ABI declared in include/svm.h.
+//
+// A context owns every device buffer of a batch of spheres (layout in
+// svm_device.h) and one HIP stream.  Host arrays cross the boundary in the
+// reference's layouts and are converted here; nothing on the per-step path
+// touches the host: step loops only enqueue kernels (or replay a captured
+// graph), and errors raised on the device are collected at the next
+// synchronising call.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/svm.h"
+#include "svm_kernels.h"
+#include "svm_delaunay.h"
+
+using svm::Dev;
+using svm::SysParams;
+
+struct svm_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    int nsys = 0;
+    std::vector<svm_system_desc> desc;
+    std::vector<SysParams> layout;   // sizes/offsets (host copy of the constant part)
+    std::vector<int> have_topology;
+    int Cs = 0, Vs = 0, cap = 0, nct = 0, nvt = 0;
+    Dev d{};
+    int pcur = 0, ncur = 0;          // current position / director buffer
+    std::string err;
+    std::vector<void*> allocs;
+    // captured step graphs keyed by (updater, steps)
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+    std::map<std::pair<int, int>, hipGraph_t> graph_defs;
+    // Delaunay triangles per sphere (from svm_build_from_cell_points)
+    std::vector<std::vector<int32_t>> triangles;
+    // profiling
+    bool profiling = false;
+    std::map<std::string, std::pair<double, long>> timing;
+    std::vector<std::string> timing_names;
+};
+
+namespace {
+
+const char* k_ok = "ok";
+
+int fail(svm_context* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIPCHECK(ctx, call)                                                                      \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(ctx, SVM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+template <typename T>
+int dalloc(svm_context* ctx, T** p, size_t n) {
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(ctx, SVM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemset(q, 0, bytes);
+    if (e != hipSuccess) return fail(ctx, SVM_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    ctx->allocs.push_back(q);
+    *p = (T*)q;
+    return SVM_OK;
+}
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+bool valid_sys(svm_context* ctx, int sys) { return ctx && sys >= 0 && sys < ctx->nsys; }
+
+// Field update of one sphere's device parameter block.
+template <typename T>
+int set_field(svm_context* ctx, int sys, size_t offset, T value) {
+    char* base = (char*)(ctx->d.sys + sys) + offset;
+    HIPCHECK(ctx, hipMemcpyAsync(base, &value, sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return SVM_OK;
+}
+
+int download_sys(svm_context* ctx, std::vector<SysParams>& out) {
+    out.resize(ctx->nsys);
+    HIPCHECK(ctx, hipMemcpyAsync(out.data(), ctx->d.sys, sizeof(SysParams) * ctx->nsys, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return SVM_OK;
+}
+
+int check_device_errors(svm_context* ctx) {
+    std::vector<SysParams> s;
+    int r = download_sys(ctx, s);
+    if (r) return r;
+    for (int i = 0; i < ctx->nsys; ++i) {
+        if (s[i].err == SVM_ERR_RING_OVERFLOW)
+            return fail(ctx, SVM_ERR_RING_OVERFLOW, "sphere " + std::to_string(i) + ": a T1 flip would exceed ring_capacity");
+        if (s[i].err == SVM_ERR_TOPOLOGY)
+            return fail(ctx, SVM_ERR_TOPOLOGY, "sphere " + std::to_string(i) + ": rings and vertex lists disagree");
+        if (s[i].err) return fail(ctx, s[i].err, "sphere " + std::to_string(i) + ": device error " + std::to_string(s[i].err));
+    }
+    return SVM_OK;
+}
+
+// Scatter a host AoS [3*nv] array into the SoA rows of sphere `sys`.
+int upload_vec3(svm_context* ctx, int sys, const double* src, double* X, double* Y, double* Z) {
+    const SysParams& L = ctx->layout[sys];
+    std::vector<double> t(3 * (size_t)L.nv);
+    for (int i = 0; i < L.nv; ++i) {
+        t[i] = src[3 * (size_t)i];
+        t[L.nv + i] = src[3 * (size_t)i + 1];
+        t[2 * (size_t)L.nv + i] = src[3 * (size_t)i + 2];
+    }
+    HIPCHECK(ctx, hipMemcpyAsync(X + L.vo, t.data(), sizeof(double) * L.nv, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(Y + L.vo, t.data() + L.nv, sizeof(double) * L.nv, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(Z + L.vo, t.data() + 2 * (size_t)L.nv, sizeof(double) * L.nv, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return SVM_OK;
+}
+
+int download_rows(svm_context* ctx, int n, int off, const double* X, const double* Y, const double* Z, double* dst) {
+    std::vector<double> t(3 * (size_t)n);
+    HIPCHECK(ctx, hipMemcpyAsync(t.data(), X + off, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(t.data() + n, Y + off, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(t.data() + 2 * (size_t)n, Z + off, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i) {
+        dst[3 * (size_t)i] = t[i];
+        dst[3 * (size_t)i + 1] = t[n + i];
+        dst[3 * (size_t)i + 2] = t[2 * (size_t)n + i];
+    }
+    return SVM_OK;
+}
+
+// Kernel launch with optional event timing (profiling mode, eager only).
+struct Timer {
+    svm_context* ctx;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timer(svm_context* c, const char* n) : ctx(c), name(n) {
+        if (ctx->profiling) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, ctx->stream);
+        }
+    }
+    ~Timer() {
+        if (ctx->profiling) {
+            (void)hipEventRecord(b, ctx->stream);
+            (void)hipEventSynchronize(b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            auto& e = ctx->timing[name];
+            e.first += ms;
+            e.second += 1;
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    }
+};
+
+int last_launch(svm_context* ctx) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ctx, SVM_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return SVM_OK;
+}
+
+// ---- step building blocks ----
+int launch_geometry(svm_context* ctx, int skip_frozen) {
+    Timer t(ctx, "k_geometry");
+    hipLaunchKernelGGL(svm::k_geometry, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, skip_frozen);
+    return last_launch(ctx);
+}
+
+template <int MODE>
+int launch_vertex(svm_context* ctx, int pout, int store_forces, const char* name) {
+    Timer t(ctx, name);
+    hipLaunchKernelGGL(svm::k_vertex<MODE>, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, pout,
+                       ctx->ncur, store_forces);
+    return last_launch(ctx);
+}
+
+int launch_t1(svm_context* ctx, int pcur, int nin, int nout, int dir, int advance_step, int fire_mode) {
+    {
+        Timer t(ctx, "k_t1_detect");
+        hipLaunchKernelGGL(svm::k_t1_detect, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d, pcur, nin, nout,
+                           dir, fire_mode);
+        int r = last_launch(ctx);
+        if (r) return r;
+    }
+    Timer t(ctx, "k_t1_resolve");
+    hipLaunchKernelGGL(svm::k_t1_resolve, dim3(ctx->nsys), dim3(SVM_TILE), 0, ctx->stream, ctx->d, pcur, nin, nout,
+                       dir, advance_step, fire_mode);
+    return last_launch(ctx);
+}
+
+// One dynamics step: geometry, force+updater+move, T1.  updater 0 = Brownian, 1 = active.
+int enqueue_step(svm_context* ctx, int updater) {
+    int r = launch_geometry(ctx, 0);
+    if (r) return r;
+    int pout = 1 - ctx->pcur;
+    if (updater == 0) r = launch_vertex<svm::VM_BROWNIAN>(ctx, pout, 0, "k_vertex_brownian");
+    else r = launch_vertex<svm::VM_ACTIVE>(ctx, pout, 0, "k_vertex_active");
+    if (r) return r;
+    int nout = updater == 1 ? 1 - ctx->ncur : ctx->ncur;
+    r = launch_t1(ctx, pout, ctx->ncur, nout, updater == 1, 1, 0);
+    if (r) return r;
+    ctx->pcur = pout;
+    ctx->ncur = nout;
+    return SVM_OK;
+}
+
+// One FIRE iteration: mixing + VV first half + move, T1, geometry, force +
+// VV second half + reductions + controller.
+int enqueue_fire_iteration(svm_context* ctx) {
+    int pout = 1 - ctx->pcur;
+    {
+        Timer t(ctx, "k_fire_vv1");
+        hipLaunchKernelGGL(svm::k_fire_vv1, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, pout);
+        int r = last_launch(ctx);
+        if (r) return r;
+    }
+    int r = launch_t1(ctx, pout, ctx->ncur, ctx->ncur, 0, 0, 1);
+    if (r) return r;
+    ctx->pcur = pout;
+    r = launch_geometry(ctx, 1);
+    if (r) return r;
+    return launch_vertex<svm::VM_FIRE>(ctx, ctx->pcur, 1, "k_vertex_fire");
+}
+
+int require_topology(svm_context* ctx) {
+    for (int s = 0; s < ctx->nsys; ++s)
+        if (!ctx->have_topology[s]) return fail(ctx, SVM_ERR_STATE, "sphere " + std::to_string(s) + " has no topology");
+    return SVM_OK;
+}
+
+}  // namespace
+
+// ======================================================================== API
+extern "C" {
+
+const char* svm_last_error(svm_context* ctx) { return ctx ? ctx->err.c_str() : k_ok; }
+
+int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** out) {
+    if (!systems || n_systems <= 0 || !out) return SVM_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SVM_ERR_NO_DEVICE;
+    svm_context* ctx = new svm_context();
+    if (hipGetDevice(&ctx->device) != hipSuccess) { delete ctx; return SVM_ERR_NO_DEVICE; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess || std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        delete ctx;
+        return SVM_ERR_NO_DEVICE;
+    }
+    ctx->nsys = n_systems;
+    ctx->desc.assign(systems, systems + n_systems);
+    ctx->layout.resize(n_systems);
+    ctx->have_topology.assign(n_systems, 0);
+    ctx->triangles.resize(n_systems);
+    int cap = 0;
+    long long Cs = 0, Vs = 0;
+    for (int s = 0; s < n_systems; ++s) {
+        const svm_system_desc& sd = systems[s];
+        if (sd.n_cells < 4 || sd.ring_capacity < 0 || sd.ring_capacity > 64) {
+            delete ctx;
+            return SVM_ERR_ARG;
+        }
+        cap = std::max(cap, sd.ring_capacity ? (int)sd.ring_capacity : 32);
+        SysParams& L = ctx->layout[s];
+        std::memset(&L, 0, sizeof(L));
+        L.nc = sd.n_cells;
+        L.nv = 2 * sd.n_cells - 4;
+        L.co = (int)Cs;
+        L.vo = (int)Vs;
+        L.ctile0 = (int)(Cs / SVM_TILE);
+        L.nctiles = round_up(L.nc, SVM_TILE) / SVM_TILE;
+        L.vtile0 = (int)(Vs / SVM_TILE);
+        L.nvtiles = round_up(L.nv, SVM_TILE) / SVM_TILE;
+        Cs += round_up(L.nc, SVM_TILE);
+        Vs += round_up(L.nv, SVM_TILE);
+    }
+    if (Cs > (1ll << 30) || Vs > (1ll << 30)) { delete ctx; return SVM_ERR_ARG; }
+    ctx->Cs = (int)Cs;
+    ctx->Vs = (int)Vs;
+    ctx->cap = cap;
+    ctx->nct = (int)(Cs / SVM_TILE);
+    ctx->nvt = (int)(Vs / SVM_TILE);
+    if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return SVM_ERR_HIP; }
+    ctx->stream = ctx->own_stream;
+
+    Dev& d = ctx->d;
+    d.nsys = n_systems;
+    d.Cs = ctx->Cs;
+    d.Vs = ctx->Vs;
+    d.cap = cap;
+    int r = 0;
+    int32_t *cts = nullptr, *vts = nullptr;
+    r |= dalloc(ctx, &d.sys, n_systems);
+    r |= dalloc(ctx, &cts, ctx->nct);
+    r |= dalloc(ctx, &vts, ctx->nvt);
+    r |= dalloc(ctx, &d.cnn, Cs);
+    r |= dalloc(ctx, &d.ring, (size_t)Cs * cap);
+    r |= dalloc(ctx, &d.A, Cs);
+    r |= dalloc(ctx, &d.P, Cs);
+    r |= dalloc(ctx, &d.cx, Cs);
+    r |= dalloc(ctx, &d.cy, Cs);
+    r |= dalloc(ctx, &d.cz, Cs);
+    r |= dalloc(ctx, &d.a0, Cs);
+    r |= dalloc(ctx, &d.p0, Cs);
+    for (int b = 0; b < 2; ++b) {
+        r |= dalloc(ctx, &d.px[b], Vs);
+        r |= dalloc(ctx, &d.py[b], Vs);
+        r |= dalloc(ctx, &d.pz[b], Vs);
+        r |= dalloc(ctx, &d.nx[b], Vs);
+        r |= dalloc(ctx, &d.ny[b], Vs);
+        r |= dalloc(ctx, &d.nz[b], Vs);
+    }
+    r |= dalloc(ctx, &d.vx, Vs);
+    r |= dalloc(ctx, &d.vy, Vs);
+    r |= dalloc(ctx, &d.vz, Vs);
+    r |= dalloc(ctx, &d.fx, Vs);
+    r |= dalloc(ctx, &d.fy, Vs);
+    r |= dalloc(ctx, &d.fz, Vs);
+    r |= dalloc(ctx, &d.vcn, Vs);
+    r |= dalloc(ctx, &d.vn, Vs);
+    r |= dalloc(ctx, &d.vln, Vs);
+    r |= dalloc(ctx, &d.t1mask, Vs / 64);
+    r |= dalloc(ctx, &d.cand, Vs);
+    r |= dalloc(ctx, &d.cpart, ctx->nct);
+    r |= dalloc(ctx, &d.vpart, 3 * (size_t)ctx->nvt);
+    r |= dalloc(ctx, &d.disp, 3 * (size_t)Vs);
+    if (r) {
+        svm_destroy(ctx);
+        return SVM_ERR_HIP;
+    }
+    d.ctile_sys = cts;
+    d.vtile_sys = vts;
+    std::vector<int32_t> hct(ctx->nct), hvt(ctx->nvt);
+    std::vector<SysParams> hs(n_systems);
+    for (int s = 0; s < n_systems; ++s) {
+        SysParams& L = ctx->layout[s];
+        for (int t = 0; t < L.nctiles; ++t) hct[L.ctile0 + t] = s;
+        for (int t = 0; t < L.nvtiles; ++t) hvt[L.vtile0 + t] = s;
+        SysParams P = L;
+        const svm_system_desc& sd = systems[s];
+        P.R = std::sqrt(sd.n_cells / (4.0 * SVM_PI));
+        P.kr = sd.kr;
+        P.thr = sd.t1_threshold;
+        P.seed = sd.seed;
+        P.dt = 0.01; P.mu = 1.0; P.T = 0.0; P.v0 = 0.01; P.tau = 1.0;   // updater defaults
+        P.f_dt0 = 0.01; P.f_alpha_start = 0.99; P.f_dt_max = 0.1; P.f_dt_inc = 1.05; P.f_dt_dec = 0.95;
+        P.f_alpha_dec = 0.9; P.f_cutoff = 1e-7; P.f_alpha_min = 0.0; P.f_dt_min = 0.01 * 0.01;
+        P.f_nmin = 5; P.f_max_iter = 1000;
+        P.f_dt = 0.01; P.f_alpha = 0.99; P.f_force_max = 100.;
+        hs[s] = P;
+    }
+    if (hipMemcpy(cts, hct.data(), sizeof(int32_t) * ctx->nct, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(vts, hvt.data(), sizeof(int32_t) * ctx->nvt, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.sys, hs.data(), sizeof(SysParams) * n_systems, hipMemcpyHostToDevice) != hipSuccess) {
+        svm_destroy(ctx);
+        return SVM_ERR_HIP;
+    }
+    // preferences
+    std::vector<double> a0(Cs, 0.0), p0(Cs, 0.0);
+    for (int s = 0; s < n_systems; ++s)
+        for (int c = 0; c < ctx->layout[s].nc; ++c) {
+            a0[ctx->layout[s].co + c] = systems[s].a0;
+            p0[ctx->layout[s].co + c] = systems[s].p0;
+        }
+    if (hipMemcpy(d.a0, a0.data(), sizeof(double) * Cs, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d.p0, p0.data(), sizeof(double) * Cs, hipMemcpyHostToDevice) != hipSuccess) {
+        svm_destroy(ctx);
+        return SVM_ERR_HIP;
+    }
+    *out = ctx;
+    return SVM_OK;
+}
+
+int svm_destroy(svm_context* ctx) {
+    if (!ctx) return SVM_ERR_ARG;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto& g : ctx->graph_defs) (void)hipGraphDestroy(g.second);
+    for (void* p : ctx->allocs) (void)hipFree(p);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return SVM_OK;
+}
+
+int svm_set_stream(svm_context* ctx, void* stream) {
+    if (!ctx) return SVM_ERR_ARG;
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto& g : ctx->graph_defs) (void)hipGraphDestroy(g.second);
+    ctx->graphs.clear();
+    ctx->graph_defs.clear();
+    return SVM_OK;
+}
+
+int svm_synchronize(svm_context* ctx) {
+    if (!ctx) return SVM_ERR_ARG;
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_device_errors(ctx);
+}
+
+int svm_get_num_vertices(svm_context* ctx, int32_t sys, int32_t* n_vertices) {
+    if (!valid_sys(ctx, sys) || !n_vertices) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    *n_vertices = ctx->layout[sys].nv;
+    return SVM_OK;
+}
+
+int svm_generate_cell_points(int32_t n_cells, uint64_t seed, double* cell_points) {
+    if (n_cells < 4 || !cell_points) return SVM_ERR_ARG;
+    svm::generate_cell_points(n_cells, seed, cell_points);
+    return SVM_OK;
+}
+
+int svm_generate_directors(int32_t n_vertices, const double* positions, uint64_t seed, double* directors) {
+    if (n_vertices < 1 || !positions || !directors) return SVM_ERR_ARG;
+    svm::generate_directors(n_vertices, positions, seed, directors);
+    return SVM_OK;
+}
+
+int svm_set_topology(svm_context* ctx, int32_t sys, int32_t n_vertices, const double* vertex_positions,
+                     const int32_t* vertex_neighbors, const int32_t* vertex_cell_neighbors,
+                     const int32_t* cell_vertices, int32_t ring_stride, const int32_t* cell_vertex_counts) {
+    if (!valid_sys(ctx, sys) || !vertex_positions || !vertex_neighbors || !vertex_cell_neighbors || !cell_vertices ||
+        !cell_vertex_counts)
+        return fail(ctx, SVM_ERR_ARG, "null argument");
+    const SysParams& L = ctx->layout[sys];
+    if (n_vertices != L.nv) return fail(ctx, SVM_ERR_TOPOLOGY, "vertex count must be 2N-4");
+    // host-side validation and conversion to global rows
+    std::vector<int4> vcn(L.nv), vn(L.nv);
+    for (int v = 0; v < L.nv; ++v) {
+        int a[3], b[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = vertex_cell_neighbors[3 * (size_t)v + k];
+            b[k] = vertex_neighbors[3 * (size_t)v + k];
+            if (a[k] < 0 || a[k] >= L.nc || b[k] < 0 || b[k] >= L.nv)
+                return fail(ctx, SVM_ERR_TOPOLOGY, "vertex list index out of range");
+        }
+        vcn[v] = make_int4(a[0] + L.co, a[1] + L.co, a[2] + L.co, sys);
+        vn[v] = make_int4(b[0] + L.vo, b[1] + L.vo, b[2] + L.vo, sys);
+    }
+    std::vector<int32_t> ring((size_t)ctx->cap * L.nc, -1), cnt(L.nc);
+    for (int c = 0; c < L.nc; ++c) {
+        int n = cell_vertex_counts[c];
+        if (n < 3 || n > ctx->cap || n > ring_stride)
+            return fail(ctx, SVM_ERR_RING_OVERFLOW, "cell vertex count outside [3, ring_capacity]");
+        cnt[c] = n;
+        for (int k = 0; k < n; ++k) {
+            int v = cell_vertices[(size_t)c * ring_stride + k];
+            if (v < 0 || v >= L.nv) return fail(ctx, SVM_ERR_TOPOLOGY, "ring index out of range");
+            ring[(size_t)k * L.nc + c] = v + L.vo;
+        }
+    }
+    Dev& d = ctx->d;
+    HIPCHECK(ctx, hipMemcpyAsync(d.vcn + L.vo, vcn.data(), sizeof(int4) * L.nv, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(d.vn + L.vo, vn.data(), sizeof(int4) * L.nv, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(d.cnn + L.co, cnt.data(), sizeof(int32_t) * L.nc, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpy2DAsync(d.ring + L.co, sizeof(int32_t) * ctx->Cs, ring.data(), sizeof(int32_t) * L.nc,
+                                   sizeof(int32_t) * L.nc, ctx->cap, hipMemcpyHostToDevice, ctx->stream));
+    int r = upload_vec3(ctx, sys, vertex_positions, d.px[ctx->pcur], d.py[ctx->pcur], d.pz[ctx->pcur]);
+    if (r) return r;
+    hipLaunchKernelGGL(svm::k_build_vln, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d, sys);
+    r = last_launch(ctx);
+    if (r) return r;
+    r = svm_synchronize(ctx);
+    if (r) return r;
+    ctx->have_topology[sys] = 1;
+    return SVM_OK;
+}
+
+int svm_get_topology(svm_context* ctx, int32_t sys, int32_t* vertex_neighbors, int32_t* vertex_cell_neighbors,
+                     int32_t* cell_vertices, int32_t ring_stride, int32_t* cell_vertex_counts) {
+    if (!valid_sys(ctx, sys) || !vertex_neighbors || !vertex_cell_neighbors || !cell_vertices || !cell_vertex_counts)
+        return fail(ctx, SVM_ERR_ARG, "null argument");
+    const SysParams& L = ctx->layout[sys];
+    Dev& d = ctx->d;
+    std::vector<int4> vcn(L.nv), vn(L.nv);
+    std::vector<int32_t> ring((size_t)ctx->cap * L.nc), cnt(L.nc);
+    HIPCHECK(ctx, hipMemcpyAsync(vcn.data(), d.vcn + L.vo, sizeof(int4) * L.nv, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(vn.data(), d.vn + L.vo, sizeof(int4) * L.nv, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(cnt.data(), d.cnn + L.co, sizeof(int32_t) * L.nc, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipMemcpy2DAsync(ring.data(), sizeof(int32_t) * L.nc, d.ring + L.co, sizeof(int32_t) * ctx->Cs,
+                                   sizeof(int32_t) * L.nc, ctx->cap, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    int maxn = 0;
+    for (int c = 0; c < L.nc; ++c) maxn = std::max(maxn, cnt[c]);
+    if (maxn > ring_stride) return fail(ctx, SVM_ERR_ARG, "ring_stride smaller than the largest cell");
+    for (int v = 0; v < L.nv; ++v) {
+        const int* a = (const int*)&vcn[v];
+        const int* b = (const int*)&vn[v];
+        for (int k = 0; k < 3; ++k) {
+            vertex_cell_neighbors[3 * (size_t)v + k] = a[k] - L.co;
+            vertex_neighbors[3 * (size_t)v + k] = b[k] - L.vo;
+        }
+    }
+    for (int c = 0; c < L.nc; ++c) {
+        cell_vertex_counts[c] = cnt[c];
+        for (int k = 0; k < ring_stride; ++k)
+            cell_vertices[(size_t)c * ring_stride + k] = k < cnt[c] ? ring[(size_t)k * L.nc + c] - L.vo : -1;
+    }
+    return SVM_OK;
+}
+
+int svm_build_from_cell_points(svm_context* ctx, int32_t sys, const double* cell_points) {
+    if (!valid_sys(ctx, sys) || !cell_points) return fail(ctx, SVM_ERR_ARG, "null argument");
+    return fail(ctx, SVM_ERR_STATE, "GPU Delaunay construction is not built yet");
+}
+
+int svm_get_delaunay_triangles(svm_context* ctx, int32_t sys, int32_t* triangles) {
+    if (!valid_sys(ctx, sys) || !triangles) return fail(ctx, SVM_ERR_ARG, "null argument");
+    if (ctx->triangles[sys].empty()) return fail(ctx, SVM_ERR_STATE, "no Delaunay triangulation for this sphere");
+    std::memcpy(triangles, ctx->triangles[sys].data(), sizeof(int32_t) * ctx->triangles[sys].size());
+    return SVM_OK;
+}
+
+#define STATE_IO(NAME, X, Y, Z)                                                                                   \
+    int svm_set_##NAME(svm_context* ctx, int32_t sys, const double* src) {                                         \
+        if (!valid_sys(ctx, sys) || !src) return fail(ctx, SVM_ERR_ARG, "null argument");                           \
+        return upload_vec3(ctx, sys, src, X, Y, Z);                                                                \
+    }                                                                                                              \
+    int svm_get_##NAME(svm_context* ctx, int32_t sys, double* dst) {                                               \
+        if (!valid_sys(ctx, sys) || !dst) return fail(ctx, SVM_ERR_ARG, "null argument");                           \
+        return download_rows(ctx, ctx->layout[sys].nv, ctx->layout[sys].vo, X, Y, Z, dst);                         \
+    }
+
+STATE_IO(positions, ctx->d.px[ctx->pcur], ctx->d.py[ctx->pcur], ctx->d.pz[ctx->pcur])
+STATE_IO(directors, ctx->d.nx[ctx->ncur], ctx->d.ny[ctx->ncur], ctx->d.nz[ctx->ncur])
+STATE_IO(velocities, ctx->d.vx, ctx->d.vy, ctx->d.vz)
+
+int svm_get_forces(svm_context* ctx, int32_t sys, double* forces) {
+    if (!valid_sys(ctx, sys) || !forces) return fail(ctx, SVM_ERR_ARG, "null argument");
+    return download_rows(ctx, ctx->layout[sys].nv, ctx->layout[sys].vo, ctx->d.fx, ctx->d.fy, ctx->d.fz, forces);
+}
+
+int svm_get_cell_positions(svm_context* ctx, int32_t sys, double* out) {
+    if (!valid_sys(ctx, sys) || !out) return fail(ctx, SVM_ERR_ARG, "null argument");
+    return download_rows(ctx, ctx->layout[sys].nc, ctx->layout[sys].co, ctx->d.cx, ctx->d.cy, ctx->d.cz, out);
+}
+
+int svm_get_area_perimeter(svm_context* ctx, int32_t sys, double* out) {
+    if (!valid_sys(ctx, sys) || !out) return fail(ctx, SVM_ERR_ARG, "null argument");
+    const SysParams& L = ctx->layout[sys];
+    std::vector<double> a(L.nc), p(L.nc);
+    HIPCHECK(ctx, hipMemcpyAsync(a.data(), ctx->d.A + L.co, sizeof(double) * L.nc, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(p.data(), ctx->d.P + L.co, sizeof(double) * L.nc, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < L.nc; ++c) { out[2 * (size_t)c] = a[c]; out[2 * (size_t)c + 1] = p[c]; }
+    return SVM_OK;
+}
+
+int svm_set_preferences(svm_context* ctx, int32_t sys, const double* prefs) {
+    if (!valid_sys(ctx, sys) || !prefs) return fail(ctx, SVM_ERR_ARG, "null argument");
+    const SysParams& L = ctx->layout[sys];
+    std::vector<double> a(L.nc), p(L.nc);
+    for (int c = 0; c < L.nc; ++c) { a[c] = prefs[2 * (size_t)c]; p[c] = prefs[2 * (size_t)c + 1]; }
+    HIPCHECK(ctx, hipMemcpyAsync(ctx->d.a0 + L.co, a.data(), sizeof(double) * L.nc, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(ctx->d.p0 + L.co, p.data(), sizeof(double) * L.nc, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return SVM_OK;
+}
+
+int svm_set_preferred_parameters(svm_context* ctx, int32_t sys, double a0, double p0) {
+    if (!valid_sys(ctx, sys)) return fail(ctx, SVM_ERR_ARG, "bad sphere index");
+    std::vector<double> pr(2 * (size_t)ctx->layout[sys].nc);
+    for (int c = 0; c < ctx->layout[sys].nc; ++c) { pr[2 * (size_t)c] = a0; pr[2 * (size_t)c + 1] = p0; }
+    return svm_set_preferences(ctx, sys, pr.data());
+}
+
+int svm_set_scalar_model_parameter(svm_context* ctx, int32_t sys, double kr) {
+    if (!valid_sys(ctx, sys)) return fail(ctx, SVM_ERR_ARG, "bad sphere index");
+    return set_field(ctx, sys, offsetof(SysParams, kr), kr);
+}
+
+int svm_set_t1_threshold(svm_context* ctx, int32_t sys, double threshold) {
+    if (!valid_sys(ctx, sys)) return fail(ctx, SVM_ERR_ARG, "bad sphere index");
+    return set_field(ctx, sys, offsetof(SysParams, thr), threshold);
+}
+
+int svm_compute_geometry(svm_context* ctx) {
+    if (!ctx) return SVM_ERR_ARG;
+    int r = require_topology(ctx);
+    if (r) return r;
+    return launch_geometry(ctx, 0);
+}
+
+int svm_compute_forces(svm_context* ctx, int32_t precision) {
+    if (!ctx) return SVM_ERR_ARG;
+    int r = require_topology(ctx);
+    if (r) return r;
+    r = launch_geometry(ctx, 0);
+    if (r) return r;
+    if (precision != 0) return fail(ctx, SVM_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32)");
+    return launch_vertex<svm::VM_FORCE>(ctx, ctx->pcur, 1, "k_vertex_force");
+}
+
+int svm_move_particles(svm_context* ctx, int32_t sys, const double* displacements, double scale) {
+    if (!valid_sys(ctx, sys) || !displacements) return fail(ctx, SVM_ERR_ARG, "null argument");
+    int r = require_topology(ctx);
+    if (r) return r;
+    const SysParams& L = ctx->layout[sys];
+    HIPCHECK(ctx, hipMemcpyAsync(ctx->d.disp + 3 * (size_t)L.vo, displacements, sizeof(double) * 3 * L.nv,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    int pout = 1 - ctx->pcur;
+    hipLaunchKernelGGL(svm::k_move_ext, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, pout, sys,
+                       scale);
+    r = last_launch(ctx);
+    if (r) return r;
+    r = launch_t1(ctx, pout, ctx->ncur, ctx->ncur, 0, 0, 0);
+    if (r) return r;
+    ctx->pcur = pout;
+    return svm_synchronize(ctx);
+}
+
+int svm_enforce_topology(svm_context* ctx) {
+    if (!ctx) return SVM_ERR_ARG;
+    int r = require_topology(ctx);
+    if (r) return r;
+    return launch_t1(ctx, ctx->pcur, ctx->ncur, ctx->ncur, 0, 0, 0);
+}
+
+int svm_get_energies(svm_context* ctx, double* potential, double* kinetic) {
+    if (!ctx) return SVM_ERR_ARG;
+    hipLaunchKernelGGL(svm::k_energy_cells, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, ctx->d);
+    int r = last_launch(ctx);
+    if (r) return r;
+    hipLaunchKernelGGL(svm::k_energy_vertices, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d);
+    r = last_launch(ctx);
+    if (r) return r;
+    std::vector<SysParams> s;
+    r = download_sys(ctx, s);
+    if (r) return r;
+    for (int i = 0; i < ctx->nsys; ++i) {
+        if (potential) potential[i] = s[i].energy_pot;
+        if (kinetic) kinetic[i] = s[i].energy_kin;
+    }
+    return SVM_OK;
+}
+
+int svm_get_counters(svm_context* ctx, int64_t* counters) {
+    if (!ctx || !counters) return SVM_ERR_ARG;
+    std::vector<SysParams> s;
+    int r = download_sys(ctx, s);
+    if (r) return r;
+    for (int i = 0; i < ctx->nsys; ++i) {
+        counters[3 * i] = s[i].n_t1;
+        counters[3 * i + 1] = s[i].n_fan;
+        counters[3 * i + 2] = s[i].n_nan;
+    }
+    return SVM_OK;
+}
+
+int svm_set_brownian(svm_context* ctx, int32_t sys, double dt, double mu, double temperature) {
+    if (!valid_sys(ctx, sys) || temperature < 0) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    int r = set_field(ctx, sys, offsetof(SysParams, dt), dt);
+    r = r ? r : set_field(ctx, sys, offsetof(SysParams, mu), mu);
+    return r ? r : set_field(ctx, sys, offsetof(SysParams, T), temperature);
+}
+
+int svm_set_active(svm_context* ctx, int32_t sys, double dt, double mu, double v0, double tau) {
+    if (!valid_sys(ctx, sys)) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    int r = set_field(ctx, sys, offsetof(SysParams, dt), dt);
+    r = r ? r : set_field(ctx, sys, offsetof(SysParams, mu), mu);
+    r = r ? r : set_field(ctx, sys, offsetof(SysParams, v0), v0);
+    return r ? r : set_field(ctx, sys, offsetof(SysParams, tau), tau);
+}
+
+static int run_steps(svm_context* ctx, int updater, int n_steps) {
+    if (!ctx || n_steps < 0) return SVM_ERR_ARG;
+    int r = require_topology(ctx);
+    if (r) return r;
+    for (int i = 0; i < n_steps; ++i) {
+        r = enqueue_step(ctx, updater);
+        if (r) return r;
+    }
+    return SVM_OK;
+}
+
+int svm_brownian_steps(svm_context* ctx, int32_t n_steps) { return run_steps(ctx, 0, n_steps); }
+int svm_active_steps(svm_context* ctx, int32_t n_steps) { return run_steps(ctx, 1, n_steps); }
+
+int svm_run_graph(svm_context* ctx, int32_t updater, int32_t steps_per_graph, int32_t n_replays) {
+    if (!ctx || (updater != 0 && updater != 1) || steps_per_graph <= 0 || (steps_per_graph & 1) || n_replays < 0)
+        return fail(ctx, SVM_ERR_ARG, "steps_per_graph must be a positive even number");
+    int r = require_topology(ctx);
+    if (r) return r;
+    // An even number of steps returns the ping-pong buffers to where they
+    // started, so a graph captured from buffer state (pcur, ncur) can be
+    // replayed whenever the context is in that state again.
+    auto key = std::make_pair((int)updater, (int)steps_per_graph * 4 + ctx->pcur * 2 + ctx->ncur);
+    auto it = ctx->graphs.find(key);
+    if (it == ctx->graphs.end()) {
+        bool prof = ctx->profiling;
+        ctx->profiling = false;
+        hipGraph_t g;
+        HIPCHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < steps_per_graph; ++i) {
+            r = enqueue_step(ctx, updater);
+            if (r) {
+                hipGraph_t junk;
+                (void)hipStreamEndCapture(ctx->stream, &junk);
+                ctx->profiling = prof;
+                return r;
+            }
+        }
+        HIPCHECK(ctx, hipStreamEndCapture(ctx->stream, &g));
+        ctx->profiling = prof;
+        hipGraphExec_t ge;
+        HIPCHECK(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        ctx->graphs[key] = ge;
+        ctx->graph_defs[key] = g;
+        it = ctx->graphs.find(key);
+    }
+    for (int i = 0; i < n_replays; ++i) HIPCHECK(ctx, hipGraphLaunch(it->second, ctx->stream));
+    return SVM_OK;
+}
+
+int svm_fire_set_parameters(svm_context* ctx, int32_t sys, const svm_fire_params* p) {
+    if (!valid_sys(ctx, sys) || !p) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    std::vector<SysParams> s;
+    int r = download_sys(ctx, s);
+    if (r) return r;
+    SysParams& S = s[sys];
+    S.f_dt0 = p->dt;
+    S.f_dt = p->dt;
+    S.f_dt_min = p->dt * .01;
+    S.f_alpha_start = p->alpha_start;
+    S.f_alpha = p->alpha_start;
+    S.f_dt_max = p->dt_max;
+    S.f_dt_inc = p->dt_inc;
+    S.f_dt_dec = p->dt_dec;
+    S.f_alpha_dec = p->alpha_dec;
+    S.f_nmin = p->n_min;
+    S.f_cutoff = p->force_cutoff;
+    S.f_alpha_min = p->alpha_min;
+    S.f_max_iter = p->max_iterations;
+    S.f_stop_rms = p->stop_rms;
+    S.f_iter = 0;
+    S.f_nsince = 0;
+    S.f_power = 0;
+    S.f_force_max = 100.;
+    S.mix_pending = 0;
+    HIPCHECK(ctx, hipMemcpyAsync(ctx->d.sys + sys, &S, sizeof(SysParams), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return SVM_OK;
+}
+
+int svm_fire_minimize(svm_context* ctx, int32_t check_every) {
+    if (!ctx) return SVM_ERR_ARG;
+    int r = require_topology(ctx);
+    if (r) return r;
+    if (check_every <= 0) check_every = 50;
+    // minimize(): forces at the start, forceMax = 110, then iterate while
+    // (iterations < max && forceMax > cutoff) (energyMinimizerFIRE.cpp:239-249)
+    std::vector<SysParams> s;
+    r = download_sys(ctx, s);
+    if (r) return r;
+    for (int i = 0; i < ctx->nsys; ++i) {
+        s[i].f_force_max = 110.0;
+        s[i].f_frozen = !((s[i].f_iter < s[i].f_max_iter) && (110.0 > s[i].f_cutoff));
+        s[i].mix_pending = 0;
+        r = set_field(ctx, i, offsetof(SysParams, f_force_max), s[i].f_force_max);
+        r = r ? r : set_field(ctx, i, offsetof(SysParams, f_frozen), s[i].f_frozen);
+        r = r ? r : set_field(ctx, i, offsetof(SysParams, mix_pending), 0);
+        if (r) return r;
+    }
+    r = launch_geometry(ctx, 0);
+    if (r) return r;
+    r = launch_vertex<svm::VM_FORCE>(ctx, ctx->pcur, 1, "k_vertex_force");
+    if (r) return r;
+    while (true) {
+        bool any = false;
+        for (int i = 0; i < ctx->nsys; ++i) any |= !s[i].f_frozen;
+        if (!any) break;
+        for (int k = 0; k < check_every; ++k) {
+            r = enqueue_fire_iteration(ctx);
+            if (r) return r;
+        }
+        r = download_sys(ctx, s);
+        if (r) return r;
+        r = check_device_errors(ctx);
+        if (r) return r;
+    }
+    hipLaunchKernelGGL(svm::k_fire_finalize, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d);
+    r = last_launch(ctx);
+    if (r) return r;
+    for (int i = 0; i < ctx->nsys; ++i) {
+        r = set_field(ctx, i, offsetof(SysParams, mix_pending), 0);
+        if (r) return r;
+    }
+    return svm_synchronize(ctx);
+}
+
+int svm_fire_get_state(svm_context* ctx, int32_t sys, svm_fire_state* st) {
+    if (!valid_sys(ctx, sys) || !st) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    std::vector<SysParams> s;
+    int r = download_sys(ctx, s);
+    if (r) return r;
+    const SysParams& S = s[sys];
+    st->iterations = S.f_iter;
+    st->n_since_negative_power = S.f_nsince;
+    st->converged = S.f_force_max <= S.f_cutoff;
+    st->reserved = 0;
+    st->dt = S.f_dt;
+    st->alpha = S.f_alpha;
+    st->force_max = S.f_force_max;
+    st->power = S.f_power;
+    return SVM_OK;
+}
+
+int svm_set_profiling(svm_context* ctx, int32_t on) {
+    if (!ctx) return SVM_ERR_ARG;
+    ctx->profiling = on != 0;
+    ctx->timing.clear();
+    return SVM_OK;
+}
+
+int svm_get_timing(svm_context* ctx, const char** names, double* ms, int32_t n, int32_t* n_written) {
+    if (!ctx || !n_written) return SVM_ERR_ARG;
+    int k = 0;
+    ctx->timing_names.clear();
+    for (auto& e : ctx->timing) ctx->timing_names.push_back(e.first);
+    for (auto& e : ctx->timing) {
+        if (k >= n) break;
+        if (names) names[k] = ctx->timing_names[k].c_str();
+        if (ms) ms[k] = e.second.second ? e.second.first / e.second.second : 0.0;
+        ++k;
+    }
+    *n_written = k;
+    return SVM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,192 @@
+/* svm.h -- C ABI of the MI355X spherical vertex-model library (libsvm.so).
+ *
+ * One context holds a batch of independent spheres ("systems") resident in
+ * HBM.  Every entry point takes plain pointers and sizes, returns an int
+ * status (SVM_OK == 0) and never throws.  Work is queued on the context's
+ * HIP stream; entry points that return values to the host synchronise that
+ * stream (their names say "get" or they document it).
+ *
+ * Host arrays use the reference's layouts:
+ *   positions / directors / velocities / forces / displacements : double[3*V]
+ *     (one dVec of three doubles per vertex, inc/dDimensionalVectorTypes.h:19-34)
+ *   vertex_neighbors, vertex_cell_neighbors : int32[3*V] (stride 3,
+ *     src/model/convexHullCGAL.cpp:131-150)
+ *   cell_vertices : int32[n_cells*ring_stride], entry (k, c) at c*ring_stride+k
+ *     (Index2D, inc/indexer.h:41-44; src/model/convexHullCGAL.cpp:163-185)
+ *   cell_vertex_counts : int32[n_cells]
+ *   area_perimeter, preferences : double[2*n_cells] as (x, y) pairs (scalar2)
+ * All indices are local to their system (0-based).
+ *
+ * The reference's kernel-caller layer each entry point replaces is cited
+ * beside it (paths relative to the reference repository root).
+ */
+#ifndef SVM_H
+#define SVM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct svm_context svm_context;
+
+enum svm_status {
+    SVM_OK = 0,
+    SVM_ERR_ARG = 1,            /* bad argument (null pointer, index, size) */
+    SVM_ERR_HIP = 2,            /* a HIP runtime call failed */
+    SVM_ERR_NO_DEVICE = 3,      /* no gfx950 device visible */
+    SVM_ERR_RING_OVERFLOW = 4,  /* a T1 would grow a cell beyond ring_capacity */
+    SVM_ERR_TOPOLOGY = 5,       /* inconsistent topology (ring/vertex lists disagree) */
+    SVM_ERR_DELAUNAY = 6,       /* spherical Delaunay construction failed */
+    SVM_ERR_STATE = 7,          /* call out of order (e.g. no topology yet) */
+    SVM_ERR_CAPACITY = 8        /* a fixed-size work buffer would overflow */
+};
+
+/* One sphere of the batch.  Mirrors the constructor arguments of
+ * sphericalVertexModel(n, noise, a0, p0, ...) (src/model/sphericalVertexModel.h:13)
+ * plus the values its constructor fixes (src/model/sphericalVertexModel.cpp:12-13,79-80). */
+typedef struct {
+    int32_t n_cells;        /* N, number of cells */
+    int32_t ring_capacity;  /* max vertices per cell the ring storage holds (0 -> 32) */
+    double a0, p0;          /* preferred area and perimeter for every cell */
+    double kr;              /* K_A (Kr); K_P is 1 */
+    double t1_threshold;    /* T1 edge-length threshold (reference default 0.1) */
+    uint64_t seed;          /* key of this sphere's Philox noise streams */
+} svm_system_desc;
+
+/* Per-sphere FIRE parameters: the arguments of
+ * energyMinimizerFIRE::setFIREParameters (src/updaters/energyMinimizerFIRE.cpp:262-273)
+ * and setMaximumIterations.  stop_rms = 0 keeps the reference stop test
+ * sqrt(sum F^2)/V <= force_cutoff (energyMinimizerFIRE.cpp:199); 1 uses the
+ * rms force sqrt(sum F^2 / V). */
+typedef struct {
+    double dt, alpha_start, dt_max, dt_inc, dt_dec, alpha_dec;
+    int32_t n_min;
+    int32_t max_iterations;
+    double force_cutoff, alpha_min;
+    int32_t stop_rms;
+    int32_t reserved;
+} svm_fire_params;
+
+/* FIRE state of one sphere (energyMinimizerFIRE.h:71-96). */
+typedef struct {
+    int32_t iterations, n_since_negative_power, converged, reserved;
+    double dt, alpha, force_max, power;
+} svm_fire_state;
+
+/* ---- lifetime ---------------------------------------------------------- */
+/* Allocate device storage for n_systems spheres on the current HIP device. */
+int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** out);
+int svm_destroy(svm_context* ctx);
+/* Use `stream` (a hipStream_t, NULL = the context's own stream) for all work. */
+int svm_set_stream(svm_context* ctx, void* stream);
+int svm_synchronize(svm_context* ctx);
+/* Vertex count (2N-4) of system `sys` once its topology is set. */
+int svm_get_num_vertices(svm_context* ctx, int32_t sys, int32_t* n_vertices);
+/* Text of the last error on this context (static storage, never NULL). */
+const char* svm_last_error(svm_context* ctx);
+
+/* ---- initial topology -------------------------------------------------- */
+/* Spherical Delaunay triangulation of the n_cells cell centres on the GPU
+ * (exact in-circle predicates), then the vertex model's topology and vertex
+ * positions (normalised triangle centroids) from it.  Replaces
+ * convexHullCGALInterface::sphericalConvexHullForVertexModel
+ * (src/model/convexHullCGAL.cpp:82-186) as called by the constructor
+ * (src/model/sphericalVertexModel.cpp:14-18).  cell_points: double[3N]. */
+int svm_build_from_cell_points(svm_context* ctx, int32_t sys, const double* cell_points);
+/* Seeded cell centres: sphericalModel::setParticlePositionsRandomly + setRadius
+ * (src/model/sphericalModel.cpp:11-51) with Philox draws keyed by `seed`. */
+int svm_generate_cell_points(int32_t n_cells, uint64_t seed, double* cell_points);
+/* Seeded unit tangent directors (src/model/sphericalVertexModel.cpp:58-73). */
+int svm_generate_directors(int32_t n_vertices, const double* positions, uint64_t seed, double* directors);
+
+/* Upload a complete topology in the reference layout (see file header). */
+int svm_set_topology(svm_context* ctx, int32_t sys, int32_t n_vertices,
+                     const double* vertex_positions, const int32_t* vertex_neighbors,
+                     const int32_t* vertex_cell_neighbors, const int32_t* cell_vertices,
+                     int32_t ring_stride, const int32_t* cell_vertex_counts);
+int svm_get_topology(svm_context* ctx, int32_t sys, int32_t* vertex_neighbors,
+                     int32_t* vertex_cell_neighbors, int32_t* cell_vertices,
+                     int32_t ring_stride, int32_t* cell_vertex_counts);
+/* Spherical Delaunay triangles of the last svm_build_from_cell_points call:
+ * int32[3*(2N-4)], each counter-clockwise seen from outside, rotated so the
+ * smallest index is first, sorted lexicographically. */
+int svm_get_delaunay_triangles(svm_context* ctx, int32_t sys, int32_t* triangles);
+
+/* ---- state ------------------------------------------------------------- */
+int svm_set_positions(svm_context* ctx, int32_t sys, const double* positions);
+int svm_get_positions(svm_context* ctx, int32_t sys, double* positions);
+int svm_set_directors(svm_context* ctx, int32_t sys, const double* directors);
+int svm_get_directors(svm_context* ctx, int32_t sys, double* directors);
+int svm_set_velocities(svm_context* ctx, int32_t sys, const double* velocities);
+int svm_get_velocities(svm_context* ctx, int32_t sys, double* velocities);
+int svm_get_forces(svm_context* ctx, int32_t sys, double* forces);
+int svm_get_cell_positions(svm_context* ctx, int32_t sys, double* cell_positions);
+int svm_get_area_perimeter(svm_context* ctx, int32_t sys, double* area_perimeter);
+/* sphericalVertexModel::setPreferredParameters (src/model/sphericalVertexModel.cpp:103-109). */
+int svm_set_preferred_parameters(svm_context* ctx, int32_t sys, double a0, double p0);
+/* Per-cell (a0, p0) pairs, double[2N]. */
+int svm_set_preferences(svm_context* ctx, int32_t sys, const double* preferences);
+/* setScalarModelParameter (Kr) and setT1Threshold (src/model/sphericalVertexModel.h:82-85). */
+int svm_set_scalar_model_parameter(svm_context* ctx, int32_t sys, double kr);
+int svm_set_t1_threshold(svm_context* ctx, int32_t sys, double threshold);
+
+/* ---- model operations (all systems of the context) ---------------------- */
+/* computeGeometry: gpu_spherical_vertex_model_geometry
+ * (src/model/sphericalVertexModel.cuh:34-48), CPU semantics of
+ * computeGeometryCPU (src/model/sphericalVertexModel.cpp:209-311). */
+int svm_compute_geometry(svm_context* ctx);
+/* computeForces: geometry then gpu_quadratic_spherical_cellular_force
+ * (src/model/sphericalVertexModel.cuh:50-63; CPU semantics
+ * src/model/sphericalVertexModel.cpp:356-420).  precision: 0 = FP64 (parity),
+ * 1 = FP32 forces from FP64 positions and cell centres. */
+int svm_compute_forces(svm_context* ctx, int32_t precision);
+/* moveParticles(displacements, scale) then enforceTopology:
+ * gpu_move_particles_on_sphere + the T1 callers
+ * (src/model/sphericalVertexModel.cuh:14-92, sphericalVertexModel.cpp:111-148).
+ * displacements: host double[3V] for system `sys`; other systems stay still. */
+int svm_move_particles(svm_context* ctx, int32_t sys, const double* displacements, double scale);
+/* enforceTopology alone (src/model/sphericalVertexModel.cpp:422-433). */
+int svm_enforce_topology(svm_context* ctx);
+/* Energies after the last geometry: potential sum Kr(A-A0)^2+(P-P0)^2
+ * (computeEnergy, sphericalVertexModel.cpp:150-160) and kinetic
+ * sum m v^2/2 (simpleModel.cpp:63-73); double[n_systems] each (may be NULL). */
+int svm_get_energies(svm_context* ctx, double* potential, double* kinetic);
+/* Counters since creation: T1 flips, fan-area fallbacks, skipped NaN
+ * contributions; int64[3*n_systems]. */
+int svm_get_counters(svm_context* ctx, int64_t* counters);
+
+/* ---- updaters ---------------------------------------------------------- */
+/* brownianDynamics (src/updaters/brownianDynamics.cpp:29-48; kernel caller
+ * gpu_brownian_eom_integration, src/updaters/brownianDynamics.cuh:15-22). */
+int svm_set_brownian(svm_context* ctx, int32_t sys, double dt, double mu, double temperature);
+int svm_brownian_steps(svm_context* ctx, int32_t n_steps);
+/* sphericalSelfPropelledParticle (src/updaters/sphericalSelfPropelledParticle.cpp:16-47,
+ * parameters src/updaters/vectorialVicsek.h:188-191; tau = 1/Dr). */
+int svm_set_active(svm_context* ctx, int32_t sys, double dt, double mu, double v0, double tau);
+int svm_active_steps(svm_context* ctx, int32_t n_steps);
+/* Capture `steps_per_graph` steps of the given updater (0 Brownian, 1 active)
+ * into a HIP graph and replay it n_replays times. */
+int svm_run_graph(svm_context* ctx, int32_t updater, int32_t steps_per_graph, int32_t n_replays);
+
+/* energyMinimizerFIRE (src/updaters/energyMinimizerFIRE.cpp:181-259,
+ * velocityVerlet.cpp:6-38; kernel callers gpu_update_velocity_FIRE,
+ * gpu_dVec_dot_products, gpu_displacement_velocity_verlet,
+ * gpu_update_velocity). */
+int svm_fire_set_parameters(svm_context* ctx, int32_t sys, const svm_fire_params* params);
+/* Minimise every sphere until its stop test passes or its iteration limit is
+ * reached; the host checks convergence every `check_every` iterations. */
+int svm_fire_minimize(svm_context* ctx, int32_t check_every);
+int svm_fire_get_state(svm_context* ctx, int32_t sys, svm_fire_state* state);
+
+/* Kernel timing: when on, every eagerly launched kernel is bracketed by HIP
+ * events on the context stream (this serialises launches; graphs are never
+ * timed this way).  svm_get_timing fills up to n (name, average ms) pairs. */
+int svm_set_profiling(svm_context* ctx, int32_t on);
+int svm_get_timing(svm_context* ctx, const char** names, double* ms, int32_t n, int32_t* n_written);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SVM_H */

@@ -1,0 +1,131 @@
+"""GPU parity: the HIP library against the CPU oracle on identical seeded inputs.
+
+Bar (FP64 parity mode): topology bit-exact; areas, perimeters, cell centres,
+forces, positions, directors and velocities bit-exact (assert_array_equal),
+which is stricter than the 1e-10 relative the path promises.  Energies are
+compared against the oracle's tree-ordered sum bit-exactly and against the
+reference's sequential sum within 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+import svmtest
+import curvedspacevertexmodel_amd as svm
+
+pytestmark = pytest.mark.gpu
+
+
+def make_ctx(topos, p0=3.8, kr=1.0, seeds=None):
+    systems = []
+    for i, t in enumerate(topos):
+        systems.append(dict(n_cells=t["cnn"].shape[0], a0=1.0, p0=p0, kr=kr, t1_threshold=0.1,
+                            seed=(seeds[i] if seeds else 1137 + i)))
+    ctx = svm.Context(systems)
+    for i, t in enumerate(topos):
+        ctx.set_topology(i, t["vpos"], t["vn"], t["vcn"], t["ring"], t["cnn"])
+        if "directors" in t:
+            ctx.set_directors(i, t["directors"])
+    return ctx
+
+
+def assert_topology_equal(ctx, sys, om):
+    vn, vcn, ring, cnn = ctx.get_topology(sys, svmtest.RING_CAP)
+    ovn, ovcn, oring, ocnn = om.topology(svmtest.RING_CAP)
+    np.testing.assert_array_equal(cnn, ocnn)
+    np.testing.assert_array_equal(vn, ovn)
+    np.testing.assert_array_equal(vcn, ovcn)
+    for c in range(len(cnn)):
+        np.testing.assert_array_equal(ring[c, :cnn[c]], oring[c, :ocnn[c]])
+
+
+@pytest.fixture(scope="module")
+def state512():
+    return svmtest.seeded_state(512, 1137)
+
+
+def test_geometry_forces_n512(state512):
+    om = svmtest.OracleModel(state512)
+    om.compute_forces()
+    ctx = make_ctx([state512])
+    ctx.compute_forces()
+    np.testing.assert_array_equal(ctx.area_perimeter(0), om.area_perimeter())
+    np.testing.assert_array_equal(ctx.cell_positions(0), om.cell_positions())
+    np.testing.assert_array_equal(ctx.forces(0), om.forces())
+    pe, ke = ctx.energies()
+    assert pe[0] == om.energy(mode=1)
+    assert abs(pe[0] - om.energy(mode=0)) <= 1e-12 * abs(om.energy(mode=0))
+    # sum of areas is the sphere's area
+    assert abs(ctx.area_perimeter(0)[:, 0].sum() - 512.0) < 1e-9
+
+
+def test_active_steps_n512(state512):
+    om = svmtest.OracleModel(state512)
+    ctx = make_ctx([state512])
+    ctx.set_active(0, dt=0.01, mu=1.0, v0=0.1, tau=1.0)
+    for chunk in range(4):
+        om.active(50, dt=0.01, mu=1.0, v0=0.1, tau=1.0, seed=1137, step0=50 * chunk)
+        ctx.active_steps(50)
+        np.testing.assert_array_equal(ctx.positions(0), om.positions())
+        np.testing.assert_array_equal(ctx.directors(0), om.directors())
+        assert_topology_equal(ctx, 0, om)
+    assert ctx.counters()[0, 0] == om.n_t1
+    assert om.n_t1 > 0, "the run should exercise T1 flips"
+
+
+def test_brownian_noise_steps_n512(state512):
+    om = svmtest.OracleModel(state512, p0=3.9)
+    ctx = make_ctx([state512], p0=3.9)
+    ctx.set_brownian(0, dt=0.01, mu=1.0, T=0.05)
+    om.brownian(60, dt=0.01, mu=1.0, T=0.05, seed=1137, step0=0)
+    ctx.brownian_steps(60)
+    np.testing.assert_array_equal(ctx.positions(0), om.positions())
+    assert_topology_equal(ctx, 0, om)
+    assert ctx.counters()[0, 0] == om.n_t1
+
+
+def test_move_particles_triggers_t1(state512):
+    rng = np.random.default_rng(7)
+    om = svmtest.OracleModel(state512)
+    ctx = make_ctx([state512])
+    disp = rng.normal(scale=0.15, size=(state512["vpos"].shape[0], 3))
+    om.move(disp, 1.0)
+    ctx.move_particles(0, disp, 1.0)
+    np.testing.assert_array_equal(ctx.positions(0), om.positions())
+    assert_topology_equal(ctx, 0, om)
+    om.move(disp, -0.5)
+    ctx.move_particles(0, disp, -0.5)
+    np.testing.assert_array_equal(ctx.positions(0), om.positions())
+    assert_topology_equal(ctx, 0, om)
+    assert om.n_t1 > 10
+
+
+def test_fire_n512(state512):
+    om = svmtest.OracleModel(state512, p0=3.6)
+    ctx = make_ctx([state512], p0=3.6)
+    fp = dict(dt=0.01, alpha_start=0.99, dt_max=0.5, dt_inc=1.1, dt_dec=0.95, alpha_dec=0.9, n_min=4,
+              force_cutoff=1e-12, alpha_min=0.0, max_iterations=600)
+    ores = om.fire(**fp)
+    ctx.fire_set_parameters(0, **fp)
+    ctx.fire_minimize(check_every=64)
+    st = ctx.fire_state(0)
+    assert st["iterations"] == ores["iterations"]
+    assert st["force_max"] == ores["force_max"]
+    assert st["dt"] == ores["dt"] and st["alpha"] == ores["alpha"]
+    np.testing.assert_array_equal(ctx.positions(0), om.positions())
+    np.testing.assert_array_equal(ctx.velocities(0), om.velocities())
+    assert_topology_equal(ctx, 0, om)
+
+
+def test_batch_equivalence_and_graph(state512):
+    s2 = svmtest.seeded_state(300, 1138)
+    s3 = svmtest.seeded_state(1200, 1139)
+    batch = make_ctx([state512, s2, s3], seeds=[1137, 1138, 1139])
+    for i in range(3):
+        batch.set_active(i, dt=0.01, mu=1.0, v0=0.05 * (i + 1), tau=1.0)
+    batch.run_graph(1, 10, 3)
+    for i, s in enumerate([state512, s2, s3]):
+        alone = make_ctx([s], seeds=[1137 + i])
+        alone.set_active(0, dt=0.01, mu=1.0, v0=0.05 * (i + 1), tau=1.0)
+        alone.active_steps(30)
+        np.testing.assert_array_equal(batch.positions(i), alone.positions(0))
+        np.testing.assert_array_equal(batch.directors(i), alone.directors(0))
