@@ -1,0 +1,190 @@
+#!/usr/bin/env python
+"""Throughput of the spherical vertex model on one MI355X.
+
+Workload (default): 256 independent spheres in one batch, N log-uniform in
+[200, 20000], p0 uniform in [3.5, 4.0], v0 uniform in [0.01, 0.5],
+Dr in {0.1, 1, 10} (tau = 1/Dr), dt = 0.01, mu = 1, K_A = K_P = 1, A0 = 1,
+FP64; active Brownian dynamics with T1 topology maintenance on every step;
+steps replayed from a captured HIP graph.  Every sphere is first stabilised
+as the reference driver does (200 Brownian T=0 steps with (A0, P0) = (1, 1),
+untimed).  A "step" is one timestep of every
+sphere; the metric is cell-steps/s = sum_N * steps / wall time, with the
+state resident in HBM before the timed region starts.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-steps/s (batched spheres)"
+PEAK_HBM_GBS = 8000.0
+# Algorithmic HBM bytes per cell per step of the vertex pass (k_vertex_active):
+# two vertices per cell, each reading its position (24 B) and director (24 B),
+# writing its new position (24 B) and reading its vertex-cell and
+# vertex-vertex lists (2 x 12 B): 2 x 96 = 192 B.
+VERTEX_BYTES_PER_CELL = 192.0
+STEP_BYTES_PER_CELL = 244.0   # whole active step (positions, directors r/w, topology)
+
+
+def batch_params(n_sys, seed=13377, nmin=200, nmax=20000):
+    """Per-sphere (N, p0, v0, Dr), drawn in sphere order from one Philox generator."""
+    g = np.random.Generator(np.random.Philox(seed))
+    out = []
+    for _ in range(n_sys):
+        u = g.random(4)
+        n = int(round(np.exp(np.log(nmin) + u[0] * (np.log(nmax) - np.log(nmin)))))
+        p0 = 3.5 + 0.5 * u[1]
+        v0 = 0.01 + 0.49 * u[2]
+        dr = [0.1, 1.0, 10.0][min(2, int(u[3] * 3))]
+        out.append((n, p0, v0, dr))
+    return out
+
+
+def stabilise(ctx, params, steps):
+    """The reference driver's stabilisation (sphericalVertex.cpp:100-110):
+    Brownian dynamics at T=0 with (A0, P0) = (1, 1), then the target (1, p0).
+    From raw random Voronoi starts the dynamics otherwise squeeze small cells
+    until they invert (the model has no T2 move)."""
+    for i in range(len(params)):
+        ctx.set_preferred_parameters(i, 1.0, 1.0)
+        ctx.set_brownian(i, dt=0.01, mu=1.0, T=0.0)
+    ctx.run_graph(0, 10, steps // 10)
+    ctx.synchronize()
+    for i, (n, p0, v0, dr) in enumerate(params):
+        ctx.set_preferred_parameters(i, 1.0, p0)
+
+
+def health(ctx, params):
+    """Geometry invariants after the run: the angle-sum areas must tile each sphere."""
+    ctx.compute_geometry()
+    dev = 0.0
+    for i, (n, *_rest) in enumerate(params):
+        dev = max(dev, abs(ctx.area_perimeter(i)[:, 0].sum() / n - 1.0))
+    c = ctx.counters()
+    return {"max_rel_area_sum_error": dev, "t1_flips": int(c[:, 0].sum()), "fan_fallbacks": int(c[:, 1].sum()),
+            "nan_contributions": int(c[:, 2].sum()), "t1_skipped_no_fourth_cell": int(c[:, 3].sum())}
+
+
+def build_batch(svm, params):
+    from scipy.spatial import ConvexHull
+    systems = [dict(n_cells=n, a0=1.0, p0=p0, kr=1.0, t1_threshold=0.1, seed=1137 + i)
+               for i, (n, p0, v0, dr) in enumerate(params)]
+    ctx = svm.Context(systems)
+    for i, (n, p0, v0, dr) in enumerate(params):
+        pts = svm.generate_cell_points(n, 1137 + i)
+        tris = ConvexHull(pts).simplices.astype(np.int32)
+        ctx.set_topology_from_triangles(i, pts, tris)
+        ctx.set_directors(i, svm.generate_directors(ctx.positions(i), 1137 + i))
+        ctx.set_active(i, dt=0.01, mu=1.0, v0=v0, tau=1.0 / dr)
+    return ctx
+
+
+def cpu_baseline(ctx, params, budget_s=12.0):
+    """The CPU oracle (a restatement of the reference's CPU path), one thread,
+    advancing the first spheres of the same batch from the same state."""
+    import ctypes as C
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import svmtest
+    L = svmtest.oracle_lib()
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    models, cells = [], 0
+    for i, (n, p0, v0, dr) in enumerate(params):
+        vn, vcn, ring, cnn = ctx.get_topology(i, 32)
+        pos = np.ascontiguousarray(ctx.positions(i))
+        h = C.c_void_p(L.or_create(C.c_int(n), C.c_int(2 * n - 4), C.c_double(np.sqrt(n / (4 * np.pi))), dp(pos),
+                                   ip(vn), ip(vcn), ip(np.ascontiguousarray(ring)), C.c_int(32), ip(cnn)))
+        L.or_set_preferences(h, dp(np.tile(np.array([1.0, p0]), n)))
+        L.or_set_directors(h, dp(np.ascontiguousarray(ctx.directors(i))))
+        models.append((h, n, v0, dr, i))
+        cells += n
+        if cells >= 20000:
+            break
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        for h, n, v0, dr, i in models:
+            L.or_active_steps(h, C.c_int(10), C.c_double(0.01), C.c_double(1.0), C.c_double(v0),
+                              C.c_double(1.0 / dr), C.c_uint64(1137 + i), C.c_uint64(steps))
+        steps += 10
+    dt = time.perf_counter() - t0
+    for h, *_ in models:
+        L.or_destroy(h)
+    return dict(value=cells * steps / dt, unit="cell-steps/s", cores=1, kind="port",
+                sample=f"first {len(models)} spheres of the batch ({cells} cells), {steps} active steps, "
+                       f"single-threaded oracle, {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--spheres", type=int, default=256)
+    ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        sys.exit(2)
+    import curvedspacevertexmodel_amd as svm
+
+    params = batch_params(args.spheres)
+    total_cells = int(sum(p[0] for p in params))
+    t_setup = time.perf_counter()
+    ctx = build_batch(svm, params)
+    t_setup = time.perf_counter() - t_setup
+    G = max(2, args.graph_steps - args.graph_steps % 2)
+    warm = max(G, args.warmup - args.warmup % G)
+    steps = max(G, args.steps - args.steps % G)
+    print(f"setup {t_setup:.1f} s: {len(params)} spheres, {total_cells} cells", flush=True)
+    stabilise(ctx, params, 200)
+    print(f"stabilised (200 Brownian steps); warmup {warm} active steps", flush=True)
+    ctx.run_graph(1, G, warm // G)
+    ctx.synchronize()
+    print("warmup done", flush=True)
+    t0 = time.perf_counter()
+    ctx.run_graph(1, G, steps // G)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = total_cells * steps / elapsed
+    print(f"timed {steps} steps in {elapsed:.3f} s", flush=True)
+
+    # per-kernel durations with HIP events on the library's stream (eager steps)
+    ctx.set_profiling(True)
+    ctx.active_steps(G)
+    ctx.synchronize()
+    timing = ctx.timing()
+    ctx.set_profiling(False)
+    dom = "k_vertex_active"
+    dom_ms = timing.get(dom, float("nan"))
+    achieved = VERTEX_BYTES_PER_CELL * total_cells / (dom_ms * 1e-3) / 1e9
+    result = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": warm, "ms_per_step": 1e3 * elapsed / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batched-active-256", "spheres": len(params), "total_cells": total_cells,
+                   "N_range": [200, 20000], "p0_range": [3.5, 4.0], "v0_range": [0.01, 0.5],
+                   "Dr": [0.1, 1, 10], "dt": 0.01, "steps_per_graph": G},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "bytes_per_launch": VERTEX_BYTES_PER_CELL * total_cells, "avg_launch_ms": dom_ms},
+        "kernel_ms": timing,
+        "health": health(ctx, params),
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(ctx, params)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

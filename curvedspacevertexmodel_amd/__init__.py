@@ -23,7 +23,7 @@ STATUS = {
 EXPORTS = [
     "svm_create", "svm_destroy", "svm_set_stream", "svm_synchronize", "svm_get_num_vertices",
     "svm_last_error", "svm_build_from_cell_points", "svm_generate_cell_points",
-    "svm_generate_directors", "svm_set_topology", "svm_get_topology", "svm_get_delaunay_triangles",
+    "svm_generate_directors", "svm_set_topology_from_triangles", "svm_set_topology", "svm_get_topology", "svm_get_delaunay_triangles",
     "svm_set_positions", "svm_get_positions", "svm_set_directors", "svm_get_directors",
     "svm_set_velocities", "svm_get_velocities", "svm_get_forces", "svm_get_cell_positions",
     "svm_get_area_perimeter", "svm_set_preferred_parameters", "svm_set_preferences",
@@ -146,6 +146,12 @@ class Context:
                                         _ip(vcn), _ip(ring), C.c_int32(ring.shape[1]), _ip(counts)),
                  "svm_set_topology")
 
+    def set_topology_from_triangles(self, sys, points, triangles):
+        pts = np.ascontiguousarray(points, np.float64)
+        tris = np.ascontiguousarray(triangles, np.int32)
+        self._ck(lib().svm_set_topology_from_triangles(self.h, C.c_int32(sys), _dp(pts), C.c_int32(tris.shape[0]),
+                                                       _ip(tris)), "svm_set_topology_from_triangles")
+
     def get_topology(self, sys, stride=32):
         nv, nc = self.n_vertices[sys], self.n_cells[sys]
         vn = np.zeros((nv, 3), np.int32)
@@ -240,7 +246,7 @@ class Context:
         return pe, ke
 
     def counters(self):
-        out = np.zeros((self.n, 3), np.int64)
+        out = np.zeros((self.n, 4), np.int64)
         self._ck(lib().svm_get_counters(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "svm_get_counters")
         return out
 

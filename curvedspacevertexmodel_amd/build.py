@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libsvm.so")
 
 SOURCES = ["svm_capi.hip"]
-HEADERS = ["svm_fp64.h", "svm_geom.h", "svm_device.h", "svm_kernels.h", "svm_delaunay.h"]
+HEADERS = ["svm_fp64.h", "svm_geom.h", "svm_device.h", "svm_kernels.h", "svm_delaunay.h", "svm_exact.h"]
 
 
 def hipcc():

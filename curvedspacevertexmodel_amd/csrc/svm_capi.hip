@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstddef>
 #include <cstdio>
@@ -485,6 +486,107 @@ int svm_set_topology(svm_context* ctx, int32_t sys, int32_t n_vertices, const do
     return SVM_OK;
 }
 
+// Vertex-model topology from a spherical Delaunay triangulation, in the
+// conventions of convexHullCGALInterface::sphericalConvexHullForVertexModel
+// (src/model/convexHullCGAL.cpp:82-186): a vertex per triangle at
+// R_hull * normalize((a + b + c) / 3) (getFacetCenter, :67-80; R_hull = |p_0|,
+// :84), summed in the triangle's circulation order; vertex slot k holds cell
+// t[k] and the triangle across the edge (t[k-1], t[k]); cell rings list the
+// triangles around each point clockwise seen from outside (the halfedge-
+// around-vertex order), starting at the lowest triangle index.  Triangles are
+// first put in canonical form: counter-clockwise seen from outside, smallest
+// index first, sorted.
+static int topology_from_triangles(svm_context* ctx, int sys, const double* pts, int ntri, const int32_t* tris_in) {
+    const SysParams& L = ctx->layout[sys];
+    const int n = L.nc;
+    if (ntri != 2 * n - 4) return fail(ctx, SVM_ERR_TOPOLOGY, "a triangulation of the sphere has 2N-4 triangles");
+    std::vector<std::array<int, 3>> T(ntri);
+    for (int t = 0; t < ntri; ++t) {
+        int a = tris_in[3 * t], b = tris_in[3 * t + 1], c = tris_in[3 * t + 2];
+        if (a < 0 || b < 0 || c < 0 || a >= n || b >= n || c >= n || a == b || b == c || a == c)
+            return fail(ctx, SVM_ERR_TOPOLOGY, "bad triangle");
+        if (svm::orient_origin(pts + 3 * a, pts + 3 * b, pts + 3 * c) < 0) std::swap(b, c);
+        int m[3] = {a, b, c};
+        int k = (a < b) ? (a < c ? 0 : 2) : (b < c ? 1 : 2);
+        T[t] = {m[k], m[(k + 1) % 3], m[(k + 2) % 3]};
+    }
+    std::sort(T.begin(), T.end());
+    std::vector<int32_t> flat(3 * (size_t)ntri);
+    for (int t = 0; t < ntri; ++t)
+        for (int k = 0; k < 3; ++k) flat[3 * (size_t)t + k] = T[t][k];
+    // directed edge a->b -> triangle; each point's out-edges
+    std::vector<int> deg(n, 0);
+    for (int t = 0; t < ntri; ++t)
+        for (int k = 0; k < 3; ++k) deg[T[t][k]]++;
+    std::vector<int> off(n + 1, 0);
+    for (int i = 0; i < n; ++i) off[i + 1] = off[i] + deg[i];
+    std::vector<int> eb(off[n]), et(off[n]), fill(off.begin(), off.end() - 1);
+    for (int t = 0; t < ntri; ++t)
+        for (int k = 0; k < 3; ++k) {
+            int a = T[t][k], b = T[t][(k + 1) % 3];
+            eb[fill[a]] = b;
+            et[fill[a]] = t;
+            fill[a]++;
+        }
+    auto tri_of = [&](int a, int b) -> int {
+        for (int e = off[a]; e < off[a + 1]; ++e)
+            if (eb[e] == b) return et[e];
+        return -1;
+    };
+    int maxdeg = 0;
+    for (int i = 0; i < n; ++i) maxdeg = std::max(maxdeg, deg[i]);
+    if (maxdeg > ctx->cap) return fail(ctx, SVM_ERR_RING_OVERFLOW, "a cell has more vertices than ring_capacity");
+    double radius = std::sqrt(pts[0] * pts[0] + pts[1] * pts[1] + pts[2] * pts[2]);
+    std::vector<double> vpos(3 * (size_t)ntri);
+    std::vector<int32_t> vn(3 * (size_t)ntri), vcn(3 * (size_t)ntri);
+    for (int t = 0; t < ntri; ++t) {
+        double c[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < 3; ++k) {
+            const double* p = pts + 3 * (size_t)T[t][k];
+            c[0] += p[0];
+            c[1] += p[1];
+            c[2] += p[2];
+        }
+        for (int k = 0; k < 3; ++k) c[k] = (1.0 / 3) * c[k];
+        double nrm = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        for (int k = 0; k < 3; ++k) vpos[3 * (size_t)t + k] = (radius / nrm) * c[k];
+        for (int k = 0; k < 3; ++k) {
+            int prev = T[t][(k + 2) % 3], cur = T[t][k];
+            vcn[3 * (size_t)t + k] = cur;
+            int nb = tri_of(cur, prev);
+            if (nb < 0) return fail(ctx, SVM_ERR_TOPOLOGY, "triangulation is not closed");
+            vn[3 * (size_t)t + k] = nb;
+        }
+    }
+    std::vector<int32_t> ring((size_t)n * ctx->cap, -1), cnt(n);
+    for (int c = 0; c < n; ++c) {
+        int start = -1, b = -1;
+        for (int e = off[c]; e < off[c + 1]; ++e)
+            if (start < 0 || et[e] < start) { start = et[e]; b = eb[e]; }
+        int t = start;
+        for (int k = 0; k < deg[c]; ++k) {
+            ring[(size_t)c * ctx->cap + k] = t;
+            int tn = tri_of(b, c);
+            if (tn < 0) return fail(ctx, SVM_ERR_TOPOLOGY, "triangulation is not closed");
+            int y = -1;
+            for (int kk = 0; kk < 3; ++kk)
+                if (T[tn][kk] == c) y = T[tn][(kk + 1) % 3];
+            t = tn;
+            b = y;
+        }
+        if (t != start) return fail(ctx, SVM_ERR_TOPOLOGY, "a point's triangles do not close into one ring");
+        cnt[c] = deg[c];
+    }
+    ctx->triangles[sys] = flat;
+    return svm_set_topology(ctx, sys, ntri, vpos.data(), vn.data(), vcn.data(), ring.data(), ctx->cap, cnt.data());
+}
+
+int svm_set_topology_from_triangles(svm_context* ctx, int32_t sys, const double* cell_points, int32_t n_triangles,
+                                    const int32_t* triangles) {
+    if (!valid_sys(ctx, sys) || !cell_points || !triangles) return fail(ctx, SVM_ERR_ARG, "null argument");
+    return topology_from_triangles(ctx, sys, cell_points, n_triangles, triangles);
+}
+
 int svm_get_topology(svm_context* ctx, int32_t sys, int32_t* vertex_neighbors, int32_t* vertex_cell_neighbors,
                      int32_t* cell_vertices, int32_t ring_stride, int32_t* cell_vertex_counts) {
     if (!valid_sys(ctx, sys) || !vertex_neighbors || !vertex_cell_neighbors || !cell_vertices || !cell_vertex_counts)
@@ -659,9 +761,10 @@ int svm_get_counters(svm_context* ctx, int64_t* counters) {
     int r = download_sys(ctx, s);
     if (r) return r;
     for (int i = 0; i < ctx->nsys; ++i) {
-        counters[3 * i] = s[i].n_t1;
-        counters[3 * i + 1] = s[i].n_fan;
-        counters[3 * i + 2] = s[i].n_nan;
+        counters[4 * i] = s[i].n_t1;
+        counters[4 * i + 1] = s[i].n_fan;
+        counters[4 * i + 2] = s[i].n_nan;
+        counters[4 * i + 3] = s[i].n_t1_skipped;
     }
     return SVM_OK;
 }

@@ -3,6 +3,7 @@
 #pragma once
 #include <stdint.h>
 #include "svm_fp64.h"
+#include "svm_exact.h"
 
 namespace svm {
 
@@ -51,6 +52,10 @@ inline void generate_directors(int nv, const double* pos, uint64_t seed, double*
         double inv = 1.0 / __builtin_sqrt(nn);
         for (int k = 0; k < 3; ++k) out[3 * (size_t)i + k] = inv * a[k];
     }
+}
+
+inline int orient_origin(const double* a, const double* b, const double* c) {
+    return orient_origin_sign(a, b, c);
 }
 
 }  // namespace svm

@@ -46,7 +46,7 @@ struct SysParams {
     int32_t counter_geo, counter_vtx;
     int32_t t1_flag, err;
     double energy_pot, energy_kin;
-    long long n_t1, n_fan, n_nan;
+    long long n_t1, n_fan, n_nan, n_t1_skipped;
 };
 
 struct Dev {

@@ -484,7 +484,11 @@ __device__ inline int& vcnref(const Dev& d, int v, int k) { return ((int*)&d.vcn
 __device__ inline int& vnref(const Dev& d, int v, int k) { return ((int*)&d.vn[v])[k]; }
 
 // getCellVertexSetForT1 (src/model/sphericalVertexModel.cpp:611-731).
-__device__ inline void cell_vertex_set(const Dev& d, int vertex1, int vertex2, int* cs, int* vs) {
+// Returns false when the four cells cannot all be determined (vertex1 and
+// vertex2 share all three cells, so there is no cell l); the flip is then
+// skipped, where the reference would read an uninitialised cell set.
+__device__ inline bool cell_vertex_set(const Dev& d, int vertex1, int vertex2, int* cs, int* vs) {
+    cs[0] = cs[1] = cs[2] = cs[3] = -1;
     int cell1 = vcnref(d, vertex1, 0), cell2 = vcnref(d, vertex1, 1), cell3 = vcnref(d, vertex1, 2);
     for (int ff = 0; ff < 3; ++ff) {
         int ctest = vcnref(d, vertex2, ff);
@@ -507,6 +511,7 @@ __device__ inline void cell_vertex_set(const Dev& d, int vertex1, int vertex2, i
         else if (vnext == vertex2) cs[2] = cell;
         else cs[1] = cell;
     }
+    if (cs[0] < 0 || cs[1] < 0 || cs[2] < 0 || cs[3] < 0) return false;
     cneigh = d.cnn[cs[1]];
     vlast = ringref(d, cs[1], cneigh - 2);
     vcur = ringref(d, cs[1], cneigh - 1);
@@ -529,6 +534,7 @@ __device__ inline void cell_vertex_set(const Dev& d, int vertex1, int vertex2, i
     }
     vs[3] = vlast;
     vs[2] = vnext;
+    return true;
 }
 
 // Rebuild vln for vertex x from the current rings; returns false if a ring
@@ -742,7 +748,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_resolve(Dev d, int pcur, int ni
         Heap heap{heapbuf, 0};
         int ci = 0, last = -1, nm = 0, nt = 0, err = 0;
         bool full_scan = false;
-        long long flips = 0;
+        long long flips = 0, skipped = 0;
         while (true) {
             int u;
             if (!full_scan) {
@@ -765,8 +771,17 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_resolve(Dev d, int pcur, int ni
                     double q[3] = {X[vertex2], Y[vertex2], Z[vertex2]};
                     double arcLength = geodesic(v1, q, pnorm(v1), pnorm(q), R);
                     if (arcLength < thr) {
-                        int cs[4] = {0, 0, 0, 0}, vs[4] = {0, 0, 0, 0};
-                        cell_vertex_set(d, gu, vertex2, cs, vs);
+                        int cs[4], vs[4] = {0, 0, 0, 0};
+                        if (!cell_vertex_set(d, gu, vertex2, cs, vs)) {
+                            ++skipped;
+                            continue;
+                        }
+                        bool inside = true;
+                        for (int k = 0; k < 4; ++k) {
+                            inside &= cs[k] >= S->co && cs[k] < S->co + S->nc;
+                            inside &= vs[k] >= vo && vs[k] < vo + nv;
+                        }
+                        if (!inside) { err = 5; break; }
                         if (d.cnn[cs[0]] == 3 || d.cnn[cs[2]] == 3) continue;
                         int r = t1_flip(d, pcur, gu, vertex2, cs, vs, v1, R, tmp);
                         if (r) { err = r; break; }
@@ -780,7 +795,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_resolve(Dev d, int pcur, int ni
                         if (!full_scan) {
                             int dirty[6] = {u, vertex2 - vo, vs[0] - vo, vs[1] - vo, vs[2] - vo, vs[3] - vo};
                             for (int k = 0; k < 6; ++k) {
-                                if (dirty[k] > u) {
+                                if (dirty[k] > u && dirty[k] < nv) {
                                     if (heap.n < SVM_HEAP_CAP) heap.push(dirty[k]);
                                     else full_scan = true;
                                 }
@@ -792,6 +807,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_resolve(Dev d, int pcur, int ni
             if (err) break;
         }
         S->n_t1 += flips;
+        S->n_t1_skipped += skipped;
         sh_nmoved = nm;
         sh_ntouched = nt;
         sh_err = err;

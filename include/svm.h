@@ -101,6 +101,13 @@ int svm_generate_cell_points(int32_t n_cells, uint64_t seed, double* cell_points
 /* Seeded unit tangent directors (src/model/sphericalVertexModel.cpp:58-73). */
 int svm_generate_directors(int32_t n_vertices, const double* positions, uint64_t seed, double* directors);
 
+/* Topology from a spherical Delaunay triangulation computed elsewhere:
+ * triangles int32[3*n_triangles] (any orientation and order; n_triangles must
+ * be 2N-4), cell_points double[3N].  Builds vertex positions and lists in the
+ * conventions of sphericalConvexHullForVertexModel
+ * (src/model/convexHullCGAL.cpp:82-186). */
+int svm_set_topology_from_triangles(svm_context* ctx, int32_t sys, const double* cell_points,
+                                    int32_t n_triangles, const int32_t* triangles);
 /* Upload a complete topology in the reference layout (see file header). */
 int svm_set_topology(svm_context* ctx, int32_t sys, int32_t n_vertices,
                      const double* vertex_positions, const int32_t* vertex_neighbors,
@@ -153,8 +160,9 @@ int svm_enforce_topology(svm_context* ctx);
  * (computeEnergy, sphericalVertexModel.cpp:150-160) and kinetic
  * sum m v^2/2 (simpleModel.cpp:63-73); double[n_systems] each (may be NULL). */
 int svm_get_energies(svm_context* ctx, double* potential, double* kinetic);
-/* Counters since creation: T1 flips, fan-area fallbacks, skipped NaN
- * contributions; int64[3*n_systems]. */
+/* Counters since creation, int64[4*n_systems]: T1 flips, fan-area
+ * fallbacks, skipped NaN force contributions, T1 flips skipped because the
+ * edge's two vertices share all three cells (no fourth cell exists). */
 int svm_get_counters(svm_context* ctx, int64_t* counters);
 
 /* ---- updaters ---------------------------------------------------------- */

@@ -244,6 +244,7 @@ struct Model {
     int nT1 = 0;                                        // flips performed (diagnostic)
     int nFanRecompute = 0;                              // fan fallbacks taken
     int nNan = 0;                                       // NaN force contributions skipped
+    int nT1Skipped = 0;                                 // flips skipped: no cell l
     int reduce_mode = 0;                                // 0: sequential sums, 1: tree_sum
 
     int cni(int k, int c) const { return c * maxDeg + k; }
@@ -399,8 +400,14 @@ struct Model {
         cellNeighbors.swap(nc);
     }
 
-    // sphericalVertexModel::getCellVertexSetForT1, src/model/sphericalVertexModel.cpp:611-731
-    void getCellVertexSetForT1(int vertex1, int vertex2, int cs[4], int vs[4], bool& growList) {
+    // sphericalVertexModel::getCellVertexSetForT1, src/model/sphericalVertexModel.cpp:611-731.
+    // Returns false when the four cells cannot all be determined: vertex1 and
+    // vertex2 then share all three cells (a cell touching both ends of the
+    // edge), there is no cell l, and the reference would read its int4
+    // cellSet uninitialised (left over from an earlier flip); such a flip is
+    // skipped here, as the triangle rule skips a flip.
+    bool getCellVertexSetForT1(int vertex1, int vertex2, int cs[4], int vs[4], bool& growList) {
+        cs[0] = cs[1] = cs[2] = cs[3] = -1;
         int cell1 = vertexCellNeighbors[3 * vertex1];
         int cell2 = vertexCellNeighbors[3 * vertex1 + 1];
         int cell3 = vertexCellNeighbors[3 * vertex1 + 2];
@@ -426,6 +433,7 @@ struct Model {
             else if (vnext == vertex2) cs[2] = cell;
             else cs[1] = cell;
         }
+        if (cs[0] < 0 || cs[1] < 0 || cs[2] < 0 || cs[3] < 0) return false;
         cneigh = cellNumberOfNeighbors[cs[1]];
         vlast = cellNeighbors[cni(cneigh - 2, cs[1])];
         vcur = cellNeighbors[cni(cneigh - 1, cs[1])];
@@ -451,6 +459,7 @@ struct Model {
         if ((int)cellNumberOfNeighbors[cs[0]] == maxDeg || (int)cellNumberOfNeighbors[cs[1]] == maxDeg ||
             (int)cellNumberOfNeighbors[cs[2]] == maxDeg || (int)cellNumberOfNeighbors[cs[3]] == maxDeg)
             growList = true;
+        return true;
     }
 
     // sphericalVertexModel::testAndPerformT1TransitionsCPU, src/model/sphericalVertexModel.cpp:439-604
@@ -466,8 +475,11 @@ struct Model {
                     sphere.geodesicDistance(v1, v2, arcLength);
                     if (arcLength < t1Threshold) {
                         bool grow = false;
-                        int cs[4] = {0, 0, 0, 0}, vs[4] = {0, 0, 0, 0};
-                        getCellVertexSetForT1(vertex1, vertex2, cs, vs, grow);
+                        int cs[4], vs[4] = {0, 0, 0, 0};
+                        if (!getCellVertexSetForT1(vertex1, vertex2, cs, vs, grow)) {
+                            ++nT1Skipped;
+                            continue;
+                        }
                         if (cellNumberOfNeighbors[cs[0]] == 3 || cellNumberOfNeighbors[cs[2]] == 3) continue;
                         if (grow) {
                             vMax += 1;
@@ -871,6 +883,7 @@ void or_fire_minimize(void* h, const double* params, double* state, int reduce_m
 int or_num_vertices(void* h) { return ((Model*)h)->N; }
 int or_max_deg(void* h) { return ((Model*)h)->maxDeg; }
 int or_num_t1(void* h) { return ((Model*)h)->nT1; }
+int or_num_t1_skipped(void* h) { return ((Model*)h)->nT1Skipped; }
 int or_num_fan(void* h) { return ((Model*)h)->nFanRecompute; }
 int or_num_nan(void* h) { return ((Model*)h)->nNan; }
 void or_get_positions(void* h, double* out) {

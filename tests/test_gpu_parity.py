@@ -129,3 +129,42 @@ def test_batch_equivalence_and_graph(state512):
         alone.active_steps(30)
         np.testing.assert_array_equal(batch.positions(i), alone.positions(0))
         np.testing.assert_array_equal(batch.directors(i), alone.directors(0))
+
+
+def test_topology_from_triangles_matches_oracle(state512):
+    """The library's triangle -> topology builder follows the same conventions as the oracle's."""
+    ctx = svm.Context([dict(n_cells=512, seed=1137)])
+    tris = state512["tris"].copy()
+    rng = np.random.default_rng(3)
+    tris = tris[rng.permutation(len(tris))]          # any order and orientation is accepted
+    flip = rng.random(len(tris)) < 0.5
+    tris[flip] = tris[flip][:, [0, 2, 1]]
+    ctx.set_topology_from_triangles(0, state512["points"], tris)
+    np.testing.assert_array_equal(ctx.delaunay_triangles(0), state512["tris"])
+    np.testing.assert_array_equal(ctx.positions(0), state512["vpos"])
+    vn, vcn, ring, cnn = ctx.get_topology(0, svmtest.RING_CAP)
+    np.testing.assert_array_equal(vn, state512["vn"])
+    np.testing.assert_array_equal(vcn, state512["vcn"])
+    np.testing.assert_array_equal(cnn, state512["cnn"])
+    for c in range(512):
+        np.testing.assert_array_equal(ring[c, :cnn[c]], state512["ring"][c, :cnn[c]])
+
+
+def test_flip_without_fourth_cell_is_skipped():
+    """Sphere 35 of the batched workload reaches, at step ~370, an edge whose two
+    vertices share all three cells; that flip is skipped on both sides and the
+    run stays bit-identical to the oracle with a valid topology."""
+    n, p0, v0, dr, seed = 2842, 3.865664562197794, 0.4494973611929375, 1.0, 1137 + 35
+    st = svmtest.seeded_state(n, seed)
+    om = svmtest.OracleModel(st, p0=p0)
+    ctx = make_ctx([st], p0=p0, seeds=[seed])
+    ctx.set_active(0, dt=0.01, mu=1.0, v0=v0, tau=1.0 / dr)
+    om.active(400, dt=0.01, mu=1.0, v0=v0, tau=1.0 / dr, seed=seed, step0=0)
+    ctx.active_steps(400)
+    ctx.synchronize()
+    np.testing.assert_array_equal(ctx.positions(0), om.positions())
+    assert_topology_equal(ctx, 0, om)
+    skipped = om.L.or_num_t1_skipped(om.h)
+    assert skipped >= 1
+    assert ctx.counters()[0, 3] == skipped
+    assert ctx.counters()[0, 0] == om.n_t1
