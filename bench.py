@@ -70,7 +70,8 @@ def health(ctx, params):
         dev = max(dev, abs(ctx.area_perimeter(i)[:, 0].sum() / n - 1.0))
     c = ctx.counters()
     return {"max_rel_area_sum_error": dev, "t1_flips": int(c[:, 0].sum()), "fan_fallbacks": int(c[:, 1].sum()),
-            "nan_contributions": int(c[:, 2].sum()), "t1_skipped_no_fourth_cell": int(c[:, 3].sum())}
+            "nan_contributions": int(c[:, 2].sum()), "t1_skipped_no_fourth_cell": int(c[:, 3].sum()),
+            "t1_sequential_sphere_steps": int(c[:, 4].sum())}
 
 
 def build_batch(svm, params):

@@ -246,7 +246,7 @@ class Context:
         return pe, ke
 
     def counters(self):
-        out = np.zeros((self.n, 4), np.int64)
+        out = np.zeros((self.n, 5), np.int64)
         self._ck(lib().svm_get_counters(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "svm_get_counters")
         return out
 

@@ -34,6 +34,7 @@ struct svm_context {
     std::vector<SysParams> layout;   // sizes/offsets (host copy of the constant part)
     std::vector<int> have_topology;
     int Cs = 0, Vs = 0, cap = 0, nct = 0, nvt = 0;
+    int cluster_blocks = 256;        // T1 dry-run grid (persistent loop over candidates)
     Dev d{};
     int pcur = 0, ncur = 0;          // current position / director buffer
     std::string err;
@@ -180,8 +181,15 @@ int last_launch(svm_context* ctx) {
 
 // ---- step building blocks ----
 int launch_geometry(svm_context* ctx, int skip_frozen) {
-    Timer t(ctx, "k_geometry");
-    hipLaunchKernelGGL(svm::k_geometry, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, skip_frozen);
+    {
+        Timer t(ctx, "k_geometry");
+        hipLaunchKernelGGL(svm::k_geometry, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur,
+                           skip_frozen);
+        int r = last_launch(ctx);
+        if (r) return r;
+    }
+    Timer t(ctx, "k_fan_fix");
+    hipLaunchKernelGGL(svm::k_fan_fix, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, skip_frozen);
     return last_launch(ctx);
 }
 
@@ -198,6 +206,13 @@ int launch_t1(svm_context* ctx, int pcur, int nin, int nout, int dir, int advanc
         Timer t(ctx, "k_t1_detect");
         hipLaunchKernelGGL(svm::k_t1_detect, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d, pcur, nin, nout,
                            dir, fire_mode);
+        int r = last_launch(ctx);
+        if (r) return r;
+    }
+    {
+        Timer t(ctx, "k_t1_cluster");
+        hipLaunchKernelGGL(svm::k_t1_cluster, dim3(ctx->cluster_blocks), dim3(64 * SVM_T1_WAVES), 0, ctx->stream,
+                           ctx->d, pcur);
         int r = last_launch(ctx);
         if (r) return r;
     }
@@ -238,7 +253,11 @@ int enqueue_fire_iteration(svm_context* ctx) {
     ctx->pcur = pout;
     r = launch_geometry(ctx, 1);
     if (r) return r;
-    return launch_vertex<svm::VM_FIRE>(ctx, ctx->pcur, 1, "k_vertex_fire");
+    r = launch_vertex<svm::VM_FIRE>(ctx, ctx->pcur, 1, "k_vertex_fire");
+    if (r) return r;
+    Timer t(ctx, "k_fire_control");
+    hipLaunchKernelGGL(svm::k_fire_control, dim3(ctx->nsys), dim3(SVM_TILE), 0, ctx->stream, ctx->d);
+    return last_launch(ctx);
 }
 
 int require_topology(svm_context* ctx) {
@@ -288,6 +307,10 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
         L.vo = (int)Vs;
         L.ctile0 = (int)(Cs / SVM_TILE);
         L.nctiles = round_up(L.nc, SVM_TILE) / SVM_TILE;
+        if (L.nctiles > SVM_MAX_TILES || round_up(2 * L.nc - 4, SVM_TILE) / SVM_TILE > SVM_MAX_TILES) {
+            delete ctx;
+            return SVM_ERR_ARG;   // more than 2^20 rows in one sphere
+        }
         L.vtile0 = (int)(Vs / SVM_TILE);
         L.nvtiles = round_up(L.nv, SVM_TILE) / SVM_TILE;
         Cs += round_up(L.nc, SVM_TILE);
@@ -343,6 +366,12 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
     r |= dalloc(ctx, &d.cpart, ctx->nct);
     r |= dalloc(ctx, &d.vpart, 3 * (size_t)ctx->nvt);
     r |= dalloc(ctx, &d.disp, 3 * (size_t)Vs);
+    r |= dalloc(ctx, &d.t1_count, 1);
+    r |= dalloc(ctx, &d.t1_gen, 1);
+    r |= dalloc(ctx, &d.t1_list, SVM_T1_MAXCL);
+    r |= dalloc(ctx, &d.t1_log, SVM_T1_MAXCL);
+    r |= dalloc(ctx, &d.vown, Vs);
+    r |= dalloc(ctx, &d.cown, Cs);
     if (r) {
         svm_destroy(ctx);
         return SVM_ERR_HIP;
@@ -366,9 +395,12 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
         P.f_alpha_dec = 0.9; P.f_cutoff = 1e-7; P.f_alpha_min = 0.0; P.f_dt_min = 0.01 * 0.01;
         P.f_nmin = 5; P.f_max_iter = 1000;
         P.f_dt = 0.01; P.f_alpha = 0.99; P.f_force_max = 100.;
+        P.t1_head = -1;
         hs[s] = P;
     }
-    if (hipMemcpy(cts, hct.data(), sizeof(int32_t) * ctx->nct, hipMemcpyHostToDevice) != hipSuccess ||
+    uint32_t gen1 = 1;   // stamps of generation 0 mean "never touched"
+    if (hipMemcpy(d.t1_gen, &gen1, sizeof(gen1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(cts, hct.data(), sizeof(int32_t) * ctx->nct, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(vts, hvt.data(), sizeof(int32_t) * ctx->nvt, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d.sys, hs.data(), sizeof(SysParams) * n_systems, hipMemcpyHostToDevice) != hipSuccess) {
         svm_destroy(ctx);
@@ -745,6 +777,9 @@ int svm_get_energies(svm_context* ctx, double* potential, double* kinetic) {
     hipLaunchKernelGGL(svm::k_energy_vertices, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d);
     r = last_launch(ctx);
     if (r) return r;
+    hipLaunchKernelGGL(svm::k_energy_fold, dim3(ctx->nsys), dim3(SVM_TILE), 0, ctx->stream, ctx->d);
+    r = last_launch(ctx);
+    if (r) return r;
     std::vector<SysParams> s;
     r = download_sys(ctx, s);
     if (r) return r;
@@ -761,10 +796,11 @@ int svm_get_counters(svm_context* ctx, int64_t* counters) {
     int r = download_sys(ctx, s);
     if (r) return r;
     for (int i = 0; i < ctx->nsys; ++i) {
-        counters[4 * i] = s[i].n_t1;
-        counters[4 * i + 1] = s[i].n_fan;
-        counters[4 * i + 2] = s[i].n_nan;
-        counters[4 * i + 3] = s[i].n_t1_skipped;
+        counters[5 * i] = s[i].n_t1;
+        counters[5 * i + 1] = s[i].n_fan;
+        counters[5 * i + 2] = s[i].n_nan;
+        counters[5 * i + 3] = s[i].n_t1_skipped;
+        counters[5 * i + 4] = s[i].n_t1_sequential;
     }
     return SVM_OK;
 }

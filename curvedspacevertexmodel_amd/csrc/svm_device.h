@@ -23,6 +23,8 @@
 
 namespace svm {
 
+struct T1Log;
+
 struct SysParams {
     // sizes and offsets
     int32_t nc, nv, co, vo;
@@ -45,8 +47,10 @@ struct SysParams {
     // reductions and diagnostics
     int32_t counter_geo, counter_vtx;
     int32_t t1_flag, err;
+    int32_t fan_flag, pad2;
+    int32_t t1_ncand, t1_done, t1_overlap, t1_head;   // dry-run bookkeeping
     double energy_pot, energy_kin;
-    long long n_t1, n_fan, n_nan, n_t1_skipped;
+    long long n_t1, n_fan, n_nan, n_t1_skipped, n_t1_sequential;
 };
 
 struct Dev {
@@ -72,6 +76,13 @@ struct Dev {
     double* cpart;                // per cell tile partial sums
     double* vpart;                // 3 per vertex tile partial sums
     double* disp;                 // AoS displacement staging (3 per vertex row)
+    // T1 dry runs
+    int32_t* t1_count;            // candidates appended this step (all spheres)
+    uint32_t* t1_gen;             // step generation of the ownership stamps
+    int32_t* t1_list;             // candidate vertex rows
+    struct T1Log* t1_log;         // per-candidate result logs
+    unsigned long long* vown;     // ownership stamps, one per vertex row
+    unsigned long long* cown;     // ownership stamps, one per cell row
 };
 
 }  // namespace svm

@@ -44,6 +44,11 @@ SVM_HD double withwords(int32_t hi, uint32_t lo) {
 }
 
 // ---------------------------------------------------------------- acos ----
+// The three argument ranges of the method share one rational approximation
+// R(z) = p(z)/q(z); it is evaluated once for the range's z and the three
+// range-specific combinations are formed and selected, so a wavefront whose
+// lanes fall in different ranges does not execute three divergent paths.
+// Each lane's result is exactly the arithmetic of its own range.
 SVM_HD double acos(double x) {
     const double pi = 3.14159265358979311600e+00;
     const double pio2_hi = 1.57079632679489655800e+00;
@@ -60,39 +65,24 @@ SVM_HD double acos(double x) {
     const double qS4 = 7.70381505559019352791e-02;
     int32_t hx = hiword(x);
     int32_t ix = hx & 0x7fffffff;
-    if (ix >= 0x3ff00000) {                     // |x| >= 1
-        if (((ix - 0x3ff00000) | (int32_t)loword(x)) == 0) {
-            if (hx > 0) return 0.0;             // acos(1) = 0
-            return pi + 2.0 * pio2_lo;          // acos(-1) = pi
-        }
-        return (x - x) / (x - x);               // |x| > 1 or NaN: NaN
+    bool small = ix < 0x3fe00000;                           // |x| < 0.5
+    double z = small ? x * x : (hx < 0 ? (1.0 + x) * 0.5 : (1.0 - x) * 0.5);
+    double p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
+    double q = 1.0 + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
+    double r = p / q;
+    double s = __builtin_sqrt(z);
+    double df = bitsd(dbits(s) & 0xffffffff00000000ull);
+    double c = (z - df * df) / (s + df);
+    double res_small = pio2_hi - (x - (pio2_lo - x * r));   // |x| < 0.5
+    double res_neg = pi - 2.0 * (s + (r * s - pio2_lo));     // x <= -0.5
+    double res_pos = 2.0 * (df + (r * s + c));               // x >= 0.5
+    double res = small ? res_small : (hx < 0 ? res_neg : res_pos);
+    if (ix <= 0x3c600000) res = pio2_hi + pio2_lo;          // |x| < 2^-57
+    if (ix >= 0x3ff00000) {                                 // |x| >= 1
+        if (((ix - 0x3ff00000) | (int32_t)loword(x)) == 0) res = hx > 0 ? 0.0 : pi + 2.0 * pio2_lo;
+        else res = (x - x) / (x - x);                       // |x| > 1 or NaN
     }
-    if (ix < 0x3fe00000) {                      // |x| < 0.5
-        if (ix <= 0x3c600000) return pio2_hi + pio2_lo;
-        double z = x * x;
-        double p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
-        double q = 1.0 + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
-        double r = p / q;
-        return pio2_hi - (x - (pio2_lo - x * r));
-    } else if (hx < 0) {                        // x < -0.5
-        double z = (1.0 + x) * 0.5;
-        double p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
-        double q = 1.0 + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
-        double s = __builtin_sqrt(z);
-        double r = p / q;
-        double w = r * s - pio2_lo;
-        return pi - 2.0 * (s + w);
-    } else {                                    // x > 0.5
-        double z = (1.0 - x) * 0.5;
-        double s = __builtin_sqrt(z);
-        double df = bitsd(dbits(s) & 0xffffffff00000000ull);
-        double c = (z - df * df) / (s + df);
-        double p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
-        double q = 1.0 + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
-        double r = p / q;
-        double w = r * s + c;
-        return 2.0 * (df + w);
-    }
+    return res;
 }
 
 // ------------------------------------------------------ sin/cos kernels ----

@@ -160,9 +160,11 @@ int svm_enforce_topology(svm_context* ctx);
  * (computeEnergy, sphericalVertexModel.cpp:150-160) and kinetic
  * sum m v^2/2 (simpleModel.cpp:63-73); double[n_systems] each (may be NULL). */
 int svm_get_energies(svm_context* ctx, double* potential, double* kinetic);
-/* Counters since creation, int64[4*n_systems]: T1 flips, fan-area
+/* Counters since creation, int64[5*n_systems]: T1 flips, fan-area
  * fallbacks, skipped NaN force contributions, T1 flips skipped because the
- * edge's two vertices share all three cells (no fourth cell exists). */
+ * edge's two vertices share all three cells (no fourth cell exists), and
+ * steps whose T1 flips were applied by the sequential pass because their
+ * candidates' neighbourhoods overlapped. */
 int svm_get_counters(svm_context* ctx, int64_t* counters);
 
 /* ---- updaters ---------------------------------------------------------- */
