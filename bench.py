@@ -69,9 +69,12 @@ def health(ctx, params):
     for i, (n, *_rest) in enumerate(params):
         dev = max(dev, abs(ctx.area_perimeter(i)[:, 0].sum() / n - 1.0))
     c = ctx.counters()
-    return {"max_rel_area_sum_error": dev, "t1_flips": int(c[:, 0].sum()), "fan_fallbacks": int(c[:, 1].sum()),
+    r = ctx.t1_diagnostics().sum(axis=0)
+    return {"t1_sequential_reasons": {"flippable_list_overflow": int(r[0]), "group_too_large": int(r[1]),
+                                      "group_out_of_room": int(r[2]), "blocked_list_overflow": int(r[3])},
+            "t1_candidates": {"flippable": int(r[4]), "blocked": int(r[5])},"max_rel_area_sum_error": dev, "t1_flips": int(c[:, 0].sum()), "fan_fallbacks": int(c[:, 1].sum()),
             "nan_contributions": int(c[:, 2].sum()), "t1_skipped_no_fourth_cell": int(c[:, 3].sum()),
-            "t1_sequential_sphere_steps": int(c[:, 4].sum())}
+            "t1_sequential_sphere_steps": int(c[:, 4].sum()), "t1_group_sphere_steps": int(c[:, 5].sum())}
 
 
 def build_batch(svm, params):

@@ -31,7 +31,7 @@ EXPORTS = [
     "svm_compute_forces", "svm_move_particles", "svm_enforce_topology", "svm_get_energies",
     "svm_get_counters", "svm_set_brownian", "svm_brownian_steps", "svm_set_active",
     "svm_active_steps", "svm_run_graph", "svm_fire_set_parameters", "svm_fire_minimize",
-    "svm_fire_get_state", "svm_set_profiling", "svm_get_timing",
+    "svm_fire_get_state", "svm_set_t1_mode", "svm_get_t1_diagnostics", "svm_set_profiling", "svm_get_timing",
 ]
 
 
@@ -246,7 +246,7 @@ class Context:
         return pe, ke
 
     def counters(self):
-        out = np.zeros((self.n, 5), np.int64)
+        out = np.zeros((self.n, 6), np.int64)
         self._ck(lib().svm_get_counters(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "svm_get_counters")
         return out
 
@@ -288,6 +288,14 @@ class Context:
 
     def set_stream(self, stream_ptr):
         self._ck(lib().svm_set_stream(self.h, C.c_void_p(stream_ptr)), "svm_set_stream")
+
+    def t1_diagnostics(self):
+        out = np.zeros((self.n, 6), np.int64)
+        self._ck(lib().svm_get_t1_diagnostics(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "diagnostics")
+        return out
+
+    def set_t1_mode(self, sequential):
+        self._ck(lib().svm_set_t1_mode(self.h, C.c_int32(1 if sequential else 0)), "svm_set_t1_mode")
 
     def set_profiling(self, on):
         self._ck(lib().svm_set_profiling(self.h, C.c_int32(1 if on else 0)), "svm_set_profiling")

@@ -35,6 +35,7 @@ struct svm_context {
     std::vector<int> have_topology;
     int Cs = 0, Vs = 0, cap = 0, nct = 0, nvt = 0;
     int cluster_blocks = 256;        // T1 dry-run grid (persistent loop over candidates)
+    int t1_sequential = 0;           // 1: skip the dry runs, always use the sequential pass
     Dev d{};
     int pcur = 0, ncur = 0;          // current position / director buffer
     std::string err;
@@ -209,16 +210,28 @@ int launch_t1(svm_context* ctx, int pcur, int nin, int nout, int dir, int advanc
         int r = last_launch(ctx);
         if (r) return r;
     }
-    {
+    if (!ctx->t1_sequential) {
         Timer t(ctx, "k_t1_cluster");
         hipLaunchKernelGGL(svm::k_t1_cluster, dim3(ctx->cluster_blocks), dim3(64 * SVM_T1_WAVES), 0, ctx->stream,
                            ctx->d, pcur);
         int r = last_launch(ctx);
         if (r) return r;
     }
+    if (!ctx->t1_sequential) {
+        Timer t(ctx, "k_t1_check");
+        hipLaunchKernelGGL(svm::k_t1_check, dim3(64), dim3(SVM_TILE), 0, ctx->stream, ctx->d, pcur);
+        int r = last_launch(ctx);
+        if (r) return r;
+    }
+    if (!ctx->t1_sequential) {
+        Timer t(ctx, "k_t1_group");
+        hipLaunchKernelGGL(svm::k_t1_group, dim3(ctx->nsys), dim3(64), 0, ctx->stream, ctx->d, pcur);
+        int r = last_launch(ctx);
+        if (r) return r;
+    }
     Timer t(ctx, "k_t1_resolve");
     hipLaunchKernelGGL(svm::k_t1_resolve, dim3(ctx->nsys), dim3(SVM_TILE), 0, ctx->stream, ctx->d, pcur, nin, nout,
-                       dir, advance_step, fire_mode);
+                       dir, advance_step, fire_mode, ctx->t1_sequential);
     return last_launch(ctx);
 }
 
@@ -367,8 +380,15 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
     r |= dalloc(ctx, &d.vpart, 3 * (size_t)ctx->nvt);
     r |= dalloc(ctx, &d.disp, 3 * (size_t)Vs);
     r |= dalloc(ctx, &d.t1_count, 1);
-    r |= dalloc(ctx, &d.t1_gen, 1);
     r |= dalloc(ctx, &d.t1_list, SVM_T1_MAXCL);
+    r |= dalloc(ctx, &d.t1_flagged, SVM_T1_MAXCL);
+    r |= dalloc(ctx, &d.t1_logged, SVM_T1_MAXCL);
+    r |= dalloc(ctx, &d.t1_total, 1);
+    r |= dalloc(ctx, &d.t1_bcount, 1);
+    r |= dalloc(ctx, &d.t1_blist, SVM_T1_MAXBL);
+    r |= dalloc(ctx, &d.t1_bflag, SVM_T1_MAXBL);
+    r |= dalloc(ctx, &d.t1_bskip, SVM_T1_MAXBL);
+    r |= dalloc(ctx, &d.t1_group, n_systems);
     r |= dalloc(ctx, &d.t1_log, SVM_T1_MAXCL);
     r |= dalloc(ctx, &d.vown, Vs);
     r |= dalloc(ctx, &d.cown, Cs);
@@ -396,11 +416,10 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
         P.f_nmin = 5; P.f_max_iter = 1000;
         P.f_dt = 0.01; P.f_alpha = 0.99; P.f_force_max = 100.;
         P.t1_head = -1;
+        P.t1_gen = 1;   // stamps of generation 0 mean "never touched"
         hs[s] = P;
     }
-    uint32_t gen1 = 1;   // stamps of generation 0 mean "never touched"
-    if (hipMemcpy(d.t1_gen, &gen1, sizeof(gen1), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(cts, hct.data(), sizeof(int32_t) * ctx->nct, hipMemcpyHostToDevice) != hipSuccess ||
+    if (hipMemcpy(cts, hct.data(), sizeof(int32_t) * ctx->nct, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(vts, hvt.data(), sizeof(int32_t) * ctx->nvt, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d.sys, hs.data(), sizeof(SysParams) * n_systems, hipMemcpyHostToDevice) != hipSuccess) {
         svm_destroy(ctx);
@@ -796,11 +815,12 @@ int svm_get_counters(svm_context* ctx, int64_t* counters) {
     int r = download_sys(ctx, s);
     if (r) return r;
     for (int i = 0; i < ctx->nsys; ++i) {
-        counters[5 * i] = s[i].n_t1;
-        counters[5 * i + 1] = s[i].n_fan;
-        counters[5 * i + 2] = s[i].n_nan;
-        counters[5 * i + 3] = s[i].n_t1_skipped;
-        counters[5 * i + 4] = s[i].n_t1_sequential;
+        counters[6 * i] = s[i].n_t1;
+        counters[6 * i + 1] = s[i].n_fan;
+        counters[6 * i + 2] = s[i].n_nan;
+        counters[6 * i + 3] = s[i].n_t1_skipped;
+        counters[6 * i + 4] = s[i].n_t1_sequential;
+        counters[6 * i + 5] = s[i].n_t1_group;
     }
     return SVM_OK;
 }
@@ -960,6 +980,29 @@ int svm_fire_get_state(svm_context* ctx, int32_t sys, svm_fire_state* st) {
     st->alpha = S.f_alpha;
     st->force_max = S.f_force_max;
     st->power = S.f_power;
+    return SVM_OK;
+}
+
+int svm_get_t1_diagnostics(svm_context* ctx, int64_t* reasons) {
+    if (!ctx || !reasons) return SVM_ERR_ARG;
+    std::vector<SysParams> s;
+    int r = download_sys(ctx, s);
+    if (r) return r;
+    for (int i = 0; i < ctx->nsys; ++i) {
+        for (int k = 0; k < 4; ++k) reasons[6 * i + k] = s[i].n_seq_reason[k];
+        reasons[6 * i + 4] = s[i].n_cand_flip;
+        reasons[6 * i + 5] = s[i].n_cand_block;
+    }
+    return SVM_OK;
+}
+
+int svm_set_t1_mode(svm_context* ctx, int32_t mode) {
+    if (!ctx || (mode != 0 && mode != 1)) return fail(ctx, SVM_ERR_ARG, "T1 mode must be 0 or 1");
+    ctx->t1_sequential = mode;
+    for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto& g : ctx->graph_defs) (void)hipGraphDestroy(g.second);
+    ctx->graphs.clear();
+    ctx->graph_defs.clear();
     return SVM_OK;
 }
 

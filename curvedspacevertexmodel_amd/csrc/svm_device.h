@@ -24,6 +24,7 @@
 namespace svm {
 
 struct T1Log;
+struct T1GroupLog;
 
 struct SysParams {
     // sizes and offsets
@@ -45,12 +46,13 @@ struct SysParams {
     double mix_alpha, mix_scaling;
     int32_t mix_pending, mix_zero;
     // reductions and diagnostics
-    int32_t counter_geo, counter_vtx;
     int32_t t1_flag, err;
-    int32_t fan_flag, pad2;
-    int32_t t1_ncand, t1_done, t1_overlap, t1_head;   // dry-run bookkeeping
+    int32_t t1_overlap, t1_overflow, t1_head, t1_gen;   // dry-run bookkeeping
+    int32_t t1_bskips, pad3;
     double energy_pot, energy_kin;
-    long long n_t1, n_fan, n_nan, n_t1_skipped, n_t1_sequential;
+    long long n_t1, n_fan, n_nan, n_t1_skipped, n_t1_sequential, n_t1_group;
+    long long n_seq_reason[4];   // sequential pass because: list overflow, group too large, group out of room, -
+    long long n_cand_flip, n_cand_block;   // candidates seen (flippable, blocked)
 };
 
 struct Dev {
@@ -78,9 +80,16 @@ struct Dev {
     double* disp;                 // AoS displacement staging (3 per vertex row)
     // T1 dry runs
     int32_t* t1_count;            // candidates appended this step (all spheres)
-    uint32_t* t1_gen;             // step generation of the ownership stamps
     int32_t* t1_list;             // candidate vertex rows
+    int32_t* t1_flagged;          // per candidate: generation in which its run overlapped
+    int32_t* t1_logged;           // per candidate: generation in which its run wrote a log
+    int32_t* t1_total;            // flippable-list length of the current step (copied before the reset)
+    int32_t* t1_bcount;           // blocked candidates appended this step
+    int32_t* t1_blist;            // blocked candidate vertex rows
+    int32_t* t1_bflag;            // per blocked candidate: generation in which it was entangled
+    int32_t* t1_bskip;            // per blocked candidate: its no-fourth-cell skips
     struct T1Log* t1_log;         // per-candidate result logs
+    struct T1GroupLog* t1_group;  // per-sphere group-run log
     unsigned long long* vown;     // ownership stamps, one per vertex row
     unsigned long long* cown;     // ownership stamps, one per cell row
 };

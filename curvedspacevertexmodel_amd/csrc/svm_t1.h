@@ -3,9 +3,10 @@
 // src/model/sphericalVertexModel.cpp:439-604): vertex1 ascending, slots 0..2,
 // every test seeing all earlier flips.
 //
-// Three launches per step:
+// Launches per step:
 //   k_t1_detect   thread per vertex: short-edge test on the moved positions
-//                 (bit mask + compact candidate list), active-director update.
+//                 (bit mask + compact lists of flippable and of blocked
+//                 candidates), active-director update.
 //   k_t1_cluster  one wavefront per candidate vertex u: a dry run of the
 //                 sequential loop restricted to u -- u's visit, then the
 //                 visits of every later vertex its flips disturbed, in index
@@ -14,6 +15,11 @@
 //                 would write.  When no two candidates of a sphere touched a
 //                 common item, their visits commute and any order gives the
 //                 sequential result.
+//   k_t1_check    blocked candidates (every short edge triangle-protected or
+//                 without a fourth cell) do nothing unless a run above
+//                 touched what they read; those that were are flagged.
+//   k_t1_group    block per sphere with flagged candidates: one dry run over
+//                 all of them together, in index order.
 //   k_t1_resolve  block per sphere: applies the logs of a sphere whose
 //                 candidates did not overlap; otherwise (or when a buffer was
 //                 too small) runs the sequential loop itself over the
@@ -27,7 +33,8 @@
 
 namespace svm {
 
-#define SVM_T1_MAXCL 4096     // candidates handled by dry runs per step
+#define SVM_T1_MAXCL 4096     // flippable candidates handled by dry runs per step
+#define SVM_T1_MAXBL 65536    // blocked candidates checked per step
 #define SVM_T1_WV 64          // vertices cached per dry run
 #define SVM_T1_WC 32          // cells cached per dry run
 #define SVM_T1_RC 32          // ring entries per cached cell
@@ -51,6 +58,23 @@ struct T1Log {
 __device__ inline int& ringref(const Dev& d, int c, int k) { return d.ring[(size_t)k * d.Cs + c]; }
 __device__ inline int& vcnref(const Dev& d, int v, int k) { return ((int*)&d.vcn[v])[k]; }
 __device__ inline int& vnref(const Dev& d, int v, int k) { return ((int*)&d.vn[v])[k]; }
+
+#define SVM_T1_GV 512         // vertices cached / logged by a group dry run
+#define SVM_T1_GC 128         // cells cached / logged by a group dry run
+#define SVM_T1_GMAX 64        // overlapping candidates one group dry run takes
+
+struct T1GroupLog {
+    int nvert, ncell, ok, pad;
+    long long flips, skipped;
+    int vid[SVM_T1_GV];
+    int vflag[SVM_T1_GV];
+    double vpos[SVM_T1_GV][3];
+    int vvn[SVM_T1_GV][3];
+    int vvcn[SVM_T1_GV][3];
+    int cid[SVM_T1_GC];
+    int ccnt[SVM_T1_GC];
+    int cring[SVM_T1_GC][SVM_T1_RC];
+};
 
 // Min-heap of local vertex indices (the disturbed vertices still to visit).
 struct Heap {
@@ -105,15 +129,24 @@ struct GlobalAcc {
 };
 
 // Private LDS copy of the vertices and cells one dry run touches (open
-// addressing on the global row), filled from HBM on first touch.  Every first
-// touch stamps the item with this dry run's (generation, id); meeting another
-// dry run's stamp of the same generation marks the sphere as overlapping.
+// addressing on the global row, WV and WC slots), filled from HBM on first
+// touch.
+//   Single-candidate dry runs (GROUP = false) stamp each first touch with
+//   (generation, id) by an atomic max; meeting another dry run's stamp of the
+//   same generation flags both runs as overlapping.
+//   Group dry runs (GROUP = true) only read the stamps: touching an item
+//   stamped by a run that was not flagged (whose result will be committed)
+//   makes the group fail.
+template <int WV, int WC, bool GROUP>
 struct CacheAcc {
     const Dev* d;
     int pcur;
-    unsigned long long stamp;
-    int32_t* overlap;
+    unsigned long long stamp;     // (generation << 32) | (id + 1)
+    int32_t* flagged;             // per candidate: generation in which it overlapped
+    int32_t* overlap;             // the sphere's overlap flag
     bool fail;
+    int* absorb;                  // group runs: foreign runs met (at most 8)
+    int* absorb_n;
     int* vkey;        // [WV]
     int* vflag;       // [WV]  1: position written, 2: lists written
     double* vp;       // [WV][3]
@@ -124,13 +157,30 @@ struct CacheAcc {
     int* ccnt;        // [WC]
     int* cring;       // [WC][RC]
 
+    static constexpr int log2i(int x) { return x <= 1 ? 0 : 1 + log2i(x >> 1); }
+
     __device__ void claim(unsigned long long* own) {
-        unsigned long long old = atomicMax(own, stamp);
-        if ((old >> 32) == (stamp >> 32) && old != stamp) *overlap = 1;
+        const unsigned gen = (unsigned)(stamp >> 32);
+        if (!GROUP) {
+            unsigned long long old = atomicMax(own, stamp);
+            if ((unsigned)(old >> 32) == gen && old != stamp) {
+                *overlap = 1;
+                flagged[(int)(stamp & 0xffffffffu) - 1] = (int)gen;
+                flagged[(int)(old & 0xffffffffu) - 1] = (int)gen;
+            }
+        } else {
+            unsigned long long old = __hip_atomic_load(own, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int o = (int)(old & 0xffffffffu) - 1;
+            if ((unsigned)(old >> 32) == gen && flagged[o] != (int)gen) {
+                // a run whose result would be committed: the group must absorb it
+                fail = true;
+                if (*absorb_n < 8) absorb[(*absorb_n)++] = o;
+            }
+        }
     }
     __device__ int vslot(int v) {
-        unsigned h = ((unsigned)v * 2654435761u) >> (32 - 6);
-        for (int probe = 0; probe < SVM_T1_WV; ++probe, h = (h + 1) & (SVM_T1_WV - 1)) {
+        unsigned h = ((unsigned)v * 2654435761u) >> (32 - log2i(WV));
+        for (int probe = 0; probe < WV; ++probe, h = (h + 1) & (WV - 1)) {
             if (vkey[h] == v) return h;
             if (vkey[h] < 0) {
                 vkey[h] = v;
@@ -149,8 +199,8 @@ struct CacheAcc {
         return 0;
     }
     __device__ int cslot(int c) {
-        unsigned h = ((unsigned)c * 2654435761u) >> (32 - 5);
-        for (int probe = 0; probe < SVM_T1_WC; ++probe, h = (h + 1) & (SVM_T1_WC - 1)) {
+        unsigned h = ((unsigned)c * 2654435761u) >> (32 - log2i(WC));
+        for (int probe = 0; probe < WC; ++probe, h = (h + 1) & (WC - 1)) {
             if (ckey[h] == c) return h;
             if (ckey[h] < 0) {
                 int n = d->cnn[c];
@@ -192,6 +242,36 @@ struct CacheAcc {
         cflag[h] = 1;
     }
     __device__ bool failed() { return fail; }
+
+    // Copy the modified items into a log; false if they do not fit.
+    template <int LV, int LC>
+    __device__ bool write_log(int* lvid, int* lvflag, double (*lvpos)[3], int (*lvvn)[3], int (*lvvcn)[3],
+                              int* lcid, int* lccnt, int (*lcring)[SVM_T1_RC], int* nvert, int* ncell) {
+        int nvl = 0, ncl = 0;
+        for (int h = 0; h < WV; ++h) {
+            if (vkey[h] < 0 || !vflag[h]) continue;
+            if (nvl == LV) return false;
+            lvid[nvl] = vkey[h];
+            lvflag[nvl] = vflag[h];
+            for (int k = 0; k < 3; ++k) {
+                lvpos[nvl][k] = vp[3 * h + k];
+                lvvn[nvl][k] = vvn[3 * h + k];
+                lvvcn[nvl][k] = vvcn[3 * h + k];
+            }
+            ++nvl;
+        }
+        for (int h = 0; h < WC; ++h) {
+            if (ckey[h] < 0 || !cflag[h]) continue;
+            if (ncl == LC) return false;
+            lcid[ncl] = ckey[h];
+            lccnt[ncl] = ccnt[h];
+            for (int k = 0; k < ccnt[h]; ++k) lcring[ncl][k] = cring[h * SVM_T1_RC + k];
+            ++ncl;
+        }
+        *nvert = nvl;
+        *ncell = ncl;
+        return true;
+    }
 };
 
 // -------------------------------------------------------- T1 logic --------
@@ -427,7 +507,8 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int nin
     const double* X = d.px[pcur];
     const double* Y = d.py[pcur];
     const double* Z = d.pz[pcur];
-    bool flag = false;
+    bool flag = false, flippable = false;
+    int skips = 0;
     if (local < S->nv) {
         double p[3] = {X[v], Y[v], Z[v]};
         double n1 = pnorm(p);
@@ -438,7 +519,16 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int nin
             if (v < w) {
                 double q[3] = {X[w], Y[w], Z[w]};
                 double arc = geodesic(p, q, n1, pnorm(q), S->R);
-                if (arc < S->thr) flag = true;
+                if (arc < S->thr) {
+                    flag = true;
+                    if (!flippable) {
+                        // would the sequential visit flip this edge, given the current state?
+                        GlobalAcc a{&d, pcur};
+                        int cs[4], vs[4];
+                        if (!cell_vertex_set(a, v, w, cs, vs)) ++skips;
+                        else if (!(d.cnn[cs[0]] == 3 || d.cnn[cs[2]] == 3)) flippable = true;
+                    }
+                }
             }
         }
         if (dir_update) {
@@ -451,33 +541,172 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int nin
         }
     }
     unsigned long long m = __ballot(flag);
+    unsigned long long mf = __ballot(flag && flippable);
+    unsigned long long mb = m & ~mf;
     const int lane = threadIdx.x & 63;
-    int base = 0;
+    int basef = 0, baseb = 0;
     if (lane == 0) {
         d.t1mask[v >> 6] = m;
-        if (m) {
-            atomicOr(&S->t1_flag, 1);
-            int cnt = __popcll(m);
-            base = atomicAdd(d.t1_count, cnt);
-            if (base + cnt > SVM_T1_MAXCL) atomicOr(&S->t1_overlap, 1);
+        if (m) atomicOr(&S->t1_flag, 1);
+        if (mf) {
+            int cnt = __popcll(mf);
+            basef = atomicAdd(d.t1_count, cnt);
+            atomicAdd((unsigned long long*)&S->n_cand_flip, (unsigned long long)cnt);
+            if (basef + cnt > SVM_T1_MAXCL) atomicOr(&S->t1_overflow, 1);
+        }
+        if (mb) {
+            int cnt = __popcll(mb);
+            baseb = atomicAdd(d.t1_bcount, cnt);
+            atomicAdd((unsigned long long*)&S->n_cand_block, (unsigned long long)cnt);
+            if (baseb + cnt > SVM_T1_MAXBL) atomicOr(&S->t1_overflow, 2);
         }
     }
-    base = __shfl(base, 0, 64);
-    if (flag) {
-        int pos = base + __popcll(m & ((1ull << lane) - 1ull));
-        if (pos < SVM_T1_MAXCL) d.t1_list[pos] = v;
+    basef = __shfl(basef, 0, 64);
+    baseb = __shfl(baseb, 0, 64);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (flag && flippable) {
+        int pos = basef + __popcll(mf & below);
+        if (pos < SVM_T1_MAXCL) {
+            // list slots are reused across steps and spheres: clear their marks
+            d.t1_list[pos] = v;
+            d.t1_flagged[pos] = 0;
+            d.t1_logged[pos] = 0;
+        }
+    } else if (flag) {
+        int pos = baseb + __popcll(mb & below);
+        if (pos < SVM_T1_MAXBL) {
+            d.t1_blist[pos] = v;
+            d.t1_bskip[pos] = skips;
+            d.t1_bflag[pos] = 0;
+        }
+    }
+}
+
+// A blocked candidate (no short edge of it can flip in the current state)
+// has a visit without writes, whose outcome stands unless a flippable
+// candidate's run touched what the visit reads: the vertex, its later
+// neighbours, its cells and the fourth cell of each short edge.  Thread per
+// blocked candidate; an entangled one is flagged, with the runs it met, for
+// the group run.  Otherwise its no-fourth-cell skips are tallied for the
+// commit.
+__global__ __launch_bounds__(SVM_TILE) void k_t1_check(Dev d, int pcur) {
+    const int total = min(*d.t1_bcount, SVM_T1_MAXBL);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        const int v = d.t1_blist[i];
+        SysParams* S = d.sys + d.vtile_sys[v / SVM_TILE];
+        const unsigned gen = (unsigned)S->t1_gen;
+        bool hit = false;
+        auto look = [&](const unsigned long long* own) {
+            unsigned long long st = *own;
+            if ((unsigned)(st >> 32) == gen) {
+                hit = true;
+                d.t1_flagged[(int)(st & 0xffffffffu) - 1] = (int)gen;
+            }
+        };
+        look(d.vown + v);
+        int4 cl = d.vcn[v];
+        look(d.cown + cl.x);
+        look(d.cown + cl.y);
+        look(d.cown + cl.z);
+        int4 nb = d.vn[v];
+        int w3[3] = {nb.x, nb.y, nb.z};
+        for (int k = 0; k < 3; ++k) {
+            int w = w3[k];
+            if (v < w) {
+                look(d.vown + w);
+                int4 cw = d.vcn[w];
+                int c3[3] = {cw.x, cw.y, cw.z};
+                for (int j = 0; j < 3; ++j)
+                    if (c3[j] != cl.x && c3[j] != cl.y && c3[j] != cl.z) look(d.cown + c3[j]);
+            }
+        }
+        if (hit) {
+            d.t1_bflag[i] = (int)gen;
+            atomicOr(&S->t1_overlap, 1);
+        } else if (d.t1_bskip[i]) {
+            atomicAdd(&S->t1_bskips, d.t1_bskip[i]);   // counted if the dry runs are committed
+        }
     }
 }
 
 // ------------------------------------------------------------ dry runs ----
-// Apply the logged results of every dry run of one sphere (whole block),
-// then rebuild vln of the vertices of modified cells and, for the active
-// updater, the directors of moved vertices.
-__device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int nin, int nout, int dir_mode) {
+// Write one log's items to HBM (whole block).
+__device__ inline void apply_items(const Dev& d, int pcur, int nvert, const int* vid, const int* vflag,
+                                   const double (*vpos)[3], const int (*vvn)[3], const int (*vvcn)[3], int ncell,
+                                   const int* cid, const int* ccnt, const int (*cring)[SVM_T1_RC]) {
+    for (int i = threadIdx.x; i < nvert; i += blockDim.x) {
+        int v = vid[i];
+        if (vflag[i] & 1) {
+            d.px[pcur][v] = vpos[i][0];
+            d.py[pcur][v] = vpos[i][1];
+            d.pz[pcur][v] = vpos[i][2];
+        }
+        if (vflag[i] & 2) {
+            d.vn[v] = make_int4(vvn[i][0], vvn[i][1], vvn[i][2], d.vn[v].w);
+            d.vcn[v] = make_int4(vvcn[i][0], vvcn[i][1], vvcn[i][2], d.vcn[v].w);
+        }
+    }
+    for (int i = threadIdx.x; i < ncell; i += blockDim.x) {
+        int c = cid[i];
+        d.cnn[c] = ccnt[i];
+        for (int k = 0; k < ccnt[i]; ++k) ringref(d, c, k) = cring[i][k];
+    }
+}
+
+// After the items are written: vln of the vertices of modified cells and,
+// for the active updater, the directors of moved vertices (whole block).
+__device__ inline void refresh_items(const Dev& d, SysParams* S, int pcur, int nin, int nout, int dir_mode, int nvert,
+                                     const int* vid, const int* vflag, int ncell, const int* cid) {
+    for (int i = threadIdx.x; i < ncell; i += blockDim.x) {
+        int c = cid[i];
+        int n = d.cnn[c];
+        for (int k = 0; k < n; ++k)
+            if (!build_vln(d, ringref(d, c, k))) atomicMax(&S->err, 5);
+    }
+    if (!dir_mode) return;
+    for (int i = threadIdx.x; i < nvert; i += blockDim.x) {
+        if (!(vflag[i] & 1)) continue;
+        int x = vid[i];
+        double p[3] = {d.px[pcur][x], d.py[pcur][x], d.pz[pcur][x]};
+        double nI[3] = {d.nx[nin][x], d.ny[nin][x], d.nz[nin][x]}, nO[3];
+        director_update(S, x - S->vo, p, nI, nO);
+        d.nx[nout][x] = nO[0]; d.ny[nout][x] = nO[1]; d.nz[nout][x] = nO[2];
+    }
+}
+
+// Commit the dry runs of one sphere (whole block): the logs of every
+// candidate run that completed and was not flagged as overlapping, plus the
+// group run's log when there was one.  Logs are applied one per thread; then
+// vln of every vertex of a modified cell and, for the active updater, the
+// director of every moved vertex are rebuilt one vertex per thread.
+#define SVM_COMMIT_VERTS 4096
+__device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int nin, int nout, int dir_mode,
+                                     int use_group) {
+    __shared__ int s_verts[SVM_COMMIT_VERTS];
+    __shared__ int s_moved[SVM_COMMIT_VERTS];
+    __shared__ int s_nverts, s_nmoved;
+    __shared__ unsigned long long s_flips, s_skipped;
     const int t = threadIdx.x, nt = blockDim.x;
-    for (int idx = S->t1_head; idx >= 0; idx = d.t1_log[idx].next) {
-        const T1Log& L = d.t1_log[idx];
-        for (int i = t; i < L.nvert; i += nt) {
+    const int gen = S->t1_gen;
+    const int vo = S->vo, nv = S->nv;
+    const T1GroupLog& G = d.t1_group[blockIdx.x];
+    if (t == 0) {
+        s_nverts = 0; s_nmoved = 0;
+        s_flips = use_group ? G.flips : 0;
+        s_skipped = (use_group ? G.skipped : 0) + S->t1_bskips;
+    }
+    __syncthreads();
+    // the flippable list's length as the dry runs saw it (the live counter is
+    // reset by this kernel)
+    const int total = min(*d.t1_total, SVM_T1_MAXCL);
+    auto committed = [&](int i) {
+        int row = d.t1_list[i];
+        return row >= vo && row < vo + nv && d.t1_logged[i] == gen && d.t1_flagged[i] != gen;
+    };
+    for (int j = t; j < total; j += nt) {
+        if (!committed(j)) continue;
+        const T1Log& L = d.t1_log[j];
+        for (int i = 0; i < L.nvert; ++i) {
             int v = L.vid[i];
             if (L.vflag[i] & 1) {
                 d.px[pcur][v] = L.vpos[i][0];
@@ -489,41 +718,62 @@ __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int n
                 d.vcn[v] = make_int4(L.vvcn[i][0], L.vvcn[i][1], L.vvcn[i][2], d.vcn[v].w);
             }
         }
-        for (int i = t; i < L.ncell; i += nt) {
+        for (int i = 0; i < L.ncell; ++i) {
             int c = L.cid[i];
             d.cnn[c] = L.ccnt[i];
             for (int k = 0; k < L.ccnt[i]; ++k) ringref(d, c, k) = L.cring[i][k];
         }
+        atomicAdd(&s_flips, (unsigned long long)L.flips);
+        atomicAdd(&s_skipped, (unsigned long long)L.skipped);
     }
-    if (t == 0) {
-        long long flips = 0, skipped = 0;
-        for (int idx = S->t1_head; idx >= 0; idx = d.t1_log[idx].next) {
-            flips += d.t1_log[idx].flips;
-            skipped += d.t1_log[idx].skipped;
-        }
-        S->n_t1 += flips;
-        S->n_t1_skipped += skipped;
-    }
+    if (use_group) apply_items(d, pcur, G.nvert, G.vid, G.vflag, G.vpos, G.vvn, G.vvcn, G.ncell, G.cid, G.ccnt, G.cring);
     __threadfence_block();
     __syncthreads();
-    for (int idx = S->t1_head; idx >= 0; idx = d.t1_log[idx].next) {
-        const T1Log& L = d.t1_log[idx];
-        for (int i = t; i < L.ncell; i += nt) {
-            int c = L.cid[i];
+    // vertices whose vln / director must be rebuilt
+    for (int j = t; j < total + (use_group ? 1 : 0); j += nt) {
+        bool grp = j == total;
+        if (!grp && !committed(j)) continue;
+        const T1Log* L = grp ? nullptr : &d.t1_log[j];
+        int ncell = grp ? G.ncell : L->ncell;
+        for (int i = 0; i < ncell; ++i) {
+            int c = grp ? G.cid[i] : L->cid[i];
             int n = d.cnn[c];
+            int k0 = atomicAdd(&s_nverts, n);
             for (int k = 0; k < n; ++k)
-                if (!build_vln(d, ringref(d, c, k))) atomicMax(&S->err, 5);
+                if (k0 + k < SVM_COMMIT_VERTS) s_verts[k0 + k] = ringref(d, c, k);
         }
         if (dir_mode) {
-            for (int i = t; i < L.nvert; i += nt) {
-                if (!(L.vflag[i] & 1)) continue;
-                int x = L.vid[i];
-                double p[3] = {d.px[pcur][x], d.py[pcur][x], d.pz[pcur][x]};
-                double nI[3] = {d.nx[nin][x], d.ny[nin][x], d.nz[nin][x]}, nO[3];
-                director_update(S, x - S->vo, p, nI, nO);
-                d.nx[nout][x] = nO[0]; d.ny[nout][x] = nO[1]; d.nz[nout][x] = nO[2];
+            int nvert = grp ? G.nvert : L->nvert;
+            for (int i = 0; i < nvert; ++i) {
+                int fl = grp ? G.vflag[i] : L->vflag[i];
+                if (!(fl & 1)) continue;
+                int k = atomicAdd(&s_nmoved, 1);
+                if (k < SVM_COMMIT_VERTS) s_moved[k] = grp ? G.vid[i] : L->vid[i];
             }
         }
+    }
+    __syncthreads();
+    if (s_nverts > SVM_COMMIT_VERTS) {
+        for (int x = vo + t; x < vo + nv; x += nt)
+            if (!build_vln(d, x)) atomicMax(&S->err, 5);
+    } else {
+        for (int i = t; i < s_nverts; i += nt)
+            if (!build_vln(d, s_verts[i])) atomicMax(&S->err, 5);
+    }
+    if (dir_mode) {
+        const int all = s_nmoved > SVM_COMMIT_VERTS;
+        const int hi = all ? nv : s_nmoved;
+        for (int i = t; i < hi; i += nt) {
+            int x = all ? vo + i : s_moved[i];
+            double p[3] = {d.px[pcur][x], d.py[pcur][x], d.pz[pcur][x]};
+            double nI[3] = {d.nx[nin][x], d.ny[nin][x], d.nz[nin][x]}, nO[3];
+            director_update(S, x - vo, p, nI, nO);
+            d.nx[nout][x] = nO[0]; d.ny[nout][x] = nO[1]; d.nz[nout][x] = nO[2];
+        }
+    }
+    if (t == 0) {
+        S->n_t1 += (long long)s_flips;
+        S->n_t1_skipped += (long long)s_skipped;
     }
     __syncthreads();
 }
@@ -540,7 +790,7 @@ __global__ __launch_bounds__(64 * SVM_T1_WAVES) void k_t1_cluster(Dev d, int pcu
     const int lane = threadIdx.x & 63;
     const int nwaves = gridDim.x * SVM_T1_WAVES;
     const int total = min(*d.t1_count, SVM_T1_MAXCL);
-    const unsigned long long gen = (unsigned long long)(*d.t1_gen) << 32;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *d.t1_total = *d.t1_count;
     for (int idx = blockIdx.x * SVM_T1_WAVES + w; idx < total; idx += nwaves) {
         const int row = d.t1_list[idx];
         const int s = d.vtile_sys[row / SVM_TILE];
@@ -550,10 +800,12 @@ __global__ __launch_bounds__(64 * SVM_T1_WAVES) void k_t1_cluster(Dev d, int pcu
         if (lane < SVM_T1_WC) s_ckey[w][lane] = -1;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (lane == 0 && !__hip_atomic_load(&S->t1_overlap, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            CacheAcc a{&d, pcur, gen | (unsigned long long)(idx + 1), &S->t1_overlap, false,
-                       s_vkey[w], s_vflag[w], s_vp[w], s_vvn[w], s_vvcn[w],
-                       s_ckey[w], s_cflag[w], s_ccnt[w], s_cring[w]};
+        if (lane == 0 && !S->t1_overflow) {
+            const unsigned gen = (unsigned)S->t1_gen;
+            CacheAcc<SVM_T1_WV, SVM_T1_WC, false> a{&d, pcur, ((unsigned long long)gen << 32) | (unsigned)(idx + 1),
+                                                   d.t1_flagged, &S->t1_overlap, false, nullptr, nullptr,
+                                                   s_vkey[w], s_vflag[w], s_vp[w], s_vvn[w], s_vvcn[w],
+                                                   s_ckey[w], s_cflag[w], s_ccnt[w], s_cring[w]};
             const int vo = S->vo, nv = S->nv;
             Heap heap{s_heap[w], 0};
             bool heap_full = false;
@@ -577,44 +829,140 @@ __global__ __launch_bounds__(64 * SVM_T1_WAVES) void k_t1_cluster(Dev d, int pcu
                 if (nxt < 0) break;
                 u = nxt;
             }
-            if (err == 4) atomicMax(&S->err, 4);
-            if (err || a.failed() || heap_full) {
-                atomicOr(&S->t1_overlap, 1);
-            } else if (st.flips || st.skipped) {
-                // log the modified items
+            bool ok = !(err || a.failed() || heap_full);
+            if (ok && (st.flips || st.skipped)) {
                 T1Log& L = d.t1_log[idx];
                 int nvl = 0, ncl = 0;
-                bool too_many = false;
-                for (int h = 0; h < SVM_T1_WV; ++h) {
-                    if (s_vkey[w][h] < 0 || !s_vflag[w][h]) continue;
-                    if (nvl == SVM_T1_LV) { too_many = true; break; }
-                    L.vid[nvl] = s_vkey[w][h];
-                    L.vflag[nvl] = s_vflag[w][h];
-                    for (int k = 0; k < 3; ++k) {
-                        L.vpos[nvl][k] = s_vp[w][3 * h + k];
-                        L.vvn[nvl][k] = s_vvn[w][3 * h + k];
-                        L.vvcn[nvl][k] = s_vvcn[w][3 * h + k];
-                    }
-                    ++nvl;
-                }
-                for (int h = 0; h < SVM_T1_WC && !too_many; ++h) {
-                    if (s_ckey[w][h] < 0 || !s_cflag[w][h]) continue;
-                    if (ncl == SVM_T1_LC) { too_many = true; break; }
-                    L.cid[ncl] = s_ckey[w][h];
-                    L.ccnt[ncl] = s_ccnt[w][h];
-                    for (int k = 0; k < s_ccnt[w][h]; ++k) L.cring[ncl][k] = s_cring[w][h * SVM_T1_RC + k];
-                    ++ncl;
-                }
-                if (too_many) {
-                    atomicOr(&S->t1_overlap, 1);
-                } else {
+                ok = a.template write_log<SVM_T1_LV, SVM_T1_LC>(L.vid, L.vflag, L.vpos, L.vvn, L.vvcn, L.cid, L.ccnt,
+                                                                L.cring, &nvl, &ncl);
+                if (ok) {
                     L.nvert = nvl;
                     L.ncell = ncl;
                     L.flips = st.flips;
                     L.skipped = st.skipped;
-                    L.next = atomicExch(&S->t1_head, idx);
+                    d.t1_logged[idx] = (int)gen;
                 }
             }
+            if (!ok) {
+                // this candidate's run could not be completed here: the group run redoes it
+                d.t1_flagged[idx] = (int)gen;
+                atomicOr(&S->t1_overlap, 1);
+            }
+        }
+    }
+}
+
+// Second round, block (one wavefront) per sphere whose single-candidate runs
+// overlapped: one dry run of the sequential loop over all the flagged
+// candidates together (in index order, with their disturbed vertices), in a
+// larger cache.  It fails if it touches an item of a run that was not
+// flagged, or runs out of room; the sphere then takes the sequential pass.
+__global__ __launch_bounds__(64) void k_t1_group(Dev d, int pcur) {
+    __shared__ int s_vkey[SVM_T1_GV], s_vflag[SVM_T1_GV];
+    __shared__ double s_vp[SVM_T1_GV * 3];
+    __shared__ int s_vvn[SVM_T1_GV * 3], s_vvcn[SVM_T1_GV * 3];
+    __shared__ int s_ckey[SVM_T1_GC], s_cflag[SVM_T1_GC], s_ccnt[SVM_T1_GC];
+    __shared__ int s_cring[SVM_T1_GC * SVM_T1_RC];
+    __shared__ int s_heap[SVM_T1_GV];
+    __shared__ int s_tmp[SVM_T1_RC + 8];
+    __shared__ int s_mem[SVM_T1_GMAX];
+    __shared__ int s_nmem;
+    const int s = blockIdx.x;
+    SysParams* S = d.sys + s;
+    const int lane = threadIdx.x;
+    T1GroupLog& G = d.t1_group[s];
+    if (!S->t1_overlap || S->t1_overflow) {
+        if (lane == 0) {
+            G.ok = 0;
+            if (S->t1_overflow & 1) S->n_seq_reason[0] += 1;
+            if (S->t1_overflow & 2) S->n_seq_reason[3] += 1;
+        }
+        return;
+    }
+    const int gen = S->t1_gen;
+    const int vo = S->vo, nv = S->nv;
+    const int total = min(*d.t1_count, SVM_T1_MAXCL);
+    if (lane == 0) s_nmem = 0;
+    __syncthreads();
+    for (int i = lane; i < total; i += 64) {
+        int row = d.t1_list[i];
+        if (row >= vo && row < vo + nv && d.t1_flagged[i] == gen) {
+            int k = atomicAdd(&s_nmem, 1);
+            if (k < SVM_T1_GMAX) s_mem[k] = row - vo;
+        }
+    }
+    const int btotal = min(*d.t1_bcount, SVM_T1_MAXBL);
+    for (int i = lane; i < btotal; i += 64) {
+        int row = d.t1_blist[i];
+        if (row >= vo && row < vo + nv && d.t1_bflag[i] == gen) {
+            int k = atomicAdd(&s_nmem, 1);
+            if (k < SVM_T1_GMAX) s_mem[k] = row - vo;
+        }
+    }
+    __syncthreads();
+    if (lane != 0) return;
+    int nmem = s_nmem;
+    G.ok = 0;
+    if (nmem > SVM_T1_GMAX) { S->n_seq_reason[1] += 1; return; }
+    int absorb[8];
+    int absorb_n = 0;
+    // retry with every committed-to-be run the group met absorbed, until the
+    // group completes without meeting one or runs out of room
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        for (int i = 1; i < nmem; ++i) {          // insertion sort
+            int x = s_mem[i], j = i - 1;
+            while (j >= 0 && s_mem[j] > x) { s_mem[j + 1] = s_mem[j]; --j; }
+            s_mem[j + 1] = x;
+        }
+        for (int i = 0; i < SVM_T1_GV; ++i) s_vkey[i] = -1;
+        for (int i = 0; i < SVM_T1_GC; ++i) s_ckey[i] = -1;
+        absorb_n = 0;
+        CacheAcc<SVM_T1_GV, SVM_T1_GC, true> a{&d, pcur, (unsigned long long)(unsigned)gen << 32, d.t1_flagged,
+                                               &S->t1_overlap, false, absorb, &absorb_n, s_vkey, s_vflag, s_vp,
+                                               s_vvn, s_vvcn, s_ckey, s_cflag, s_ccnt, s_cring};
+        Heap heap;
+        heap.a = s_heap;
+        heap.n = 0;
+        bool heap_full = false;
+        VisitStats st{0, 0};
+        int mi = 0, last = -1, err = 0;
+        while (true) {
+            int cu = mi < nmem ? s_mem[mi] : 0x7fffffff;
+            int hu = heap.n > 0 ? heap.a[0] : 0x7fffffff;
+            if (cu == 0x7fffffff && hu == 0x7fffffff) break;
+            int u;
+            if (hu < cu) { u = hu; heap.pop(); }
+            else { u = cu; ++mi; }
+            if (u <= last) continue;
+            last = u;
+            err = visit_vertex(a, vo + u, u, vo, nv, S->co, S->nc, S->R, S->thr, d.cap, s_tmp, st,
+                               [&](int x) {
+                                   if (heap.n < SVM_T1_GV) heap.push(x);
+                                   else heap_full = true;
+                               },
+                               [&](int, int, const int*) {});
+            if (err || a.failed() || heap_full) break;
+        }
+        if (!(err || a.failed() || heap_full)) {
+            int nvl = 0, ncl = 0;
+            if (!a.template write_log<SVM_T1_GV, SVM_T1_GC>(G.vid, G.vflag, G.vpos, G.vvn, G.vvcn, G.cid, G.ccnt,
+                                                            G.cring, &nvl, &ncl))
+                return;
+            G.nvert = nvl;
+            G.ncell = ncl;
+            G.flips = st.flips;
+            G.skipped = st.skipped;
+            G.ok = 1;
+            return;
+        }
+        // only the meeting of committed-to-be runs can be repaired
+        if (err || heap_full || absorb_n == 0 || absorb_n == 8) { S->n_seq_reason[2] += 1; return; }
+        for (int k = 0; k < absorb_n; ++k) {
+            int o = absorb[k];
+            if (d.t1_flagged[o] == gen) continue;
+            if (nmem == SVM_T1_GMAX) { S->n_seq_reason[1] += 1; return; }
+            d.t1_flagged[o] = gen;
+            s_mem[nmem++] = d.t1_list[o] - vo;
         }
     }
 }
@@ -627,21 +975,23 @@ __global__ __launch_bounds__(64 * SVM_T1_WAVES) void k_t1_cluster(Dev d, int pcu
 // the heap fills up, every remaining vertex is visited in order (still
 // exact).
 __global__ __launch_bounds__(SVM_TILE) void k_t1_resolve(Dev d, int pcur, int nin, int nout, int dir_mode,
-                                                         int advance_step, int fire_mode) {
+                                                         int advance_step, int fire_mode, int force_sequential) {
     __shared__ int heapbuf[SVM_HEAP_CAP];
     __shared__ int moved[SVM_LIST_CAP];
     __shared__ int touched[SVM_LIST_CAP];
     __shared__ int scan[SVM_TILE];
     __shared__ int tmp[72];
-    __shared__ int sh_run, sh_commit, sh_nmoved, sh_ntouched;
+    __shared__ int sh_run, sh_commit, sh_group, sh_nmoved, sh_ntouched;
     const int s = blockIdx.x;
     SysParams* S = d.sys + s;
     const int t = threadIdx.x;
     if (t == 0) {
         bool frozen = fire_mode && S->f_frozen;
         int any = S->t1_flag;
-        sh_run = !frozen && any && S->t1_overlap;
-        sh_commit = !frozen && any && !S->t1_overlap && S->t1_head >= 0;
+        int group_ok = !force_sequential && S->t1_overlap && !S->t1_overflow && d.t1_group[s].ok;
+        sh_run = !frozen && any && (force_sequential || S->t1_overflow || (S->t1_overlap && !group_ok));
+        sh_commit = !frozen && any && !sh_run;
+        sh_group = group_ok;
         if (!frozen) {
             S->t1_flag = 0;
             if (fire_mode) S->mix_pending = 0;
@@ -649,17 +999,22 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_resolve(Dev d, int pcur, int ni
         sh_nmoved = 0;
         sh_ntouched = 0;
         if (s == 0) {
-            // the dry-run list and generation of this step are used up
-            *d.t1_count = 0;
-            *d.t1_gen += 1;
+            *d.t1_count = 0;   // the candidate lists of this step are used up
+            *d.t1_bcount = 0;
         }
     }
     __syncthreads();
     if (!sh_run) {
-        if (sh_commit) commit_sphere(d, S, pcur, nin, nout, dir_mode);
+        if (sh_commit) {
+            commit_sphere(d, S, pcur, nin, nout, dir_mode, sh_group);
+            if (t == 0 && sh_group) S->n_t1_group += 1;
+        }
         if (t == 0) {
             S->t1_overlap = 0;
+            S->t1_overflow = 0;
             S->t1_head = -1;
+            S->t1_bskips = 0;
+            S->t1_gen += 1;
             if (advance_step) S->step += 1;
         }
         return;
@@ -765,7 +1120,10 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_resolve(Dev d, int pcur, int ni
     __syncthreads();
     if (t == 0) {
         S->t1_overlap = 0;
+        S->t1_overflow = 0;
         S->t1_head = -1;
+        S->t1_bskips = 0;
+        S->t1_gen += 1;
         if (advance_step) S->step += 1;
     }
 }

@@ -160,11 +160,11 @@ int svm_enforce_topology(svm_context* ctx);
  * (computeEnergy, sphericalVertexModel.cpp:150-160) and kinetic
  * sum m v^2/2 (simpleModel.cpp:63-73); double[n_systems] each (may be NULL). */
 int svm_get_energies(svm_context* ctx, double* potential, double* kinetic);
-/* Counters since creation, int64[5*n_systems]: T1 flips, fan-area
+/* Counters since creation, int64[6*n_systems]: T1 flips, fan-area
  * fallbacks, skipped NaN force contributions, T1 flips skipped because the
- * edge's two vertices share all three cells (no fourth cell exists), and
- * steps whose T1 flips were applied by the sequential pass because their
- * candidates' neighbourhoods overlapped. */
+ * edge's two vertices share all three cells (no fourth cell exists), steps
+ * whose T1 flips were applied by the sequential pass, and steps whose
+ * overlapping candidates were resolved together by one group dry run. */
 int svm_get_counters(svm_context* ctx, int64_t* counters);
 
 /* ---- updaters ---------------------------------------------------------- */
@@ -190,6 +190,17 @@ int svm_fire_set_parameters(svm_context* ctx, int32_t sys, const svm_fire_params
 int svm_fire_minimize(svm_context* ctx, int32_t check_every);
 int svm_fire_get_state(svm_context* ctx, int32_t sys, svm_fire_state* state);
 
+/* T1 path: 0 (default) resolves each step's flips by per-candidate dry runs,
+ * a group run for overlapping candidates, and the sequential pass only when
+ * those cannot complete; 1 always uses the sequential pass.  Both give the
+ * reference's sequential order exactly; 1 exists to show that. */
+int svm_set_t1_mode(svm_context* ctx, int32_t mode);
+/* T1 diagnostics, int64[6*n_systems]: steps whose sequential pass ran
+ * because the flippable-candidate list overflowed, the overlapping candidates
+ * were too many for one group, the group run ran out of room, the
+ * blocked-candidate list overflowed; then the totals of flippable and of
+ * blocked candidates seen. */
+int svm_get_t1_diagnostics(svm_context* ctx, int64_t* reasons);
 /* Kernel timing: when on, every eagerly launched kernel is bracketed by HIP
  * events on the context stream (this serialises launches; graphs are never
  * timed this way).  svm_get_timing fills up to n (name, average ms) pairs. */
