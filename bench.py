@@ -84,6 +84,7 @@ def build_batch(svm, params):
     ctx = svm.Context(systems)
     for i, (n, p0, v0, dr) in enumerate(params):
         pts = svm.generate_cell_points(n, 1137 + i)
+        pts = pts[svm.spatial_order(pts)]   # spatially coherent labels
         tris = ConvexHull(pts).simplices.astype(np.int32)
         ctx.set_topology_from_triangles(i, pts, tris)
         ctx.set_directors(i, svm.generate_directors(ctx.positions(i), 1137 + i))
@@ -126,6 +127,14 @@ def cpu_baseline(ctx, params, budget_s=12.0):
                        f"single-threaded oracle, {dt:.1f} s")
 
 
+def copy_probe(svm, n_bytes=1 << 28):
+    """A plain streaming copy of n_bytes (read n_bytes, write n_bytes), three
+    launches: the yardstick for the FETCH_SIZE / WRITE_SIZE counters."""
+    ms = svm.copy_probe(n_bytes, 3)
+    print(f"copy probe: 3 x k_copy_probe of {n_bytes} B, {ms:.4f} ms each, "
+          f"{2 * n_bytes / ms / 1e6:.0f} GB/s", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +143,9 @@ def main():
     ap.add_argument("--spheres", type=int, default=256)
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the run, 3 launches of a plain 256 MiB -> 256 MiB FP64 copy kernel "
+                         "(calibrates the HBM byte counters of a profiler)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -188,6 +200,8 @@ def main():
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ctx, params)
     print(json.dumps(result), flush=True)
+    if args.copy_probe:
+        copy_probe(svm)
 
 
 if __name__ == "__main__":

@@ -32,6 +32,7 @@ EXPORTS = [
     "svm_get_counters", "svm_set_brownian", "svm_brownian_steps", "svm_set_active",
     "svm_active_steps", "svm_run_graph", "svm_fire_set_parameters", "svm_fire_minimize",
     "svm_fire_get_state", "svm_set_t1_mode", "svm_get_t1_diagnostics", "svm_set_profiling", "svm_get_timing",
+    "svm_copy_probe", "svm_spatial_order",
 ]
 
 
@@ -99,6 +100,25 @@ def generate_directors(positions, seed):
     if r:
         raise SVMError(r, "svm_generate_directors")
     return out
+
+
+def spatial_order(points):
+    """Permutation that relabels points in Z-order (svm_spatial_order)."""
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    out = np.empty(len(pts), np.int32)
+    rc = lib().svm_spatial_order(C.c_int32(len(pts)), _dp(pts), _ip(out))
+    if rc != 0:
+        raise SVMError(rc, "svm_spatial_order")
+    return out
+
+
+def copy_probe(n_bytes, reps=3):
+    """Mean milliseconds of one plain device copy of n_bytes (k_copy_probe)."""
+    ms = C.c_double(0.0)
+    rc = lib().svm_copy_probe(C.c_int64(n_bytes), C.c_int32(reps), C.byref(ms))
+    if rc != 0:
+        raise SVMError(rc, "svm_copy_probe")
+    return ms.value
 
 
 class Context:

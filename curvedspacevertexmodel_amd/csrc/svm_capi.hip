@@ -18,6 +18,10 @@
 #include <string>
 #include <vector>
 
+// Grid size for kernels that map workgroups to tiles with xcd_tile().
+static inline int xcd_grid(int ntiles) { return (ntiles + 7) / 8 * 8; }
+
+
 #include "../../include/svm.h"
 #include "svm_kernels.h"
 #include "svm_delaunay.h"
@@ -184,20 +188,20 @@ int last_launch(svm_context* ctx) {
 int launch_geometry(svm_context* ctx, int skip_frozen) {
     {
         Timer t(ctx, "k_geometry");
-        hipLaunchKernelGGL(svm::k_geometry, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur,
+        hipLaunchKernelGGL(svm::k_geometry, dim3(xcd_grid(ctx->nct)), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur,
                            skip_frozen);
         int r = last_launch(ctx);
         if (r) return r;
     }
     Timer t(ctx, "k_fan_fix");
-    hipLaunchKernelGGL(svm::k_fan_fix, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, skip_frozen);
+    hipLaunchKernelGGL(svm::k_fan_fix, dim3(xcd_grid(ctx->nct)), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, skip_frozen);
     return last_launch(ctx);
 }
 
 template <int MODE>
 int launch_vertex(svm_context* ctx, int pout, int store_forces, const char* name) {
     Timer t(ctx, name);
-    hipLaunchKernelGGL(svm::k_vertex<MODE>, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, pout,
+    hipLaunchKernelGGL(svm::k_vertex<MODE>, dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, pout,
                        ctx->ncur, store_forces);
     return last_launch(ctx);
 }
@@ -205,7 +209,7 @@ int launch_vertex(svm_context* ctx, int pout, int store_forces, const char* name
 int launch_t1(svm_context* ctx, int pcur, int nin, int nout, int dir, int advance_step, int fire_mode) {
     {
         Timer t(ctx, "k_t1_detect");
-        hipLaunchKernelGGL(svm::k_t1_detect, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d, pcur, nin, nout,
+        hipLaunchKernelGGL(svm::k_t1_detect, dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0, ctx->stream, ctx->d, pcur, nin, nout,
                            dir, fire_mode);
         int r = last_launch(ctx);
         if (r) return r;
@@ -257,7 +261,7 @@ int enqueue_fire_iteration(svm_context* ctx) {
     int pout = 1 - ctx->pcur;
     {
         Timer t(ctx, "k_fire_vv1");
-        hipLaunchKernelGGL(svm::k_fire_vv1, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, pout);
+        hipLaunchKernelGGL(svm::k_fire_vv1, dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, pout);
         int r = last_launch(ctx);
         if (r) return r;
     }
@@ -282,6 +286,13 @@ int require_topology(svm_context* ctx) {
 }  // namespace
 
 // ======================================================================== API
+// A plain streaming copy (n doubles read, n written): the yardstick that
+// calibrates a profiler's HBM byte counters against known traffic.
+__global__ __launch_bounds__(256) void k_copy_probe(const double* __restrict__ a, double* __restrict__ b, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
 extern "C" {
 
 const char* svm_last_error(svm_context* ctx) { return ctx ? ctx->err.c_str() : k_ok; }
@@ -478,6 +489,39 @@ int svm_get_num_vertices(svm_context* ctx, int32_t sys, int32_t* n_vertices) {
 int svm_generate_cell_points(int32_t n_cells, uint64_t seed, double* cell_points) {
     if (n_cells < 4 || !cell_points) return SVM_ERR_ARG;
     svm::generate_cell_points(n_cells, seed, cell_points);
+    return SVM_OK;
+}
+
+static uint64_t spread3(uint64_t x) {   // 21 bits -> every third bit of 63
+    x &= 0x1fffffull;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+int svm_spatial_order(int32_t n_points, const double* points, int32_t* order) {
+    if (n_points < 1 || !points || !order) return SVM_ERR_ARG;
+    double lo[3] = {points[0], points[1], points[2]}, hi[3] = {points[0], points[1], points[2]};
+    for (int i = 0; i < n_points; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], points[3 * (size_t)i + k]);
+            hi[k] = std::max(hi[k], points[3 * (size_t)i + k]);
+        }
+    const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+    std::vector<std::pair<uint64_t, int32_t>> key(n_points);
+    for (int i = 0; i < n_points; ++i) {
+        uint64_t q[3];
+        for (int k = 0; k < 3; ++k) {
+            double u = (points[3 * (size_t)i + k] - lo[k]) / span;
+            q[k] = (uint64_t)std::min(2097151.0, std::max(0.0, u * 2097152.0));
+        }
+        key[i] = {spread3(q[0]) | spread3(q[1]) << 1 | spread3(q[2]) << 2, i};
+    }
+    std::sort(key.begin(), key.end());
+    for (int i = 0; i < n_points; ++i) order[i] = key[i].second;
     return SVM_OK;
 }
 
@@ -1026,6 +1070,36 @@ int svm_get_timing(svm_context* ctx, const char** names, double* ms, int32_t n, 
     }
     *n_written = k;
     return SVM_OK;
+}
+
+int svm_copy_probe(int64_t n_bytes, int32_t reps, double* ms_per_copy) {
+    if (n_bytes < 8 || reps < 1 || !ms_per_copy) return SVM_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SVM_ERR_NO_DEVICE;
+    const size_t n = (size_t)(n_bytes / 8);
+    double *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SVM_OK;
+    if (hipMalloc(&a, n * 8) != hipSuccess || hipMalloc(&b, n * 8) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = SVM_ERR_HIP;
+    } else {
+        const int threads = 256;
+        const int blocks = (int)std::min<size_t>((n + threads - 1) / threads, 16384);
+        hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(threads), 0, nullptr, (const double*)a, a, n);  // touch
+        hipEventRecord(e0, nullptr);
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(threads), 0, nullptr, (const double*)a, b, n);
+        hipEventRecord(e1, nullptr);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = SVM_ERR_HIP;
+        *ms_per_copy = ms / reps;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    return rc;
 }
 
 }  // extern "C"

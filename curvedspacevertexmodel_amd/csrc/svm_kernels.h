@@ -24,6 +24,17 @@
 
 namespace svm {
 
+// Workgroups are dealt round-robin to the 8 XCDs (workgroup b runs on XCD
+// b % 8), each with its own L2.  Giving XCD x the contiguous tile range
+// [x*per, (x+1)*per) keeps a sphere's rows, and the rows its tiles gather, in
+// one L2.  Grids are launched with 8*per workgroups (xcd_grid); the few
+// spare workgroups get a tile >= ntiles and return at once.
+__device__ inline int xcd_tile(int ntiles) {
+    const int per = (ntiles + 7) >> 3;
+    return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+}
+
+
 enum VertexMode { VM_FORCE = 0, VM_BROWNIAN = 1, VM_ACTIVE = 2, VM_FIRE = 3, VM_FORCE_F32 = 4 };
 
 // ------------------------------------------------------------ reductions --
@@ -84,7 +95,8 @@ __device__ inline double dot0(const double* a, const double* b) {
 // and recomputeAreasCPU (:162-207): one thread per cell.
 __global__ __launch_bounds__(SVM_TILE) void k_geometry(Dev d, int pin, int skip_frozen) {
     __shared__ double red[SVM_TILE + 1];
-    const int tile = blockIdx.x;
+    const int tile = xcd_tile(d.Cs / SVM_TILE);
+    if (tile >= d.Cs / SVM_TILE) return;
     const int s = d.ctile_sys[tile];
     SysParams* S = d.sys + s;
     if (skip_frozen && S->f_frozen) return;
@@ -155,7 +167,8 @@ __global__ __launch_bounds__(SVM_TILE) void k_geometry(Dev d, int pin, int skip_
 // decision without any cross-block synchronisation.
 __global__ __launch_bounds__(SVM_TILE) void k_fan_fix(Dev d, int pin, int skip_frozen) {
     __shared__ double buf[SVM_MAX_TILES];
-    const int tile = blockIdx.x;
+    const int tile = xcd_tile(d.Cs / SVM_TILE);
+    if (tile >= d.Cs / SVM_TILE) return;
     SysParams* S = d.sys + d.ctile_sys[tile];
     if (skip_frozen && S->f_frozen) return;
     const double R = S->R;
@@ -235,7 +248,8 @@ __device__ inline int vertex_force(const Dev& d, const SysParams* S, const doubl
 template <int MODE>
 __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, int nin, int store_forces) {
     __shared__ double red[SVM_TILE + 1];
-    const int tile = blockIdx.x;
+    const int tile = xcd_tile(d.Vs / SVM_TILE);
+    if (tile >= d.Vs / SVM_TILE) return;
     const int s = d.vtile_sys[tile];
     SysParams* S = d.sys + s;
     if (MODE == VM_FIRE && S->f_frozen) return;
@@ -361,7 +375,8 @@ __device__ inline void fire_mix(const Dev& d, const SysParams* S, int v, double*
 // First half of velocityVerlet::integrateEOMCPU (src/updaters/velocityVerlet.cpp:14-23)
 // preceded by the previous iteration's FIRE mixing, then the move.
 __global__ __launch_bounds__(SVM_TILE) void k_fire_vv1(Dev d, int pin, int pout) {
-    const int tile = blockIdx.x;
+    const int tile = xcd_tile(d.Vs / SVM_TILE);
+    if (tile >= d.Vs / SVM_TILE) return;
     const int s = d.vtile_sys[tile];
     SysParams* S = d.sys + s;
     const int v = tile * SVM_TILE + threadIdx.x;

@@ -35,8 +35,8 @@ namespace svm {
 
 #define SVM_T1_MAXCL 4096     // flippable candidates handled by dry runs per step
 #define SVM_T1_MAXBL 65536    // blocked candidates checked per step
-#define SVM_T1_WV 64          // vertices cached per dry run
-#define SVM_T1_WC 32          // cells cached per dry run
+#define SVM_T1_WV 128         // vertices cached per dry run
+#define SVM_T1_WC 64          // cells cached per dry run
 #define SVM_T1_RC 32          // ring entries per cached cell
 #define SVM_T1_LV 32          // modified vertices logged per dry run
 #define SVM_T1_LC 16          // modified cells logged per dry run
@@ -59,9 +59,10 @@ __device__ inline int& ringref(const Dev& d, int c, int k) { return d.ring[(size
 __device__ inline int& vcnref(const Dev& d, int v, int k) { return ((int*)&d.vcn[v])[k]; }
 __device__ inline int& vnref(const Dev& d, int v, int k) { return ((int*)&d.vn[v])[k]; }
 
-#define SVM_T1_GV 512         // vertices cached / logged by a group dry run
-#define SVM_T1_GC 128         // cells cached / logged by a group dry run
+#define SVM_T1_GV 1024        // vertices cached / logged by a group dry run
+#define SVM_T1_GC 256         // cells cached / logged by a group dry run
 #define SVM_T1_GMAX 64        // overlapping candidates one group dry run takes
+#define SVM_T1_GABS 32        // foreign runs a group attempt can absorb
 
 struct T1GroupLog {
     int nvert, ncell, ok, pad;
@@ -80,7 +81,7 @@ struct T1GroupLog {
 struct Heap {
     int* a;
     int n;
-    __device__ void push(int x) {
+    __device__ __attribute__((always_inline)) void push(int x) {
         int i = n++;
         while (i > 0) {
             int p = (i - 1) >> 1;
@@ -90,7 +91,7 @@ struct Heap {
         }
         a[i] = x;
     }
-    __device__ int pop() {
+    __device__ __attribute__((always_inline)) int pop() {
         int top = a[0];
         int x = a[--n];
         int i = 0;
@@ -113,164 +114,322 @@ struct Heap {
 struct GlobalAcc {
     const Dev* d;
     int pcur;
-    __device__ void pos(int v, double* p) { p[0] = d->px[pcur][v]; p[1] = d->py[pcur][v]; p[2] = d->pz[pcur][v]; }
-    __device__ void set_pos(int v, const double* p) {
+    __device__ __attribute__((always_inline)) void pos(int v, double* p) { p[0] = d->px[pcur][v]; p[1] = d->py[pcur][v]; p[2] = d->pz[pcur][v]; }
+    __device__ __attribute__((always_inline)) void set_pos(int v, const double* p) {
         d->px[pcur][v] = p[0]; d->py[pcur][v] = p[1]; d->pz[pcur][v] = p[2];
     }
-    __device__ int vn(int v, int k) { return vnref(*d, v, k); }
-    __device__ void set_vn(int v, int k, int x) { vnref(*d, v, k) = x; }
-    __device__ int vcn(int v, int k) { return vcnref(*d, v, k); }
-    __device__ void set_vcn(int v, int k, int x) { vcnref(*d, v, k) = x; }
-    __device__ int cnn(int c) { return d->cnn[c]; }
-    __device__ void set_cnn(int c, int x) { d->cnn[c] = x; }
-    __device__ int ring(int c, int k) { return ringref(*d, c, k); }
-    __device__ void set_ring(int c, int k, int x) { ringref(*d, c, k) = x; }
-    __device__ bool failed() { return false; }
+    __device__ __attribute__((always_inline)) int vn(int v, int k) { return vnref(*d, v, k); }
+    __device__ __attribute__((always_inline)) void set_vn(int v, int k, int x) { vnref(*d, v, k) = x; }
+    __device__ __attribute__((always_inline)) int vcn(int v, int k) { return vcnref(*d, v, k); }
+    __device__ __attribute__((always_inline)) void set_vcn(int v, int k, int x) { vcnref(*d, v, k) = x; }
+    __device__ __attribute__((always_inline)) int cnn(int c) { return d->cnn[c]; }
+    __device__ __attribute__((always_inline)) void set_cnn(int c, int x) { d->cnn[c] = x; }
+    __device__ __attribute__((always_inline)) int ring(int c, int k) { return ringref(*d, c, k); }
+    __device__ __attribute__((always_inline)) void set_ring(int c, int k, int x) { ringref(*d, c, k) = x; }
+    __device__ __attribute__((always_inline)) bool failed() { return false; }
 };
 
-// Private LDS copy of the vertices and cells one dry run touches (open
-// addressing on the global row, WV and WC slots), filled from HBM on first
-// touch.
-//   Single-candidate dry runs (GROUP = false) stamp each first touch with
-//   (generation, id) by an atomic max; meeting another dry run's stamp of the
-//   same generation flags both runs as overlapping.
-//   Group dry runs (GROUP = true) only read the stamps: touching an item
-//   stamped by a run that was not flagged (whose result will be committed)
-//   makes the group fail.
-template <int WV, int WC, bool GROUP>
+// Private LDS copy of the vertices and cells one dry run reads and writes
+// (open addressing on the global row; WV vertex and WC cell slots).
+//   * prefetch(): all lanes of the run's wavefront load the neighbourhood of
+//     the seed vertices in parallel -- each seed, its three neighbours, the
+//     cells of those, and every vertex of those cells' rings -- so the walk,
+//     which lane 0 does alone, mostly hits LDS.  What it misses it loads on
+//     first touch.
+//   * Every slot the walk uses is marked touched.  Ownership is settled after
+//     the walk, by all lanes in one pass over the touched slots: single-
+//     candidate runs stamp them (stamp_touched), group runs check them
+//     (check_touched).  Whether two runs touched a common item does not
+//     depend on when during the walks they claimed it.
+struct WaveSync {
+    __device__ __attribute__((always_inline)) void operator()() const {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+};
+
+template <int WV, int WC>
 struct CacheAcc {
     const Dev* d;
     int pcur;
-    unsigned long long stamp;     // (generation << 32) | (id + 1)
-    int32_t* flagged;             // per candidate: generation in which it overlapped
-    int32_t* overlap;             // the sphere's overlap flag
     bool fail;
-    int* absorb;                  // group runs: foreign runs met (at most 8)
-    int* absorb_n;
-    int* vkey;        // [WV]
-    int* vflag;       // [WV]  1: position written, 2: lists written
+    int* vkey;        // [WV] global row, or -1
+    int* vflag;       // [WV] 1: position written, 2: lists written
+    int* vuse;        // [WV] 1: loaded, 2: touched by the walk
     double* vp;       // [WV][3]
     int* vvn;         // [WV][3]
     int* vvcn;        // [WV][3]
     int* ckey;        // [WC]
-    int* cflag;       // [WC]
+    int* cflag;       // [WC] 1: written
+    int* cuse;        // [WC] 1: loaded, 2: touched, 4: ring longer than a slot holds
     int* ccnt;        // [WC]
     int* cring;       // [WC][RC]
+    int* vfill;       // occupied vertex slots
+    int* cfill;       // occupied cell slots
+    // the walk's last two lookups of each table (key, slot), in registers
+    int mvk0 = -1, mvh0 = 0, mvk1 = -1, mvh1 = 0;
+    int mck0 = -1, mch0 = 0, mck1 = -1, mch1 = 0;
 
     static constexpr int log2i(int x) { return x <= 1 ? 0 : 1 + log2i(x >> 1); }
+    __device__ __attribute__((always_inline)) static unsigned vhash(int v) { return ((unsigned)v * 2654435761u) >> (32 - log2i(WV)); }
+    __device__ __attribute__((always_inline)) static unsigned chash(int c) { return ((unsigned)c * 2654435761u) >> (32 - log2i(WC)); }
 
-    __device__ void claim(unsigned long long* own) {
+    // all lanes: empty tables
+    __device__ __attribute__((always_inline)) void reset(int lane, int nl) {
+        for (int h = lane; h < WV; h += nl) { vkey[h] = -1; vuse[h] = 0; }
+        for (int h = lane; h < WC; h += nl) { ckey[h] = -1; cuse[h] = 0; }
+        if (lane == 0) { *vfill = 0; *cfill = 0; }
+    }
+    __device__ __attribute__((always_inline)) void load_v(int h, int v) {
+        vp[3 * h] = d->px[pcur][v];
+        vp[3 * h + 1] = d->py[pcur][v];
+        vp[3 * h + 2] = d->pz[pcur][v];
+        int4 a = d->vn[v], b = d->vcn[v];
+        vvn[3 * h] = a.x; vvn[3 * h + 1] = a.y; vvn[3 * h + 2] = a.z;
+        vvcn[3 * h] = b.x; vvcn[3 * h + 1] = b.y; vvcn[3 * h + 2] = b.z;
+        vflag[h] = 0;
+    }
+    // sets cuse[h] to 1 (loaded) or 4 (too long)
+    __device__ __attribute__((always_inline)) void load_c(int h, int c) {
+        int n = d->cnn[c];
+        cflag[h] = 0;
+        if (n > SVM_T1_RC - 1) { ccnt[h] = 0; cuse[h] = 4; return; }
+        ccnt[h] = n;
+        for (int k = 0; k < n; ++k) cring[h * SVM_T1_RC + k] = ringref(*d, c, k);
+        cuse[h] = 1;
+    }
+
+    // ---- the walk (lane 0 only)
+    // Every key in a table is loaded (prefetch loads what it inserts), so a
+    // hit only marks the slot touched.
+    __device__ __attribute__((always_inline)) int vslot(int v) {
+        if (v == mvk0) return mvh0;
+        if (v == mvk1) return mvh1;
+        unsigned h = vhash(v);
+        int found = -1;
+        for (int probe = 0; probe < WV; ++probe, h = (h + 1) & (WV - 1)) {
+            const int k = vkey[h];
+            if (k == v) {
+                vuse[h] = 3;
+                found = (int)h;
+                break;
+            }
+            if (k < 0) {
+                vkey[h] = v;
+                ++*vfill;
+                load_v(h, v);
+                vuse[h] = 3;
+                found = (int)h;
+                break;
+            }
+        }
+        if (found < 0) { fail = true; return 0; }
+        mvk1 = mvk0; mvh1 = mvh0;
+        mvk0 = v; mvh0 = found;
+        return found;
+    }
+    __device__ __attribute__((always_inline)) int cslot(int c) {
+        if (c == mck0) return mch0;
+        if (c == mck1) return mch1;
+        unsigned h = chash(c);
+        int found = -1;
+        for (int probe = 0; probe < WC; ++probe, h = (h + 1) & (WC - 1)) {
+            const int k = ckey[h];
+            if (k == c) {
+                found = (int)h;
+                break;
+            }
+            if (k < 0) {
+                ckey[h] = c;
+                ++*cfill;
+                load_c(h, c);
+                found = (int)h;
+                break;
+            }
+        }
+        if (found < 0 || (cuse[found] & 4)) { fail = true; return 0; }
+        cuse[found] = 3;
+        mck1 = mck0; mch1 = mch0;
+        mck0 = c; mch0 = found;
+        return found;
+    }
+    // forget the memo (after restore(), the slots are untouched again)
+    __device__ __attribute__((always_inline)) void forget() { mvk0 = mvk1 = mck0 = mck1 = -1; }
+    __device__ __attribute__((always_inline)) void pos(int v, double* p) {
+        int h = vslot(v);
+        p[0] = vp[3 * h]; p[1] = vp[3 * h + 1]; p[2] = vp[3 * h + 2];
+    }
+    __device__ __attribute__((always_inline)) void set_pos(int v, const double* p) {
+        int h = vslot(v);
+        vp[3 * h] = p[0]; vp[3 * h + 1] = p[1]; vp[3 * h + 2] = p[2];
+        vflag[h] |= 1;
+    }
+    __device__ __attribute__((always_inline)) int vn(int v, int k) { return vvn[3 * vslot(v) + k]; }
+    __device__ __attribute__((always_inline)) void set_vn(int v, int k, int x) { int h = vslot(v); vvn[3 * h + k] = x; vflag[h] |= 2; }
+    __device__ __attribute__((always_inline)) int vcn(int v, int k) { return vvcn[3 * vslot(v) + k]; }
+    __device__ __attribute__((always_inline)) void set_vcn(int v, int k, int x) { int h = vslot(v); vvcn[3 * h + k] = x; vflag[h] |= 2; }
+    __device__ __attribute__((always_inline)) int cnn(int c) { return ccnt[cslot(c)]; }
+    __device__ __attribute__((always_inline)) void set_cnn(int c, int x) {
+        int h = cslot(c);
+        if (x > SVM_T1_RC - 1) fail = true;
+        else { ccnt[h] = x; cflag[h] = 1; }
+    }
+    __device__ __attribute__((always_inline)) int ring(int c, int k) { return cring[cslot(c) * SVM_T1_RC + (k & (SVM_T1_RC - 1))]; }
+    __device__ __attribute__((always_inline)) void set_ring(int c, int k, int x) {
+        int h = cslot(c);
+        cring[h * SVM_T1_RC + (k & (SVM_T1_RC - 1))] = x;
+        cflag[h] = 1;
+    }
+    __device__ __attribute__((always_inline)) bool failed() { return fail; }
+
+    // ---- cooperative parts (every lane of the wavefront)
+    // Insert `key` from many lanes at once.  New entries are limited to
+    // `limit` occupied slots in all, so that prefetching leaves the walk room.
+    template <int W>
+    __device__ __attribute__((always_inline)) static void par_insert(int* keys, unsigned h, int key, int* fill, int limit) {
+        for (int probe = 0; probe < W; ++probe, h = (h + 1) & (W - 1)) {
+            int k = __hip_atomic_load(keys + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (k == key) return;
+            if (k < 0) {
+                if (atomicAdd(fill, 1) >= limit) { atomicSub(fill, 1); return; }
+                int old = atomicCAS(keys + h, -1, key);
+                if (old < 0) return;
+                atomicSub(fill, 1);
+                if (old == key) return;
+            }
+        }
+    }
+    // Load the neighbourhood of the seed vertices (local indices seeds[i] +
+    // vo), within the budgets vlimit / climit of occupied slots.
+    template <class Sync>
+    __device__ __attribute__((always_inline)) void prefetch(const int* seeds, int n, int vo, int lane, int nl, int vlimit, int climit, Sync sync) {
+        for (int i = lane; i < n; i += nl) {
+            const int u = vo + seeds[i];
+            par_insert<WV>(vkey, vhash(u), u, vfill, vlimit);
+            int4 nb = d->vn[u];
+            par_insert<WV>(vkey, vhash(nb.x), nb.x, vfill, vlimit);
+            par_insert<WV>(vkey, vhash(nb.y), nb.y, vfill, vlimit);
+            par_insert<WV>(vkey, vhash(nb.z), nb.z, vfill, vlimit);
+        }
+        sync();
+        for (int h = lane; h < WV; h += nl) {
+            const int v = vkey[h];
+            if (v < 0 || (vuse[h] & 1)) continue;
+            load_v(h, v);
+            vuse[h] = 1;
+            for (int k = 0; k < 3; ++k) {
+                const int c = vvcn[3 * h + k];
+                par_insert<WC>(ckey, chash(c), c, cfill, climit);
+            }
+        }
+        sync();
+        for (int h = lane; h < WC; h += nl) {
+            const int c = ckey[h];
+            if (c < 0 || cuse[h]) continue;
+            load_c(h, c);
+            for (int k = 0; k < ccnt[h]; ++k) {
+                const int x = cring[h * SVM_T1_RC + k];
+                par_insert<WV>(vkey, vhash(x), x, vfill, vlimit);
+            }
+        }
+        sync();
+        for (int h = lane; h < WV; h += nl) {
+            const int v = vkey[h];
+            if (v < 0 || (vuse[h] & 1)) continue;
+            load_v(h, v);
+            vuse[h] = 1;
+        }
+        sync();
+    }
+    // Single-candidate runs: stamp every touched item with (generation, id)
+    // by an atomic max; meeting another run's stamp of the same generation
+    // flags both runs as overlapping.
+    __device__ __attribute__((always_inline)) void stamp_touched(unsigned long long stamp, int32_t* flagged, int32_t* overlap, int lane, int nl) {
         const unsigned gen = (unsigned)(stamp >> 32);
-        if (!GROUP) {
+        for (int h = lane; h < WV + WC; h += nl) {
+            unsigned long long* own;
+            if (h < WV) {
+                if (vkey[h] < 0 || !(vuse[h] & 2)) continue;
+                own = d->vown + vkey[h];
+            } else {
+                const int j = h - WV;
+                if (ckey[j] < 0 || !(cuse[j] & 2)) continue;
+                own = d->cown + ckey[j];
+            }
             unsigned long long old = atomicMax(own, stamp);
             if ((unsigned)(old >> 32) == gen && old != stamp) {
                 *overlap = 1;
                 flagged[(int)(stamp & 0xffffffffu) - 1] = (int)gen;
                 flagged[(int)(old & 0xffffffffu) - 1] = (int)gen;
             }
-        } else {
-            unsigned long long old = __hip_atomic_load(own, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int o = (int)(old & 0xffffffffu) - 1;
-            if ((unsigned)(old >> 32) == gen && flagged[o] != (int)gen) {
-                // a run whose result would be committed: the group must absorb it
-                fail = true;
-                if (*absorb_n < 8) absorb[(*absorb_n)++] = o;
-            }
         }
     }
-    __device__ int vslot(int v) {
-        unsigned h = ((unsigned)v * 2654435761u) >> (32 - log2i(WV));
-        for (int probe = 0; probe < WV; ++probe, h = (h + 1) & (WV - 1)) {
-            if (vkey[h] == v) return h;
-            if (vkey[h] < 0) {
-                vkey[h] = v;
-                vflag[h] = 0;
-                vp[3 * h] = d->px[pcur][v];
-                vp[3 * h + 1] = d->py[pcur][v];
-                vp[3 * h + 2] = d->pz[pcur][v];
-                int4 a = d->vn[v], b = d->vcn[v];
-                vvn[3 * h] = a.x; vvn[3 * h + 1] = a.y; vvn[3 * h + 2] = a.z;
-                vvcn[3 * h] = b.x; vvcn[3 * h + 1] = b.y; vvcn[3 * h + 2] = b.z;
-                claim(d->vown + v);
-                return h;
+    // Group runs: list (up to amax, count in *nabs) the runs that touched what
+    // the group touched and whose results would be committed (not flagged).
+    __device__ __attribute__((always_inline)) void check_touched(int gen, const int32_t* flagged, int* absorb, int* nabs, int amax, int lane,
+                                  int nl) {
+        for (int h = lane; h < WV + WC; h += nl) {
+            const unsigned long long* own;
+            if (h < WV) {
+                if (vkey[h] < 0 || !(vuse[h] & 2)) continue;
+                own = d->vown + vkey[h];
+            } else {
+                const int j = h - WV;
+                if (ckey[j] < 0 || !(cuse[j] & 2)) continue;
+                own = d->cown + ckey[j];
             }
+            unsigned long long st = *own;
+            if ((int)(st >> 32) != gen) continue;
+            const int o = (int)(st & 0xffffffffu) - 1;
+            if (flagged[o] == gen) continue;
+            int k = atomicAdd(nabs, 1);
+            if (k < amax) absorb[k] = o;
         }
-        fail = true;
-        return 0;
     }
-    __device__ int cslot(int c) {
-        unsigned h = ((unsigned)c * 2654435761u) >> (32 - log2i(WC));
-        for (int probe = 0; probe < WC; ++probe, h = (h + 1) & (WC - 1)) {
-            if (ckey[h] == c) return h;
-            if (ckey[h] < 0) {
-                int n = d->cnn[c];
-                if (n > SVM_T1_RC - 1) { fail = true; return 0; }
-                ckey[h] = c;
-                cflag[h] = 0;
-                ccnt[h] = n;
-                for (int k = 0; k < n; ++k) cring[h * SVM_T1_RC + k] = ringref(*d, c, k);
-                claim(d->cown + c);
-                return h;
-            }
+    // Undo the walk's writes and touches (the loaded copies stay).
+    __device__ __attribute__((always_inline)) void restore(int lane, int nl) {
+        forget();
+        for (int h = lane; h < WV; h += nl) {
+            if (vkey[h] < 0) continue;
+            if (vflag[h]) load_v(h, vkey[h]);
+            vuse[h] &= 1;
         }
-        fail = true;
-        return 0;
+        for (int h = lane; h < WC; h += nl) {
+            if (ckey[h] < 0) continue;
+            if (cflag[h]) load_c(h, ckey[h]);
+            else cuse[h] &= ~2;
+        }
     }
-    __device__ void pos(int v, double* p) {
-        int h = vslot(v);
-        p[0] = vp[3 * h]; p[1] = vp[3 * h + 1]; p[2] = vp[3 * h + 2];
-    }
-    __device__ void set_pos(int v, const double* p) {
-        int h = vslot(v);
-        vp[3 * h] = p[0]; vp[3 * h + 1] = p[1]; vp[3 * h + 2] = p[2];
-        vflag[h] |= 1;
-    }
-    __device__ int vn(int v, int k) { return vvn[3 * vslot(v) + k]; }
-    __device__ void set_vn(int v, int k, int x) { int h = vslot(v); vvn[3 * h + k] = x; vflag[h] |= 2; }
-    __device__ int vcn(int v, int k) { return vvcn[3 * vslot(v) + k]; }
-    __device__ void set_vcn(int v, int k, int x) { int h = vslot(v); vvcn[3 * h + k] = x; vflag[h] |= 2; }
-    __device__ int cnn(int c) { return ccnt[cslot(c)]; }
-    __device__ void set_cnn(int c, int x) {
-        int h = cslot(c);
-        if (x > SVM_T1_RC - 1) fail = true;
-        else { ccnt[h] = x; cflag[h] = 1; }
-    }
-    __device__ int ring(int c, int k) { return cring[cslot(c) * SVM_T1_RC + (k & (SVM_T1_RC - 1))]; }
-    __device__ void set_ring(int c, int k, int x) {
-        int h = cslot(c);
-        cring[h * SVM_T1_RC + (k & (SVM_T1_RC - 1))] = x;
-        cflag[h] = 1;
-    }
-    __device__ bool failed() { return fail; }
-
-    // Copy the modified items into a log; false if they do not fit.
-    template <int LV, int LC>
-    __device__ bool write_log(int* lvid, int* lvflag, double (*lvpos)[3], int (*lvvn)[3], int (*lvvcn)[3],
-                              int* lcid, int* lccnt, int (*lcring)[SVM_T1_RC], int* nvert, int* ncell) {
-        int nvl = 0, ncl = 0;
-        for (int h = 0; h < WV; ++h) {
+    // Copy the modified items into a log (all lanes; order of entries is
+    // free since the items are distinct).  False if they do not fit.
+    template <int LV, int LC, class Sync>
+    __device__ __attribute__((always_inline)) bool write_log(int* lvid, int* lvflag, double (*lvpos)[3], int (*lvvn)[3], int (*lvvcn)[3], int* lcid,
+                              int* lccnt, int (*lcring)[SVM_T1_RC], int* nvert, int* ncell, int lane, int nl,
+                              Sync sync) {
+        if (lane == 0) { *nvert = 0; *ncell = 0; }
+        sync();
+        for (int h = lane; h < WV; h += nl) {
             if (vkey[h] < 0 || !vflag[h]) continue;
-            if (nvl == LV) return false;
-            lvid[nvl] = vkey[h];
-            lvflag[nvl] = vflag[h];
+            const int i = atomicAdd(nvert, 1);
+            if (i >= LV) continue;
+            lvid[i] = vkey[h];
+            lvflag[i] = vflag[h];
             for (int k = 0; k < 3; ++k) {
-                lvpos[nvl][k] = vp[3 * h + k];
-                lvvn[nvl][k] = vvn[3 * h + k];
-                lvvcn[nvl][k] = vvcn[3 * h + k];
+                lvpos[i][k] = vp[3 * h + k];
+                lvvn[i][k] = vvn[3 * h + k];
+                lvvcn[i][k] = vvcn[3 * h + k];
             }
-            ++nvl;
         }
-        for (int h = 0; h < WC; ++h) {
+        for (int h = lane; h < WC; h += nl) {
             if (ckey[h] < 0 || !cflag[h]) continue;
-            if (ncl == LC) return false;
-            lcid[ncl] = ckey[h];
-            lccnt[ncl] = ccnt[h];
-            for (int k = 0; k < ccnt[h]; ++k) lcring[ncl][k] = cring[h * SVM_T1_RC + k];
-            ++ncl;
+            const int i = atomicAdd(ncell, 1);
+            if (i >= LC) continue;
+            lcid[i] = ckey[h];
+            lccnt[i] = ccnt[h];
+            for (int k = 0; k < ccnt[h]; ++k) lcring[i][k] = cring[h * SVM_T1_RC + k];
         }
-        *nvert = nvl;
-        *ncell = ncl;
-        return true;
+        sync();
+        return *nvert <= LV && *ncell <= LC;
     }
 };
 
@@ -280,7 +439,7 @@ struct CacheAcc {
 // vertex2 share all three cells, so there is no cell l); the flip is then
 // skipped, where the reference would read an uninitialised cell set.
 template <class A>
-__device__ bool cell_vertex_set(A& a, int vertex1, int vertex2, int* cs, int* vs) {
+__device__ __attribute__((always_inline)) bool cell_vertex_set(A& a, int vertex1, int vertex2, int* cs, int* vs) {
     cs[0] = cs[1] = cs[2] = cs[3] = -1;
     int cell1 = a.vcn(vertex1, 0), cell2 = a.vcn(vertex1, 1), cell3 = a.vcn(vertex1, 2);
     for (int ff = 0; ff < 3; ++ff) {
@@ -335,7 +494,7 @@ __device__ bool cell_vertex_set(A& a, int vertex1, int vertex2, int* cs, int* vs
 // v1 is the caller's copy of vertex1's position.  Returns 0, or
 // SVM_ERR_RING_OVERFLOW (nothing modified) when a cell would outgrow `cap`.
 template <class A>
-__device__ int t1_flip(A& a, int vertex1, int vertex2, const int* cs, const int* vs, double* v1, double R,
+__device__ __attribute__((always_inline)) int t1_flip(A& a, int vertex1, int vertex2, const int* cs, const int* vs, double* v1, double R,
                        int cap, int* tmp) {
     if (a.cnn(cs[1]) + 1 > cap || a.cnn(cs[3]) + 1 > cap) return 4;
     double v2[3], vC[3];
@@ -417,7 +576,7 @@ struct VisitStats {
 // receives every later vertex (local index > u) a flip disturbed; `on_flip`
 // receives each flip's (vertex1, vertex2, cell set).  Returns 0 or an error.
 template <class A, class Push, class OnFlip>
-__device__ int visit_vertex(A& a, int gu, int u, int vo, int nv, int co, int nc, double R, double thr, int cap,
+__device__ __attribute__((always_inline)) int visit_vertex(A& a, int gu, int u, int vo, int nv, int co, int nc, double R, double thr, int cap,
                             int* tmp, VisitStats& st, Push push, OnFlip on_flip) {
     double v1[3];
     a.pos(gu, v1);
@@ -498,7 +657,8 @@ __global__ __launch_bounds__(SVM_TILE) void k_build_vln(Dev d, int only) {
 // runs); the active updater's director step is fused here.
 __global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int nin, int nout, int dir_update,
                                                         int skip_frozen) {
-    const int tile = blockIdx.x;
+    const int tile = xcd_tile(d.Vs / SVM_TILE);
+    if (tile >= d.Vs / SVM_TILE) return;
     const int s = d.vtile_sys[tile];
     SysParams* S = d.sys + s;
     if (skip_frozen && S->f_frozen) return;
@@ -779,34 +939,37 @@ __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int n
 }
 
 __global__ __launch_bounds__(64 * SVM_T1_WAVES) void k_t1_cluster(Dev d, int pcur) {
-    __shared__ int s_vkey[SVM_T1_WAVES][SVM_T1_WV], s_vflag[SVM_T1_WAVES][SVM_T1_WV];
+    __shared__ int s_vkey[SVM_T1_WAVES][SVM_T1_WV], s_vflag[SVM_T1_WAVES][SVM_T1_WV], s_vuse[SVM_T1_WAVES][SVM_T1_WV];
     __shared__ double s_vp[SVM_T1_WAVES][SVM_T1_WV * 3];
     __shared__ int s_vvn[SVM_T1_WAVES][SVM_T1_WV * 3], s_vvcn[SVM_T1_WAVES][SVM_T1_WV * 3];
-    __shared__ int s_ckey[SVM_T1_WAVES][SVM_T1_WC], s_cflag[SVM_T1_WAVES][SVM_T1_WC], s_ccnt[SVM_T1_WAVES][SVM_T1_WC];
+    __shared__ int s_ckey[SVM_T1_WAVES][SVM_T1_WC], s_cflag[SVM_T1_WAVES][SVM_T1_WC];
+    __shared__ int s_cuse[SVM_T1_WAVES][SVM_T1_WC], s_ccnt[SVM_T1_WAVES][SVM_T1_WC];
     __shared__ int s_cring[SVM_T1_WAVES][SVM_T1_WC * SVM_T1_RC];
     __shared__ int s_heap[SVM_T1_WAVES][64];
     __shared__ int s_tmp[SVM_T1_WAVES][SVM_T1_RC + 8];
+    __shared__ int s_misc[SVM_T1_WAVES][8];   // vfill, cfill, seed, outcome, nvert, ncell
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
+    const WaveSync sync;
     const int nwaves = gridDim.x * SVM_T1_WAVES;
     const int total = min(*d.t1_count, SVM_T1_MAXCL);
     if (blockIdx.x == 0 && threadIdx.x == 0) *d.t1_total = *d.t1_count;
+    int* misc = s_misc[w];
     for (int idx = blockIdx.x * SVM_T1_WAVES + w; idx < total; idx += nwaves) {
         const int row = d.t1_list[idx];
-        const int s = d.vtile_sys[row / SVM_TILE];
-        SysParams* S = d.sys + s;
-        // reset the private cache
-        s_vkey[w][lane] = -1;
-        if (lane < SVM_T1_WC) s_ckey[w][lane] = -1;
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        if (lane == 0 && !S->t1_overflow) {
-            const unsigned gen = (unsigned)S->t1_gen;
-            CacheAcc<SVM_T1_WV, SVM_T1_WC, false> a{&d, pcur, ((unsigned long long)gen << 32) | (unsigned)(idx + 1),
-                                                   d.t1_flagged, &S->t1_overlap, false, nullptr, nullptr,
-                                                   s_vkey[w], s_vflag[w], s_vp[w], s_vvn[w], s_vvcn[w],
-                                                   s_ckey[w], s_cflag[w], s_ccnt[w], s_cring[w]};
-            const int vo = S->vo, nv = S->nv;
+        SysParams* S = d.sys + d.vtile_sys[row / SVM_TILE];
+        if (S->t1_overflow) continue;   // the sphere takes the sequential pass
+        const unsigned gen = (unsigned)S->t1_gen;
+        const int vo = S->vo, nv = S->nv;
+        CacheAcc<SVM_T1_WV, SVM_T1_WC> a{&d, pcur, false, s_vkey[w], s_vflag[w], s_vuse[w], s_vp[w], s_vvn[w],
+                                         s_vvcn[w], s_ckey[w], s_cflag[w], s_cuse[w], s_ccnt[w], s_cring[w],
+                                         misc + 0, misc + 1};
+        a.reset(lane, 64);
+        if (lane == 0) misc[2] = row - vo;
+        sync();
+        a.prefetch(misc + 2, 1, vo, lane, 64, SVM_T1_WV * 5 / 8, SVM_T1_WC * 5 / 8, sync);
+        T1Log& L = d.t1_log[idx];
+        if (lane == 0) {
             Heap heap{s_heap[w], 0};
             bool heap_full = false;
             VisitStats st{0, 0};
@@ -829,47 +992,61 @@ __global__ __launch_bounds__(64 * SVM_T1_WAVES) void k_t1_cluster(Dev d, int pcu
                 if (nxt < 0) break;
                 u = nxt;
             }
-            bool ok = !(err || a.failed() || heap_full);
-            if (ok && (st.flips || st.skipped)) {
-                T1Log& L = d.t1_log[idx];
-                int nvl = 0, ncl = 0;
-                ok = a.template write_log<SVM_T1_LV, SVM_T1_LC>(L.vid, L.vflag, L.vpos, L.vvn, L.vvcn, L.cid, L.ccnt,
-                                                                L.cring, &nvl, &ncl);
-                if (ok) {
-                    L.nvert = nvl;
-                    L.ncell = ncl;
-                    L.flips = st.flips;
-                    L.skipped = st.skipped;
-                    d.t1_logged[idx] = (int)gen;
-                }
-            }
+            const bool ok = !(err || a.failed() || heap_full);
+            misc[3] = !ok ? 0 : ((st.flips || st.skipped) ? 2 : 1);
+            L.flips = st.flips;
+            L.skipped = st.skipped;
+        }
+        sync();
+        const int outcome = misc[3];
+        a.stamp_touched(((unsigned long long)gen << 32) | (unsigned)(idx + 1), d.t1_flagged, &S->t1_overlap, lane,
+                        64);
+        bool ok = outcome != 0;
+        if (outcome == 2)
+            ok = a.template write_log<SVM_T1_LV, SVM_T1_LC>(L.vid, L.vflag, L.vpos, L.vvn, L.vvcn, L.cid, L.ccnt,
+                                                            L.cring, misc + 4, misc + 5, lane, 64, sync);
+        if (lane == 0) {
             if (!ok) {
                 // this candidate's run could not be completed here: the group run redoes it
                 d.t1_flagged[idx] = (int)gen;
-                atomicOr(&S->t1_overlap, 1);
+                S->t1_overlap = 1;
+            } else if (outcome == 2) {
+                L.nvert = misc[4];
+                L.ncell = misc[5];
+                d.t1_logged[idx] = (int)gen;
             }
         }
+        sync();
     }
 }
+
+struct BlockSync {
+    __device__ void operator()() const { __syncthreads(); }
+};
 
 // Second round, block (one wavefront) per sphere whose single-candidate runs
 // overlapped: one dry run of the sequential loop over all the flagged
 // candidates together (in index order, with their disturbed vertices), in a
-// larger cache.  It fails if it touches an item of a run that was not
-// flagged, or runs out of room; the sphere then takes the sequential pass.
+// larger cache.  Touching an item of a run that was not flagged means that
+// run interacts: it is absorbed into the group and the group retries (the
+// cache keeps what it loaded).  Out of room, or too many retries, and the
+// sphere takes the sequential pass.
 __global__ __launch_bounds__(64) void k_t1_group(Dev d, int pcur) {
-    __shared__ int s_vkey[SVM_T1_GV], s_vflag[SVM_T1_GV];
+    __shared__ int s_vkey[SVM_T1_GV], s_vflag[SVM_T1_GV], s_vuse[SVM_T1_GV];
     __shared__ double s_vp[SVM_T1_GV * 3];
     __shared__ int s_vvn[SVM_T1_GV * 3], s_vvcn[SVM_T1_GV * 3];
-    __shared__ int s_ckey[SVM_T1_GC], s_cflag[SVM_T1_GC], s_ccnt[SVM_T1_GC];
+    __shared__ int s_ckey[SVM_T1_GC], s_cflag[SVM_T1_GC], s_cuse[SVM_T1_GC], s_ccnt[SVM_T1_GC];
     __shared__ int s_cring[SVM_T1_GC * SVM_T1_RC];
     __shared__ int s_heap[SVM_T1_GV];
     __shared__ int s_tmp[SVM_T1_RC + 8];
     __shared__ int s_mem[SVM_T1_GMAX];
-    __shared__ int s_nmem;
+    __shared__ int s_absorb[SVM_T1_GABS];
+    __shared__ int s_nmem, s_nabs, s_vfill, s_cfill, s_status, s_nlv, s_nlc;
+    __shared__ long long s_flips, s_skipped;
     const int s = blockIdx.x;
     SysParams* S = d.sys + s;
     const int lane = threadIdx.x;
+    const BlockSync sync;
     T1GroupLog& G = d.t1_group[s];
     if (!S->t1_overlap || S->t1_overflow) {
         if (lane == 0) {
@@ -900,71 +1077,98 @@ __global__ __launch_bounds__(64) void k_t1_group(Dev d, int pcur) {
         }
     }
     __syncthreads();
-    if (lane != 0) return;
     int nmem = s_nmem;
-    G.ok = 0;
-    if (nmem > SVM_T1_GMAX) { S->n_seq_reason[1] += 1; return; }
-    int absorb[8];
-    int absorb_n = 0;
-    // retry with every committed-to-be run the group met absorbed, until the
-    // group completes without meeting one or runs out of room
+    if (nmem > SVM_T1_GMAX) {
+        if (lane == 0) { G.ok = 0; S->n_seq_reason[1] += 1; }
+        return;
+    }
+    CacheAcc<SVM_T1_GV, SVM_T1_GC> a{&d, pcur, false, s_vkey, s_vflag, s_vuse, s_vp, s_vvn, s_vvcn,
+                                     s_ckey, s_cflag, s_cuse, s_ccnt, s_cring, &s_vfill, &s_cfill};
+    a.reset(lane, 64);
+    if (lane == 0) G.ok = 0;
+    __syncthreads();
+    int seeded = 0;
     for (int attempt = 0; attempt < 8; ++attempt) {
-        for (int i = 1; i < nmem; ++i) {          // insertion sort
-            int x = s_mem[i], j = i - 1;
-            while (j >= 0 && s_mem[j] > x) { s_mem[j + 1] = s_mem[j]; --j; }
-            s_mem[j + 1] = x;
+        a.prefetch(s_mem + seeded, nmem - seeded, vo, lane, 64, SVM_T1_GV * 5 / 8, SVM_T1_GC * 5 / 8, sync);
+        seeded = nmem;
+        if (lane == 0) {
+            for (int i = 1; i < nmem; ++i) {          // insertion sort
+                int x = s_mem[i], j = i - 1;
+                while (j >= 0 && s_mem[j] > x) { s_mem[j + 1] = s_mem[j]; --j; }
+                s_mem[j + 1] = x;
+            }
+            a.fail = false;
+            Heap heap;
+            heap.a = s_heap;
+            heap.n = 0;
+            bool heap_full = false;
+            VisitStats st{0, 0};
+            int mi = 0, last = -1, err = 0;
+            while (true) {
+                int cu = mi < nmem ? s_mem[mi] : 0x7fffffff;
+                int hu = heap.n > 0 ? heap.a[0] : 0x7fffffff;
+                if (cu == 0x7fffffff && hu == 0x7fffffff) break;
+                int u;
+                if (hu < cu) { u = hu; heap.pop(); }
+                else { u = cu; ++mi; }
+                if (u <= last) continue;
+                last = u;
+                err = visit_vertex(a, vo + u, u, vo, nv, S->co, S->nc, S->R, S->thr, d.cap, s_tmp, st,
+                                   [&](int x) {
+                                       if (heap.n < SVM_T1_GV) heap.push(x);
+                                       else heap_full = true;
+                                   },
+                                   [&](int, int, const int*) {});
+                if (err || a.failed() || heap_full) break;
+            }
+            s_status = (err || a.failed() || heap_full) ? 1 : 0;
+            s_flips = st.flips;
+            s_skipped = st.skipped;
+            s_nabs = 0;
         }
-        for (int i = 0; i < SVM_T1_GV; ++i) s_vkey[i] = -1;
-        for (int i = 0; i < SVM_T1_GC; ++i) s_ckey[i] = -1;
-        absorb_n = 0;
-        CacheAcc<SVM_T1_GV, SVM_T1_GC, true> a{&d, pcur, (unsigned long long)(unsigned)gen << 32, d.t1_flagged,
-                                               &S->t1_overlap, false, absorb, &absorb_n, s_vkey, s_vflag, s_vp,
-                                               s_vvn, s_vvcn, s_ckey, s_cflag, s_ccnt, s_cring};
-        Heap heap;
-        heap.a = s_heap;
-        heap.n = 0;
-        bool heap_full = false;
-        VisitStats st{0, 0};
-        int mi = 0, last = -1, err = 0;
-        while (true) {
-            int cu = mi < nmem ? s_mem[mi] : 0x7fffffff;
-            int hu = heap.n > 0 ? heap.a[0] : 0x7fffffff;
-            if (cu == 0x7fffffff && hu == 0x7fffffff) break;
-            int u;
-            if (hu < cu) { u = hu; heap.pop(); }
-            else { u = cu; ++mi; }
-            if (u <= last) continue;
-            last = u;
-            err = visit_vertex(a, vo + u, u, vo, nv, S->co, S->nc, S->R, S->thr, d.cap, s_tmp, st,
-                               [&](int x) {
-                                   if (heap.n < SVM_T1_GV) heap.push(x);
-                                   else heap_full = true;
-                               },
-                               [&](int, int, const int*) {});
-            if (err || a.failed() || heap_full) break;
-        }
-        if (!(err || a.failed() || heap_full)) {
-            int nvl = 0, ncl = 0;
-            if (!a.template write_log<SVM_T1_GV, SVM_T1_GC>(G.vid, G.vflag, G.vpos, G.vvn, G.vvcn, G.cid, G.ccnt,
-                                                            G.cring, &nvl, &ncl))
-                return;
-            G.nvert = nvl;
-            G.ncell = ncl;
-            G.flips = st.flips;
-            G.skipped = st.skipped;
-            G.ok = 1;
+        __syncthreads();
+        if (s_status) {
+            if (lane == 0) S->n_seq_reason[2] += 1;   // out of room (or a ring overflow)
             return;
         }
-        // only the meeting of committed-to-be runs can be repaired
-        if (err || heap_full || absorb_n == 0 || absorb_n == 8) { S->n_seq_reason[2] += 1; return; }
-        for (int k = 0; k < absorb_n; ++k) {
-            int o = absorb[k];
-            if (d.t1_flagged[o] == gen) continue;
-            if (nmem == SVM_T1_GMAX) { S->n_seq_reason[1] += 1; return; }
-            d.t1_flagged[o] = gen;
-            s_mem[nmem++] = d.t1_list[o] - vo;
+        a.check_touched(gen, d.t1_flagged, s_absorb, &s_nabs, SVM_T1_GABS, lane, 64);
+        __syncthreads();
+        const int nabs = s_nabs;
+        if (nabs == 0) {
+            const bool fit = a.template write_log<SVM_T1_GV, SVM_T1_GC>(G.vid, G.vflag, G.vpos, G.vvn, G.vvcn, G.cid,
+                                                                        G.ccnt, G.cring, &s_nlv, &s_nlc, lane, 64,
+                                                                        sync);
+            if (lane == 0 && fit) {
+                G.nvert = s_nlv;
+                G.ncell = s_nlc;
+                G.flips = s_flips;
+                G.skipped = s_skipped;
+                G.ok = 1;
+            }
+            return;
         }
+        if (nabs > SVM_T1_GABS) {
+            if (lane == 0) S->n_seq_reason[2] += 1;
+            return;
+        }
+        if (lane == 0) {
+            s_status = 0;
+            for (int k = 0; k < nabs; ++k) {
+                int o = s_absorb[k];
+                if (d.t1_flagged[o] == gen) continue;
+                if (nmem == SVM_T1_GMAX) { S->n_seq_reason[1] += 1; s_status = 1; break; }
+                d.t1_flagged[o] = gen;
+                s_mem[nmem++] = d.t1_list[o] - vo;
+            }
+            s_nmem = nmem;
+        }
+        __syncthreads();
+        if (s_status) return;
+        nmem = s_nmem;
+        a.restore(lane, 64);
+        __syncthreads();
     }
+    if (lane == 0) S->n_seq_reason[2] += 1;
 }
 
 // ------------------------------------------------------ sequential pass ----

@@ -98,6 +98,14 @@ int svm_build_from_cell_points(svm_context* ctx, int32_t sys, const double* cell
 /* Seeded cell centres: sphericalModel::setParticlePositionsRandomly + setRadius
  * (src/model/sphericalModel.cpp:11-51) with Philox draws keyed by `seed`. */
 int svm_generate_cell_points(int32_t n_cells, uint64_t seed, double* cell_points);
+/* A spatially coherent labelling of points (no counterpart in the reference,
+ * whose labels are the generator's draw order): order[i] is the index of the
+ * point that should get label i, sorting by the Z-order (Morton) key of the
+ * points in their bounding cube, ties by index.  Relabelling the cell points
+ * this way before building the topology puts the cells of a 256-row tile, and
+ * (through the canonical triangle order) their vertices, close together in
+ * memory.  The model itself is unchanged; only labels move. */
+int svm_spatial_order(int32_t n_points, const double* points, int32_t* order);
 /* Seeded unit tangent directors (src/model/sphericalVertexModel.cpp:58-73). */
 int svm_generate_directors(int32_t n_vertices, const double* positions, uint64_t seed, double* directors);
 
@@ -206,6 +214,13 @@ int svm_get_t1_diagnostics(svm_context* ctx, int64_t* reasons);
  * timed this way).  svm_get_timing fills up to n (name, average ms) pairs. */
 int svm_set_profiling(svm_context* ctx, int32_t on);
 int svm_get_timing(svm_context* ctx, const char** names, double* ms, int32_t n, int32_t* n_written);
+
+/* Streaming-copy yardstick (no counterpart in the reference): allocates two
+ * n_bytes buffers on the current device, copies one into the other `reps`
+ * times with a plain HIP kernel (k_copy_probe) and returns the mean time per
+ * copy.  Used to calibrate profiler byte counters (FETCH_SIZE / WRITE_SIZE)
+ * and as the achievable-bandwidth reference. */
+int svm_copy_probe(int64_t n_bytes, int32_t reps, double* ms_per_copy);
 
 #ifdef __cplusplus
 }
