@@ -62,17 +62,23 @@ def stabilise(ctx, params, steps):
         ctx.set_preferred_parameters(i, 1.0, p0)
 
 
-def health(ctx, params):
-    """Geometry invariants after the run: the angle-sum areas must tile each sphere."""
+def health(ctx, params, base):
+    """Geometry invariants after the run (the angle-sum areas must tile each
+    sphere) and the T1 counters accumulated since `base` (a snapshot of
+    counters and diagnostics taken before the timed steps)."""
     ctx.compute_geometry()
     dev = 0.0
     for i, (n, *_rest) in enumerate(params):
         dev = max(dev, abs(ctx.area_perimeter(i)[:, 0].sum() / n - 1.0))
-    c = ctx.counters()
-    r = ctx.t1_diagnostics().sum(axis=0)
+    c = ctx.counters() - base[0]
+    dg = ctx.t1_diagnostics()
+    r = (dg - base[1]).sum(axis=0)
     return {"t1_sequential_reasons": {"flippable_list_overflow": int(r[0]), "group_too_large": int(r[1]),
                                       "group_out_of_room": int(r[2]), "blocked_list_overflow": int(r[3])},
-            "t1_candidates": {"flippable": int(r[4]), "blocked": int(r[5])},"max_rel_area_sum_error": dev, "t1_flips": int(c[:, 0].sum()), "fan_fallbacks": int(c[:, 1].sum()),
+            "t1_candidates": {"flippable": int(r[4]), "blocked": int(r[5])},
+            "t1_dry_runs": {"group_members": int(r[6]), "group_attempts": int(r[7]), "group_visits": int(r[8]),
+                            "group_longest_walk": int(dg[:, 9].max()), "single_visits": int(r[10]),
+                            "single_longest_walk": int(dg[:, 11].max())},"max_rel_area_sum_error": dev, "t1_flips": int(c[:, 0].sum()), "fan_fallbacks": int(c[:, 1].sum()),
             "nan_contributions": int(c[:, 2].sum()), "t1_skipped_no_fourth_cell": int(c[:, 3].sum()),
             "t1_sequential_sphere_steps": int(c[:, 4].sum()), "t1_group_sphere_steps": int(c[:, 5].sum())}
 
@@ -167,6 +173,7 @@ def main():
     ctx.run_graph(1, G, warm // G)
     ctx.synchronize()
     print("warmup done", flush=True)
+    base = (ctx.counters(), ctx.t1_diagnostics())
     t0 = time.perf_counter()
     ctx.run_graph(1, G, steps // G)
     ctx.synchronize()
@@ -195,7 +202,7 @@ def main():
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None,
                      "bytes_per_launch": VERTEX_BYTES_PER_CELL * total_cells, "avg_launch_ms": dom_ms},
         "kernel_ms": timing,
-        "health": health(ctx, params),
+        "health": health(ctx, params, base),
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ctx, params)

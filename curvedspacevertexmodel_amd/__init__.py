@@ -266,7 +266,7 @@ class Context:
         return pe, ke
 
     def counters(self):
-        out = np.zeros((self.n, 6), np.int64)
+        out = np.zeros((self.n, 12), np.int64)
         self._ck(lib().svm_get_counters(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "svm_get_counters")
         return out
 
@@ -310,7 +310,7 @@ class Context:
         self._ck(lib().svm_set_stream(self.h, C.c_void_p(stream_ptr)), "svm_set_stream")
 
     def t1_diagnostics(self):
-        out = np.zeros((self.n, 6), np.int64)
+        out = np.zeros((self.n, 12), np.int64)
         self._ck(lib().svm_get_t1_diagnostics(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "diagnostics")
         return out
 

@@ -420,6 +420,7 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
         P.R = std::sqrt(sd.n_cells / (4.0 * SVM_PI));
         P.kr = sd.kr;
         P.thr = sd.t1_threshold;
+        P.thr_cos = svm::t1_cos_cut(P.thr, P.R);
         P.seed = sd.seed;
         P.dt = 0.01; P.mu = 1.0; P.T = 0.0; P.v0 = 0.01; P.tau = 1.0;   // updater defaults
         P.f_dt0 = 0.01; P.f_alpha_start = 0.99; P.f_dt_max = 0.1; P.f_dt_inc = 1.05; P.f_dt_dec = 0.95;
@@ -787,7 +788,10 @@ int svm_set_scalar_model_parameter(svm_context* ctx, int32_t sys, double kr) {
 
 int svm_set_t1_threshold(svm_context* ctx, int32_t sys, double threshold) {
     if (!valid_sys(ctx, sys)) return fail(ctx, SVM_ERR_ARG, "bad sphere index");
-    return set_field(ctx, sys, offsetof(SysParams, thr), threshold);
+    if (!(threshold >= 0.0)) return fail(ctx, SVM_ERR_ARG, "the T1 threshold must be >= 0");
+    const double R = std::sqrt(ctx->layout[sys].nc / (4.0 * SVM_PI));
+    int r = set_field(ctx, sys, offsetof(SysParams, thr), threshold);
+    return r ? r : set_field(ctx, sys, offsetof(SysParams, thr_cos), svm::t1_cos_cut(threshold, R));
 }
 
 int svm_compute_geometry(svm_context* ctx) {
@@ -1033,9 +1037,16 @@ int svm_get_t1_diagnostics(svm_context* ctx, int64_t* reasons) {
     int r = download_sys(ctx, s);
     if (r) return r;
     for (int i = 0; i < ctx->nsys; ++i) {
-        for (int k = 0; k < 4; ++k) reasons[6 * i + k] = s[i].n_seq_reason[k];
-        reasons[6 * i + 4] = s[i].n_cand_flip;
-        reasons[6 * i + 5] = s[i].n_cand_block;
+        int64_t* r = reasons + SVM_T1_DIAG_COLS * (size_t)i;
+        for (int k = 0; k < 4; ++k) r[k] = s[i].n_seq_reason[k];
+        r[4] = s[i].n_cand_flip;
+        r[5] = s[i].n_cand_block;
+        r[6] = s[i].n_grp_members;
+        r[7] = s[i].n_grp_attempts;
+        r[8] = s[i].n_grp_visits;
+        r[9] = s[i].max_grp_visits;
+        r[10] = s[i].n_cl_visits;
+        r[11] = s[i].max_cl_visits;
     }
     return SVM_OK;
 }

@@ -32,6 +32,7 @@ struct SysParams {
     int32_t ctile0, nctiles, vtile0, nvtiles;
     // model
     double R, kr, thr;
+    double thr_cos;   // t1_cos_cut(thr, R): edges with a smaller cosine are not short
     uint64_t seed;
     // Brownian / active dynamics
     double dt, mu, T, v0, tau;
@@ -53,6 +54,8 @@ struct SysParams {
     long long n_t1, n_fan, n_nan, n_t1_skipped, n_t1_sequential, n_t1_group;
     long long n_seq_reason[4];   // sequential pass because: list overflow, group too large, group out of room, -
     long long n_cand_flip, n_cand_block;   // candidates seen (flippable, blocked)
+    long long n_grp_members, n_grp_attempts, n_grp_visits, max_grp_visits;   // group dry runs
+    long long n_cl_visits, max_cl_visits;                                      // single-candidate dry runs
 };
 
 struct Dev {

@@ -168,6 +168,22 @@ SVM_HD double geodesic(const double* p1, const double* p2, double n1, double n2,
     return radius * svm::acos(numerator / denominator);
 }
 
+// The T1 edge test `geodesic(p1, p2, n1, n2, R) < thr`
+// (src/model/sphericalVertexModel.cpp:478-480) with a pre-filter that leaves
+// the decision unchanged.  acos is decreasing with slope magnitude >= 1, so a
+// cosine x <= cos(thr/R) - 1e-9 gives acos(x) >= thr/R + 1e-9, i.e. an arc at
+// least 1e-9 R above thr -- many orders beyond the <= 1 ulp errors of the
+// exact expression.  Only the remaining edges evaluate acos.
+SVM_HD double t1_cos_cut(double thr, double radius) { return svm::cos(thr / radius) - 1e-9; }
+SVM_HD bool short_edge(const double* p1, const double* p2, double n1, double n2, double radius, double thr,
+                       double cos_cut) {
+    double numerator = p1[0] * p2[0] + p1[1] * p2[1] + p1[2] * p2[2];
+    double denominator = n1 * n2;
+    double x = numerator / denominator;
+    if (x <= cos_cut) return false;
+    return radius * svm::acos(x) < thr;
+}
+
 SVM_HD double pnorm(const double* p) { return __builtin_sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]); }
 
 // sphericalTriangleArea (sphericalDomain.h:135-166).

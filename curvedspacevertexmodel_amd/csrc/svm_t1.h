@@ -576,7 +576,7 @@ struct VisitStats {
 // receives every later vertex (local index > u) a flip disturbed; `on_flip`
 // receives each flip's (vertex1, vertex2, cell set).  Returns 0 or an error.
 template <class A, class Push, class OnFlip>
-__device__ __attribute__((always_inline)) int visit_vertex(A& a, int gu, int u, int vo, int nv, int co, int nc, double R, double thr, int cap,
+__device__ __attribute__((always_inline)) int visit_vertex(A& a, int gu, int u, int vo, int nv, int co, int nc, double R, double thr, double cos_cut, int cap,
                             int* tmp, VisitStats& st, Push push, OnFlip on_flip) {
     double v1[3];
     a.pos(gu, v1);
@@ -585,8 +585,7 @@ __device__ __attribute__((always_inline)) int visit_vertex(A& a, int gu, int u, 
         if (gu < vertex2) {
             double q[3];
             a.pos(vertex2, q);
-            double arcLength = geodesic(v1, q, pnorm(v1), pnorm(q), R);
-            if (arcLength < thr) {
+            if (short_edge(v1, q, pnorm(v1), pnorm(q), R, thr, cos_cut)) {
                 int cs[4], vs[4] = {0, 0, 0, 0};
                 if (!cell_vertex_set(a, gu, vertex2, cs, vs)) {
                     ++st.skipped;
@@ -678,8 +677,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int nin
             int w = w3[k];
             if (v < w) {
                 double q[3] = {X[w], Y[w], Z[w]};
-                double arc = geodesic(p, q, n1, pnorm(q), S->R);
-                if (arc < S->thr) {
+                if (short_edge(p, q, n1, pnorm(q), S->R, S->thr, S->thr_cos)) {
                     flag = true;
                     if (!flippable) {
                         // would the sequential visit flip this edge, given the current state?
@@ -973,11 +971,12 @@ __global__ __launch_bounds__(64 * SVM_T1_WAVES) void k_t1_cluster(Dev d, int pcu
             Heap heap{s_heap[w], 0};
             bool heap_full = false;
             VisitStats st{0, 0};
-            int err = 0, last = -1;
+            int err = 0, last = -1, nvis = 0;
             int u = row - vo;
             while (true) {
                 last = u;
-                err = visit_vertex(a, vo + u, u, vo, nv, S->co, S->nc, S->R, S->thr, d.cap, s_tmp[w], st,
+                ++nvis;
+                err = visit_vertex(a, vo + u, u, vo, nv, S->co, S->nc, S->R, S->thr, S->thr_cos, d.cap, s_tmp[w], st,
                                    [&](int x) {
                                        if (heap.n < 64) heap.push(x);
                                        else heap_full = true;
@@ -994,6 +993,8 @@ __global__ __launch_bounds__(64 * SVM_T1_WAVES) void k_t1_cluster(Dev d, int pcu
             }
             const bool ok = !(err || a.failed() || heap_full);
             misc[3] = !ok ? 0 : ((st.flips || st.skipped) ? 2 : 1);
+            atomicAdd((unsigned long long*)&S->n_cl_visits, (unsigned long long)nvis);
+            atomicMax((unsigned long long*)&S->max_cl_visits, (unsigned long long)nvis);
             L.flips = st.flips;
             L.skipped = st.skipped;
         }
@@ -1103,7 +1104,7 @@ __global__ __launch_bounds__(64) void k_t1_group(Dev d, int pcur) {
             heap.n = 0;
             bool heap_full = false;
             VisitStats st{0, 0};
-            int mi = 0, last = -1, err = 0;
+            int mi = 0, last = -1, err = 0, nvis = 0;
             while (true) {
                 int cu = mi < nmem ? s_mem[mi] : 0x7fffffff;
                 int hu = heap.n > 0 ? heap.a[0] : 0x7fffffff;
@@ -1113,7 +1114,8 @@ __global__ __launch_bounds__(64) void k_t1_group(Dev d, int pcur) {
                 else { u = cu; ++mi; }
                 if (u <= last) continue;
                 last = u;
-                err = visit_vertex(a, vo + u, u, vo, nv, S->co, S->nc, S->R, S->thr, d.cap, s_tmp, st,
+                ++nvis;
+                err = visit_vertex(a, vo + u, u, vo, nv, S->co, S->nc, S->R, S->thr, S->thr_cos, d.cap, s_tmp, st,
                                    [&](int x) {
                                        if (heap.n < SVM_T1_GV) heap.push(x);
                                        else heap_full = true;
@@ -1122,6 +1124,10 @@ __global__ __launch_bounds__(64) void k_t1_group(Dev d, int pcur) {
                 if (err || a.failed() || heap_full) break;
             }
             s_status = (err || a.failed() || heap_full) ? 1 : 0;
+            S->n_grp_attempts += 1;
+            S->n_grp_visits += nvis;
+            if (attempt == 0) S->n_grp_members += nmem;
+            if (nvis > S->max_grp_visits) S->max_grp_visits = nvis;
             s_flips = st.flips;
             s_skipped = st.skipped;
             s_nabs = 0;
@@ -1284,7 +1290,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_resolve(Dev d, int pcur, int ni
                 if (u >= nv) break;
             }
             last = u;
-            err = visit_vertex(a, vo + u, u, vo, nv, S->co, S->nc, S->R, S->thr, d.cap, tmp, st, push, on_flip);
+            err = visit_vertex(a, vo + u, u, vo, nv, S->co, S->nc, S->R, S->thr, S->thr_cos, d.cap, tmp, st, push, on_flip);
             if (err) break;
         }
         S->n_t1 += st.flips;

@@ -207,7 +207,11 @@ int svm_set_t1_mode(svm_context* ctx, int32_t mode);
  * because the flippable-candidate list overflowed, the overlapping candidates
  * were too many for one group, the group run ran out of room, the
  * blocked-candidate list overflowed; then the totals of flippable and of
- * blocked candidates seen. */
+ * blocked candidates seen; then, for group dry runs, the totals of members,
+ * attempts and vertex visits and the longest walk; then, for single-candidate
+ * dry runs, the total of visits and the longest walk.
+ * `reasons` holds SVM_T1_DIAG_COLS int64 per sphere. */
+#define SVM_T1_DIAG_COLS 12
 int svm_get_t1_diagnostics(svm_context* ctx, int64_t* reasons);
 /* Kernel timing: when on, every eagerly launched kernel is bracketed by HIP
  * events on the context stream (this serialises launches; graphs are never
