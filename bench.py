@@ -98,39 +98,83 @@ def build_batch(svm, params):
     return ctx
 
 
-def cpu_baseline(ctx, params, budget_s=12.0):
-    """The CPU oracle (a restatement of the reference's CPU path), one thread,
-    advancing the first spheres of the same batch from the same state."""
+def cpu_baseline(ctx, params, budget_s=12.0, threads=None):
+    """The CPU oracle (a restatement of the reference's CPU path, which is
+    single-threaded) on the host cores: independent spheres of the same batch,
+    from the same state, spread over `threads` worker threads (each advances
+    its own spheres; ctypes releases the GIL), then the same on one thread."""
     import ctypes as C
+    from concurrent.futures import ThreadPoolExecutor
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import svmtest
     L = svmtest.oracle_lib()
     dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
     ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
-    models, cells = [], 0
-    for i, (n, p0, v0, dr) in enumerate(params):
+    if threads is None:
+        threads = max(1, min(16, os.cpu_count() or 1))
+
+    def make(i):
+        n, p0, v0, dr = params[i]
         vn, vcn, ring, cnn = ctx.get_topology(i, 32)
         pos = np.ascontiguousarray(ctx.positions(i))
         h = C.c_void_p(L.or_create(C.c_int(n), C.c_int(2 * n - 4), C.c_double(np.sqrt(n / (4 * np.pi))), dp(pos),
                                    ip(vn), ip(vcn), ip(np.ascontiguousarray(ring)), C.c_int(32), ip(cnn)))
         L.or_set_preferences(h, dp(np.tile(np.array([1.0, p0]), n)))
         L.or_set_directors(h, dp(np.ascontiguousarray(ctx.directors(i))))
-        models.append((h, n, v0, dr, i))
-        cells += n
+        return (h, n, v0, dr, i)
+
+    def run(models, budget):
+        steps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget:
+            for h, n, v0, dr, i in models:
+                L.or_active_steps(h, C.c_int(10), C.c_double(0.01), C.c_double(1.0), C.c_double(v0),
+                                  C.c_double(1.0 / dr), C.c_uint64(1137 + i), C.c_uint64(steps))
+            steps += 10
+        return sum(m[1] for m in models) * steps, time.perf_counter() - t0
+
+    # one group of spheres (>= 20000 cells) per thread, in batch order
+    groups, cur, cells, i = [], [], 0, 0
+    while len(groups) < threads and i < len(params):
+        cur.append(make(i))
+        cells += params[i][0]
+        i += 1
         if cells >= 20000:
-            break
-    steps, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        for h, n, v0, dr, i in models:
-            L.or_active_steps(h, C.c_int(10), C.c_double(0.01), C.c_double(1.0), C.c_double(v0),
-                              C.c_double(1.0 / dr), C.c_uint64(1137 + i), C.c_uint64(steps))
-        steps += 10
-    dt = time.perf_counter() - t0
-    for h, *_ in models:
-        L.or_destroy(h)
-    return dict(value=cells * steps / dt, unit="cell-steps/s", cores=1, kind="port",
-                sample=f"first {len(models)} spheres of the batch ({cells} cells), {steps} active steps, "
-                       f"single-threaded oracle, {dt:.1f} s")
+            groups.append(cur)
+            cur, cells = [], 0
+    if cur:
+        groups.append(cur)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(len(groups)) as ex:
+        res = list(ex.map(lambda g: run(g, budget_s), groups))
+    wall = time.perf_counter() - t0
+    multi = sum(r[0] for r in res) / wall
+    single_work, single_t = run(groups[0], budget_s / 4)
+    for g in groups:
+        for h, *_ in g:
+            L.or_destroy(h)
+    ncells = sum(m[1] for g in groups for m in g)
+    return dict(value=multi, unit="cell-steps/s", cores=len(groups), kind="port",
+                single_thread_value=single_work / single_t,
+                sample=f"{sum(len(g) for g in groups)} spheres of the batch ({ncells} cells) in {len(groups)} "
+                       f"threads, each advancing its own spheres with the oracle's active step for "
+                       f"{budget_s:.0f} s; single-thread figure: the first thread's spheres alone")
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_batched-active-256.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain copy
+    of known size); None if there is no such measurement."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            f = line.split()
+            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+                return float(f[2])
+    except OSError:
+        pass
+    return None
 
 
 def copy_probe(svm, n_bytes=1 << 28):
@@ -189,7 +233,10 @@ def main():
     timing = ctx.timing()
     ctx.set_profiling(False)
     dom = "k_vertex_active"
+    # mean of the dominant kernel's launches in the eager profiled steps (HIP events around
+    # each launch on the library's stream); also 20 launches back to back between two events
     dom_ms = timing.get(dom, float("nan"))
+    b2b_ms = ctx.time_vertex_pass(1, 20)
     achieved = VERTEX_BYTES_PER_CELL * total_cells / (dom_ms * 1e-3) / 1e9
     result = {
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": steps,
@@ -199,8 +246,9 @@ def main():
                    "N_range": [200, 20000], "p0_range": [3.5, 4.0], "v0_range": [0.01, 0.5],
                    "Dr": [0.1, 1, 10], "dt": 0.01, "steps_per_graph": G},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "bytes_per_launch": VERTEX_BYTES_PER_CELL * total_cells, "avg_launch_ms": dom_ms},
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(dom),
+                     "bytes_per_launch": VERTEX_BYTES_PER_CELL * total_cells, "avg_launch_ms": dom_ms,
+                     "back_to_back_ms": b2b_ms},
         "kernel_ms": timing,
         "health": health(ctx, params, base),
     }

@@ -32,7 +32,8 @@ EXPORTS = [
     "svm_get_counters", "svm_set_brownian", "svm_brownian_steps", "svm_set_active",
     "svm_active_steps", "svm_run_graph", "svm_fire_set_parameters", "svm_fire_minimize",
     "svm_fire_get_state", "svm_set_t1_mode", "svm_get_t1_diagnostics", "svm_set_profiling", "svm_get_timing",
-    "svm_copy_probe", "svm_spatial_order",
+    "svm_copy_probe", "svm_spatial_order", "svm_time_vertex_pass",
+    "svm_set_force_precision", "svm_get_force_error",
 ]
 
 
@@ -308,6 +309,23 @@ class Context:
 
     def set_stream(self, stream_ptr):
         self._ck(lib().svm_set_stream(self.h, C.c_void_p(stream_ptr)), "svm_set_stream")
+
+    def set_force_precision(self, precision):
+        """0: FP64 forces (parity), 1: FP32 forces for every later force evaluation."""
+        self._ck(lib().svm_set_force_precision(self.h, C.c_int32(precision)), "svm_set_force_precision")
+
+    def force_error(self):
+        """Per sphere: max|dF|/max|F|, rms|dF|/rms|F|, max|dF|, rms|F| of FP32 vs FP64 forces."""
+        out = np.zeros((self.n, 4), np.float64)
+        self._ck(lib().svm_get_force_error(self.h, _dp(out)), "svm_get_force_error")
+        return out
+
+    def time_vertex_pass(self, updater, reps=20):
+        """Mean ms of one launch of the fused vertex pass (0 Brownian, 1 active)."""
+        ms = C.c_double(0.0)
+        self._ck(lib().svm_time_vertex_pass(self.h, C.c_int32(updater), C.c_int32(reps), C.byref(ms)),
+                 "svm_time_vertex_pass")
+        return ms.value
 
     def t1_diagnostics(self):
         out = np.zeros((self.n, 12), np.int64)

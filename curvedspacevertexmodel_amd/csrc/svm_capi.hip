@@ -49,6 +49,8 @@ struct svm_context {
     std::map<std::pair<int, int>, hipGraph_t> graph_defs;
     // Delaunay triangles per sphere (from svm_build_from_cell_points)
     std::vector<std::vector<int32_t>> triangles;
+    // force arithmetic of every force evaluation: FP64 (parity) or FP32
+    int force_fp32 = 0;
     // profiling
     bool profiling = false;
     std::map<std::string, std::pair<double, long>> timing;
@@ -201,8 +203,12 @@ int launch_geometry(svm_context* ctx, int skip_frozen) {
 template <int MODE>
 int launch_vertex(svm_context* ctx, int pout, int store_forces, const char* name) {
     Timer t(ctx, name);
-    hipLaunchKernelGGL(svm::k_vertex<MODE>, dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, pout,
-                       ctx->ncur, store_forces);
+    if (ctx->force_fp32)
+        hipLaunchKernelGGL((svm::k_vertex<MODE, float>), dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0, ctx->stream,
+                           ctx->d, ctx->pcur, pout, ctx->ncur, store_forces);
+    else
+        hipLaunchKernelGGL((svm::k_vertex<MODE, double>), dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0, ctx->stream,
+                           ctx->d, ctx->pcur, pout, ctx->ncur, store_forces);
     return last_launch(ctx);
 }
 
@@ -805,10 +811,58 @@ int svm_compute_forces(svm_context* ctx, int32_t precision) {
     if (!ctx) return SVM_ERR_ARG;
     int r = require_topology(ctx);
     if (r) return r;
+    if (precision != 0 && precision != 1) return fail(ctx, SVM_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32)");
     r = launch_geometry(ctx, 0);
     if (r) return r;
-    if (precision != 0) return fail(ctx, SVM_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32)");
-    return launch_vertex<svm::VM_FORCE>(ctx, ctx->pcur, 1, "k_vertex_force");
+    const int keep = ctx->force_fp32;
+    ctx->force_fp32 = precision;
+    r = launch_vertex<svm::VM_FORCE>(ctx, ctx->pcur, 1, "k_vertex_force");
+    ctx->force_fp32 = keep;
+    return r;
+}
+
+int svm_set_force_precision(svm_context* ctx, int32_t precision) {
+    if (!ctx || (precision != 0 && precision != 1))
+        return fail(ctx, SVM_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32)");
+    if (ctx->force_fp32 != precision) {
+        ctx->force_fp32 = precision;
+        for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+        ctx->graphs.clear();
+    }
+    return SVM_OK;
+}
+
+int svm_get_force_error(svm_context* ctx, double* stats) {
+    if (!ctx || !stats) return SVM_ERR_ARG;
+    // forces of every sphere as AoS rows, FP64 then FP32
+    std::vector<double> f64(3 * (size_t)ctx->Vs), f32(3 * (size_t)ctx->Vs);
+    int r = svm_compute_forces(ctx, 0);
+    if (!r) r = download_rows(ctx, ctx->Vs, 0, ctx->d.fx, ctx->d.fy, ctx->d.fz, f64.data());
+    if (!r) r = svm_compute_forces(ctx, 1);
+    if (!r) r = download_rows(ctx, ctx->Vs, 0, ctx->d.fx, ctx->d.fy, ctx->d.fz, f32.data());
+    if (r) return r;
+    for (int s = 0; s < ctx->nsys; ++s) {
+        const SysParams& L = ctx->layout[s];
+        double maxf = 0.0, sf = 0.0, se = 0.0, maxe = 0.0;
+        for (int v = L.vo; v < L.vo + L.nv; ++v) {
+            double a = 0.0, e = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                const double x = f64[3 * (size_t)v + k], y = f32[3 * (size_t)v + k];
+                a += x * x;
+                e += (y - x) * (y - x);
+            }
+            sf += a;
+            se += e;
+            maxf = std::max(maxf, std::sqrt(a));
+            maxe = std::max(maxe, std::sqrt(e));
+        }
+        double* o = stats + 4 * (size_t)s;
+        o[0] = maxf > 0 ? maxe / maxf : 0.0;          // max |dF| / max |F|
+        o[1] = sf > 0 ? std::sqrt(se / sf) : 0.0;      // rms |dF| / rms |F|
+        o[2] = maxe;                                   // max |dF|
+        o[3] = std::sqrt(sf / L.nv);                   // rms |F| (FP64)
+    }
+    return SVM_OK;
 }
 
 int svm_move_particles(svm_context* ctx, int32_t sys, const double* displacements, double scale) {
@@ -1081,6 +1135,41 @@ int svm_get_timing(svm_context* ctx, const char** names, double* ms, int32_t n, 
     }
     *n_written = k;
     return SVM_OK;
+}
+
+int svm_time_vertex_pass(svm_context* ctx, int32_t updater, int32_t reps, double* ms_per_launch) {
+    if (!ctx || (updater != 0 && updater != 1) || reps < 1 || !ms_per_launch) return SVM_ERR_ARG;
+    int r = require_topology(ctx);
+    if (r) return r;
+    std::vector<SysParams> saved;
+    r = download_sys(ctx, saved);
+    if (r) return r;
+    const int was_profiling = ctx->profiling;
+    ctx->profiling = 0;
+    r = launch_geometry(ctx, 0);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!r && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) r = SVM_ERR_HIP;
+    const int pout = 1 - ctx->pcur;
+    if (!r) {
+        (void)hipEventRecord(e0, ctx->stream);
+        for (int i = 0; i < reps && !r; ++i)
+            r = updater == 0 ? launch_vertex<svm::VM_BROWNIAN>(ctx, pout, 0, "k_vertex_brownian")
+                             : launch_vertex<svm::VM_ACTIVE>(ctx, pout, 0, "k_vertex_active");
+        (void)hipEventRecord(e1, ctx->stream);
+        float ms = 0.f;
+        if (!r && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            r = SVM_ERR_HIP;
+        *ms_per_launch = (double)ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    ctx->profiling = was_profiling;
+    // the pass is a pure function of the current state; only its counters moved
+    if (hipMemcpyAsync(ctx->d.sys, saved.data(), sizeof(SysParams) * ctx->nsys, hipMemcpyHostToDevice,
+                       ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        r = r ? r : SVM_ERR_HIP;
+    return r;
 }
 
 int svm_copy_probe(int64_t n_bytes, int32_t reps, double* ms_per_copy) {

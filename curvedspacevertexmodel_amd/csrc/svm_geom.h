@@ -12,102 +12,114 @@
 
 namespace svm {
 
+// Scalar-type-generic square root: the FP64 path (parity) and the FP32 force
+// variant share the expression trees below.
+SVM_HD double vsqrt(double x) { return __builtin_sqrt(x); }
+SVM_HD float vsqrt(float x) { return __builtin_sqrtf(x); }
+
 // Spherical coordinates of a point, exactly as each reference gradient
 // recomputes them (sphericalDomain.h:250-261, 350-365, 218-225).
-struct PQ {
-    double x, y, z;
-    double cosT, sinT, cosP, sinP;
+template <class T>
+struct PQT {
+    T x, y, z;
+    T cosT, sinT, cosP, sinP;
 };
+using PQ = PQT<double>;
 
-SVM_HD PQ make_pq(double x, double y, double z) {
-    PQ q;
+template <class T>
+SVM_HD PQT<T> make_pq(T x, T y, T z) {
+    PQT<T> q;
     q.x = x; q.y = y; q.z = z;
-    double r = __builtin_sqrt(x * x + y * y + z * z);
+    T r = vsqrt(x * x + y * y + z * z);
     q.cosT = z / r;
-    q.sinT = __builtin_sqrt(1.0 - q.cosT * q.cosT);
-    double s = __builtin_sqrt(x * x + y * y);
+    q.sinT = vsqrt(T(1.0) - q.cosT * q.cosT);
+    T s = vsqrt(x * x + y * y);
     q.cosP = x / s;
     q.sinP = y / s;
     return q;
 }
 
 // thetaHat, phiHat at q (sphericalDomain.h:226-231).
-SVM_HD void basis(const PQ& q, double* tH, double* pH) {
+template <class T>
+SVM_HD void basis(const PQT<T>& q, T* tH, T* pH) {
     tH[0] = q.cosT * q.cosP;
     tH[1] = q.cosT * q.sinP;
     tH[2] = -q.sinT;
     pH[0] = -q.sinP;
     pH[1] = q.cosP;
-    pH[2] = 0;
+    pH[2] = T(0);
 }
 
 // d(arc length)/d v1 (sphericalDomain.h:248-273), v1 = a, v2 = b.
-SVM_HD void grad_geodesic(const PQ& a, const PQ& b, const double* tH, const double* pH, double* d) {
-    double sinP1MinusP2 = a.sinP * b.cosP - a.cosP * b.sinP;
-    double cosP1MinusP2 = a.cosP * b.cosP + a.sinP * b.sinP;
-    double denomPart = a.cosT * b.cosT + cosP1MinusP2 * a.sinT * b.sinT;
-    double denomInverse = 1. / __builtin_sqrt(1 - denomPart * denomPart);
-    double gradTheta = 1.0 * (b.cosT * a.sinT - a.cosT * cosP1MinusP2 * b.sinT) * denomInverse;
-    double gradPhi = b.sinT * sinP1MinusP2 * denomInverse;
+template <class T>
+SVM_HD void grad_geodesic(const PQT<T>& a, const PQT<T>& b, const T* tH, const T* pH, T* d) {
+    T sinP1MinusP2 = a.sinP * b.cosP - a.cosP * b.sinP;
+    T cosP1MinusP2 = a.cosP * b.cosP + a.sinP * b.sinP;
+    T denomPart = a.cosT * b.cosT + cosP1MinusP2 * a.sinT * b.sinT;
+    T denomInverse = T(1.) / vsqrt(T(1) - denomPart * denomPart);
+    T gradTheta = T(1.0) * (b.cosT * a.sinT - a.cosT * cosP1MinusP2 * b.sinT) * denomInverse;
+    T gradPhi = b.sinT * sinP1MinusP2 * denomInverse;
     d[0] = gradTheta * tH[0] + gradPhi * pH[0];
     d[1] = gradTheta * tH[1] + gradPhi * pH[1];
     d[2] = gradTheta * tH[2] + gradPhi * pH[2];
 }
 
 // d(spherical triangle area)/d v1 (sphericalDomain.h:347-411), v1=a, v2=b, v3=c.
-SVM_HD void grad_triangle(const PQ& a, const PQ& b, const PQ& c, const double* tH, const double* pH,
-                          double radius, double* d) {
-    double cosT1 = a.cosT, cosT2 = b.cosT, cosT3 = c.cosT;
-    double sinT1 = a.sinT, sinT2 = b.sinT, sinT3 = c.sinT;
-    double cosP1 = a.cosP, cosP2 = b.cosP, cosP3 = c.cosP;
-    double sinP1 = a.sinP, sinP2 = b.sinP, sinP3 = c.sinP;
-    double sinP1MinusP2 = sinP1 * cosP2 - cosP1 * sinP2;
-    double sinP1MinusP3 = sinP1 * cosP3 - cosP1 * sinP3;
-    double cosP1MinusP2 = cosP1 * cosP2 + sinP1 * sinP2;
-    double cosP1MinusP3 = cosP1 * cosP3 + sinP1 * sinP3;
-    double cosP2MinusP3 = cosP2 * cosP3 + sinP2 * sinP3;
-    double s12 = cosT1 * cosT2 + cosP1MinusP2 * sinT1 * sinT2;
-    double s13 = cosT1 * cosT3 + cosP1MinusP3 * sinT1 * sinT3;
-    double s23 = cosT2 * cosT3 + cosP2MinusP3 * sinT2 * sinT3;
-    double d12 = cosT1 * cosP1MinusP2 * sinT2 - cosT2 * sinT1;
-    double d13 = cosT1 * cosP1MinusP3 * sinT3 - cosT3 * sinT1;
-    double g = 1 - s12 * s12 - s13 * s13 - s23 * s23 + 2.0 * s12 * s13 * s23;
-    double denom1 = __builtin_sqrt((-1. + s12 * s12) * (-1. + s12 * s12) * g);
-    double denom2 = __builtin_sqrt((-1. + s13 * s13) * (-1. + s13 * s13) * g);
-    double denom3 = __builtin_sqrt((s12 * s12 - 1.) * (s13 * s13 - 1.) * (s12 * s12 - 1.) * (s13 * s13 - 1.) * g);
-    double gradTheta = (d13 * (s12 * s12 - 1.0) + d12 * (s23 - s12 * s13)) / denom1
-                     + (d12 * (s13 * s13 - 1.0) + d13 * (s23 - s12 * s13)) / denom2
-                     - (d12 * (s13 * s13 - 1.) * (s13 - s12 * s23) + d13 * (s12 * s12 - 1.0) * (s12 - s13 * s23)) / denom3;
+template <class T>
+SVM_HD void grad_triangle(const PQT<T>& a, const PQT<T>& b, const PQT<T>& c, const T* tH, const T* pH, T radius,
+                          T* d) {
+    T cosT1 = a.cosT, cosT2 = b.cosT, cosT3 = c.cosT;
+    T sinT1 = a.sinT, sinT2 = b.sinT, sinT3 = c.sinT;
+    T cosP1 = a.cosP, cosP2 = b.cosP, cosP3 = c.cosP;
+    T sinP1 = a.sinP, sinP2 = b.sinP, sinP3 = c.sinP;
+    T sinP1MinusP2 = sinP1 * cosP2 - cosP1 * sinP2;
+    T sinP1MinusP3 = sinP1 * cosP3 - cosP1 * sinP3;
+    T cosP1MinusP2 = cosP1 * cosP2 + sinP1 * sinP2;
+    T cosP1MinusP3 = cosP1 * cosP3 + sinP1 * sinP3;
+    T cosP2MinusP3 = cosP2 * cosP3 + sinP2 * sinP3;
+    T s12 = cosT1 * cosT2 + cosP1MinusP2 * sinT1 * sinT2;
+    T s13 = cosT1 * cosT3 + cosP1MinusP3 * sinT1 * sinT3;
+    T s23 = cosT2 * cosT3 + cosP2MinusP3 * sinT2 * sinT3;
+    T d12 = cosT1 * cosP1MinusP2 * sinT2 - cosT2 * sinT1;
+    T d13 = cosT1 * cosP1MinusP3 * sinT3 - cosT3 * sinT1;
+    T g = T(1) - s12 * s12 - s13 * s13 - s23 * s23 + T(2.0) * s12 * s13 * s23;
+    T denom1 = vsqrt((T(-1.) + s12 * s12) * (T(-1.) + s12 * s12) * g);
+    T denom2 = vsqrt((T(-1.) + s13 * s13) * (T(-1.) + s13 * s13) * g);
+    T denom3 = vsqrt((s12 * s12 - T(1.)) * (s13 * s13 - T(1.)) * (s12 * s12 - T(1.)) * (s13 * s13 - T(1.)) * g);
+    T gradTheta = (d13 * (s12 * s12 - T(1.0)) + d12 * (s23 - s12 * s13)) / denom1
+                + (d12 * (s13 * s13 - T(1.0)) + d13 * (s23 - s12 * s13)) / denom2
+                - (d12 * (s13 * s13 - T(1.)) * (s13 - s12 * s23) + d13 * (s12 * s12 - T(1.0)) * (s12 - s13 * s23)) /
+                      denom3;
     gradTheta *= radius;
-    double gradPhi = ((s12 * s13 - s23) * sinP1MinusP2 * sinT2 - (-1 + s12 * s12) * sinP1MinusP3 * sinT3) / (denom1)
-                   - ((-1 + s13 * s13) * sinP1MinusP2 * sinT2 + (-(s12 * s13) + s23) * sinP1MinusP3 * sinT3) / (denom2)
-                   + ((-1 + s13 * s13) * (s13 - s12 * s23) * sinP1MinusP2 * sinT2 + (-1 + s12 * s12) * (s12 - s13 * s23) *
-                      sinP1MinusP3 * sinT3) / (denom3);
+    T gradPhi = ((s12 * s13 - s23) * sinP1MinusP2 * sinT2 - (T(-1) + s12 * s12) * sinP1MinusP3 * sinT3) / (denom1)
+              - ((T(-1) + s13 * s13) * sinP1MinusP2 * sinT2 + (-(s12 * s13) + s23) * sinP1MinusP3 * sinT3) / (denom2)
+              + ((T(-1) + s13 * s13) * (s13 - s12 * s23) * sinP1MinusP2 * sinT2 + (T(-1) + s12 * s12) * (s12 - s13 * s23) *
+                 sinP1MinusP3 * sinT3) / (denom3);
     gradPhi *= radius;
-    double determinant = a.x * (b.y * c.z - b.z * c.y)
-                       + a.y * (b.z * c.x - b.x * c.z)
-                       + a.z * (b.x * c.y - b.y * c.x);
+    T determinant = a.x * (b.y * c.z - b.z * c.y)
+                  + a.y * (b.z * c.x - b.x * c.z)
+                  + a.z * (b.x * c.y - b.y * c.x);
     d[0] = gradTheta * tH[0] + gradPhi * pH[0];
     d[1] = gradTheta * tH[1] + gradPhi * pH[1];
     d[2] = gradTheta * tH[2] + gradPhi * pH[2];
     if (determinant < 0) {
-        d[0] = -1. * d[0];
-        d[1] = -1. * d[1];
-        d[2] = -1. * d[2];
+        d[0] = T(-1.) * d[0];
+        d[1] = T(-1.) * d[1];
+        d[2] = T(-1.) * d[2];
     }
 }
 
 // Force contribution of one (vertex, cell) pair: the body of the reference's
 // per-cell loop (sphericalVertexModel.cpp:386-413).  Returns false when the
 // contribution is NaN and must be skipped.
-SVM_HD bool pair_force(const PQ& cur, const PQ& last, const PQ& next, const PQ& cpos,
-                       double areaDifference, double perimeterDifference, double Kr, double radius,
-                       double* fSet) {
-    double tH[3], pH[3], t[3];
+template <class T>
+SVM_HD bool pair_force(const PQT<T>& cur, const PQT<T>& last, const PQT<T>& next, const PQT<T>& cpos,
+                       T areaDifference, T perimeterDifference, T Kr, T radius, T* fSet) {
+    T tH[3], pH[3], t[3];
     basis(cur, tH, pH);
-    fSet[0] = 0.0; fSet[1] = 0.0; fSet[2] = 0.0;
-    double cp = 2.0 * perimeterDifference;
-    double ca = 2.0 * Kr * areaDifference;
+    fSet[0] = T(0.0); fSet[1] = T(0.0); fSet[2] = T(0.0);
+    T cp = T(2.0) * perimeterDifference;
+    T ca = T(2.0) * Kr * areaDifference;
     grad_geodesic(cur, last, tH, pH, t);
     fSet[0] -= cp * t[0]; fSet[1] -= cp * t[1]; fSet[2] -= cp * t[2];
     grad_geodesic(cur, next, tH, pH, t);
@@ -117,6 +129,52 @@ SVM_HD bool pair_force(const PQ& cur, const PQ& last, const PQ& next, const PQ& 
     grad_triangle(cur, cpos, next, tH, pH, radius, t);
     fSet[0] -= ca * t[0]; fSet[1] -= ca * t[1]; fSet[2] -= ca * t[2];
     return !(fSet[0] != fSet[0]);
+}
+
+// ------------------------------------------------------ FP32 force path ----
+// The same tangent-plane gradients as grad_geodesic / grad_triangle, written
+// in a form that single precision evaluates well.  The reference's trig-basis
+// expressions cancel catastrophically in FP32 for the small angles between
+// neighbouring points (the Gram determinant g of a small triangle is ~1e-5
+// from O(1) terms).  Here every quantity is built from the unit vector a of
+// the vertex and the scaled differences e = (u - v)/R of its neighbours,
+// formed in FP64, so no O(1) terms cancel.
+struct f3 {
+    float x, y, z;
+};
+SVM_HD f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+SVM_HD f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+SVM_HD f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
+SVM_HD float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+SVM_HD f3 cross3(f3 a, f3 b) { return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// d(arc length |v u|)/d v on the sphere: -(e - (a.e) a) / |a x e|
+// (the angle alpha between a and a + e has sin(alpha) = |a x e|).
+SVM_HD f3 grad_arc32(f3 a, f3 e) {
+    f3 t = e - dot3(a, e) * a;
+    f3 x = cross3(a, e);
+    return (-1.0f / __builtin_sqrtf(dot3(x, x))) * t;
+}
+
+// d(signed area of the spherical triangle v, b, c)/d v for b = a + eb,
+// c = a + ec: tan(E/2) = N/D with N = a.(b x c) = a.(eb x ec) and
+// D = 1 + a.b + b.c + c.a = 4 - (|eb|^2 + |ec|^2 + |eb - ec|^2)/2, so
+// grad_a E = 2 (D (b x c) - N (b + c)) / (N^2 + D^2), projected on the
+// tangent plane; area = R^2 E and d/dv = (1/R) d/da.  The fan triangles
+// (v, last, C) and (v, C, next) of a clockwise ring have N > 0; one that
+// folds over (a non-convex cell) has N < 0 and counts negatively, so the fan
+// areas keep summing to the cell's angle-sum area -- as in the reference's
+// chain rule (checked against finite differences of the energy).
+SVM_HD f3 grad_tri32(f3 a, f3 eb, f3 ec, float radius) {
+    f3 bxc = cross3(a, ec - eb) + cross3(eb, ec);
+    float N = dot3(a, cross3(eb, ec));
+    f3 dbc = eb - ec;
+    float D = 4.0f - 0.5f * (dot3(eb, eb) + dot3(ec, ec) + dot3(dbc, dbc));
+    f3 tb = bxc - N * a;                       // tangent part of b x c (a.(b x c) = N)
+    f3 s = eb + ec;
+    f3 ts = s - dot3(a, s) * a;                // tangent part of b + c
+    float k = 2.0f * radius / (N * N + D * D);
+    return k * (D * tb - N * ts);
 }
 
 // sphericalDomain::move(p, disp) (sphericalDomain.h:203-209): project the

@@ -200,38 +200,43 @@ __global__ __launch_bounds__(SVM_TILE) void k_fan_fix(Dev d, int pin, int skip_f
 }
 
 // --------------------------------------------------------------- vertices --
-__device__ inline PQ pq_at(const double* X, const double* Y, const double* Z, int v) {
-    return make_pq(X[v], Y[v], Z[v]);
+template <class T>
+__device__ inline PQT<T> pq_at(const double* X, const double* Y, const double* Z, int v) {
+    return make_pq<T>((T)X[v], (T)Y[v], (T)Z[v]);
 }
 
-__device__ inline const PQ& pick3(int k, const PQ& a, const PQ& b, const PQ& c) {
+template <class T>
+__device__ inline const PQT<T>& pick3(int k, const PQT<T>& a, const PQT<T>& b, const PQT<T>& c) {
     return k == 0 ? a : (k == 1 ? b : c);
 }
 
 // Force on vertex v (sphericalVertexModel::computeForceCPU,
 // src/model/sphericalVertexModel.cpp:380-416).  Returns the number of skipped
-// NaN contributions.
+// NaN contributions.  T = double is the parity path; T = float evaluates the
+// same expression trees in FP32 from the FP64 positions, cell centres and
+// (A - A0, P - P0) differences.
+template <class T>
 __device__ inline int vertex_force(const Dev& d, const SysParams* S, const double* X, const double* Y,
                                    const double* Z, int v, double* f) {
-    const double R = S->R, Kr = S->kr;
-    PQ qs = pq_at(X, Y, Z, v);
+    const T R = (T)S->R, Kr = (T)S->kr;
+    PQT<T> qs = pq_at<T>(X, Y, Z, v);
     int4 nb = d.vn[v];
     int4 cl = d.vcn[v];
     int ln = d.vln[v];
-    PQ q0 = pq_at(X, Y, Z, nb.x);
-    PQ q1 = pq_at(X, Y, Z, nb.y);
-    PQ q2 = pq_at(X, Y, Z, nb.z);
-    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    PQT<T> q0 = pq_at<T>(X, Y, Z, nb.x);
+    PQT<T> q1 = pq_at<T>(X, Y, Z, nb.y);
+    PQT<T> q2 = pq_at<T>(X, Y, Z, nb.z);
+    T f0 = T(0.0), f1 = T(0.0), f2 = T(0.0);
     int nan = 0;
 #pragma unroll
     for (int sl = 0; sl < 3; ++sl) {
         int c = sl == 0 ? cl.x : (sl == 1 ? cl.y : cl.z);
         int ls = (ln >> (4 * sl)) & 3;
         int ns = (ln >> (4 * sl + 2)) & 3;
-        double areaDifference = d.A[c] - d.a0[c];
-        double perimeterDifference = d.P[c] - d.p0[c];
-        PQ qc = make_pq(d.cx[c], d.cy[c], d.cz[c]);
-        double fs[3];
+        T areaDifference = (T)(d.A[c] - d.a0[c]);
+        T perimeterDifference = (T)(d.P[c] - d.p0[c]);
+        PQT<T> qc = make_pq<T>((T)d.cx[c], (T)d.cy[c], (T)d.cz[c]);
+        T fs[3];
         if (pair_force(qs, pick3(ls, q0, q1, q2), pick3(ns, q0, q1, q2), qc, areaDifference, perimeterDifference,
                        Kr, R, fs)) {
             f0 = f0 + fs[0];
@@ -241,11 +246,52 @@ __device__ inline int vertex_force(const Dev& d, const SysParams* S, const doubl
             ++nan;
         }
     }
-    f[0] = f0; f[1] = f1; f[2] = f2;
+    f[0] = (double)f0; f[1] = (double)f1; f[2] = (double)f2;
     return nan;
 }
 
-template <int MODE>
+// FP32 force variant: the force of the same energy, from the vector forms
+// grad_arc32 / grad_tri32 (no basis singularity at the poles).
+template <>
+__device__ inline int vertex_force<float>(const Dev& d, const SysParams* S, const double* X, const double* Y,
+                                          const double* Z, int v, double* f) {
+    const double R = S->R, iR = 1.0 / R;
+    const float Kr = (float)S->kr, Rf = (float)R;
+    const double px = X[v], py = Y[v], pz = Z[v];
+    const f3 a{(float)(px * iR), (float)(py * iR), (float)(pz * iR)};
+    auto rel = [&](double x, double y, double z) {
+        return f3{(float)((x - px) * iR), (float)((y - py) * iR), (float)((z - pz) * iR)};
+    };
+    int4 nb = d.vn[v];
+    int4 cl = d.vcn[v];
+    int ln = d.vln[v];
+    const f3 e0 = rel(X[nb.x], Y[nb.x], Z[nb.x]);
+    const f3 e1 = rel(X[nb.y], Y[nb.y], Z[nb.y]);
+    const f3 e2 = rel(X[nb.z], Y[nb.z], Z[nb.z]);
+    const f3 g0 = grad_arc32(a, e0), g1 = grad_arc32(a, e1), g2 = grad_arc32(a, e2);
+    f3 F{0.0f, 0.0f, 0.0f};
+    int nan = 0;
+#pragma unroll
+    for (int sl = 0; sl < 3; ++sl) {
+        int c = sl == 0 ? cl.x : (sl == 1 ? cl.y : cl.z);
+        int ls = (ln >> (4 * sl)) & 3;
+        int ns = (ln >> (4 * sl + 2)) & 3;
+        const float cp = 2.0f * (float)(d.P[c] - d.p0[c]);
+        const float ca = 2.0f * Kr * (float)(d.A[c] - d.a0[c]);
+        const f3 ec = rel(d.cx[c], d.cy[c], d.cz[c]);
+        const f3 el = ls == 0 ? e0 : (ls == 1 ? e1 : e2);
+        const f3 en = ns == 0 ? e0 : (ns == 1 ? e1 : e2);
+        const f3 gl = ls == 0 ? g0 : (ls == 1 ? g1 : g2);
+        const f3 gn = ns == 0 ? g0 : (ns == 1 ? g1 : g2);
+        f3 fs = (-cp) * (gl + gn) - ca * (grad_tri32(a, el, ec, Rf) + grad_tri32(a, ec, en, Rf));
+        if (fs.x != fs.x) ++nan;
+        else F = F + fs;
+    }
+    f[0] = (double)F.x; f[1] = (double)F.y; f[2] = (double)F.z;
+    return nan;
+}
+
+template <int MODE, class FT>
 __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, int nin, int store_forces) {
     __shared__ double red[SVM_TILE + 1];
     const int tile = xcd_tile(d.Vs / SVM_TILE);
@@ -263,7 +309,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, i
     double tp = 0.0, tf = 0.0, tv = 0.0;
     int nan = 0;
     if (valid) {
-        nan = vertex_force(d, S, X, Y, Z, v, f);
+        nan = vertex_force<FT>(d, S, X, Y, Z, v, f);
         if (MODE == VM_FORCE || store_forces) {
             d.fx[v] = f[0]; d.fy[v] = f[1]; d.fz[v] = f[2];
         }

@@ -157,6 +157,15 @@ int svm_compute_geometry(svm_context* ctx);
  * src/model/sphericalVertexModel.cpp:356-420).  precision: 0 = FP64 (parity),
  * 1 = FP32 forces from FP64 positions and cell centres. */
 int svm_compute_forces(svm_context* ctx, int32_t precision);
+/* Force arithmetic of every later force evaluation, the updaters included
+ * (the SCALARFLOAT build switch, inc/std_include.h:66-78, made a run-time
+ * choice): 0 = FP64, 1 = FP32 forces.  Positions, cell centres, areas,
+ * perimeters, T1 tests and updates stay FP64. */
+int svm_set_force_precision(svm_context* ctx, int32_t precision);
+/* Error of the FP32 forces against the FP64 forces on the current state, per
+ * sphere (4 doubles each): max|dF|/max|F|, rms|dF|/rms|F|, max|dF|, rms|F|.
+ * Leaves the FP32 forces in the force buffer. */
+int svm_get_force_error(svm_context* ctx, double* stats);
 /* moveParticles(displacements, scale) then enforceTopology:
  * gpu_move_particles_on_sphere + the T1 callers
  * (src/model/sphericalVertexModel.cuh:14-92, sphericalVertexModel.cpp:111-148).
@@ -218,6 +227,14 @@ int svm_get_t1_diagnostics(svm_context* ctx, int64_t* reasons);
  * timed this way).  svm_get_timing fills up to n (name, average ms) pairs. */
 int svm_set_profiling(svm_context* ctx, int32_t on);
 int svm_get_timing(svm_context* ctx, const char** names, double* ms, int32_t n, int32_t* n_written);
+
+/* Kernel timing without launch gaps (no counterpart in the reference): the
+ * fused vertex pass of `updater` (0 Brownian, 1 active; force + update +
+ * move, k_vertex) launched `reps` times back to back between two HIP events
+ * on the context's stream.  The pass is a pure function of the current state
+ * (it writes the spare position buffer, which the next step overwrites), so
+ * the model state is unchanged; its counters are restored. */
+int svm_time_vertex_pass(svm_context* ctx, int32_t updater, int32_t reps, double* ms_per_launch);
 
 /* Streaming-copy yardstick (no counterpart in the reference): allocates two
  * n_bytes buffers on the current device, copies one into the other `reps`
