@@ -33,7 +33,8 @@ EXPORTS = [
     "svm_active_steps", "svm_run_graph", "svm_fire_set_parameters", "svm_fire_minimize",
     "svm_fire_get_state", "svm_set_t1_mode", "svm_get_t1_diagnostics", "svm_set_profiling", "svm_get_timing",
     "svm_copy_probe", "svm_spatial_order", "svm_time_vertex_pass",
-    "svm_set_force_precision", "svm_get_force_error",
+    "svm_set_force_precision", "svm_get_force_error", "svm_delaunay_triangles",
+    "svm_cell_list",
 ]
 
 
@@ -111,6 +112,34 @@ def spatial_order(points):
     if rc != 0:
         raise SVMError(rc, "svm_spatial_order")
     return out
+
+
+def delaunay_triangles(points):
+    """Spherical Delaunay triangles of points (N x 3) on the GPU, int32 (2N-4) x 3,
+    CCW seen from outside, smallest index first, sorted lexicographically."""
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    out = np.empty((2 * len(pts) - 4, 3), np.int32)
+    rc = lib().svm_delaunay_triangles(C.c_int32(len(pts)), _dp(pts), _ip(out))
+    if rc != 0:
+        raise SVMError(rc, "svm_delaunay_triangles")
+    return out
+
+
+def cell_list(points, cell_size):
+    """GPU cell list: (G, bin_of, bin_start, sorted_points); see svm_cell_list."""
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    G = C.c_int32(0)
+    rc = lib().svm_cell_list(C.c_int32(len(pts)), _dp(pts), C.c_double(cell_size), C.byref(G), None, None, None)
+    if rc != 0:
+        raise SVMError(rc, "svm_cell_list")
+    bin_of = np.empty(len(pts), np.int32)
+    start = np.empty(G.value ** 3 + 1, np.int32)
+    order = np.empty(len(pts), np.int32)
+    rc = lib().svm_cell_list(C.c_int32(len(pts)), _dp(pts), C.c_double(cell_size), C.byref(G), _ip(bin_of),
+                             _ip(start), _ip(order))
+    if rc != 0:
+        raise SVMError(rc, "svm_cell_list")
+    return G.value, bin_of, start, order
 
 
 def copy_probe(n_bytes, reps=3):

@@ -25,6 +25,7 @@ static inline int xcd_grid(int ntiles) { return (ntiles + 7) / 8 * 8; }
 #include "../../include/svm.h"
 #include "svm_kernels.h"
 #include "svm_delaunay.h"
+#include "svm_voronoi.h"
 
 using svm::Dev;
 using svm::SysParams;
@@ -292,6 +293,184 @@ int require_topology(svm_context* ctx) {
 }  // namespace
 
 // ======================================================================== API
+namespace {
+
+// Device scratch of one triangulation, freed when the holder goes.
+struct DevScratch {
+    std::vector<void*> ptrs;
+    ~DevScratch() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    T* get(size_t n) {
+        void* p = nullptr;
+        if (hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) return nullptr;
+        ptrs.push_back(p);
+        return (T*)p;
+    }
+};
+
+// exclusive scan of n device ints into out (recursive over blocks of 1024)
+bool dl_scan(hipStream_t st, DevScratch& m, const int* in, int n, int* out) {
+    const int nb = (n + 1023) / 1024;
+    int* sums = m.get<int>(nb);
+    if (!sums) return false;
+    hipLaunchKernelGGL(svm::k_scan_block, dim3(nb), dim3(1024), 0, st, in, n, out, sums);
+    if (nb > 1) {
+        int* offs = m.get<int>(nb);
+        if (!offs || !dl_scan(st, m, sums, nb, offs)) return false;
+        hipLaunchKernelGGL(svm::k_scan_add, dim3(nb), dim3(1024), 0, st, out, n, (const int*)offs);
+    }
+    return hipGetLastError() == hipSuccess;
+}
+
+// Cell list of n device points (AoS) in a uniform cube grid over
+// [-Rs, Rs]^3 with bins of side about `h`: start[b] .. start[b+1] index the
+// points of bin b in `list`, in index order.
+struct DlCells {
+    svm::DlGrid g;
+    int nbins = 0;
+    int* bin_of = nullptr;
+    int* start = nullptr;
+    int* list = nullptr;
+};
+
+bool dl_cell_list(hipStream_t st, DevScratch& m, const double* P, int n, double Rs, double h, DlCells& c) {
+    c.g.lo = -Rs * (1.0 + 1e-9);
+    c.g.G = std::max(1, std::min(256, (int)std::ceil(2.0 * Rs * (1.0 + 1e-9) / h)));
+    c.g.h = 2.0 * Rs * (1.0 + 1e-9) / c.g.G;
+    c.nbins = c.g.G * c.g.G * c.g.G;
+    c.bin_of = m.get<int>(n);
+    int* count = m.get<int>(c.nbins + 1);
+    c.start = m.get<int>(c.nbins + 1);
+    int* fill = m.get<int>(c.nbins);
+    c.list = m.get<int>(n);
+    if (!c.bin_of || !count || !c.start || !fill || !c.list) return false;
+    if (hipMemsetAsync(count, 0, sizeof(int) * (c.nbins + 1), st) != hipSuccess ||
+        hipMemsetAsync(fill, 0, sizeof(int) * c.nbins, st) != hipSuccess)
+        return false;
+    const int tb = 256;
+    hipLaunchKernelGGL(svm::k_dl_bin, dim3((n + tb - 1) / tb), dim3(tb), 0, st, P, n, c.g, c.bin_of, count);
+    if (!dl_scan(st, m, count, c.nbins + 1, c.start)) return false;
+    hipLaunchKernelGGL(svm::k_dl_fill, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, (const int*)c.bin_of,
+                       (const int*)c.start, fill, c.list);
+    hipLaunchKernelGGL(svm::k_dl_sort_bins, dim3((c.nbins + tb - 1) / tb), dim3(tb), 0, st, c.nbins,
+                       (const int*)c.start, c.list);
+    return hipGetLastError() == hipSuccess;
+}
+
+struct DlOut {
+    double* P = nullptr;   // points, AoS
+    int* T = nullptr;      // canonical triangles
+    int* off = nullptr;    // first triangle of each point's slice
+    int* nbr = nullptr;    // CCW Delaunay neighbours, cap per point
+    int* deg = nullptr;
+    int* tid = nullptr;    // triangle of each star wedge
+    int* err = nullptr;
+    int ntri = 0;
+};
+
+// Spherical Delaunay triangulation of n points (host, AoS) on stream `st`,
+// exact (see svm_voronoi.h).  Returns SVM_OK or an error code with `msg`.
+int dl_triangulate(hipStream_t st, DevScratch& m, const double* pts, int n, int cap, DlOut& o, std::string& msg) {
+    if (n < 4) { msg = "a spherical triangulation needs at least 4 points"; return SVM_ERR_ARG; }
+    double Rs = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double* p = pts + 3 * (size_t)i;
+        double r = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+        if (!(r > 0.0) || !std::isfinite(r)) { msg = "points must be finite and away from the origin"; return SVM_ERR_ARG; }
+        Rs = std::max(Rs, r);
+    }
+    const double spacing = std::sqrt(4.0 * SVM_PI * Rs * Rs / n);   // mean distance between neighbours
+    const double r0 = 3.5 * spacing;
+    o.P = m.get<double>(3 * (size_t)n);
+    if (!o.P || hipMemcpyAsync(o.P, pts, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess) {
+        msg = "device copy failed";
+        return SVM_ERR_HIP;
+    }
+    DlCells cl;
+    if (!dl_cell_list(st, m, o.P, n, Rs, r0, cl)) { msg = "cell list failed"; return SVM_ERR_HIP; }
+    const svm::DlGrid g = cl.g;
+    const int* start = cl.start;
+    const int* list = cl.list;
+    o.nbr = m.get<int>((size_t)n * cap);
+    o.deg = m.get<int>(n);
+    int* status = m.get<int>(n);
+    int* cnt = m.get<int>(n + 1);
+    o.off = m.get<int>(n + 1);
+    o.tid = m.get<int>((size_t)n * cap);
+    o.err = m.get<int>(1);
+    if (!o.nbr || !o.deg || !status || !cnt || !o.off || !o.tid || !o.err) {
+        msg = "device allocation failed";
+        return SVM_ERR_HIP;
+    }
+    const int tb = 256;
+    auto blocks = [&](long long x) { return dim3((unsigned)((x + tb - 1) / tb)); };
+    if (hipMemsetAsync(o.err, 0, sizeof(int), st) != hipSuccess ||
+        hipMemsetAsync(cnt, 0, sizeof(int) * (n + 1), st) != hipSuccess) {
+        msg = "device copy failed";
+        return SVM_ERR_HIP;
+    }
+    hipLaunchKernelGGL(svm::k_fill_int, blocks(n), dim3(tb), 0, st, status, n, (int)svm::DL_PENDING);
+    // stars, widening the search for the points that need it
+    std::vector<int> hs(n);
+    for (int pass = 0; pass < 3; ++pass) {
+        const double r = r0 * (1 + pass);
+        hipLaunchKernelGGL(svm::k_dl_star, dim3((n + SVM_DL_WAVES - 1) / SVM_DL_WAVES), dim3(64 * SVM_DL_WAVES), 0, st,
+                           (const double*)o.P, n, g, (const int*)start, (const int*)list, r, Rs, cap, o.nbr, o.deg,
+                           status);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(hs.data(), status, sizeof(int) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            msg = "star kernel failed";
+            return SVM_ERR_HIP;
+        }
+        int retry = 0, worst = 0;
+        for (int i = 0; i < n; ++i) {
+            if (hs[i] == svm::DL_RETRY) ++retry;
+            else if (hs[i] != svm::DL_OK) worst = hs[i];
+        }
+        if (worst) {
+            msg = worst == svm::DL_DEGENERATE ? "degenerate input: four co-circular points"
+                  : worst == svm::DL_OPEN     ? "a cell has more Delaunay neighbours than ring_capacity"
+                  : worst == svm::DL_CROWDED  ? "points too crowded for the neighbour search"
+                                              : "inconsistent triangulation";
+            return worst == svm::DL_OPEN ? SVM_ERR_RING_OVERFLOW : SVM_ERR_TOPOLOGY;
+        }
+        if (!retry) break;
+        if (pass == 2) { msg = "neighbour search did not converge"; return SVM_ERR_TOPOLOGY; }
+    }
+    // canonical triangles
+    hipLaunchKernelGGL(svm::k_dl_tri_count, blocks(n), dim3(tb), 0, st, n, cap, (const int*)o.nbr,
+                       (const int*)o.deg, cnt);
+    if (!dl_scan(st, m, cnt, n + 1, o.off)) { msg = "scan failed"; return SVM_ERR_HIP; }
+    int total = 0;
+    if (hipMemcpyAsync(&total, o.off + n, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        msg = "device copy failed";
+        return SVM_ERR_HIP;
+    }
+    if (total != 2 * n - 4) { msg = "triangle count is not 2N-4"; return SVM_ERR_TOPOLOGY; }
+    o.ntri = total;
+    o.T = m.get<int>(3 * (size_t)total);
+    if (!o.T) { msg = "device allocation failed"; return SVM_ERR_HIP; }
+    hipLaunchKernelGGL(svm::k_dl_tri_write, blocks(n), dim3(tb), 0, st, n, cap, (const int*)o.nbr,
+                       (const int*)o.deg, (const int*)o.off, o.T);
+    hipLaunchKernelGGL(svm::k_dl_tid, blocks(n), dim3(tb), 0, st, n, cap, (const int*)o.nbr, (const int*)o.deg,
+                       (const int*)o.T, (const int*)o.off, o.tid, o.err);
+    int herr = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&herr, o.err, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        msg = "triangle kernels failed";
+        return SVM_ERR_HIP;
+    }
+    if (herr) { msg = "stars of neighbouring points disagree"; return SVM_ERR_TOPOLOGY; }
+    return SVM_OK;
+}
+
+}  // namespace
+
 // A plain streaming copy (n doubles read, n written): the yardstick that
 // calibrates a profiler's HBM byte counters against known traffic.
 __global__ __launch_bounds__(256) void k_copy_probe(const double* __restrict__ a, double* __restrict__ b, size_t n) {
@@ -724,7 +903,89 @@ int svm_get_topology(svm_context* ctx, int32_t sys, int32_t* vertex_neighbors, i
 
 int svm_build_from_cell_points(svm_context* ctx, int32_t sys, const double* cell_points) {
     if (!valid_sys(ctx, sys) || !cell_points) return fail(ctx, SVM_ERR_ARG, "null argument");
-    return fail(ctx, SVM_ERR_STATE, "GPU Delaunay construction is not built yet");
+    const SysParams& L = ctx->layout[sys];
+    const int n = L.nc;
+    DevScratch m;
+    DlOut o;
+    std::string msg;
+    int r = dl_triangulate(ctx->stream, m, cell_points, n, ctx->cap, o, msg);
+    if (r) return fail(ctx, r, msg);
+    const double radius =
+        std::sqrt(cell_points[0] * cell_points[0] + cell_points[1] * cell_points[1] + cell_points[2] * cell_points[2]);
+    const int tb = 256;
+    hipLaunchKernelGGL(svm::k_dl_vertices, dim3((o.ntri + tb - 1) / tb), dim3(tb), 0, ctx->stream, ctx->d, sys,
+                       ctx->pcur, (const double*)o.P, o.ntri, ctx->cap, (const int*)o.T, (const int*)o.nbr,
+                       (const int*)o.deg, (const int*)o.tid, radius, o.err);
+    hipLaunchKernelGGL(svm::k_dl_rings, dim3((n + tb - 1) / tb), dim3(tb), 0, ctx->stream, ctx->d, sys, n, ctx->cap,
+                       (const int*)o.deg, (const int*)o.tid);
+    hipLaunchKernelGGL(svm::k_build_vln, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d, sys);
+    r = last_launch(ctx);
+    if (r) return r;
+    int herr = 0;
+    std::vector<int32_t> T(3 * (size_t)o.ntri);
+    HIPCHECK(ctx, hipMemcpyAsync(&herr, o.err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(T.data(), o.T, sizeof(int32_t) * T.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (herr) return fail(ctx, SVM_ERR_TOPOLOGY, "triangulation is not closed");
+    r = check_device_errors(ctx);
+    if (r) return r;
+    ctx->triangles[sys] = std::move(T);
+    ctx->have_topology[sys] = 1;
+    return SVM_OK;
+}
+
+int svm_cell_list(int32_t n_points, const double* points, double cell_size, int32_t* grid_dim, int32_t* bin_of,
+                  int32_t* bin_start, int32_t* sorted_points) {
+    if (n_points < 1 || !points || !(cell_size > 0.0) || !grid_dim) return SVM_ERR_ARG;
+    double Rs = 0.0;
+    for (int i = 0; i < 3 * n_points; ++i) {
+        if (!std::isfinite(points[i])) return SVM_ERR_ARG;
+        Rs = std::max(Rs, std::fabs(points[i]));
+    }
+    if (!(Rs > 0.0)) return SVM_ERR_ARG;
+    const int G = std::max(1, std::min(256, (int)std::ceil(2.0 * Rs * (1.0 + 1e-9) / cell_size)));
+    *grid_dim = G;
+    if (!bin_of || !bin_start || !sorted_points) return SVM_OK;   // size query
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SVM_ERR_NO_DEVICE;
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return SVM_ERR_HIP;
+    int r = SVM_OK;
+    {
+        DevScratch m;
+        DlCells c;
+        double* P = m.get<double>(3 * (size_t)n_points);
+        if (!P || hipMemcpyAsync(P, points, sizeof(double) * 3 * (size_t)n_points, hipMemcpyHostToDevice, st) != hipSuccess ||
+            !dl_cell_list(st, m, P, n_points, Rs, cell_size, c) ||
+            hipMemcpyAsync(bin_of, c.bin_of, sizeof(int) * n_points, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(bin_start, c.start, sizeof(int) * (c.nbins + 1), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(sorted_points, c.list, sizeof(int) * n_points, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            r = SVM_ERR_HIP;
+    }
+    (void)hipStreamDestroy(st);
+    return r;
+}
+
+int svm_delaunay_triangles(int32_t n_points, const double* points, int32_t* triangles) {
+    if (n_points < 4 || !points || !triangles) return SVM_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SVM_ERR_NO_DEVICE;
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return SVM_ERR_HIP;
+    int r;
+    {
+        DevScratch m;
+        DlOut o;
+        std::string msg;
+        r = dl_triangulate(st, m, points, n_points, 64, o, msg);
+        if (!r && (hipMemcpyAsync(triangles, o.T, sizeof(int32_t) * 3 * (size_t)o.ntri, hipMemcpyDeviceToHost, st) !=
+                       hipSuccess ||
+                   hipStreamSynchronize(st) != hipSuccess))
+            r = SVM_ERR_HIP;
+    }
+    (void)hipStreamDestroy(st);
+    return r;
 }
 
 int svm_get_delaunay_triangles(svm_context* ctx, int32_t sys, int32_t* triangles) {

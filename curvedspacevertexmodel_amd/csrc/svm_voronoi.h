@@ -1,0 +1,408 @@
+// svm_voronoi.h -- the spherical Delaunay triangulation of cell points on the
+// GPU, and the vertex-model topology built from it.
+//
+// Reference behaviour: convexHullCGALInterface::sphericalConvexHullForVertexModel
+// (src/model/convexHullCGAL.cpp:82-186) takes the convex hull of the cell
+// points (for points on a sphere, the Delaunay triangulation); every hull
+// facet becomes a Voronoi vertex at R * normalize(a + b + c); cells list their
+// vertices clockwise seen from outside.  sphericalVoronoi::convexHull
+// (src/model/sphericalVoronoi.cpp:8-12) is the same triangulation, redone at
+// every move.  The neighbour search follows the role of hyperrectangularCellList
+// (src/utilities/hyperrectangularCellList.cpp:14-51, 165-342): points binned in
+// a uniform cube grid, candidates read from the bins around a point.
+//
+// Method (own design):
+//  * cell list: bin = floor((x - lo) / h) per axis; counting sort; each bin's
+//    points sorted by index, so the list does not depend on atomic order.
+//  * star of point i, one wavefront per point: the candidates within chord
+//    distance r; the nearest is a Delaunay neighbour; then gift wrapping
+//    around i: the next neighbour after j is the candidate k with no other
+//    candidate beyond the plane (i, j, k), found by a tournament across the
+//    lanes whose comparison is the exact orient3d predicate (svm_exact.h).
+//    Every facet is then checked against every candidate (exactly), and its
+//    circumcircle must lie well inside the search ball, so no point outside
+//    the candidates can lie in it; otherwise the point is redone with a
+//    larger r.  Four co-circular points (an exact zero) are reported as
+//    degenerate input.
+//  * triangles numbered as the host canonical form does: CCW seen from
+//    outside, smallest cell first, sorted lexicographically; rings walk the
+//    triangles around a cell starting from its smallest triangle.
+#pragma once
+#include "svm_exact.h"
+
+namespace svm {
+
+#define SVM_DL_MAXC 512        // candidates one star can hold (8 per lane)
+#define SVM_DL_WAVES 4         // stars per block
+
+enum {
+    DL_OK = 0,
+    DL_RETRY = 1,        // circumcircle not inside the search ball: redo with a larger radius
+    DL_CROWDED = 2,      // more than SVM_DL_MAXC candidates
+    DL_OPEN = 3,         // the star did not close within the ring capacity
+    DL_INVALID = 4,      // a facet failed the exact check
+    DL_DEGENERATE = 5,   // four co-circular points
+    DL_PENDING = 6,      // not computed yet
+};
+
+struct DlGrid {
+    double lo, h;
+    int G;
+};
+
+// ------------------------------------------------------------- cell list --
+__device__ inline int dl_axis(double x, const DlGrid& g) {
+    int k = (int)__builtin_floor((x - g.lo) / g.h);
+    return k < 0 ? 0 : (k >= g.G ? g.G - 1 : k);
+}
+
+__global__ __launch_bounds__(256) void k_dl_bin(const double* P, int n, DlGrid g, int* bin_of, int* count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int b = (dl_axis(P[3 * (size_t)i], g) * g.G + dl_axis(P[3 * (size_t)i + 1], g)) * g.G +
+            dl_axis(P[3 * (size_t)i + 2], g);
+    bin_of[i] = b;
+    atomicAdd(count + b, 1);
+}
+
+__global__ __launch_bounds__(256) void k_dl_fill(int n, const int* bin_of, const int* start, int* fill, int* list) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int b = bin_of[i];
+    list[start[b] + atomicAdd(fill + b, 1)] = i;
+}
+
+// each bin's points in index order
+__global__ __launch_bounds__(256) void k_dl_sort_bins(int nbins, const int* start, int* list) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbins) return;
+    int lo = start[b], hi = start[b + 1];
+    for (int i = lo + 1; i < hi; ++i) {
+        int x = list[i], j = i - 1;
+        while (j >= lo && list[j] > x) { list[j + 1] = list[j]; --j; }
+        list[j + 1] = x;
+    }
+}
+
+// exclusive scan in blocks of 1024: out[i] = sum of in[<i] within the block,
+// block totals to sums[block]
+__global__ __launch_bounds__(1024) void k_scan_block(const int* in, int n, int* out, int* sums) {
+    __shared__ int s[1024];
+    int i = blockIdx.x * 1024 + threadIdx.x;
+    int x = i < n ? in[i] : 0;
+    s[threadIdx.x] = x;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int add = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += add;
+        __syncthreads();
+    }
+    if (i < n) out[i] = s[threadIdx.x] - x;
+    if (threadIdx.x == 1023) sums[blockIdx.x] = s[1023];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_add(int* out, int n, const int* offs) {
+    int i = blockIdx.x * 1024 + threadIdx.x;
+    if (i < n) out[i] += offs[blockIdx.x];
+}
+
+// ------------------------------------------------------------------ stars --
+__device__ inline void dl_load(const double* P, int i, double* p) {
+    p[0] = P[3 * (size_t)i];
+    p[1] = P[3 * (size_t)i + 1];
+    p[2] = P[3 * (size_t)i + 2];
+}
+
+// Does candidate x beat candidate k as the wrap point after edge (pi, pj)?
+// Exactly: x lies beyond the plane (pi, pj, pk).  A tie (co-planar) is
+// broken by index and reported.
+__device__ inline bool dl_beats(const double* pi, const double* pj, const double* pk, int k, const double* px, int x,
+                                bool& degenerate) {
+    int o = orient3d_sign(pi, pj, pk, px);
+    if (o == 0) {
+        degenerate = true;
+        return x < k;
+    }
+    return o > 0;
+}
+
+// Star of every point i with status[i] == DL_PENDING or DL_RETRY.  Writes the
+// CCW ring of Delaunay neighbours (nbr[i*cap ..], deg[i]) and the status.
+__global__ __launch_bounds__(64 * SVM_DL_WAVES) void k_dl_star(const double* P, int n, DlGrid g, const int* start,
+                                                               const int* list, double r, double R, int cap,
+                                                               int* nbr, int* deg, int* status) {
+    __shared__ int s_idx[SVM_DL_WAVES][SVM_DL_MAXC];
+    __shared__ int s_ring[SVM_DL_WAVES][64];
+    __shared__ int s_n[SVM_DL_WAVES];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * SVM_DL_WAVES + w;
+    if (i >= n) return;
+    const int st0 = status[i];
+    if (st0 != DL_PENDING && st0 != DL_RETRY) return;
+    double pi[3];
+    dl_load(P, i, pi);
+    // ---- gather the candidates within chord distance r
+    if (lane == 0) s_n[w] = 0;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const int wr = (int)__builtin_ceil(r / g.h);
+    const int gx = dl_axis(pi[0], g), gy = dl_axis(pi[1], g), gz = dl_axis(pi[2], g);
+    const int span = 2 * wr + 1, nb = span * span * span;
+    const double r2 = r * r;
+    for (int b = lane; b < nb; b += 64) {
+        int x = gx + b % span - wr, y = gy + (b / span) % span - wr, z = gz + b / (span * span) - wr;
+        if (x < 0 || y < 0 || z < 0 || x >= g.G || y >= g.G || z >= g.G) continue;
+        int bin = (x * g.G + y) * g.G + z;
+        for (int e = start[bin]; e < start[bin + 1]; ++e) {
+            int q = list[e];
+            if (q == i) continue;
+            double dx = P[3 * (size_t)q] - pi[0], dy = P[3 * (size_t)q + 1] - pi[1], dz = P[3 * (size_t)q + 2] - pi[2];
+            if (dx * dx + dy * dy + dz * dz < r2) {
+                int k = atomicAdd(&s_n[w], 1);
+                if (k < SVM_DL_MAXC) s_idx[w][k] = q;
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const int nc = s_n[w];
+    if (nc > SVM_DL_MAXC || nc < 3) {
+        if (lane == 0) status[i] = nc < 3 ? DL_RETRY : DL_CROWDED;
+        return;
+    }
+    const int* cand = s_idx[w];
+    // ---- nearest candidate (squared distance, then index)
+    double bd = 1e300;
+    int bj = 0x7fffffff;
+    for (int c = lane; c < nc; c += 64) {
+        int q = cand[c];
+        double dx = P[3 * (size_t)q] - pi[0], dy = P[3 * (size_t)q + 1] - pi[1], dz = P[3 * (size_t)q + 2] - pi[2];
+        double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < bd || (d2 == bd && q < bj)) { bd = d2; bj = q; }
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        double od = __shfl_xor(bd, m, 64);
+        int oj = __shfl_xor(bj, m, 64);
+        if (od < bd || (od == bd && oj < bj)) { bd = od; bj = oj; }
+    }
+    const int j0 = bj;
+    // ---- gift wrapping around i
+    bool degenerate = false;
+    int status_out = DL_OK;
+    int cur = j0, d = 0;
+    int* ring = s_ring[w];   // cap <= 64
+    double pc[3];
+    dl_load(P, cur, pc);
+    while (true) {
+        if (d >= cap) { status_out = DL_OPEN; break; }
+        if (lane == 0) ring[d] = cur;
+        ++d;
+        // tournament: each lane its candidates, then across lanes
+        int bk = -1;
+        double pk[3] = {0.0, 0.0, 0.0};
+        for (int c = lane; c < nc; c += 64) {
+            int q = cand[c];
+            if (q == cur) continue;
+            double pq[3];
+            dl_load(P, q, pq);
+            if (bk < 0 || dl_beats(pi, pc, pk, bk, pq, q, degenerate)) {
+                bk = q;
+                pk[0] = pq[0]; pk[1] = pq[1]; pk[2] = pq[2];
+            }
+        }
+        for (int m = 1; m < 64; m <<= 1) {
+            int ok = __shfl_xor(bk, m, 64);
+            double o0 = __shfl_xor(pk[0], m, 64), o1 = __shfl_xor(pk[1], m, 64), o2 = __shfl_xor(pk[2], m, 64);
+            if (ok < 0) continue;
+            double po[3] = {o0, o1, o2};
+            bool take;
+            if (bk < 0) take = true;
+            else if (ok == bk) take = false;
+            else {
+                // both lanes of the pair decide the same way: compare with the smaller index first
+                if (bk < ok) take = dl_beats(pi, pc, pk, bk, po, ok, degenerate);
+                else take = !dl_beats(pi, pc, po, ok, pk, bk, degenerate);
+            }
+            if (take) { bk = ok; pk[0] = o0; pk[1] = o1; pk[2] = o2; }
+        }
+        if (bk < 0) { status_out = DL_INVALID; break; }
+        if (bk == j0) {
+            if (d < 3) status_out = DL_INVALID;
+            break;
+        }
+        cur = bk;
+        pc[0] = pk[0]; pc[1] = pk[1]; pc[2] = pk[2];
+    }
+    // ---- verification of every facet (i, ring[s], ring[s+1])
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    bool bad = false, far = false;
+    if (status_out == DL_OK) {
+        for (int s = 0; s < d; ++s) {
+            int a = ring[s], b = ring[(s + 1) % d];
+            double pa[3], pb[3];
+            dl_load(P, a, pa);
+            dl_load(P, b, pb);
+            if (lane == 0) {
+                if (orient_origin_sign(pi, pa, pb) <= 0) bad = true;
+                // circumradius of (pi, pa, pb); the cap beyond the facet lies within 2.25 rho of pi
+                double u[3] = {pa[0] - pi[0], pa[1] - pi[1], pa[2] - pi[2]};
+                double v[3] = {pb[0] - pi[0], pb[1] - pi[1], pb[2] - pi[2]};
+                double wv[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+                double cx = u[1] * v[2] - u[2] * v[1], cy = u[2] * v[0] - u[0] * v[2], cz = u[0] * v[1] - u[1] * v[0];
+                double area2 = __builtin_sqrt(cx * cx + cy * cy + cz * cz);
+                double rho = __builtin_sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) *
+                             __builtin_sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) *
+                             __builtin_sqrt(wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2]) / (2.0 * area2);
+                if (!(2.25 * rho * (1.0 + 1e-6) < r) && r < 2.0 * R) far = true;
+            }
+            for (int c = lane; c < nc; c += 64) {
+                int q = cand[c];
+                if (q == a || q == b) continue;
+                double pq[3];
+                dl_load(P, q, pq);
+                int o = orient3d_sign(pi, pa, pb, pq);
+                if (o > 0) bad = true;
+                if (o == 0) degenerate = true;
+            }
+        }
+    }
+    unsigned long long any_bad = __ballot(bad), any_far = __ballot(far), any_deg = __ballot(degenerate);
+    if (lane == 0) {
+        int st = status_out;
+        if (st == DL_OK) {
+            if (any_deg) st = DL_DEGENERATE;
+            else if (any_bad) st = DL_INVALID;
+            else if (any_far) st = DL_RETRY;
+        }
+        status[i] = st;
+        deg[i] = d;
+    }
+    for (int s = lane; s < d; s += 64) nbr[(size_t)i * cap + s] = ring[s];
+}
+
+__global__ __launch_bounds__(256) void k_fill_int(int* a, int n, int value) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = value;
+}
+
+// ---------------------------------------------------------- triangles ----
+// Canonical triangles of point m: (m, a, b) for consecutive ring neighbours
+// a, b both above m (CCW order kept), sorted by a.
+__global__ __launch_bounds__(256) void k_dl_tri_count(int n, int cap, const int* nbr, const int* deg, int* cnt) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const int* rg = nbr + (size_t)m * cap;
+    int d = deg[m], c = 0;
+    for (int j = 0; j < d; ++j) c += (rg[j] > m && rg[(j + 1) % d] > m);
+    cnt[m] = c;
+}
+
+__global__ __launch_bounds__(256) void k_dl_tri_write(int n, int cap, const int* nbr, const int* deg, const int* off,
+                                                      int* T) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const int* rg = nbr + (size_t)m * cap;
+    int d = deg[m];
+    int base = off[m], c = 0;
+    for (int j = 0; j < d; ++j) {
+        int a = rg[j], b = rg[(j + 1) % d];
+        if (a > m && b > m) {
+            // insertion by a into this point's slice
+            int k = c++;
+            while (k > 0 && T[3 * (size_t)(base + k - 1) + 1] > a) {
+                for (int e = 0; e < 3; ++e) T[3 * (size_t)(base + k) + e] = T[3 * (size_t)(base + k - 1) + e];
+                --k;
+            }
+            T[3 * (size_t)(base + k)] = m;
+            T[3 * (size_t)(base + k) + 1] = a;
+            T[3 * (size_t)(base + k) + 2] = b;
+        }
+    }
+}
+
+// index of the canonical triangle (m, a, .) in m's slice; -1 if absent
+__device__ inline int dl_tri_index(const int* T, const int* off, int m, int a) {
+    for (int t = off[m]; t < off[m + 1]; ++t)
+        if (T[3 * (size_t)t + 1] == a) return t;
+    return -1;
+}
+
+// Per point c: the global index of each triangle (c, r_j, r_{j+1}) of its star.
+__global__ __launch_bounds__(256) void k_dl_tid(int n, int cap, const int* nbr, const int* deg, const int* T,
+                                                const int* off, int* tid, int* err) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int* rg = nbr + (size_t)c * cap;
+    int d = deg[c];
+    for (int j = 0; j < d; ++j) {
+        int a = rg[j], b = rg[(j + 1) % d];
+        int m, s;
+        if (c < a && c < b) { m = c; s = a; }
+        else if (a < b) { m = a; s = b; }
+        else { m = b; s = c; }
+        int t = dl_tri_index(T, off, m, s);
+        if (t < 0) atomicOr(err, 1);   // stars of neighbouring points disagree
+        tid[(size_t)c * cap + j] = t;
+    }
+}
+
+// Topology of sphere rows: per triangle t (vertex vo + t) the position
+// R_hull * normalize(centroid) (convexHullCGAL.cpp:118-123; same operation
+// order as the host path), vertex->cell and vertex->vertex lists; per cell
+// its ring, starting at its smallest triangle and walking against the CCW
+// star order (clockwise seen from outside).
+__global__ __launch_bounds__(256) void k_dl_vertices(Dev d, int sys, int pout, const double* P, int ntri, int cap,
+                                                     const int* T, const int* nbr, const int* deg, const int* tid,
+                                                     double radius, int* err) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntri) return;
+    const SysParams& S = d.sys[sys];
+    int c3[3] = {T[3 * (size_t)t], T[3 * (size_t)t + 1], T[3 * (size_t)t + 2]};
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 3; ++k) {
+        const double* p = P + 3 * (size_t)c3[k];
+        c[0] += p[0];
+        c[1] += p[1];
+        c[2] += p[2];
+    }
+    for (int k = 0; k < 3; ++k) c[k] = (1.0 / 3) * c[k];
+    double nrm = __builtin_sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    const int v = S.vo + t;
+    d.px[pout][v] = (radius / nrm) * c[0];
+    d.py[pout][v] = (radius / nrm) * c[1];
+    d.pz[pout][v] = (radius / nrm) * c[2];
+    int vn3[3];
+    for (int k = 0; k < 3; ++k) {
+        int cur = c3[k], prev = c3[(k + 2) % 3];
+        // the triangle with directed edge cur -> prev: (cur, prev, .) in cur's star
+        const int* rg = nbr + (size_t)cur * cap;
+        int j = -1;
+        for (int e = 0; e < deg[cur]; ++e)
+            if (rg[e] == prev) { j = e; break; }
+        if (j < 0) { atomicOr(err, 2); vn3[k] = 0; continue; }
+        vn3[k] = tid[(size_t)cur * cap + j];
+    }
+    d.vcn[v] = make_int4(c3[0] + S.co, c3[1] + S.co, c3[2] + S.co, sys);
+    d.vn[v] = make_int4(vn3[0] + S.vo, vn3[1] + S.vo, vn3[2] + S.vo, sys);
+}
+
+__global__ __launch_bounds__(256) void k_dl_rings(Dev d, int sys, int n, int cap, const int* deg, const int* tid) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const SysParams& S = d.sys[sys];
+    const int dg = deg[c];
+    const int* td = tid + (size_t)c * cap;
+    int js = 0;
+    for (int j = 1; j < dg; ++j)
+        if (td[j] < td[js]) js = j;
+    for (int k = 0; k < d.cap; ++k) {
+        int x = -1;
+        if (k < dg) x = S.vo + td[((js - k) % dg + dg) % dg];
+        d.ring[(size_t)k * d.Cs + S.co + c] = x;
+    }
+    d.cnn[S.co + c] = dg;
+}
+
+}  // namespace svm

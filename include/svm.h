@@ -95,6 +95,24 @@ const char* svm_last_error(svm_context* ctx);
  * (src/model/convexHullCGAL.cpp:82-186) as called by the constructor
  * (src/model/sphericalVertexModel.cpp:14-18).  cell_points: double[3N]. */
 int svm_build_from_cell_points(svm_context* ctx, int32_t sys, const double* cell_points);
+/* Spherical Delaunay triangulation of n_points points on the GPU, with no
+ * model attached: the per-move retriangulation of sphericalVoronoi::convexHull
+ * (src/model/sphericalVoronoi.cpp:8-12, src/model/convexHullCGAL.cpp:15-65)
+ * and its neighbour search (hyperrectangularCellList,
+ * src/utilities/hyperrectangularCellList.cpp:14-51, 165-342).  Exact
+ * predicates; four co-circular points are rejected (SVM_ERR_TOPOLOGY).
+ * triangles: int32[3*(2N-4)], counter-clockwise seen from outside, smallest
+ * index first, sorted lexicographically. */
+int svm_delaunay_triangles(int32_t n_points, const double* points, int32_t* triangles);
+/* Cell list of points on the GPU (the binning of hyperrectangularCellList,
+ * src/utilities/hyperrectangularCellList.cpp:165-342, on a cube grid
+ * [-L, L]^3 with L = (1 + 1e-9) * max |coordinate|): grid_dim G bins per axis,
+ * G = min(256, ceil(2L / cell_size)), bin side 2L/G; bin of point i is
+ * (gx*G + gy)*G + gz.  bin_start: int32[G^3 + 1], sorted_points: the points of
+ * bin b at sorted_points[bin_start[b] .. bin_start[b+1]), in index order.
+ * With bin_of, bin_start or sorted_points NULL only grid_dim is returned. */
+int svm_cell_list(int32_t n_points, const double* points, double cell_size, int32_t* grid_dim, int32_t* bin_of,
+                  int32_t* bin_start, int32_t* sorted_points);
 /* Seeded cell centres: sphericalModel::setParticlePositionsRandomly + setRadius
  * (src/model/sphericalModel.cpp:11-51) with Philox draws keyed by `seed`. */
 int svm_generate_cell_points(int32_t n_cells, uint64_t seed, double* cell_points);

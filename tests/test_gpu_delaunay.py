@@ -1,0 +1,95 @@
+"""GPU spherical Delaunay triangulation, cell list and topology construction.
+
+Bar: bit-exact.  The triangles equal the committed Qhull fixtures (which the
+CPU tests pin by brute force and exact empty-circumcircle checks); the
+topology built on the GPU (vertex positions, vertex lists, cell rings) equals
+the CPU construction from Qhull triangles array for array; the cell list
+equals a brute-force binning.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import svmtest
+import curvedspacevertexmodel_amd as svm
+from test_gpu_parity import make_ctx, assert_topology_equal
+from test_oracle import load_fixture
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIXTURES = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "delaunay_N*.txt")))
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[os.path.basename(p) for p in FIXTURES])
+def test_triangles_match_fixtures(path):
+    pts, tris, _ = load_fixture(path)
+    print(f"fixture {os.path.basename(path)}", flush=True)
+    np.testing.assert_array_equal(svm.delaunay_triangles(pts), tris)
+
+
+def test_triangles_of_scaled_and_rotated_points():
+    """Points not at unit density (radius 1) and rotated: same combinatorics."""
+    pts, tris, _ = load_fixture(os.path.join(ROOT, "tests", "golden", "delaunay_N512_s1137.txt"))
+    unit = pts / np.linalg.norm(pts[0])
+    np.testing.assert_array_equal(svm.delaunay_triangles(unit), tris)
+    th = 0.7
+    Q = np.array([[np.cos(th), -np.sin(th), 0.0], [np.sin(th), np.cos(th), 0.0], [0.0, 0.0, 1.0]])
+    rot = pts @ Q.T
+    np.testing.assert_array_equal(svm.delaunay_triangles(rot), svmtest.qhull_triangles(rot))
+
+
+def test_large_triangulation_against_qhull():
+    n = 100000
+    pts = svmtest.cell_points(n, 4242)
+    t = svm.delaunay_triangles(pts)
+    np.testing.assert_array_equal(t, svmtest.qhull_triangles(pts))
+
+
+def test_cell_list_matches_brute_force():
+    pts = svmtest.cell_points(5000, 77)
+    h = 2.5
+    G, bin_of, start, order = svm.cell_list(pts, h)
+    L = (1.0 + 1e-9) * np.abs(pts).max()
+    side = 2 * L / G
+    g = np.clip(np.floor((pts + L) / side).astype(np.int64), 0, G - 1)
+    ref = (g[:, 0] * G + g[:, 1]) * G + g[:, 2]
+    np.testing.assert_array_equal(bin_of, ref)
+    counts = np.bincount(ref, minlength=G ** 3)
+    np.testing.assert_array_equal(np.diff(start), counts)
+    # each bin lists its points in index order (the sort is stable by index)
+    np.testing.assert_array_equal(order, np.argsort(ref, kind="stable"))
+    # and the construction is deterministic
+    G2, b2, s2, o2 = svm.cell_list(pts, h)
+    assert G2 == G and np.array_equal(o2, order) and np.array_equal(s2, start)
+
+
+@pytest.mark.parametrize("n,seed", [(512, 1137), (3000, 1201)])
+def test_build_from_cell_points_matches_cpu_construction(n, seed):
+    st = svmtest.seeded_state(n, seed)
+    om = svmtest.OracleModel(st)
+    systems = [dict(n_cells=n, a0=1.0, p0=3.8, kr=1.0, t1_threshold=0.1, seed=seed)]
+    ctx = svm.Context(systems)
+    ctx.build_from_cell_points(0, st["points"])
+    assert_topology_equal(ctx, 0, om)
+    np.testing.assert_array_equal(ctx.positions(0), om.positions())
+    np.testing.assert_array_equal(ctx.delaunay_triangles(0), st["tris"])
+    # the same dynamics as a context given the CPU-built topology
+    ref = make_ctx([st], seeds=[seed])
+    for c in (ctx, ref):
+        c.set_directors(0, st["directors"])
+        c.set_active(0, dt=0.01, mu=1.0, v0=0.1, tau=1.0)
+        c.active_steps(20)
+    np.testing.assert_array_equal(ctx.positions(0), ref.positions(0))
+
+
+def test_build_batch_of_spheres():
+    sizes = [(200, 31), (700, 32), (1500, 33)]
+    states = [svmtest.seeded_state(n, s) for n, s in sizes]
+    ctx = svm.Context([dict(n_cells=n, a0=1.0, p0=3.8, kr=1.0, t1_threshold=0.1, seed=s) for n, s in sizes])
+    for i, st in enumerate(states):
+        ctx.build_from_cell_points(i, st["points"])
+    for i, st in enumerate(states):
+        assert_topology_equal(ctx, i, svmtest.OracleModel(st))
