@@ -34,7 +34,9 @@ EXPORTS = [
     "svm_fire_get_state", "svm_set_t1_mode", "svm_get_t1_diagnostics", "svm_set_profiling", "svm_get_timing",
     "svm_copy_probe", "svm_spatial_order", "svm_time_vertex_pass",
     "svm_set_force_precision", "svm_get_force_error", "svm_delaunay_triangles",
-    "svm_cell_list",
+    "svm_cell_list", "svm_mlp_create", "svm_mlp_destroy", "svm_mlp_set_parameters", "svm_mlp_get_parameters",
+    "svm_mlp_forward", "svm_mlp_gradient", "svm_mlp_train", "svm_mlp_time_forward", "svm_cell_descriptors",
+    "svm_warm_start",
 ]
 
 
@@ -140,6 +142,81 @@ def cell_list(points, cell_size):
     if rc != 0:
         raise SVMError(rc, "svm_cell_list")
     return G.value, bin_of, start, order
+
+
+MLP_NPARAM = 10448
+MLP_NDESC = 12
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class MLP:
+    """The per-cell warm-start network (svm_mlp_*): 12 -> 64 -> 64 -> 64 -> 2,
+    bf16 weights on MFMA, trained on the GPU."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        rc = lib().svm_mlp_create(C.byref(self.h))
+        if rc != 0:
+            raise SVMError(rc, "svm_mlp_create")
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().svm_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            raise SVMError(rc, what)
+
+    def set_parameters(self, theta, mu, sigma, out_scale=1.0):
+        th = np.ascontiguousarray(theta, np.float32)
+        mu = np.ascontiguousarray(mu, np.float32)
+        sg = np.ascontiguousarray(sigma, np.float32)
+        assert th.size == MLP_NPARAM and mu.size == MLP_NDESC and sg.size == MLP_NDESC
+        self._ck(lib().svm_mlp_set_parameters(self.h, _fp(th), _fp(mu), _fp(sg), C.c_float(out_scale)),
+                 "svm_mlp_set_parameters")
+
+    def get_parameters(self):
+        th = np.zeros(MLP_NPARAM, np.float32)
+        mu = np.zeros(MLP_NDESC, np.float32)
+        sg = np.zeros(MLP_NDESC, np.float32)
+        sc = C.c_float(0)
+        self._ck(lib().svm_mlp_get_parameters(self.h, _fp(th), _fp(mu), _fp(sg), C.byref(sc)),
+                 "svm_mlp_get_parameters")
+        return th, mu, sg, sc.value
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], 2), np.float32)
+        self._ck(lib().svm_mlp_forward(self.h, _fp(X), C.c_int32(X.shape[0]), _fp(Y)), "svm_mlp_forward")
+        return Y
+
+    def gradient(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = C.c_double(0)
+        self._ck(lib().svm_mlp_gradient(self.h, _fp(X), _fp(T), C.c_int32(X.shape[0]), _fp(g), C.byref(loss)),
+                 "svm_mlp_gradient")
+        return g, loss.value
+
+    def train(self, X, T, epochs, batch=4096, lr=1e-3, seed=1):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        losses = np.zeros(epochs, np.float64)
+        self._ck(lib().svm_mlp_train(self.h, _fp(X), _fp(T), C.c_int32(X.shape[0]), C.c_int32(epochs),
+                                     C.c_int32(batch), C.c_float(lr), C.c_uint64(seed), _dp(losses)),
+                 "svm_mlp_train")
+        return losses
+
+    def time_forward(self, n, reps=20):
+        ms = C.c_double(0)
+        self._ck(lib().svm_mlp_time_forward(self.h, C.c_int32(n), C.c_int32(reps), C.byref(ms)),
+                 "svm_mlp_time_forward")
+        return ms.value
 
 
 def copy_probe(n_bytes, reps=3):
@@ -338,6 +415,17 @@ class Context:
 
     def set_stream(self, stream_ptr):
         self._ck(lib().svm_set_stream(self.h, C.c_void_p(stream_ptr)), "svm_set_stream")
+
+    def cell_descriptors(self, sys):
+        """(X float32 [N][16], frames float32 [N][6]) of sphere sys."""
+        n = self.n_cells[sys]
+        X = np.zeros((n, 16), np.float32)
+        F = np.zeros((n, 6), np.float32)
+        self._ck(lib().svm_cell_descriptors(self.h, C.c_int32(sys), _fp(X), _fp(F)), "svm_cell_descriptors")
+        return X, F
+
+    def warm_start(self, mlp, scale=1.0):
+        self._ck(lib().svm_warm_start(self.h, mlp.h, C.c_double(scale)), "svm_warm_start")
 
     def set_force_precision(self, precision):
         """0: FP64 forces (parity), 1: FP32 forces for every later force evaluation."""

@@ -26,6 +26,7 @@ static inline int xcd_grid(int ntiles) { return (ntiles + 7) / 8 * 8; }
 #include "svm_kernels.h"
 #include "svm_delaunay.h"
 #include "svm_voronoi.h"
+#include "svm_mlp.h"
 
 using svm::Dev;
 using svm::SysParams;
@@ -470,6 +471,25 @@ int dl_triangulate(hipStream_t st, DevScratch& m, const double* pts, int n, int 
 }
 
 }  // namespace
+
+// rows idx[0..n) of src (width floats each) into dst
+__global__ __launch_bounds__(256) void k_gather_rows(const float* src, const int* idx, int n, int width, float* dst) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (size_t)n * width) return;
+    const int r = (int)(e / width), k = (int)(e % width);
+    dst[e] = src[(size_t)idx[r] * width + k];
+}
+
+// deterministic pseudo-random descriptors in [-1, 1) for timing runs
+__global__ __launch_bounds__(256) void k_fill_hash(float* x, size_t n) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    unsigned h = (unsigned)e * 2654435761u;
+    h ^= h >> 15;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    x[e] = (float)(h & 0xffffff) * (2.0f / 16777216.0f) - 1.0f;
+}
 
 // A plain streaming copy (n doubles read, n written): the yardstick that
 // calibrates a profiler's HBM byte counters against known traffic.
@@ -1461,6 +1481,363 @@ int svm_copy_probe(int64_t n_bytes, int32_t reps, double* ms_per_copy) {
     if (a) hipFree(a);
     if (b) hipFree(b);
     return rc;
+}
+
+}  // extern "C"
+
+// ============================================================ learned warm start
+struct svm_mlp {
+    float* theta = nullptr;           // FP32 master weights [SVM_MLP_NP]
+    unsigned short* wb = nullptr;     // bf16 copies: w1 [1024], w2 [4096], w3 [4096], w4 [1024]
+    float* musig = nullptr;           // mu [16], sigma [16]
+    float* mom = nullptr;
+    float* vel = nullptr;
+    float* grad = nullptr;
+    float out_scale = 1.0f;
+    long long adam_t = 0;
+    hipStream_t stream = nullptr;
+    // workspace, grown on demand
+    size_t cap = 0;
+    float *X = nullptr, *T = nullptr, *Y = nullptr, *Z = nullptr, *H = nullptr, *D = nullptr, *partial = nullptr,
+          *lossp = nullptr;
+    int* idx = nullptr;
+};
+
+namespace {
+
+svm::MlpParams mlp_params(const svm_mlp* m) {
+    svm::MlpParams p;
+    p.theta = m->theta;
+    p.w1 = m->wb;
+    p.w2 = m->wb + 1024;
+    p.w3 = m->wb + 5120;
+    p.w4 = m->wb + 9216;
+    p.mu = m->musig;
+    p.sig = m->musig + 16;
+    p.out_scale = m->out_scale;
+    return p;
+}
+
+int mlp_refresh_bf16(svm_mlp* m) {
+    hipLaunchKernelGGL(svm::k_mlp_to_bf16, dim3(16), dim3(256), 0, m->stream, (const float*)m->theta, m->wb,
+                       m->wb + 1024, m->wb + 5120, m->wb + 9216);
+    return hipGetLastError() == hipSuccess ? SVM_OK : SVM_ERR_HIP;
+}
+
+int mlp_reserve(svm_mlp* m, size_t n) {
+    if (n <= m->cap) return SVM_OK;
+    void* old[9] = {m->X, m->T, m->Y, m->Z, m->H, m->D, m->partial, m->lossp, m->idx};
+    (void)hipStreamSynchronize(m->stream);
+    for (void* p : old)
+        if (p) (void)hipFree(p);
+    const size_t chunks = (n + 63) / 64, blocks = (n + 255) / 256;
+    if (hipMalloc(&m->X, sizeof(float) * 16 * n) != hipSuccess || hipMalloc(&m->T, sizeof(float) * 2 * n) != hipSuccess ||
+        hipMalloc(&m->Y, sizeof(float) * 2 * n) != hipSuccess ||
+        hipMalloc(&m->Z, sizeof(float) * SVM_MLP_ZS * n) != hipSuccess ||
+        hipMalloc(&m->H, sizeof(float) * SVM_MLP_ZS * n) != hipSuccess ||
+        hipMalloc(&m->D, sizeof(float) * SVM_MLP_DS * n) != hipSuccess ||
+        hipMalloc(&m->partial, sizeof(float) * SVM_MLP_NP * chunks) != hipSuccess ||
+        hipMalloc(&m->lossp, sizeof(float) * blocks) != hipSuccess || hipMalloc(&m->idx, sizeof(int) * n) != hipSuccess) {
+        m->cap = 0;
+        return SVM_ERR_HIP;
+    }
+    m->cap = n;
+    return SVM_OK;
+}
+
+// forward on device rows X[0..n) -> Y; with training buffers when `train`
+int mlp_forward_dev(svm_mlp* m, const float* X, int n, float* Y, bool train) {
+    const int blocks = std::max(1, std::min(4096, (n + 63) / 64));
+    hipLaunchKernelGGL(svm::k_mlp_forward, dim3(blocks), dim3(256), 0, m->stream, mlp_params(m), X, n, Y,
+                       train ? m->Z : nullptr, train ? m->H : nullptr);
+    return hipGetLastError() == hipSuccess ? SVM_OK : SVM_ERR_HIP;
+}
+
+// gradient of (1/n) sum |Y - T|^2 over device rows X, T [0..n) into m->grad;
+// the per-block loss partials stay in m->lossp
+int mlp_gradient_dev(svm_mlp* m, const float* X, const float* T, int n) {
+    int r = mlp_forward_dev(m, X, n, m->Y, true);
+    if (r) return r;
+    const int blocks = (n + 255) / 256, chunks = (n + 63) / 64;
+    hipLaunchKernelGGL(svm::k_mlp_backward, dim3(blocks), dim3(256), 0, m->stream, mlp_params(m), (const float*)m->Z,
+                       (const float*)m->Y, T, n, 1.0f / n, m->D, m->lossp);
+    hipLaunchKernelGGL(svm::k_mlp_wgrad, dim3(chunks), dim3(256), 0, m->stream, mlp_params(m), X, (const float*)m->H,
+                       (const float*)m->D, n, m->partial);
+    hipLaunchKernelGGL(svm::k_mlp_reduce, dim3((SVM_MLP_NP + 255) / 256), dim3(256), 0, m->stream,
+                       (const float*)m->partial, chunks, m->grad);
+    return hipGetLastError() == hipSuccess ? SVM_OK : SVM_ERR_HIP;
+}
+
+double mlp_loss_host(svm_mlp* m, int n) {
+    const int blocks = (n + 255) / 256;
+    std::vector<float> lp(blocks);
+    if (hipMemcpyAsync(lp.data(), m->lossp, sizeof(float) * blocks, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+        hipStreamSynchronize(m->stream) != hipSuccess)
+        return -1.0;
+    double s = 0.0;
+    for (float x : lp) s += x;
+    return s / n;
+}
+
+}  // namespace
+
+extern "C" {
+
+int svm_mlp_create(svm_mlp** out) {
+    if (!out) return SVM_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SVM_ERR_NO_DEVICE;
+    svm_mlp* m = new svm_mlp();
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&m->theta, sizeof(float) * SVM_MLP_NP) != hipSuccess ||
+        hipMalloc(&m->wb, sizeof(unsigned short) * 10240) != hipSuccess ||
+        hipMalloc(&m->musig, sizeof(float) * 32) != hipSuccess ||
+        hipMalloc(&m->mom, sizeof(float) * SVM_MLP_NP) != hipSuccess ||
+        hipMalloc(&m->vel, sizeof(float) * SVM_MLP_NP) != hipSuccess ||
+        hipMalloc(&m->grad, sizeof(float) * SVM_MLP_NP) != hipSuccess) {
+        svm_mlp_destroy(m);
+        return SVM_ERR_HIP;
+    }
+    std::vector<float> ms(32, 0.0f);
+    for (int k = 16; k < 32; ++k) ms[k] = 1.0f;
+    if (hipMemsetAsync(m->theta, 0, sizeof(float) * SVM_MLP_NP, m->stream) != hipSuccess ||
+        hipMemsetAsync(m->mom, 0, sizeof(float) * SVM_MLP_NP, m->stream) != hipSuccess ||
+        hipMemsetAsync(m->vel, 0, sizeof(float) * SVM_MLP_NP, m->stream) != hipSuccess ||
+        hipMemcpyAsync(m->musig, ms.data(), sizeof(float) * 32, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+        mlp_refresh_bf16(m) != SVM_OK || hipStreamSynchronize(m->stream) != hipSuccess) {
+        svm_mlp_destroy(m);
+        return SVM_ERR_HIP;
+    }
+    *out = m;
+    return SVM_OK;
+}
+
+int svm_mlp_destroy(svm_mlp* m) {
+    if (!m) return SVM_ERR_ARG;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    void* ptrs[] = {m->theta, m->wb, m->musig, m->mom, m->vel, m->grad, m->X, m->T, m->Y, m->Z, m->H, m->D,
+                    m->partial, m->lossp, m->idx};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return SVM_OK;
+}
+
+int svm_mlp_set_parameters(svm_mlp* m, const float* theta, const float* mu, const float* sigma, float out_scale) {
+    if (!m || !theta || !mu || !sigma) return SVM_ERR_ARG;
+    std::vector<float> th(theta, theta + SVM_MLP_NP), ms(32);
+    // structural zeros: inputs 12..15 of layer 1, outputs 2..15 of layer 4
+    for (int o = 0; o < 64; ++o)
+        for (int k = SVM_MLP_IN; k < 16; ++k) th[SVM_MLP_W1 + o * 16 + k] = 0.0f;
+    for (int o = 2; o < 16; ++o) {
+        for (int k = 0; k < 64; ++k) th[SVM_MLP_W4 + o * 64 + k] = 0.0f;
+        th[SVM_MLP_B4 + o] = 0.0f;
+    }
+    for (int k = 0; k < 16; ++k) {
+        ms[k] = k < SVM_MLP_IN ? mu[k] : 0.0f;
+        ms[16 + k] = k < SVM_MLP_IN ? sigma[k] : 1.0f;
+        if (!(ms[16 + k] > 0.0f)) return SVM_ERR_ARG;
+    }
+    m->out_scale = out_scale;
+    m->adam_t = 0;
+    if (hipMemcpyAsync(m->theta, th.data(), sizeof(float) * SVM_MLP_NP, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+        hipMemcpyAsync(m->musig, ms.data(), sizeof(float) * 32, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+        hipMemsetAsync(m->mom, 0, sizeof(float) * SVM_MLP_NP, m->stream) != hipSuccess ||
+        hipMemsetAsync(m->vel, 0, sizeof(float) * SVM_MLP_NP, m->stream) != hipSuccess || mlp_refresh_bf16(m) ||
+        hipStreamSynchronize(m->stream) != hipSuccess)
+        return SVM_ERR_HIP;
+    return SVM_OK;
+}
+
+int svm_mlp_get_parameters(svm_mlp* m, float* theta, float* mu, float* sigma, float* out_scale) {
+    if (!m || !theta) return SVM_ERR_ARG;
+    std::vector<float> ms(32);
+    if (hipMemcpyAsync(theta, m->theta, sizeof(float) * SVM_MLP_NP, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+        hipMemcpyAsync(ms.data(), m->musig, sizeof(float) * 32, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+        hipStreamSynchronize(m->stream) != hipSuccess)
+        return SVM_ERR_HIP;
+    for (int k = 0; k < SVM_MLP_IN; ++k) {
+        if (mu) mu[k] = ms[k];
+        if (sigma) sigma[k] = ms[16 + k];
+    }
+    if (out_scale) *out_scale = m->out_scale;
+    return SVM_OK;
+}
+
+int svm_mlp_forward(svm_mlp* m, const float* X, int32_t n, float* Y) {
+    if (!m || !X || !Y || n < 1) return SVM_ERR_ARG;
+    if (mlp_reserve(m, n)) return SVM_ERR_HIP;
+    if (hipMemcpyAsync(m->X, X, sizeof(float) * 16 * n, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+        mlp_forward_dev(m, m->X, n, m->Y, false) ||
+        hipMemcpyAsync(Y, m->Y, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+        hipStreamSynchronize(m->stream) != hipSuccess)
+        return SVM_ERR_HIP;
+    return SVM_OK;
+}
+
+int svm_mlp_gradient(svm_mlp* m, const float* X, const float* T, int32_t n, float* grad, double* loss) {
+    if (!m || !X || !T || !grad || n < 1) return SVM_ERR_ARG;
+    if (mlp_reserve(m, n)) return SVM_ERR_HIP;
+    if (hipMemcpyAsync(m->X, X, sizeof(float) * 16 * n, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+        hipMemcpyAsync(m->T, T, sizeof(float) * 2 * n, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+        mlp_gradient_dev(m, m->X, m->T, n) ||
+        hipMemcpyAsync(grad, m->grad, sizeof(float) * SVM_MLP_NP, hipMemcpyDeviceToHost, m->stream) != hipSuccess)
+        return SVM_ERR_HIP;
+    const double l = mlp_loss_host(m, n);
+    if (l < 0) return SVM_ERR_HIP;
+    if (loss) *loss = l;
+    return SVM_OK;
+}
+
+int svm_mlp_train(svm_mlp* m, const float* X, const float* T, int32_t n, int32_t epochs, int32_t batch, float lr,
+                  uint64_t seed, double* losses) {
+    if (!m || !X || !T || n < 1 || epochs < 1 || batch < 1 || !(lr > 0.0f)) return SVM_ERR_ARG;
+    batch = std::min(batch, n);
+    // the whole set plus one gathered batch
+    float *Xall = nullptr, *Tall = nullptr;
+    if (mlp_reserve(m, batch)) return SVM_ERR_HIP;
+    if (hipMalloc(&Xall, sizeof(float) * 16 * (size_t)n) != hipSuccess ||
+        hipMalloc(&Tall, sizeof(float) * 2 * (size_t)n) != hipSuccess) {
+        if (Xall) (void)hipFree(Xall);
+        return SVM_ERR_HIP;
+    }
+    int r = SVM_OK;
+    if (hipMemcpyAsync(Xall, X, sizeof(float) * 16 * (size_t)n, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+        hipMemcpyAsync(Tall, T, sizeof(float) * 2 * (size_t)n, hipMemcpyHostToDevice, m->stream) != hipSuccess)
+        r = SVM_ERR_HIP;
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    uint64_t state = seed * 0x9e3779b97f4a7c15ull + 1;
+    auto next = [&]() {   // splitmix64
+        uint64_t z = (state += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        return z ^ (z >> 31);
+    };
+    const float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+    for (int ep = 0; ep < epochs && !r; ++ep) {
+        for (int i = n - 1; i > 0; --i) std::swap(perm[i], perm[(int)(next() % (uint64_t)(i + 1))]);
+        double lsum = 0.0;
+        for (int b0 = 0; b0 < n && !r; b0 += batch) {
+            const int nb = std::min(batch, n - b0);
+            const int tb = 256;
+            if (hipMemcpyAsync(m->idx, perm.data() + b0, sizeof(int) * nb, hipMemcpyHostToDevice, m->stream) !=
+                hipSuccess) {
+                r = SVM_ERR_HIP;
+                break;
+            }
+            hipLaunchKernelGGL(k_gather_rows, dim3((nb * 16 + tb - 1) / tb), dim3(tb), 0, m->stream,
+                               (const float*)Xall, (const int*)m->idx, nb, 16, m->X);
+            hipLaunchKernelGGL(k_gather_rows, dim3((nb * 2 + tb - 1) / tb), dim3(tb), 0, m->stream,
+                               (const float*)Tall, (const int*)m->idx, nb, 2, m->T);
+            r = mlp_gradient_dev(m, m->X, m->T, nb);
+            if (r) break;
+            m->adam_t += 1;
+            const float c1 = 1.0f - std::pow(beta1, (float)m->adam_t), c2 = 1.0f - std::pow(beta2, (float)m->adam_t);
+            hipLaunchKernelGGL(svm::k_mlp_adam, dim3((SVM_MLP_NP + 255) / 256), dim3(256), 0, m->stream, m->theta,
+                               (const float*)m->grad, m->mom, m->vel, lr, beta1, beta2, eps, c1, c2);
+            r = mlp_refresh_bf16(m);
+            if (r) break;
+            const double l = mlp_loss_host(m, nb);
+            if (l < 0) { r = SVM_ERR_HIP; break; }
+            lsum += l * nb;
+        }
+        if (losses) losses[ep] = lsum / n;
+    }
+    (void)hipStreamSynchronize(m->stream);
+    (void)hipFree(Xall);
+    (void)hipFree(Tall);
+    return r;
+}
+
+int svm_mlp_time_forward(svm_mlp* m, int32_t n, int32_t reps, double* ms_per_launch) {
+    if (!m || n < 16 || reps < 1 || !ms_per_launch) return SVM_ERR_ARG;
+    if (mlp_reserve(m, n)) return SVM_ERR_HIP;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int r = SVM_OK;
+    hipLaunchKernelGGL(k_fill_hash, dim3((unsigned)((16 * (size_t)n + 255) / 256)), dim3(256), 0, m->stream, m->X,
+                       16 * (size_t)n);
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) r = SVM_ERR_HIP;
+    if (!r) {
+        r = mlp_forward_dev(m, m->X, n, m->Y, false);   // warm
+        (void)hipEventRecord(e0, m->stream);
+        for (int i = 0; i < reps && !r; ++i) r = mlp_forward_dev(m, m->X, n, m->Y, false);
+        (void)hipEventRecord(e1, m->stream);
+        float t = 0.f;
+        if (!r && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess)) r = SVM_ERR_HIP;
+        *ms_per_launch = (double)t / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return r;
+}
+
+int svm_cell_descriptors(svm_context* ctx, int32_t sys, float* X, float* frames) {
+    if (!valid_sys(ctx, sys) || !X) return fail(ctx, SVM_ERR_ARG, "null argument");
+    int r = require_topology(ctx);
+    if (r) return r;
+    r = svm_compute_forces(ctx, 0);
+    if (r) return r;
+    float *dX = nullptr, *dF = nullptr;
+    if (hipMalloc(&dX, sizeof(float) * 16 * (size_t)ctx->Cs) != hipSuccess ||
+        hipMalloc(&dF, sizeof(float) * 6 * (size_t)ctx->Cs) != hipSuccess) {
+        if (dX) (void)hipFree(dX);
+        return fail(ctx, SVM_ERR_HIP, "device allocation failed");
+    }
+    const SysParams& L = ctx->layout[sys];
+    hipLaunchKernelGGL(svm::k_cell_descriptors, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, dX,
+                       dF);
+    r = last_launch(ctx);
+    if (!r && (hipMemcpyAsync(X, dX + 16 * (size_t)L.co, sizeof(float) * 16 * L.nc, hipMemcpyDeviceToHost,
+                              ctx->stream) != hipSuccess ||
+               (frames && hipMemcpyAsync(frames, dF + 6 * (size_t)L.co, sizeof(float) * 6 * L.nc,
+                                         hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) ||
+               hipStreamSynchronize(ctx->stream) != hipSuccess))
+        r = fail(ctx, SVM_ERR_HIP, "descriptor copy failed");
+    (void)hipFree(dX);
+    (void)hipFree(dF);
+    return r;
+}
+
+int svm_warm_start(svm_context* ctx, svm_mlp* m, double scale) {
+    if (!ctx || !m) return SVM_ERR_ARG;
+    int r = require_topology(ctx);
+    if (r) return r;
+    r = svm_compute_forces(ctx, 0);
+    if (r) return r;
+    float *dX = nullptr, *dF = nullptr, *dY = nullptr;
+    if (hipMalloc(&dX, sizeof(float) * 16 * (size_t)ctx->Cs) != hipSuccess ||
+        hipMalloc(&dF, sizeof(float) * 6 * (size_t)ctx->Cs) != hipSuccess ||
+        hipMalloc(&dY, sizeof(float) * 2 * (size_t)ctx->Cs) != hipSuccess) {
+        if (dX) (void)hipFree(dX);
+        if (dF) (void)hipFree(dF);
+        return fail(ctx, SVM_ERR_HIP, "device allocation failed");
+    }
+    // the network runs on the context's stream
+    hipStream_t keep = m->stream;
+    m->stream = ctx->stream;
+    hipLaunchKernelGGL(svm::k_cell_descriptors, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, dX,
+                       dF);
+    r = last_launch(ctx);
+    if (!r) r = mlp_forward_dev(m, dX, ctx->Cs, dY, false);
+    m->stream = keep;
+    if (!r) {
+        hipLaunchKernelGGL(svm::k_warm_disp, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d,
+                           (const float*)dY, (const float*)dF, scale);
+        const int pout = 1 - ctx->pcur;
+        hipLaunchKernelGGL(svm::k_move_ext, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, pout, -1,
+                           1.0);
+        r = last_launch(ctx);
+        if (!r) r = launch_t1(ctx, pout, ctx->ncur, ctx->ncur, 0, 0, 0);
+        if (!r) ctx->pcur = pout;
+    }
+    if (!r) r = svm_synchronize(ctx);
+    (void)hipFree(dX);
+    (void)hipFree(dF);
+    (void)hipFree(dY);
+    return r;
 }
 
 }  // extern "C"

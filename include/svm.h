@@ -246,6 +246,45 @@ int svm_get_t1_diagnostics(svm_context* ctx, int64_t* reasons);
 int svm_set_profiling(svm_context* ctx, int32_t on);
 int svm_get_timing(svm_context* ctx, const char** names, double* ms, int32_t n, int32_t* n_written);
 
+/* ---- learned warm start for FIRE ----------------------------------------
+ * No counterpart in the reference: a per-cell MLP (12 shape descriptors ->
+ * 3 x 64 SiLU -> 2-D tangent displacement of the cell), bf16 weights on MFMA
+ * with FP32 accumulation, trained on the GPU (FP32 backward, Adam) on FIRE
+ * trajectories.  svm_warm_start moves every vertex by the mean of its three
+ * cells' predicted displacements before FIRE (energyMinimizerFIRE, unchanged,
+ * remains the fallback and does the convergence).  Parameter vector layout
+ * (SVM_MLP_NP floats): W1 [64][16] (inputs 12..15 zero), b1 [64],
+ * W2 [64][64], b2 [64], W3 [64][64], b3 [64], W4 [16][64] (rows 2..15 zero),
+ * b4 [16] (entries 2..15 zero); a layer computes z = W a + b. */
+#define SVM_MLP_NPARAM 10448
+#define SVM_MLP_NDESC 12
+typedef struct svm_mlp svm_mlp;
+int svm_mlp_create(svm_mlp** out);
+int svm_mlp_destroy(svm_mlp* mlp);
+/* theta: float[SVM_MLP_NPARAM]; mu, sigma: float[12] input normalisation
+ * x' = (x - mu) / sigma; the output is out_scale * (W4 a3 + b4).  Resets Adam. */
+int svm_mlp_set_parameters(svm_mlp* mlp, const float* theta, const float* mu, const float* sigma, float out_scale);
+int svm_mlp_get_parameters(svm_mlp* mlp, float* theta, float* mu, float* sigma, float* out_scale);
+/* X: float[n][16] raw descriptors (12 used) -> Y: float[n][2]. */
+int svm_mlp_forward(svm_mlp* mlp, const float* X, int32_t n, float* Y);
+/* Gradient of L = (1/n) sum_i |Y_i - T_i|^2 with respect to theta (as used
+ * in the bf16 forward pass), and L. */
+int svm_mlp_gradient(svm_mlp* mlp, const float* X, const float* T, int32_t n, float* grad, double* loss);
+/* `epochs` passes of Adam (lr, beta 0.9 / 0.999) over minibatches of `batch`
+ * rows in an order shuffled per epoch from `seed`; losses[epoch] (optional)
+ * is the mean minibatch loss of that epoch. */
+int svm_mlp_train(svm_mlp* mlp, const float* X, const float* T, int32_t n, int32_t epochs, int32_t batch, float lr,
+                  uint64_t seed, double* losses);
+/* Mean time of the forward kernel over n synthetic rows (device resident). */
+int svm_mlp_time_forward(svm_mlp* mlp, int32_t n, int32_t reps, double* ms_per_launch);
+/* The 12 descriptors of every cell of sphere `sys` (float[N][16]) after a
+ * force evaluation, and the cells' tangent frames (float[N][6]: e1, e2) in
+ * which the network's displacements are expressed. */
+int svm_cell_descriptors(svm_context* ctx, int32_t sys, float* X, float* frames);
+/* Move every vertex of every sphere by scale x (mean of its three cells'
+ * predicted displacements), then enforce the topology (T1). */
+int svm_warm_start(svm_context* ctx, svm_mlp* mlp, double scale);
+
 /* Kernel timing without launch gaps (no counterpart in the reference): the
  * fused vertex pass of `updater` (0 Brownian, 1 active; force + update +
  * move, k_vertex) launched `reps` times back to back between two HIP events
