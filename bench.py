@@ -24,7 +24,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = "cell-steps/s (batched spheres)"
+METRIC = "cell-steps/s (batched spheres)"   # the FIRE learned-vs-plain part is the "fire" object
 PEAK_HBM_GBS = 8000.0
 # Algorithmic HBM bytes per cell per step of the vertex pass (k_vertex_active):
 # two vertices per cell, each reading its position (24 B) and director (24 B),
@@ -189,6 +189,43 @@ def copy_probe(svm, n_bytes=1 << 28):
           f"{2 * n_bytes / ms / 1e6:.0f} GB/s", flush=True)
 
 
+def fire_comparison(svm, n_sys=1024, cap=10000):
+    """Batched FIRE from seeded uniform random starts (1024 spheres, N
+    log-uniform in [500, 5000], p0 uniform in [3.6, 3.9]) to rms force
+    < 1e-10, plain and after the learned warm start (weights trained on
+    held-out seeds, curvedspacevertexmodel_amd/warmstart_weights.npz).  FIRE
+    parameters of the reference's minimisation driver; iterations capped at
+    `cap` per sphere (spheres with p0 above ~3.81 are floppy and keep flipping
+    without reaching 1e-10)."""
+    from curvedspacevertexmodel_amd import warmstart as ws
+    params = ws.batch_params(n_sys)
+    net, scale = ws.load()
+    res = {}
+    energies = {}
+    for mode in ("plain", "learned"):
+        ctx = ws.fire_batch(params, seed0=1137, cutoff=1e-10, max_iterations=cap)
+        wall, its, conv, fmax = ws.run_fire(ctx, n_sys, net if mode == "learned" else None, scale)
+        ctx.compute_geometry()
+        energies[mode] = (np.array(ctx.energies()[0]), conv)
+        res[mode] = {"wall_s": wall, "converged_fraction": float(conv.mean()),
+                     "iterations_mean_converged": float(its[conv].mean()) if conv.any() else None,
+                     "iterations_median_converged": float(np.median(its[conv])) if conv.any() else None,
+                     "iterations_total": int(its.sum())}
+        print(f"FIRE {mode}: {wall:.2f} s, converged {conv.mean():.3f}", flush=True)
+    both = energies["plain"][1] & energies["learned"][1]
+    ep, el = energies["plain"][0][both], energies["learned"][0][both]
+    rel = np.abs(el - ep) / np.maximum(np.abs(ep), 1e-300)
+    total = int(sum(n for n, _ in params))
+    ms = net.time_forward(total, reps=10)
+    return {"workload": "batched-fire-1024", "spheres": n_sys, "total_cells": total, "N_range": [500, 5000],
+            "p0_range": [3.6, 3.9], "stop": "rms force < 1e-10", "iteration_cap": cap, **res,
+            "speedup_wall": res["plain"]["wall_s"] / res["learned"]["wall_s"],
+            "final_energy_rel_diff_median": float(np.median(rel)) if both.any() else None,
+            "final_energy_within_1e-9": float(np.mean(rel <= 1e-9)) if both.any() else None,
+            "mlp_forward": {"cells": total, "ms": ms, "tflops": 18176.0 * total / (ms * 1e-3) / 1e12,
+                            "peak_tflops": 2500.0, "frac": 18176.0 * total / (ms * 1e-3) / 1e12 / 2500.0}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +234,9 @@ def main():
     ap.add_argument("--spheres", type=int, default=256)
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fire", action="store_true", help="skip the learned-vs-plain FIRE comparison")
+    ap.add_argument("--fire-spheres", type=int, default=1024)
+    ap.add_argument("--fire-cap", type=int, default=10000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, 3 launches of a plain 256 MiB -> 256 MiB FP64 copy kernel "
                          "(calibrates the HBM byte counters of a profiler)")
@@ -260,6 +300,9 @@ def main():
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ctx, params)
+    del ctx
+    if not args.no_fire:
+        result["fire"] = fire_comparison(svm, args.fire_spheres, args.fire_cap)
     print(json.dumps(result), flush=True)
     if args.copy_probe:
         copy_probe(svm)

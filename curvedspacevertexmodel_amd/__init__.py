@@ -36,7 +36,7 @@ EXPORTS = [
     "svm_set_force_precision", "svm_get_force_error", "svm_delaunay_triangles",
     "svm_cell_list", "svm_mlp_create", "svm_mlp_destroy", "svm_mlp_set_parameters", "svm_mlp_get_parameters",
     "svm_mlp_forward", "svm_mlp_gradient", "svm_mlp_train", "svm_mlp_time_forward", "svm_cell_descriptors",
-    "svm_warm_start",
+    "svm_warm_start", "svm_fire_set_max_iterations",
 ]
 
 
@@ -401,6 +401,10 @@ class Context:
         p = FireParams(dt, alpha_start, dt_max, dt_inc, dt_dec, alpha_dec, n_min, max_iterations, force_cutoff,
                        alpha_min, stop_rms, 0)
         self._ck(lib().svm_fire_set_parameters(self.h, C.c_int32(sys), C.byref(p)), "svm_fire_set_parameters")
+
+    def fire_set_max_iterations(self, sys, max_iterations):
+        self._ck(lib().svm_fire_set_max_iterations(self.h, C.c_int32(sys), C.c_int32(max_iterations)),
+                 "svm_fire_set_max_iterations")
 
     def fire_minimize(self, check_every=50):
         self._ck(lib().svm_fire_minimize(self.h, C.c_int32(check_every)), "svm_fire_minimize")

@@ -415,8 +415,11 @@ int dl_triangulate(hipStream_t st, DevScratch& m, const double* pts, int n, int 
     hipLaunchKernelGGL(svm::k_fill_int, blocks(n), dim3(tb), 0, st, status, n, (int)svm::DL_PENDING);
     // stars, widening the search for the points that need it
     std::vector<int> hs(n);
-    for (int pass = 0; pass < 3; ++pass) {
-        const double r = r0 * (1 + pass);
+    // search radii r0, 2 r0, 3 r0 (about 40, 150 and 350 candidates at uniform
+    // density), then the whole sphere when it holds few enough points
+    for (int pass = 0; pass < 4; ++pass) {
+        if (pass == 3 && n - 1 > SVM_DL_MAXC) { msg = "neighbour search did not converge"; return SVM_ERR_TOPOLOGY; }
+        const double r = pass < 3 ? r0 * (1 + pass) : 2.0 * Rs * (1.0 + 1e-9);
         hipLaunchKernelGGL(svm::k_dl_star, dim3((n + SVM_DL_WAVES - 1) / SVM_DL_WAVES), dim3(64 * SVM_DL_WAVES), 0, st,
                            (const double*)o.P, n, g, (const int*)start, (const int*)list, r, Rs, cap, o.nbr, o.deg,
                            status);
@@ -426,20 +429,22 @@ int dl_triangulate(hipStream_t st, DevScratch& m, const double* pts, int n, int 
             msg = "star kernel failed";
             return SVM_ERR_HIP;
         }
-        int retry = 0, worst = 0;
+        int retry = 0, worst = 0, where = -1;
         for (int i = 0; i < n; ++i) {
             if (hs[i] == svm::DL_RETRY) ++retry;
-            else if (hs[i] != svm::DL_OK) worst = hs[i];
+            else if (hs[i] != svm::DL_OK && where < 0) { worst = hs[i]; where = i; }
         }
         if (worst) {
             msg = worst == svm::DL_DEGENERATE ? "degenerate input: four co-circular points"
                   : worst == svm::DL_OPEN     ? "a cell has more Delaunay neighbours than ring_capacity"
                   : worst == svm::DL_CROWDED  ? "points too crowded for the neighbour search"
                                               : "inconsistent triangulation";
+            msg += " (point " + std::to_string(where) + ", status " + std::to_string(worst) + ", pass " +
+                   std::to_string(pass) + ")";
             return worst == svm::DL_OPEN ? SVM_ERR_RING_OVERFLOW : SVM_ERR_TOPOLOGY;
         }
         if (!retry) break;
-        if (pass == 2) { msg = "neighbour search did not converge"; return SVM_ERR_TOPOLOGY; }
+        if (pass == 3) { msg = "neighbour search did not converge"; return SVM_ERR_TOPOLOGY; }
     }
     // canonical triangles
     hipLaunchKernelGGL(svm::k_dl_tri_count, blocks(n), dim3(tb), 0, st, n, cap, (const int*)o.nbr,
@@ -1301,6 +1306,11 @@ int svm_fire_set_parameters(svm_context* ctx, int32_t sys, const svm_fire_params
     HIPCHECK(ctx, hipMemcpyAsync(ctx->d.sys + sys, &S, sizeof(SysParams), hipMemcpyHostToDevice, ctx->stream));
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     return SVM_OK;
+}
+
+int svm_fire_set_max_iterations(svm_context* ctx, int32_t sys, int32_t max_iterations) {
+    if (!valid_sys(ctx, sys) || max_iterations < 0) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    return set_field(ctx, sys, offsetof(SysParams, f_max_iter), (int32_t)max_iterations);
 }
 
 int svm_fire_minimize(svm_context* ctx, int32_t check_every) {

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES) void k_dl_star(const double* P, 
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const int nc = s_n[w];
     if (nc > SVM_DL_MAXC || nc < 3) {
-        if (lane == 0) status[i] = nc < 3 ? DL_RETRY : DL_CROWDED;
+        if (lane == 0) status[i] = nc < 3 ? (r < 2.0 * R ? DL_RETRY : DL_INVALID) : DL_CROWDED;
         return;
     }
     const int* cand = s_idx[w];
@@ -270,11 +270,18 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES) void k_dl_star(const double* P, 
     }
     unsigned long long any_bad = __ballot(bad), any_far = __ballot(far), any_deg = __ballot(degenerate);
     if (lane == 0) {
+        // A star is proven only when every facet passes the exact checks and
+        // its circumcircle lies inside the search ball.  Any failure while
+        // the search ball does not cover the sphere may come from a missing
+        // candidate: search wider.
+        const bool partial = r < 2.0 * R;
         int st = status_out;
         if (st == DL_OK) {
-            if (any_deg) st = DL_DEGENERATE;
-            else if (any_bad) st = DL_INVALID;
+            if (any_bad) st = partial ? DL_RETRY : (any_deg ? DL_DEGENERATE : DL_INVALID);
             else if (any_far) st = DL_RETRY;
+            else if (any_deg) st = DL_DEGENERATE;
+        } else if (partial) {
+            st = DL_RETRY;
         }
         status[i] = st;
         deg[i] = d;

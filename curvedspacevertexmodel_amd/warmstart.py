@@ -1,0 +1,150 @@
+"""Learned warm start for FIRE: data, training and evaluation drivers.
+
+The network and its kernels live in the library (svm_mlp_*, svm_warm_start);
+this module drives them:
+
+  * ``fire_batch`` builds a batch of spheres from seeded uniform random cell
+    points (GPU Delaunay), with the FIRE parameters of the reference's
+    minimisation driver (minimization.cpp:111-114: dt 0.001, alpha 0.99,
+    dt_max 50 dt, dt_inc 1.1, dt_dec 0.95, alpha_dec 0.9, n_min 4,
+    alpha_min 0) and the rms-force stop test.
+  * ``training_set`` minimises held-out spheres with plain FIRE and records,
+    per cell, the descriptors of the start and the displacement of the cell
+    centre to the minimum, in the cell's tangent frame.
+  * ``train`` fits the network on the GPU (Adam, svm_mlp_train).
+  * ``run_fire`` minimises a batch, optionally after the warm start, and
+    reports wall time and per-sphere iterations.
+
+    python -m curvedspacevertexmodel_amd.warmstart train [--out FILE]
+"""
+import argparse
+import os
+import time
+
+import numpy as np
+
+import curvedspacevertexmodel_amd as svm
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(HERE, "warmstart_weights.npz")
+
+FIRE_DT = 0.001
+FIRE = dict(dt=FIRE_DT, alpha_start=0.99, dt_max=50 * FIRE_DT, dt_inc=1.1, dt_dec=0.95, alpha_dec=0.9, n_min=4,
+            alpha_min=0.0)
+TRAIN_SEED0 = 900001    # training spheres use seeds disjoint from the evaluation batch (1137 + i)
+
+
+def batch_params(n_sys, seed=13377, nmin=500, nmax=5000, p0_range=(3.6, 3.9)):
+    """Per-sphere (N, p0): N log-uniform in [nmin, nmax], p0 uniform, one Philox generator."""
+    g = np.random.Generator(np.random.Philox(seed))
+    out = []
+    for _ in range(n_sys):
+        u = g.random(2)
+        n = int(round(np.exp(np.log(nmin) + u[0] * (np.log(nmax) - np.log(nmin)))))
+        out.append((n, p0_range[0] + (p0_range[1] - p0_range[0]) * u[1]))
+    return out
+
+
+def fire_batch(params, seed0=1137, cutoff=1e-10, max_iterations=20000):
+    systems = [dict(n_cells=n, a0=1.0, p0=p0, kr=1.0, t1_threshold=0.1, seed=seed0 + i)
+               for i, (n, p0) in enumerate(params)]
+    ctx = svm.Context(systems)
+    for i, (n, p0) in enumerate(params):
+        pts = svm.generate_cell_points(n, seed0 + i)
+        ctx.build_from_cell_points(i, pts[svm.spatial_order(pts)])
+        ctx.fire_set_parameters(i, max_iterations=max_iterations, force_cutoff=cutoff, stop_rms=1, **FIRE)
+    return ctx
+
+
+def run_fire(ctx, n_sys, net=None, scale=1.0, check_every=100):
+    """Minimise every sphere; returns wall seconds (warm start included) and
+    per-sphere (iterations, converged, rms force)."""
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    if net is not None:
+        ctx.warm_start(net, scale)
+    ctx.fire_minimize(check_every)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    st = [ctx.fire_state(i) for i in range(n_sys)]
+    its = np.array([s["iterations"] for s in st])
+    conv = np.array([bool(s["converged"]) for s in st])
+    fmax = np.array([s["force_max"] for s in st])
+    return wall, its, conv, fmax
+
+
+def training_set(n_sys=48, seed0=TRAIN_SEED0, cutoff=1e-8, max_iterations=20000):
+    """Descriptors X [cells][16] at random starts and target displacements
+    T [cells][2] of the cell centres to the FIRE minimum, in the start frames."""
+    params = batch_params(n_sys, seed=seed0)
+    ctx = fire_batch(params, seed0=seed0, cutoff=cutoff, max_iterations=max_iterations)
+    X, T, frames, C0 = [], [], [], []
+    for i in range(n_sys):
+        x, f = ctx.cell_descriptors(i)
+        X.append(x)
+        frames.append(f)
+        C0.append(ctx.cell_positions(i))
+    run_fire(ctx, n_sys)
+    ctx.compute_geometry()
+    for i in range(n_sys):
+        d = ctx.cell_positions(i) - C0[i]
+        f = frames[i].astype(np.float64)
+        T.append(np.stack([np.sum(d * f[:, :3], 1), np.sum(d * f[:, 3:], 1)], 1).astype(np.float32))
+    return np.concatenate(X), np.concatenate(T)
+
+
+def train(X, T, epochs=40, batch=4096, lr=1e-3, seed=1):
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "..", "oracle"))
+    from mlp_oracle import init_theta   # initialiser only
+    mu = X[:, :12].mean(0)
+    sg = X[:, :12].std(0) + 1e-6
+    scale = float(T.std()) + 1e-12
+    net = svm.MLP()
+    net.set_parameters(init_theta(seed), mu, sg, scale)
+    losses = net.train(X, T, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return net, losses
+
+
+def save(net, path=WEIGHTS, **meta):
+    th, mu, sg, sc = net.get_parameters()
+    np.savez(path, theta=th, mu=mu, sigma=sg, out_scale=np.float32(sc), **meta)
+
+
+def load(path=WEIGHTS):
+    z = np.load(path, allow_pickle=False)
+    net = svm.MLP()
+    net.set_parameters(z["theta"], z["mu"], z["sigma"], float(z["out_scale"]))
+    scale = float(z["warm_scale"]) if "warm_scale" in z else 1.0
+    return net, scale
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=["train"])
+    ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--spheres", type=int, default=48)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--validate", type=int, default=32, help="held-out spheres used to pick the step scale")
+    args = ap.parse_args()
+    t0 = time.perf_counter()
+    X, T = training_set(args.spheres)
+    print(f"training set: {len(X)} cells from {args.spheres} spheres, {time.perf_counter() - t0:.1f} s", flush=True)
+    net, losses = train(X, T, epochs=args.epochs)
+    print("loss per epoch:", " ".join(f"{x:.4g}" for x in losses), flush=True)
+    # pick the warm-start step scale on validation spheres (seeds disjoint from both sets)
+    vparams = batch_params(args.validate, seed=TRAIN_SEED0 + 7919)
+    best = None
+    for scale in (0.0, 0.5, 1.0):
+        ctx = fire_batch(vparams, seed0=TRAIN_SEED0 + 50000)
+        wall, its, conv, _ = run_fire(ctx, len(vparams), net if scale > 0 else None, scale)
+        print(f"validation scale {scale}: wall {wall:.2f} s, iterations mean {its.mean():.0f} max {its.max()}, "
+              f"converged {conv.mean():.2f}", flush=True)
+        if scale > 0 and (best is None or wall < best[0]):
+            best = (wall, scale)
+    save(net, args.out, warm_scale=np.float32(best[1]), train_losses=losses, train_cells=np.int64(len(X)))
+    print(f"saved {args.out} (warm-start scale {best[1]})")
+
+
+if __name__ == "__main__":
+    main()

@@ -223,6 +223,10 @@ int svm_fire_set_parameters(svm_context* ctx, int32_t sys, const svm_fire_params
 /* Minimise every sphere until its stop test passes or its iteration limit is
  * reached; the host checks convergence every `check_every` iterations. */
 int svm_fire_minimize(svm_context* ctx, int32_t check_every);
+/* updater::setMaximumIterations (src/updaters/baseUpdater.h:78): the
+ * iteration limit only; the minimiser's state (iterations done, dt, alpha,
+ * velocities) is kept, so a later svm_fire_minimize continues. */
+int svm_fire_set_max_iterations(svm_context* ctx, int32_t sys, int32_t max_iterations);
 int svm_fire_get_state(svm_context* ctx, int32_t sys, svm_fire_state* state);
 
 /* T1 path: 0 (default) resolves each step's flips by per-candidate dry runs,

@@ -93,3 +93,21 @@ def test_build_batch_of_spheres():
         ctx.build_from_cell_points(i, st["points"])
     for i, st in enumerate(states):
         assert_topology_equal(ctx, i, svmtest.OracleModel(st))
+
+
+def test_large_empty_circle_needs_a_wider_search():
+    """Sphere 100 of the batched-FIRE workload (N = 2513, seed 1237, spatially
+    ordered): point 682 has Delaunay neighbours beyond the first search radius
+    (a large empty circle), so its first star is unproven and is redone wider."""
+    pts = svm.generate_cell_points(2513, 1237)
+    pts = pts[svm.spatial_order(pts)]
+    np.testing.assert_array_equal(svm.delaunay_triangles(pts), svmtest.qhull_triangles(pts))
+
+
+def test_many_random_spheres_against_qhull():
+    from curvedspacevertexmodel_amd import warmstart as ws
+    for i, (n, _) in enumerate(ws.batch_params(96, seed=2024)):
+        pts = svm.generate_cell_points(n, 5000 + i)
+        if i % 2:
+            pts = pts[svm.spatial_order(pts)]
+        np.testing.assert_array_equal(svm.delaunay_triangles(pts), svmtest.qhull_triangles(pts), err_msg=f"sphere {i}")
