@@ -235,6 +235,7 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fire", action="store_true", help="skip the learned-vs-plain FIRE comparison")
+    ap.add_argument("--autotune", type=int, default=1, help="1: pick the launch schedule with svm_autotune")
     ap.add_argument("--fire-spheres", type=int, default=1024)
     ap.add_argument("--fire-cap", type=int, default=10000)
     ap.add_argument("--copy-probe", action="store_true",
@@ -260,6 +261,12 @@ def main():
     print(f"stabilised (200 Brownian steps); warmup {warm} active steps", flush=True)
     ctx.run_graph(1, G, warm // G)
     ctx.synchronize()
+    tune_table = None
+    if args.autotune:
+        tune_table = ctx.autotune(updater=1, steps_per_trial=60)
+        G = ctx.tuning()["graph_steps"]
+        steps = max(G, args.steps - args.steps % G)
+        print(f"autotuned: {ctx.tuning()}", flush=True)
     print("warmup done", flush=True)
     base = (ctx.counters(), ctx.t1_diagnostics())
     t0 = time.perf_counter()
@@ -288,7 +295,8 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "batched-active-256", "spheres": len(params), "total_cells": total_cells,
                    "N_range": [200, 20000], "p0_range": [3.5, 4.0], "v0_range": [0.01, 0.5],
-                   "Dr": [0.1, 1, 10], "dt": 0.01, "steps_per_graph": G},
+                   "Dr": [0.1, 1, 10], "dt": 0.01, "steps_per_graph": G, "schedule": ctx.tuning(),
+                   "autotune_table": tune_table.tolist() if tune_table is not None else None},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(dom),
                      "bytes_per_launch": VERTEX_BYTES_PER_CELL * total_cells, "avg_launch_ms": dom_ms,

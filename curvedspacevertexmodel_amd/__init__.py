@@ -36,7 +36,7 @@ EXPORTS = [
     "svm_set_force_precision", "svm_get_force_error", "svm_delaunay_triangles",
     "svm_cell_list", "svm_mlp_create", "svm_mlp_destroy", "svm_mlp_set_parameters", "svm_mlp_get_parameters",
     "svm_mlp_forward", "svm_mlp_gradient", "svm_mlp_train", "svm_mlp_time_forward", "svm_cell_descriptors",
-    "svm_warm_start", "svm_fire_set_max_iterations",
+    "svm_warm_start", "svm_fire_set_max_iterations", "svm_set_tuning", "svm_get_tuning", "svm_autotune",
 ]
 
 
@@ -419,6 +419,23 @@ class Context:
 
     def set_stream(self, stream_ptr):
         self._ck(lib().svm_set_stream(self.h, C.c_void_p(stream_ptr)), "svm_set_stream")
+
+    def set_tuning(self, cluster_blocks=256, graph_steps=10, xcd_mapping=1):
+        self._ck(lib().svm_set_tuning(self.h, C.c_int32(cluster_blocks), C.c_int32(graph_steps),
+                                      C.c_int32(xcd_mapping)), "svm_set_tuning")
+
+    def tuning(self):
+        a, b, c = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+        self._ck(lib().svm_get_tuning(self.h, C.byref(a), C.byref(b), C.byref(c)), "svm_get_tuning")
+        return dict(cluster_blocks=a.value, graph_steps=b.value, xcd_mapping=c.value)
+
+    def autotune(self, updater=1, steps_per_trial=60):
+        """Returns the trial table [(cluster_blocks, graph_steps, xcd_mapping, ms/step)]."""
+        tab = np.zeros((32, 4), np.float64)
+        n = C.c_int32(0)
+        self._ck(lib().svm_autotune(self.h, C.c_int32(updater), C.c_int32(steps_per_trial), _dp(tab),
+                                    C.c_int32(32), C.byref(n)), "svm_autotune")
+        return tab[:n.value]
 
     def cell_descriptors(self, sys):
         """(X float32 [N][16], frames float32 [N][6]) of sphere sys."""

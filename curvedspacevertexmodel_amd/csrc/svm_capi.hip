@@ -41,6 +41,7 @@ struct svm_context {
     std::vector<int> have_topology;
     int Cs = 0, Vs = 0, cap = 0, nct = 0, nvt = 0;
     int cluster_blocks = 256;        // T1 dry-run grid (persistent loop over candidates)
+    int graph_steps = 10;            // steps per captured graph when the caller passes 0
     int t1_sequential = 0;           // 1: skip the dry runs, always use the sequential pass
     Dev d{};
     int pcur = 0, ncur = 0;          // current position / director buffer
@@ -181,6 +182,15 @@ struct Timer {
         }
     }
 };
+
+// Captured graphs hold kernel arguments and launch shapes: drop them when a
+// setting they baked in changes.
+void drop_graphs(svm_context* ctx) {
+    for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto& g : ctx->graph_defs) (void)hipGraphDestroy(g.second);
+    ctx->graphs.clear();
+    ctx->graph_defs.clear();
+}
 
 int last_launch(svm_context* ctx) {
     hipError_t e = hipGetLastError();
@@ -562,6 +572,7 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
     Dev& d = ctx->d;
     d.nsys = n_systems;
     d.Cs = ctx->Cs;
+    d.xcd = 1;
     d.Vs = ctx->Vs;
     d.cap = cap;
     int r = 0;
@@ -678,10 +689,7 @@ int svm_set_stream(svm_context* ctx, void* stream) {
     if (!ctx) return SVM_ERR_ARG;
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
-    for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
-    for (auto& g : ctx->graph_defs) (void)hipGraphDestroy(g.second);
-    ctx->graphs.clear();
-    ctx->graph_defs.clear();
+    drop_graphs(ctx);
     return SVM_OK;
 }
 
@@ -1112,8 +1120,7 @@ int svm_set_force_precision(svm_context* ctx, int32_t precision) {
         return fail(ctx, SVM_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32)");
     if (ctx->force_fp32 != precision) {
         ctx->force_fp32 = precision;
-        for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
-        ctx->graphs.clear();
+        drop_graphs(ctx);
     }
     return SVM_OK;
 }
@@ -1243,6 +1250,7 @@ int svm_brownian_steps(svm_context* ctx, int32_t n_steps) { return run_steps(ctx
 int svm_active_steps(svm_context* ctx, int32_t n_steps) { return run_steps(ctx, 1, n_steps); }
 
 int svm_run_graph(svm_context* ctx, int32_t updater, int32_t steps_per_graph, int32_t n_replays) {
+    if (ctx && steps_per_graph == 0) steps_per_graph = ctx->graph_steps;
     if (!ctx || (updater != 0 && updater != 1) || steps_per_graph <= 0 || (steps_per_graph & 1) || n_replays < 0)
         return fail(ctx, SVM_ERR_ARG, "steps_per_graph must be a positive even number");
     int r = require_topology(ctx);
@@ -1399,10 +1407,7 @@ int svm_get_t1_diagnostics(svm_context* ctx, int64_t* reasons) {
 int svm_set_t1_mode(svm_context* ctx, int32_t mode) {
     if (!ctx || (mode != 0 && mode != 1)) return fail(ctx, SVM_ERR_ARG, "T1 mode must be 0 or 1");
     ctx->t1_sequential = mode;
-    for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
-    for (auto& g : ctx->graph_defs) (void)hipGraphDestroy(g.second);
-    ctx->graphs.clear();
-    ctx->graph_defs.clear();
+    drop_graphs(ctx);
     return SVM_OK;
 }
 
@@ -1461,6 +1466,95 @@ int svm_time_vertex_pass(svm_context* ctx, int32_t updater, int32_t reps, double
         hipStreamSynchronize(ctx->stream) != hipSuccess)
         r = r ? r : SVM_ERR_HIP;
     return r;
+}
+
+int svm_set_tuning(svm_context* ctx, int32_t cluster_blocks, int32_t graph_steps, int32_t xcd_mapping) {
+    if (!ctx || cluster_blocks < 1 || cluster_blocks > 65536 || graph_steps < 2 || (graph_steps & 1) ||
+        (xcd_mapping != 0 && xcd_mapping != 1))
+        return fail(ctx, SVM_ERR_ARG, "bad tuning values");
+    ctx->cluster_blocks = cluster_blocks;
+    ctx->graph_steps = graph_steps;
+    ctx->d.xcd = xcd_mapping;
+    drop_graphs(ctx);
+    return SVM_OK;
+}
+
+int svm_get_tuning(svm_context* ctx, int32_t* cluster_blocks, int32_t* graph_steps, int32_t* xcd_mapping) {
+    if (!ctx) return SVM_ERR_ARG;
+    if (cluster_blocks) *cluster_blocks = ctx->cluster_blocks;
+    if (graph_steps) *graph_steps = ctx->graph_steps;
+    if (xcd_mapping) *xcd_mapping = ctx->d.xcd;
+    return SVM_OK;
+}
+
+int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, double* table, int32_t table_rows,
+                 int32_t* rows_written) {
+    if (!ctx || (updater != 0 && updater != 1) || steps_per_trial < 6) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    int r = require_topology(ctx);
+    if (r) return r;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return fail(ctx, SVM_ERR_HIP, "event creation failed");
+    int nrows = 0;
+    // ms per step of `steps` steps under the current settings: median of three
+    // samples (kernelTuner's median-of-samples, src/utilities/kernelTuner.cpp:35-126)
+    auto trial = [&](int G, double* ms) -> int {
+        const int reps = std::max(1, steps_per_trial / (3 * G));
+        double samples[3];
+        int rr = svm_run_graph(ctx, updater, G, 1);   // capture + one warm replay
+        if (rr) return rr;
+        for (int k = 0; k < 3; ++k) {
+            (void)hipEventRecord(e0, ctx->stream);
+            rr = svm_run_graph(ctx, updater, G, reps);
+            if (rr) return rr;
+            (void)hipEventRecord(e1, ctx->stream);
+            float t = 0.f;
+            if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+                return SVM_ERR_HIP;
+            samples[k] = (double)t / (reps * G);
+        }
+        std::sort(samples, samples + 3);
+        *ms = samples[1];
+        return SVM_OK;
+    };
+    auto record = [&](int cb, int G, int xcd, double ms) {
+        if (table && nrows < table_rows) {
+            table[4 * nrows] = cb;
+            table[4 * nrows + 1] = G;
+            table[4 * nrows + 2] = xcd;
+            table[4 * nrows + 3] = ms;
+        }
+        ++nrows;
+    };
+    int best_cb = ctx->cluster_blocks, best_G = ctx->graph_steps, best_x = ctx->d.xcd;
+    double best = 1e300;
+    // kernel schedules at the default graph length
+    const int cbs[] = {128, 256, 512, 1024};
+    for (int xcd = 0; xcd <= 1 && !r; ++xcd)
+        for (int cb : cbs) {
+            r = svm_set_tuning(ctx, cb, 10, xcd);
+            double ms = 0.0;
+            if (!r) r = trial(10, &ms);
+            if (r) break;
+            record(cb, 10, xcd, ms);
+            if (ms < best) { best = ms; best_cb = cb; best_x = xcd; best_G = 10; }
+        }
+    // then the graph length
+    const int Gs[] = {4, 20, 40};
+    for (int G : Gs) {
+        if (r) break;
+        r = svm_set_tuning(ctx, best_cb, G, best_x);
+        double ms = 0.0;
+        if (!r) r = trial(G, &ms);
+        if (r) break;
+        record(best_cb, G, best_x, ms);
+        if (ms < best) { best = ms; best_G = G; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r) return r;
+    if (rows_written) *rows_written = nrows;
+    return svm_set_tuning(ctx, best_cb, best_G, best_x);
 }
 
 int svm_copy_probe(int64_t n_bytes, int32_t reps, double* ms_per_copy) {

@@ -60,6 +60,7 @@ struct SysParams {
 
 struct Dev {
     int32_t nsys, Cs, Vs, cap;
+    int32_t xcd;                  // 1: tiles dealt to XCDs in contiguous ranges (xcd_tile)
     SysParams* sys;
     const int32_t* ctile_sys;   // cell tile -> sphere
     const int32_t* vtile_sys;   // vertex tile -> sphere

@@ -29,7 +29,8 @@ namespace svm {
 // [x*per, (x+1)*per) keeps a sphere's rows, and the rows its tiles gather, in
 // one L2.  Grids are launched with 8*per workgroups (xcd_grid); the few
 // spare workgroups get a tile >= ntiles and return at once.
-__device__ inline int xcd_tile(int ntiles) {
+__device__ inline int xcd_tile(const Dev& d, int ntiles) {
+    if (!d.xcd) return (int)blockIdx.x;
     const int per = (ntiles + 7) >> 3;
     return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
 }
@@ -95,7 +96,7 @@ __device__ inline double dot0(const double* a, const double* b) {
 // and recomputeAreasCPU (:162-207): one thread per cell.
 __global__ __launch_bounds__(SVM_TILE) void k_geometry(Dev d, int pin, int skip_frozen) {
     __shared__ double red[SVM_TILE + 1];
-    const int tile = xcd_tile(d.Cs / SVM_TILE);
+    const int tile = xcd_tile(d, d.Cs / SVM_TILE);
     if (tile >= d.Cs / SVM_TILE) return;
     const int s = d.ctile_sys[tile];
     SysParams* S = d.sys + s;
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_geometry(Dev d, int pin, int skip_
 // decision without any cross-block synchronisation.
 __global__ __launch_bounds__(SVM_TILE) void k_fan_fix(Dev d, int pin, int skip_frozen) {
     __shared__ double buf[SVM_MAX_TILES];
-    const int tile = xcd_tile(d.Cs / SVM_TILE);
+    const int tile = xcd_tile(d, d.Cs / SVM_TILE);
     if (tile >= d.Cs / SVM_TILE) return;
     SysParams* S = d.sys + d.ctile_sys[tile];
     if (skip_frozen && S->f_frozen) return;
@@ -294,7 +295,7 @@ __device__ inline int vertex_force<float>(const Dev& d, const SysParams* S, cons
 template <int MODE, class FT>
 __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, int nin, int store_forces) {
     __shared__ double red[SVM_TILE + 1];
-    const int tile = xcd_tile(d.Vs / SVM_TILE);
+    const int tile = xcd_tile(d, d.Vs / SVM_TILE);
     if (tile >= d.Vs / SVM_TILE) return;
     const int s = d.vtile_sys[tile];
     SysParams* S = d.sys + s;
@@ -421,7 +422,7 @@ __device__ inline void fire_mix(const Dev& d, const SysParams* S, int v, double*
 // First half of velocityVerlet::integrateEOMCPU (src/updaters/velocityVerlet.cpp:14-23)
 // preceded by the previous iteration's FIRE mixing, then the move.
 __global__ __launch_bounds__(SVM_TILE) void k_fire_vv1(Dev d, int pin, int pout) {
-    const int tile = xcd_tile(d.Vs / SVM_TILE);
+    const int tile = xcd_tile(d, d.Vs / SVM_TILE);
     if (tile >= d.Vs / SVM_TILE) return;
     const int s = d.vtile_sys[tile];
     SysParams* S = d.sys + s;

@@ -656,7 +656,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_build_vln(Dev d, int only) {
 // runs); the active updater's director step is fused here.
 __global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int nin, int nout, int dir_update,
                                                         int skip_frozen) {
-    const int tile = xcd_tile(d.Vs / SVM_TILE);
+    const int tile = xcd_tile(d, d.Vs / SVM_TILE);
     if (tile >= d.Vs / SVM_TILE) return;
     const int s = d.vtile_sys[tile];
     SysParams* S = d.sys + s;

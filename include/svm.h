@@ -289,6 +289,25 @@ int svm_cell_descriptors(svm_context* ctx, int32_t sys, float* X, float* frames)
  * predicted displacements), then enforce the topology (T1). */
 int svm_warm_start(svm_context* ctx, svm_mlp* mlp, double scale);
 
+/* ---- autotuning ----------------------------------------------------------
+ * The launch schedule of a step: the grid of the T1 single-candidate dry runs
+ * (cluster_blocks), the number of steps captured per HIP graph replayed by
+ * svm_run_graph with steps_per_graph = 0 (graph_steps, even), and whether
+ * 256-row tiles are dealt to the chip's 8 XCDs in contiguous ranges
+ * (xcd_mapping).  None of them changes any result, bit for bit. */
+int svm_set_tuning(svm_context* ctx, int32_t cluster_blocks, int32_t graph_steps, int32_t xcd_mapping);
+int svm_get_tuning(svm_context* ctx, int32_t* cluster_blocks, int32_t* graph_steps, int32_t* xcd_mapping);
+/* Project autotuner, in the manner of kernelTuner's median-of-samples scan
+ * (src/utilities/kernelTuner.cpp:35-126) but over the whole step: times
+ * real steps of `updater` (0 Brownian, 1 active) under each candidate
+ * schedule -- cluster_blocks in {128, 256, 512, 1024} x xcd_mapping in {0, 1}
+ * at 10 steps per graph, then graph_steps in {4, 20, 40} -- keeps the fastest
+ * and reports every trial in table (rows of 4 doubles: cluster_blocks,
+ * graph_steps, xcd_mapping, ms per step).  The steps advance the model
+ * (about steps_per_trial per candidate), exactly as untuned steps would. */
+int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, double* table, int32_t table_rows,
+                 int32_t* rows_written);
+
 /* Kernel timing without launch gaps (no counterpart in the reference): the
  * fused vertex pass of `updater` (0 Brownian, 1 active; force + update +
  * move, k_vertex) launched `reps` times back to back between two HIP events

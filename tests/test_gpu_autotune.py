@@ -1,0 +1,56 @@
+"""Autotuner: every schedule gives the same trajectory, bit for bit, and the
+tuner's steps are ordinary steps."""
+import numpy as np
+import pytest
+
+import svmtest
+from test_gpu_parity import make_ctx
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def states():
+    return [svmtest.seeded_state(n, 1400 + i) for i, n in enumerate((300, 900, 2000))]
+
+
+def run(states, steps, tuning=None):
+    ctx = make_ctx(states, p0=3.6)
+    for i in range(len(states)):
+        ctx.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
+    if tuning:
+        ctx.set_tuning(**tuning)
+    ctx.run_graph(1, 0, steps // ctx.tuning()["graph_steps"])
+    return ctx, [ctx.positions(i) for i in range(len(states))]
+
+
+def test_schedules_give_identical_trajectories(states):
+    _, ref = run(states, 40)
+    for t in (dict(cluster_blocks=64, graph_steps=4, xcd_mapping=0),
+              dict(cluster_blocks=1024, graph_steps=20, xcd_mapping=1),
+              dict(cluster_blocks=128, graph_steps=8, xcd_mapping=0)):
+        _, got = run(states, 40, t)
+        for a, b in zip(got, ref):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_autotune_steps_are_ordinary_steps(states):
+    ctx = make_ctx(states, p0=3.6)
+    for i in range(len(states)):
+        ctx.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
+    tab = ctx.autotune(updater=1, steps_per_trial=30)
+    assert len(tab) == 11 and (tab[:, 3] > 0).all()
+    t = ctx.tuning()
+    best = tab[np.argmin(tab[:, 3])]
+    assert (t["cluster_blocks"], t["graph_steps"], t["xcd_mapping"]) == tuple(int(x) for x in best[:3])
+    # the number of steps the tuner took, from the trial table: capture warm-up + 3 samples per candidate
+    steps = 0
+    for cb, G, x, ms in tab:
+        G = int(G)
+        steps += G + 3 * max(1, 30 // (3 * G)) * G
+    ref = make_ctx(states, p0=3.6)
+    for i in range(len(states)):
+        ref.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
+    ref.active_steps(steps)
+    for i in range(len(states)):
+        np.testing.assert_array_equal(ctx.positions(i), ref.positions(i))
