@@ -446,15 +446,19 @@ class Context:
         return X, F
 
     def warm_start(self, mlp, scale=1.0):
-        self._ck(lib().svm_warm_start(self.h, mlp.h, C.c_double(scale)), "svm_warm_start")
+        """Returns the number of spheres whose warm-start move was undone (a cell folded)."""
+        rev = C.c_int32(0)
+        self._ck(lib().svm_warm_start(self.h, mlp.h, C.c_double(scale), C.byref(rev)), "svm_warm_start")
+        return rev.value
 
     def set_force_precision(self, precision):
-        """0: FP64 forces (parity), 1: FP32 forces for every later force evaluation."""
+        """0: FP64 reference expressions (parity), 1: FP32, 2: FP64 vector form."""
         self._ck(lib().svm_set_force_precision(self.h, C.c_int32(precision)), "svm_set_force_precision")
 
     def force_error(self):
-        """Per sphere: max|dF|/max|F|, rms|dF|/rms|F|, max|dF|, rms|F| of FP32 vs FP64 forces."""
-        out = np.zeros((self.n, 4), np.float64)
+        """Per sphere: FP32 vs FP64 reference expressions (max|dF|/max|F|, rms|dF|/rms|F|, max|dF|,
+        rms|F|), FP32 vs FP64 vector form (max, rms), FP64 vector form vs reference (rms)."""
+        out = np.zeros((self.n, 7), np.float64)
         self._ck(lib().svm_get_force_error(self.h, _dp(out)), "svm_get_force_error")
         return out
 

@@ -52,7 +52,8 @@ struct svm_context {
     std::map<std::pair<int, int>, hipGraph_t> graph_defs;
     // Delaunay triangles per sphere (from svm_build_from_cell_points)
     std::vector<std::vector<int32_t>> triangles;
-    // force arithmetic of every force evaluation: FP64 (parity) or FP32
+    // force arithmetic of every force evaluation: 0 FP64 reference expressions
+    // (parity), 1 FP32 vector form, 2 FP64 vector form
     int force_fp32 = 0;
     // profiling
     bool profiling = false;
@@ -215,9 +216,12 @@ int launch_geometry(svm_context* ctx, int skip_frozen) {
 template <int MODE>
 int launch_vertex(svm_context* ctx, int pout, int store_forces, const char* name) {
     Timer t(ctx, name);
-    if (ctx->force_fp32)
+    if (ctx->force_fp32 == 1)
         hipLaunchKernelGGL((svm::k_vertex<MODE, float>), dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0, ctx->stream,
                            ctx->d, ctx->pcur, pout, ctx->ncur, store_forces);
+    else if (ctx->force_fp32 == 2)
+        hipLaunchKernelGGL((svm::k_vertex<MODE, svm::VecF64>), dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0,
+                           ctx->stream, ctx->d, ctx->pcur, pout, ctx->ncur, store_forces);
     else
         hipLaunchKernelGGL((svm::k_vertex<MODE, double>), dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0, ctx->stream,
                            ctx->d, ctx->pcur, pout, ctx->ncur, store_forces);
@@ -1105,7 +1109,8 @@ int svm_compute_forces(svm_context* ctx, int32_t precision) {
     if (!ctx) return SVM_ERR_ARG;
     int r = require_topology(ctx);
     if (r) return r;
-    if (precision != 0 && precision != 1) return fail(ctx, SVM_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32)");
+    if (precision < 0 || precision > 2)
+        return fail(ctx, SVM_ERR_ARG, "precision must be 0 (FP64 reference expressions), 1 (FP32) or 2 (FP64 vector form)");
     r = launch_geometry(ctx, 0);
     if (r) return r;
     const int keep = ctx->force_fp32;
@@ -1116,8 +1121,8 @@ int svm_compute_forces(svm_context* ctx, int32_t precision) {
 }
 
 int svm_set_force_precision(svm_context* ctx, int32_t precision) {
-    if (!ctx || (precision != 0 && precision != 1))
-        return fail(ctx, SVM_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32)");
+    if (!ctx || precision < 0 || precision > 2)
+        return fail(ctx, SVM_ERR_ARG, "precision must be 0 (FP64 reference expressions), 1 (FP32) or 2 (FP64 vector form)");
     if (ctx->force_fp32 != precision) {
         ctx->force_fp32 = precision;
         drop_graphs(ctx);
@@ -1127,33 +1132,44 @@ int svm_set_force_precision(svm_context* ctx, int32_t precision) {
 
 int svm_get_force_error(svm_context* ctx, double* stats) {
     if (!ctx || !stats) return SVM_ERR_ARG;
-    // forces of every sphere as AoS rows, FP64 then FP32
-    std::vector<double> f64(3 * (size_t)ctx->Vs), f32(3 * (size_t)ctx->Vs);
+    // forces of every sphere as AoS rows: FP64 reference expressions, FP64
+    // vector form, then FP32 (left in the force buffer)
+    std::vector<double> f64(3 * (size_t)ctx->Vs), fv(3 * (size_t)ctx->Vs), f32(3 * (size_t)ctx->Vs);
     int r = svm_compute_forces(ctx, 0);
     if (!r) r = download_rows(ctx, ctx->Vs, 0, ctx->d.fx, ctx->d.fy, ctx->d.fz, f64.data());
+    if (!r) r = svm_compute_forces(ctx, 2);
+    if (!r) r = download_rows(ctx, ctx->Vs, 0, ctx->d.fx, ctx->d.fy, ctx->d.fz, fv.data());
     if (!r) r = svm_compute_forces(ctx, 1);
     if (!r) r = download_rows(ctx, ctx->Vs, 0, ctx->d.fx, ctx->d.fy, ctx->d.fz, f32.data());
     if (r) return r;
     for (int s = 0; s < ctx->nsys; ++s) {
         const SysParams& L = ctx->layout[s];
-        double maxf = 0.0, sf = 0.0, se = 0.0, maxe = 0.0;
+        double maxf = 0.0, sf = 0.0, se = 0.0, maxe = 0.0, sev = 0.0, maxev = 0.0, ser = 0.0;
         for (int v = L.vo; v < L.vo + L.nv; ++v) {
-            double a = 0.0, e = 0.0;
+            double a = 0.0, e = 0.0, ev = 0.0, er = 0.0;
             for (int k = 0; k < 3; ++k) {
-                const double x = f64[3 * (size_t)v + k], y = f32[3 * (size_t)v + k];
+                const double x = f64[3 * (size_t)v + k], y = f32[3 * (size_t)v + k], z = fv[3 * (size_t)v + k];
                 a += x * x;
                 e += (y - x) * (y - x);
+                ev += (y - z) * (y - z);
+                er += (z - x) * (z - x);
             }
             sf += a;
             se += e;
+            sev += ev;
+            ser += er;
             maxf = std::max(maxf, std::sqrt(a));
             maxe = std::max(maxe, std::sqrt(e));
+            maxev = std::max(maxev, std::sqrt(ev));
         }
-        double* o = stats + 4 * (size_t)s;
-        o[0] = maxf > 0 ? maxe / maxf : 0.0;          // max |dF| / max |F|
-        o[1] = sf > 0 ? std::sqrt(se / sf) : 0.0;      // rms |dF| / rms |F|
-        o[2] = maxe;                                   // max |dF|
+        double* o = stats + SVM_FORCE_ERR_COLS * (size_t)s;
+        o[0] = maxf > 0 ? maxe / maxf : 0.0;          // FP32 vs FP64 reference: max |dF| / max |F|
+        o[1] = sf > 0 ? std::sqrt(se / sf) : 0.0;      //   rms |dF| / rms |F|
+        o[2] = maxe;                                   //   max |dF|
         o[3] = std::sqrt(sf / L.nv);                   // rms |F| (FP64)
+        o[4] = maxf > 0 ? maxev / maxf : 0.0;         // FP32 vs FP64 vector form: max, rms (relative)
+        o[5] = sf > 0 ? std::sqrt(sev / sf) : 0.0;
+        o[6] = sf > 0 ? std::sqrt(ser / sf) : 0.0;     // FP64 vector form vs FP64 reference, rms
     }
     return SVM_OK;
 }
@@ -1905,43 +1921,100 @@ int svm_cell_descriptors(svm_context* ctx, int32_t sys, float* X, float* frames)
     return r;
 }
 
-int svm_warm_start(svm_context* ctx, svm_mlp* m, double scale) {
+int svm_warm_start(svm_context* ctx, svm_mlp* m, double scale, int32_t* reverted) {
     if (!ctx || !m) return SVM_ERR_ARG;
     int r = require_topology(ctx);
     if (r) return r;
     r = svm_compute_forces(ctx, 0);
     if (r) return r;
-    float *dX = nullptr, *dF = nullptr, *dY = nullptr;
-    if (hipMalloc(&dX, sizeof(float) * 16 * (size_t)ctx->Cs) != hipSuccess ||
-        hipMalloc(&dF, sizeof(float) * 6 * (size_t)ctx->Cs) != hipSuccess ||
-        hipMalloc(&dY, sizeof(float) * 2 * (size_t)ctx->Cs) != hipSuccess) {
-        if (dX) (void)hipFree(dX);
-        if (dF) (void)hipFree(dF);
+    Dev& d = ctx->d;
+    const size_t Vs = ctx->Vs, Cs = ctx->Cs, cap = ctx->cap;
+    DevScratch w;
+    float* dX = w.get<float>(16 * Cs);
+    float* dF = w.get<float>(6 * Cs);
+    float* dY = w.get<float>(2 * Cs);
+    // the state the move may change, to put back per sphere
+    double* sp = w.get<double>(3 * Vs);
+    int4* svn = w.get<int4>(Vs);
+    int4* svcn = w.get<int4>(Vs);
+    int32_t* svln = w.get<int32_t>(Vs);
+    int32_t* scnn = w.get<int32_t>(Cs);
+    int32_t* sring = w.get<int32_t>(cap * Cs);
+    if (!dX || !dF || !dY || !sp || !svn || !svcn || !svln || !scnn || !sring)
         return fail(ctx, SVM_ERR_HIP, "device allocation failed");
+    const int p0 = ctx->pcur;
+    // how far each sphere's areas are from tiling it before the move (a raw
+    // random start can already hold folded cells)
+    std::vector<double> A(Cs), err0(ctx->nsys);
+    HIPCHECK(ctx, hipMemcpyAsync(A.data(), d.A, sizeof(double) * Cs, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    auto tiling_error = [&](int s, bool* ok) {
+        const SysParams& L = ctx->layout[s];
+        double sum = 0.0;
+        *ok = true;
+        for (int c = L.co; c < L.co + L.nc; ++c) {
+            *ok = *ok && A[c] > 0.0 && std::isfinite(A[c]);
+            sum += A[c];
+        }
+        return std::fabs(sum / (double)L.nc - 1.0);   // R = sqrt(N / 4 pi): the sphere's area is N
+    };
+    for (int s = 0; s < ctx->nsys; ++s) {
+        bool ok;
+        err0[s] = tiling_error(s, &ok);
     }
+    HIPCHECK(ctx, hipMemcpyAsync(sp, d.px[p0], sizeof(double) * Vs, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(sp + Vs, d.py[p0], sizeof(double) * Vs, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(sp + 2 * Vs, d.pz[p0], sizeof(double) * Vs, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(svn, d.vn, sizeof(int4) * Vs, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(svcn, d.vcn, sizeof(int4) * Vs, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(svln, d.vln, sizeof(int32_t) * Vs, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(scnn, d.cnn, sizeof(int32_t) * Cs, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(sring, d.ring, sizeof(int32_t) * cap * Cs, hipMemcpyDeviceToDevice, ctx->stream));
     // the network runs on the context's stream
     hipStream_t keep = m->stream;
     m->stream = ctx->stream;
-    hipLaunchKernelGGL(svm::k_cell_descriptors, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, dX,
-                       dF);
+    hipLaunchKernelGGL(svm::k_cell_descriptors, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, d, p0, dX, dF);
     r = last_launch(ctx);
     if (!r) r = mlp_forward_dev(m, dX, ctx->Cs, dY, false);
     m->stream = keep;
-    if (!r) {
-        hipLaunchKernelGGL(svm::k_warm_disp, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d,
-                           (const float*)dY, (const float*)dF, scale);
-        const int pout = 1 - ctx->pcur;
-        hipLaunchKernelGGL(svm::k_move_ext, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, pout, -1,
-                           1.0);
-        r = last_launch(ctx);
-        if (!r) r = launch_t1(ctx, pout, ctx->ncur, ctx->ncur, 0, 0, 0);
-        if (!r) ctx->pcur = pout;
+    if (r) return r;
+    hipLaunchKernelGGL(svm::k_warm_disp, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, d, p0, (const float*)dY,
+                       (const float*)dF, scale, 0.3);
+    const int pout = 1 - p0;
+    hipLaunchKernelGGL(svm::k_move_ext, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, d, p0, pout, -1, 1.0);
+    r = last_launch(ctx);
+    if (!r) r = launch_t1(ctx, pout, ctx->ncur, ctx->ncur, 0, 0, 0);
+    if (r) return r;
+    ctx->pcur = pout;
+    // a sphere whose angle-sum areas tile it worse than before (a cell
+    // folded) gets its state back: the warm start never leaves a sphere worse
+    // than it was
+    r = launch_geometry(ctx, 0);
+    if (r) return r;
+    HIPCHECK(ctx, hipMemcpyAsync(A.data(), d.A, sizeof(double) * Cs, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    int nrev = 0;
+    for (int s = 0; s < ctx->nsys; ++s) {
+        const SysParams& L = ctx->layout[s];
+        bool ok;
+        const double err1 = tiling_error(s, &ok);
+        if (ok && err1 <= err0[s] + 1e-8) continue;
+        ++nrev;
+        const size_t vo = L.vo, nv = L.nv, co = L.co, nc = L.nc;
+        HIPCHECK(ctx, hipMemcpyAsync(d.px[pout] + vo, sp + vo, sizeof(double) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHECK(ctx, hipMemcpyAsync(d.py[pout] + vo, sp + Vs + vo, sizeof(double) * nv, hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+        HIPCHECK(ctx, hipMemcpyAsync(d.pz[pout] + vo, sp + 2 * Vs + vo, sizeof(double) * nv, hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+        HIPCHECK(ctx, hipMemcpyAsync(d.vn + vo, svn + vo, sizeof(int4) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHECK(ctx, hipMemcpyAsync(d.vcn + vo, svcn + vo, sizeof(int4) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHECK(ctx, hipMemcpyAsync(d.vln + vo, svln + vo, sizeof(int32_t) * nv, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHECK(ctx, hipMemcpyAsync(d.cnn + co, scnn + co, sizeof(int32_t) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHECK(ctx, hipMemcpy2DAsync(d.ring + co, sizeof(int32_t) * Cs, sring + co, sizeof(int32_t) * Cs,
+                                       sizeof(int32_t) * nc, cap, hipMemcpyDeviceToDevice, ctx->stream));
     }
-    if (!r) r = svm_synchronize(ctx);
-    (void)hipFree(dX);
-    (void)hipFree(dF);
-    (void)hipFree(dY);
-    return r;
+    if (reverted) *reverted = nrev;
+    return svm_synchronize(ctx);
 }
 
 }  // extern "C"

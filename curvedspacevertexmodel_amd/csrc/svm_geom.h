@@ -131,29 +131,36 @@ SVM_HD bool pair_force(const PQT<T>& cur, const PQT<T>& last, const PQT<T>& next
     return !(fSet[0] != fSet[0]);
 }
 
-// ------------------------------------------------------ FP32 force path ----
+// ------------------------------------------------- vector-form gradients ----
 // The same tangent-plane gradients as grad_geodesic / grad_triangle, written
-// in a form that single precision evaluates well.  The reference's trig-basis
+// in a form that evaluates well in single precision (the FP32 force path)
+// and, in double precision, stays accurate on large spheres where the
+// reference's expressions lose digits.  The reference's trig-basis
 // expressions cancel catastrophically in FP32 for the small angles between
 // neighbouring points (the Gram determinant g of a small triangle is ~1e-5
 // from O(1) terms).  Here every quantity is built from the unit vector a of
 // the vertex and the scaled differences e = (u - v)/R of its neighbours,
 // formed in FP64, so no O(1) terms cancel.
-struct f3 {
-    float x, y, z;
+template <class T>
+struct V3 {
+    T x, y, z;
 };
-SVM_HD f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
-SVM_HD f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
-SVM_HD f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
-SVM_HD float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-SVM_HD f3 cross3(f3 a, f3 b) { return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+template <class T> SVM_HD V3<T> operator+(V3<T> a, V3<T> b) { return V3<T>{a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> SVM_HD V3<T> operator-(V3<T> a, V3<T> b) { return V3<T>{a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> SVM_HD V3<T> operator*(T s, V3<T> a) { return V3<T>{s * a.x, s * a.y, s * a.z}; }
+template <class T> SVM_HD T dot3(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> SVM_HD V3<T> cross3(V3<T> a, V3<T> b) {
+    return V3<T>{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+using f3 = V3<float>;
 
 // d(arc length |v u|)/d v on the sphere: -(e - (a.e) a) / |a x e|
 // (the angle alpha between a and a + e has sin(alpha) = |a x e|).
-SVM_HD f3 grad_arc32(f3 a, f3 e) {
-    f3 t = e - dot3(a, e) * a;
-    f3 x = cross3(a, e);
-    return (-1.0f / __builtin_sqrtf(dot3(x, x))) * t;
+template <class T>
+SVM_HD V3<T> grad_arc_v(V3<T> a, V3<T> e) {
+    V3<T> t = e - dot3(a, e) * a;
+    V3<T> x = cross3(a, e);
+    return (T(-1) / vsqrt(dot3(x, x))) * t;
 }
 
 // d(signed area of the spherical triangle v, b, c)/d v for b = a + eb,
@@ -165,15 +172,16 @@ SVM_HD f3 grad_arc32(f3 a, f3 e) {
 // folds over (a non-convex cell) has N < 0 and counts negatively, so the fan
 // areas keep summing to the cell's angle-sum area -- as in the reference's
 // chain rule (checked against finite differences of the energy).
-SVM_HD f3 grad_tri32(f3 a, f3 eb, f3 ec, float radius) {
-    f3 bxc = cross3(a, ec - eb) + cross3(eb, ec);
-    float N = dot3(a, cross3(eb, ec));
-    f3 dbc = eb - ec;
-    float D = 4.0f - 0.5f * (dot3(eb, eb) + dot3(ec, ec) + dot3(dbc, dbc));
-    f3 tb = bxc - N * a;                       // tangent part of b x c (a.(b x c) = N)
-    f3 s = eb + ec;
-    f3 ts = s - dot3(a, s) * a;                // tangent part of b + c
-    float k = 2.0f * radius / (N * N + D * D);
+template <class T>
+SVM_HD V3<T> grad_tri_v(V3<T> a, V3<T> eb, V3<T> ec, T radius) {
+    V3<T> bxc = cross3(a, ec - eb) + cross3(eb, ec);
+    T N = dot3(a, cross3(eb, ec));
+    V3<T> dbc = eb - ec;
+    T D = T(4) - T(0.5) * (dot3(eb, eb) + dot3(ec, ec) + dot3(dbc, dbc));
+    V3<T> tb = bxc - N * a;                       // tangent part of b x c (a.(b x c) = N)
+    V3<T> s = eb + ec;
+    V3<T> ts = s - dot3(a, s) * a;                // tangent part of b + c
+    T k = T(2) * radius / (N * N + D * D);
     return k * (D * tb - N * ts);
 }
 

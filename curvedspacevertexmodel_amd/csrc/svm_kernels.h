@@ -251,45 +251,64 @@ __device__ inline int vertex_force(const Dev& d, const SysParams* S, const doubl
     return nan;
 }
 
-// FP32 force variant: the force of the same energy, from the vector forms
-// grad_arc32 / grad_tri32 (no basis singularity at the poles).
-template <>
-__device__ inline int vertex_force<float>(const Dev& d, const SysParams* S, const double* X, const double* Y,
-                                          const double* Z, int v, double* f) {
+// Force of the same energy from the vector-form gradients (grad_arc_v /
+// grad_tri_v; no basis singularity at the poles), in T arithmetic from FP64
+// positions and geometry: T = float is the FP32 force variant, T = double a
+// well-conditioned FP64 variant.
+template <class T>
+__device__ inline int vertex_force_vec(const Dev& d, const SysParams* S, const double* X, const double* Y,
+                                       const double* Z, int v, double* f) {
     const double R = S->R, iR = 1.0 / R;
-    const float Kr = (float)S->kr, Rf = (float)R;
+    const T Kr = (T)S->kr, Rt = (T)R;
     const double px = X[v], py = Y[v], pz = Z[v];
-    const f3 a{(float)(px * iR), (float)(py * iR), (float)(pz * iR)};
+    const V3<T> a{(T)(px * iR), (T)(py * iR), (T)(pz * iR)};
     auto rel = [&](double x, double y, double z) {
-        return f3{(float)((x - px) * iR), (float)((y - py) * iR), (float)((z - pz) * iR)};
+        return V3<T>{(T)((x - px) * iR), (T)((y - py) * iR), (T)((z - pz) * iR)};
     };
     int4 nb = d.vn[v];
     int4 cl = d.vcn[v];
     int ln = d.vln[v];
-    const f3 e0 = rel(X[nb.x], Y[nb.x], Z[nb.x]);
-    const f3 e1 = rel(X[nb.y], Y[nb.y], Z[nb.y]);
-    const f3 e2 = rel(X[nb.z], Y[nb.z], Z[nb.z]);
-    const f3 g0 = grad_arc32(a, e0), g1 = grad_arc32(a, e1), g2 = grad_arc32(a, e2);
-    f3 F{0.0f, 0.0f, 0.0f};
+    const V3<T> e0 = rel(X[nb.x], Y[nb.x], Z[nb.x]);
+    const V3<T> e1 = rel(X[nb.y], Y[nb.y], Z[nb.y]);
+    const V3<T> e2 = rel(X[nb.z], Y[nb.z], Z[nb.z]);
+    const V3<T> g0 = grad_arc_v(a, e0), g1 = grad_arc_v(a, e1), g2 = grad_arc_v(a, e2);
+    V3<T> F{T(0), T(0), T(0)};
     int nan = 0;
 #pragma unroll
     for (int sl = 0; sl < 3; ++sl) {
         int c = sl == 0 ? cl.x : (sl == 1 ? cl.y : cl.z);
         int ls = (ln >> (4 * sl)) & 3;
         int ns = (ln >> (4 * sl + 2)) & 3;
-        const float cp = 2.0f * (float)(d.P[c] - d.p0[c]);
-        const float ca = 2.0f * Kr * (float)(d.A[c] - d.a0[c]);
-        const f3 ec = rel(d.cx[c], d.cy[c], d.cz[c]);
-        const f3 el = ls == 0 ? e0 : (ls == 1 ? e1 : e2);
-        const f3 en = ns == 0 ? e0 : (ns == 1 ? e1 : e2);
-        const f3 gl = ls == 0 ? g0 : (ls == 1 ? g1 : g2);
-        const f3 gn = ns == 0 ? g0 : (ns == 1 ? g1 : g2);
-        f3 fs = (-cp) * (gl + gn) - ca * (grad_tri32(a, el, ec, Rf) + grad_tri32(a, ec, en, Rf));
-        if (fs.x != fs.x) ++nan;
+        const T cp = T(2) * (T)(d.P[c] - d.p0[c]);
+        const T ca = T(2) * Kr * (T)(d.A[c] - d.a0[c]);
+        const V3<T> ec = rel(d.cx[c], d.cy[c], d.cz[c]);
+        const V3<T> el = ls == 0 ? e0 : (ls == 1 ? e1 : e2);
+        const V3<T> en = ns == 0 ? e0 : (ns == 1 ? e1 : e2);
+        const V3<T> gl = ls == 0 ? g0 : (ls == 1 ? g1 : g2);
+        const V3<T> gn = ns == 0 ? g0 : (ns == 1 ? g1 : g2);
+        V3<T> fs = (-cp) * (gl + gn) - ca * (grad_tri_v(a, el, ec, Rt) + grad_tri_v(a, ec, en, Rt));
+        // skipped like the reference's NaN contributions; here a degenerate
+        // edge (coincident vertices) gives +-inf, which x - x turns into NaN
+        if (!(fs.x - fs.x == T(0) && fs.y - fs.y == T(0) && fs.z - fs.z == T(0))) ++nan;
         else F = F + fs;
     }
     f[0] = (double)F.x; f[1] = (double)F.y; f[2] = (double)F.z;
     return nan;
+}
+
+template <>
+__device__ inline int vertex_force<float>(const Dev& d, const SysParams* S, const double* X, const double* Y,
+                                          const double* Z, int v, double* f) {
+    return vertex_force_vec<float>(d, S, X, Y, Z, v, f);
+}
+
+// Force arithmetic selector of k_vertex: double (reference expressions,
+// parity), float (vector form, FP32), VecF64 (vector form, FP64).
+struct VecF64 {};
+template <>
+__device__ inline int vertex_force<VecF64>(const Dev& d, const SysParams* S, const double* X, const double* Y,
+                                           const double* Z, int v, double* f) {
+    return vertex_force_vec<double>(d, S, X, Y, Z, v, f);
 }
 
 template <int MODE, class FT>

@@ -413,8 +413,12 @@ __global__ __launch_bounds__(SVM_TILE) void k_cell_descriptors(Dev d, int pin, f
 }
 
 // Warm-start displacement of each vertex: the mean over its three cells of
-// the predicted cell displacement y0 e1 + y1 e2, times `scale`, into d.disp.
-__global__ __launch_bounds__(SVM_TILE) void k_warm_disp(Dev d, const float* Y, const float* frame, double scale) {
+// the predicted cell displacement y0 e1 + y1 e2, times `scale`, into d.disp;
+// its length is capped at `cap` times the vertex's shortest edge, so the move
+// cannot fold a cell (vertices moving by less than a third of their edges
+// keep every triangle's orientation; FIRE does the rest).
+__global__ __launch_bounds__(SVM_TILE) void k_warm_disp(Dev d, int pin, const float* Y, const float* frame,
+                                                        double scale, double cap) {
     const int v = blockIdx.x * SVM_TILE + threadIdx.x;
     if (v >= d.Vs) return;
     const SysParams* S = d.sys + d.vtile_sys[v / SVM_TILE];
@@ -428,6 +432,20 @@ __global__ __launch_bounds__(SVM_TILE) void k_warm_disp(Dev d, const float* Y, c
             for (int j = 0; j < 3; ++j) out[j] += y0 * fr[j] + y1 * fr[3 + j];
         }
         for (int j = 0; j < 3; ++j) out[j] *= scale / 3.0;
+        const double* X = d.px[pin];
+        const double* Yp = d.py[pin];
+        const double* Zp = d.pz[pin];
+        int4 nb = d.vn[v];
+        int n3[3] = {nb.x, nb.y, nb.z};
+        double emin = 1e300;
+        for (int k = 0; k < 3; ++k) {
+            const double dx = X[n3[k]] - X[v], dy = Yp[n3[k]] - Yp[v], dz = Zp[n3[k]] - Zp[v];
+            emin = fmin(emin, __builtin_sqrt(dx * dx + dy * dy + dz * dz));
+        }
+        const double len = __builtin_sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
+        const double lim = cap * emin;
+        if (len > lim && len > 0.0)
+            for (int j = 0; j < 3; ++j) out[j] *= lim / len;
     }
     for (int j = 0; j < 3; ++j) d.disp[3 * (size_t)v + j] = out[j];
 }

@@ -33,7 +33,7 @@
 
 namespace svm {
 
-#define SVM_T1_MAXCL 4096     // flippable candidates handled by dry runs per step
+#define SVM_T1_MAXCL 16384    // flippable candidates handled by dry runs per step
 #define SVM_T1_MAXBL 65536    // blocked candidates checked per step
 #define SVM_T1_WV 128         // vertices cached per dry run
 #define SVM_T1_WC 64          // cells cached per dry run

@@ -62,7 +62,7 @@ def run_fire(ctx, n_sys, net=None, scale=1.0, check_every=100):
     ctx.synchronize()
     t0 = time.perf_counter()
     if net is not None:
-        ctx.warm_start(net, scale)
+        ctx.warm_start(net, scale)   # undone per sphere where a cell would fold
     ctx.fire_minimize(check_every)
     ctx.synchronize()
     wall = time.perf_counter() - t0

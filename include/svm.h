@@ -177,12 +177,19 @@ int svm_compute_geometry(svm_context* ctx);
 int svm_compute_forces(svm_context* ctx, int32_t precision);
 /* Force arithmetic of every later force evaluation, the updaters included
  * (the SCALARFLOAT build switch, inc/std_include.h:66-78, made a run-time
- * choice): 0 = FP64, 1 = FP32 forces.  Positions, cell centres, areas,
+ * choice): 0 = FP64 with the reference's expressions (the parity path),
+ * 1 = FP32, 2 = FP64; 1 and 2 use a vector form of the same gradients that
+ * stays well conditioned where the reference's expressions cancel (small
+ * fan triangles on large spheres).  Positions, cell centres, areas,
  * perimeters, T1 tests and updates stay FP64. */
 int svm_set_force_precision(svm_context* ctx, int32_t precision);
-/* Error of the FP32 forces against the FP64 forces on the current state, per
- * sphere (4 doubles each): max|dF|/max|F|, rms|dF|/rms|F|, max|dF|, rms|F|.
- * Leaves the FP32 forces in the force buffer. */
+/* Error of the FP32 forces on the current state, per sphere
+ * (SVM_FORCE_ERR_COLS doubles each): against the FP64 reference expressions
+ * max|dF|/max|F|, rms|dF|/rms|F|, max|dF|, rms|F|; against the FP64 vector
+ * form max and rms (relative); and the FP64 vector form against the FP64
+ * reference expressions, rms (relative).  Leaves the FP32 forces in the
+ * force buffer. */
+#define SVM_FORCE_ERR_COLS 7
 int svm_get_force_error(svm_context* ctx, double* stats);
 /* moveParticles(displacements, scale) then enforceTopology:
  * gpu_move_particles_on_sphere + the T1 callers
@@ -286,8 +293,13 @@ int svm_mlp_time_forward(svm_mlp* mlp, int32_t n, int32_t reps, double* ms_per_l
  * which the network's displacements are expressed. */
 int svm_cell_descriptors(svm_context* ctx, int32_t sys, float* X, float* frames);
 /* Move every vertex of every sphere by scale x (mean of its three cells'
- * predicted displacements), then enforce the topology (T1). */
-int svm_warm_start(svm_context* ctx, svm_mlp* mlp, double scale);
+ * predicted displacements), each move capped at 0.3 x the vertex's shortest
+ * edge, then enforce the topology (T1).  A sphere whose cell areas tile it
+ * worse afterwards (|sum A / 4 pi R^2 - 1| grown by more than 1e-8: a cell
+ * folded) gets its positions and topology back; `reverted` (optional) counts
+ * those spheres.
+ * The warm start never leaves a sphere worse than plain FIRE would start. */
+int svm_warm_start(svm_context* ctx, svm_mlp* mlp, double scale, int32_t* reverted);
 
 /* ---- autotuning ----------------------------------------------------------
  * The launch schedule of a step: the grid of the T1 single-candidate dry runs

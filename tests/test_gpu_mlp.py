@@ -146,6 +146,11 @@ def test_warm_start_moves_vertices_by_the_mean_cell_displacement():
     for v in range(len(p0)):
         cs = st["vcn"][v]
         disp[v] = np.mean([0.01 * F[c, :3] + (-0.02) * F[c, 3:] for c in cs], axis=0)
+        # capped at 0.3 x the shortest edge
+        emin = min(np.linalg.norm(p0[w] - p0[v]) for w in st["vn"][v])
+        ln = np.linalg.norm(disp[v])
+        if ln > 0.3 * emin:
+            disp[v] *= 0.3 * emin / ln
     R = np.linalg.norm(p0[0])
     exp = p0 + disp - np.sum(disp * p0, 1)[:, None] * p0 / R ** 2
     exp *= R / np.linalg.norm(exp, axis=1)[:, None]

@@ -80,3 +80,22 @@ def test_fp32_dynamics_track_fp64(states):
     ctx.set_active(0, dt=0.01, mu=1.0, v0=0.1, tau=1.0)
     ctx.active_steps(5)
     np.testing.assert_array_equal(ctx.positions(0), runs[0])
+
+
+def test_fp64_vector_form_against_numpy(states):
+    """precision 2: the vector-form gradients in FP64 on the GPU equal the same
+    formulas in numpy (tests/svmtest.vector_forces) to rounding."""
+    ctx = make_ctx(states)
+    ctx.compute_forces(precision=2)
+    rep = ctx.force_error()
+    for i, st in enumerate(states):
+        om = svmtest.OracleModel(st)
+        om.compute_forces()
+        fv = svmtest.vector_forces(om.positions(), om.cell_positions(), om.area_perimeter(), st["vn"], st["vcn"],
+                                   st["ring"], st["cnn"], st["R"])
+        ctx.compute_forces(precision=2)
+        g = ctx.forces(i)
+        assert np.linalg.norm(g - fv) / np.linalg.norm(fv) < 1e-12
+        # FP32 against the FP64 vector form, and the two FP64 forms against each other, on these small spheres
+        assert rep[i, 5] < 1e-5
+        assert rep[i, 6] < 1e-8

@@ -413,3 +413,18 @@ def test_library_reports_no_device_without_gpu():
     with pytest.raises(svm.SVMError) as e:
         svm.Context([dict(n_cells=64)])
     assert e.value.code == 3
+
+
+def test_vector_form_forces_equal_reference_forces_on_small_spheres():
+    """The FP64 vector-form forces (the FP32 path's formulas) equal the
+    reference's trig-basis forces where the latter are well conditioned, and
+    both equal minus the energy gradient (pinned above)."""
+    for n, seed in ((64, 1141), (500, 1143)):
+        st = svmtest.seeded_state(n, seed)
+        om = svmtest.OracleModel(st)
+        om.compute_forces()
+        F = om.forces()
+        Fv = svmtest.vector_forces(om.positions(), om.cell_positions(), om.area_perimeter(), st["vn"], st["vcn"],
+                                   st["ring"], st["cnn"], st["R"])
+        rel = np.linalg.norm(Fv - F) / np.linalg.norm(F)
+        assert rel < 1e-8, rel
