@@ -445,10 +445,12 @@ class Context:
         self._ck(lib().svm_cell_descriptors(self.h, C.c_int32(sys), _fp(X), _fp(F)), "svm_cell_descriptors")
         return X, F
 
-    def warm_start(self, mlp, scale=1.0):
-        """Returns the number of spheres whose warm-start move was undone (a cell folded)."""
+    def warm_start(self, mlp, scale=1.0, max_step=0.3):
+        """Move by the network's displacements, each capped at max_step x the vertex's shortest edge
+        (<= 0: no cap).  Returns the number of spheres whose move was undone (a cell folded)."""
         rev = C.c_int32(0)
-        self._ck(lib().svm_warm_start(self.h, mlp.h, C.c_double(scale), C.byref(rev)), "svm_warm_start")
+        self._ck(lib().svm_warm_start(self.h, mlp.h, C.c_double(scale), C.c_double(max_step), C.byref(rev)),
+                 "svm_warm_start")
         return rev.value
 
     def set_force_precision(self, precision):

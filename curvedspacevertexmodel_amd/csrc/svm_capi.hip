@@ -1921,8 +1921,8 @@ int svm_cell_descriptors(svm_context* ctx, int32_t sys, float* X, float* frames)
     return r;
 }
 
-int svm_warm_start(svm_context* ctx, svm_mlp* m, double scale, int32_t* reverted) {
-    if (!ctx || !m) return SVM_ERR_ARG;
+int svm_warm_start(svm_context* ctx, svm_mlp* m, double scale, double max_step, int32_t* reverted) {
+    if (!ctx || !m || !(max_step == max_step)) return SVM_ERR_ARG;
     int r = require_topology(ctx);
     if (r) return r;
     r = svm_compute_forces(ctx, 0);
@@ -1979,7 +1979,7 @@ int svm_warm_start(svm_context* ctx, svm_mlp* m, double scale, int32_t* reverted
     m->stream = keep;
     if (r) return r;
     hipLaunchKernelGGL(svm::k_warm_disp, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, d, p0, (const float*)dY,
-                       (const float*)dF, scale, 0.3);
+                       (const float*)dF, scale, max_step > 0.0 ? max_step : 0.0);
     const int pout = 1 - p0;
     hipLaunchKernelGGL(svm::k_move_ext, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, d, p0, pout, -1, 1.0);
     r = last_launch(ctx);

@@ -414,9 +414,9 @@ __global__ __launch_bounds__(SVM_TILE) void k_cell_descriptors(Dev d, int pin, f
 
 // Warm-start displacement of each vertex: the mean over its three cells of
 // the predicted cell displacement y0 e1 + y1 e2, times `scale`, into d.disp;
-// its length is capped at `cap` times the vertex's shortest edge, so the move
-// cannot fold a cell (vertices moving by less than a third of their edges
-// keep every triangle's orientation; FIRE does the rest).
+// its length is capped at `cap` times the vertex's shortest edge (no cap for
+// cap = 0); with a cap below 1/3 no move can fold a cell (vertices moving by
+// less than a third of their edges keep every triangle's orientation).
 __global__ __launch_bounds__(SVM_TILE) void k_warm_disp(Dev d, int pin, const float* Y, const float* frame,
                                                         double scale, double cap) {
     const int v = blockIdx.x * SVM_TILE + threadIdx.x;
@@ -443,8 +443,8 @@ __global__ __launch_bounds__(SVM_TILE) void k_warm_disp(Dev d, int pin, const fl
             emin = fmin(emin, __builtin_sqrt(dx * dx + dy * dy + dz * dz));
         }
         const double len = __builtin_sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
-        const double lim = cap * emin;
-        if (len > lim && len > 0.0)
+        const double lim = cap * emin;             // cap 0: no limit
+        if (cap > 0.0 && len > lim && len > 0.0)
             for (int j = 0; j < 3; ++j) out[j] *= lim / len;
     }
     for (int j = 0; j < 3; ++j) d.disp[3 * (size_t)v + j] = out[j];

@@ -293,13 +293,13 @@ int svm_mlp_time_forward(svm_mlp* mlp, int32_t n, int32_t reps, double* ms_per_l
  * which the network's displacements are expressed. */
 int svm_cell_descriptors(svm_context* ctx, int32_t sys, float* X, float* frames);
 /* Move every vertex of every sphere by scale x (mean of its three cells'
- * predicted displacements), each move capped at 0.3 x the vertex's shortest
- * edge, then enforce the topology (T1).  A sphere whose cell areas tile it
+ * predicted displacements), each move capped at max_step x the vertex's
+ * shortest edge (max_step <= 0: no cap), then enforce the topology (T1).  A sphere whose cell areas tile it
  * worse afterwards (|sum A / 4 pi R^2 - 1| grown by more than 1e-8: a cell
  * folded) gets its positions and topology back; `reverted` (optional) counts
  * those spheres.
  * The warm start never leaves a sphere worse than plain FIRE would start. */
-int svm_warm_start(svm_context* ctx, svm_mlp* mlp, double scale, int32_t* reverted);
+int svm_warm_start(svm_context* ctx, svm_mlp* mlp, double scale, double max_step, int32_t* reverted);
 
 /* ---- autotuning ----------------------------------------------------------
  * The launch schedule of a step: the grid of the T1 single-candidate dry runs
