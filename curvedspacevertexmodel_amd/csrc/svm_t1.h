@@ -613,24 +613,44 @@ __device__ __attribute__((always_inline)) int visit_vertex(A& a, int gu, int u, 
 }
 
 // Rebuild vln for vertex x from the current rings; returns false if a ring
-// neighbour is not in x's vertex-neighbour list.
+// neighbour is not in x's vertex-neighbour list.  The loads are issued in
+// three dependent rounds (vcn/vn, the three counts, eight ring entries of each
+// cell), not one ring entry at a time.
 __device__ inline bool build_vln(const Dev& d, int x) {
-    int4 cl = d.vcn[x];
-    int4 nb = d.vn[x];
-    int cells[3] = {cl.x, cl.y, cl.z};
+    const int4 cl = d.vcn[x];
+    const int4 nb = d.vn[x];
+    const int cells[3] = {cl.x, cl.y, cl.z};
+    int n3[3];
+#pragma unroll
+    for (int sl = 0; sl < 3; ++sl) n3[sl] = d.cnn[cells[sl]];
+    int e[3][8];
+#pragma unroll
+    for (int sl = 0; sl < 3; ++sl)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) e[sl][j] = j < n3[sl] ? ringref(d, cells[sl], j) : -1;
     int out = 0;
     bool ok = true;
+#pragma unroll
     for (int sl = 0; sl < 3; ++sl) {
-        int c = cells[sl];
-        int n = d.cnn[c];
+        const int c = cells[sl], n = n3[sl];
         int k = -1;
-        for (int j = 0; j < n; ++j)
-            if (ringref(d, c, j) == x) { k = j; break; }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (k < 0 && e[sl][j] == x) k = j;
+        for (int j = 8; k < 0 && j < n; ++j)
+            if (ringref(d, c, j) == x) k = j;
         if (k < 0) { ok = false; continue; }
-        int prev = ringref(d, c, (k + n - 1) % n);
-        int next = ringref(d, c, (k + 1) % n);
-        int ls = prev == nb.x ? 0 : (prev == nb.y ? 1 : (prev == nb.z ? 2 : -1));
-        int ns = next == nb.x ? 0 : (next == nb.y ? 1 : (next == nb.z ? 2 : -1));
+        const int kp = k == 0 ? n - 1 : k - 1, kn = k + 1 == n ? 0 : k + 1;
+        int prev = -1, next = -1;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            prev = j == kp ? e[sl][j] : prev;
+            next = j == kn ? e[sl][j] : next;
+        }
+        if (kp >= 8) prev = ringref(d, c, kp);
+        if (kn >= 8) next = ringref(d, c, kn);
+        const int ls = prev == nb.x ? 0 : (prev == nb.y ? 1 : (prev == nb.z ? 2 : -1));
+        const int ns = next == nb.x ? 0 : (next == nb.y ? 1 : (next == nb.z ? 2 : -1));
         if (ls < 0 || ns < 0) { ok = false; continue; }
         out |= (ls << (4 * sl)) | (ns << (4 * sl + 2));
     }
@@ -789,127 +809,175 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_check(Dev d, int pcur) {
 
 // ------------------------------------------------------------ dry runs ----
 // Write one log's items to HBM (whole block).
-__device__ inline void apply_items(const Dev& d, int pcur, int nvert, const int* vid, const int* vflag,
-                                   const double (*vpos)[3], const int (*vvn)[3], const int (*vvcn)[3], int ncell,
-                                   const int* cid, const int* ccnt, const int (*cring)[SVM_T1_RC]) {
-    for (int i = threadIdx.x; i < nvert; i += blockDim.x) {
-        int v = vid[i];
-        if (vflag[i] & 1) {
-            d.px[pcur][v] = vpos[i][0];
-            d.py[pcur][v] = vpos[i][1];
-            d.pz[pcur][v] = vpos[i][2];
+// One item of a dry-run log written to the topology, with every load issued
+// before any store (a store to the topology could alias the next load as far
+// as the compiler knows, which would otherwise serialise the items on memory
+// latency).  Item i < nvert is vertex i; nvert + j is cell j.  Vertices of a
+// written cell go to `verts` (once each, through the LDS bitmap `seen` over
+// the sphere's local vertex ids when it is given) and moved vertices to
+// `moved`, for the rebuild that follows.
+template <int RC>
+__device__ __attribute__((always_inline)) inline void apply_item(
+    const Dev& d, int pcur, int item, int nvert, const int* vid, const int* vflag, const double (*vpos)[3],
+    const int (*vvn)[3], const int (*vvcn)[3], const int* cid, const int* ccnt, const int (*cring)[RC], int dir_mode,
+    int vo, unsigned* seen, int seen_words, int* verts, int* nverts, int* moved, int* nmoved, int cap) {
+    if (item < nvert) {
+        const int v = vid[item], fl = vflag[item];
+        const double p0 = vpos[item][0], p1 = vpos[item][1], p2 = vpos[item][2];
+        const int n0 = vvn[item][0], n1 = vvn[item][1], n2 = vvn[item][2];
+        const int c0 = vvcn[item][0], c1 = vvcn[item][1], c2 = vvcn[item][2];
+        if (fl & 1) {
+            d.px[pcur][v] = p0;
+            d.py[pcur][v] = p1;
+            d.pz[pcur][v] = p2;
+            if (dir_mode) {
+                const int slot = atomicAdd(nmoved, 1);
+                if (slot < cap) moved[slot] = v;
+            }
         }
-        if (vflag[i] & 2) {
-            d.vn[v] = make_int4(vvn[i][0], vvn[i][1], vvn[i][2], d.vn[v].w);
-            d.vcn[v] = make_int4(vvcn[i][0], vvcn[i][1], vvcn[i][2], d.vcn[v].w);
+        if (fl & 2) {
+            vnref(d, v, 0) = n0; vnref(d, v, 1) = n1; vnref(d, v, 2) = n2;
+            vcnref(d, v, 0) = c0; vcnref(d, v, 1) = c1; vcnref(d, v, 2) = c2;
         }
+        return;
     }
-    for (int i = threadIdx.x; i < ncell; i += blockDim.x) {
-        int c = cid[i];
-        d.cnn[c] = ccnt[i];
-        for (int k = 0; k < ccnt[i]; ++k) ringref(d, c, k) = cring[i][k];
+    const int i = item - nvert;
+    const int c = cid[i], cnt = ccnt[i];
+    int e[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) e[k] = cring[i][k];
+    auto add = [&](int x) {
+        const unsigned lx = (unsigned)(x - vo);
+        if (seen && (lx >> 5) < (unsigned)seen_words) {
+            const unsigned bit = 1u << (lx & 31);
+            if (atomicOr(&seen[lx >> 5], bit) & bit) return;
+        }
+        const int slot = atomicAdd(nverts, 1);
+        if (slot < cap) verts[slot] = x;
+    };
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (k < cnt) { ringref(d, c, k) = e[k]; add(e[k]); }
+    for (int k = 8; k < cnt; ++k) {
+        const int x = cring[i][k];
+        ringref(d, c, k) = x;
+        add(x);
     }
-}
-
-// After the items are written: vln of the vertices of modified cells and,
-// for the active updater, the directors of moved vertices (whole block).
-__device__ inline void refresh_items(const Dev& d, SysParams* S, int pcur, int nin, int nout, int dir_mode, int nvert,
-                                     const int* vid, const int* vflag, int ncell, const int* cid) {
-    for (int i = threadIdx.x; i < ncell; i += blockDim.x) {
-        int c = cid[i];
-        int n = d.cnn[c];
-        for (int k = 0; k < n; ++k)
-            if (!build_vln(d, ringref(d, c, k))) atomicMax(&S->err, 5);
-    }
-    if (!dir_mode) return;
-    for (int i = threadIdx.x; i < nvert; i += blockDim.x) {
-        if (!(vflag[i] & 1)) continue;
-        int x = vid[i];
-        double p[3] = {d.px[pcur][x], d.py[pcur][x], d.pz[pcur][x]};
-        double nI[3] = {d.nx[nin][x], d.ny[nin][x], d.nz[nin][x]}, nO[3];
-        director_update(S, x - S->vo, p, nI, nO);
-        d.nx[nout][x] = nO[0]; d.ny[nout][x] = nO[1]; d.nz[nout][x] = nO[2];
-    }
+    d.cnn[c] = cnt;
 }
 
 // Commit the dry runs of one sphere (whole block): the logs of every
 // candidate run that completed and was not flagged as overlapping, plus the
-// group run's log when there was one.  Logs are applied one per thread; then
-// vln of every vertex of a modified cell and, for the active updater, the
-// director of every moved vertex are rebuilt one vertex per thread.
+// group run's log when there was one.  These logs write disjoint vertices and
+// cells, so their items are applied in parallel, one item per thread; then
+// vln of every vertex of a written cell and, for the active updater, the
+// director of every moved vertex are rebuilt one vertex per thread.  Every
+// write is a pure function of the logs, so the result does not depend on the
+// order.
 #define SVM_COMMIT_VERTS 4096
+#define SVM_COMMIT_LOGS 1024
+#define SVM_COMMIT_SEEN 4096      // bitmap words: spheres of up to 131072 vertices are deduplicated
 __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int nin, int nout, int dir_mode,
                                      int use_group) {
     __shared__ int s_verts[SVM_COMMIT_VERTS];
     __shared__ int s_moved[SVM_COMMIT_VERTS];
-    __shared__ int s_nverts, s_nmoved;
+    __shared__ int s_logs[SVM_COMMIT_LOGS];
+    __shared__ int s_lstart[SVM_COMMIT_LOGS + 1];    // prefix sums of items per log
+    __shared__ unsigned s_seen[SVM_COMMIT_SEEN];
+    __shared__ int s_part[SVM_TILE];
+    __shared__ int s_nverts, s_nmoved, s_nlogs;
     __shared__ unsigned long long s_flips, s_skipped;
     const int t = threadIdx.x, nt = blockDim.x;
     const int gen = S->t1_gen;
     const int vo = S->vo, nv = S->nv;
     const T1GroupLog& G = d.t1_group[blockIdx.x];
+    const int seen_words = (nv + 31) >> 5;
+    const bool dedup = seen_words <= SVM_COMMIT_SEEN;
     if (t == 0) {
-        s_nverts = 0; s_nmoved = 0;
+        s_nverts = 0; s_nmoved = 0; s_nlogs = 0;
         s_flips = use_group ? G.flips : 0;
         s_skipped = (use_group ? G.skipped : 0) + S->t1_bskips;
     }
+    if (dedup)
+        for (int w = t; w < seen_words; w += nt) s_seen[w] = 0u;
     __syncthreads();
     // the flippable list's length as the dry runs saw it (the live counter is
     // reset by this kernel)
     const int total = min(*d.t1_total, SVM_T1_MAXCL);
-    auto committed = [&](int i) {
-        int row = d.t1_list[i];
-        return row >= vo && row < vo + nv && d.t1_logged[i] == gen && d.t1_flagged[i] != gen;
-    };
     for (int j = t; j < total; j += nt) {
-        if (!committed(j)) continue;
+        const int row = d.t1_list[j];
+        if (!(row >= vo && row < vo + nv && d.t1_logged[j] == gen && d.t1_flagged[j] != gen)) continue;
+        const int slot = atomicAdd(&s_nlogs, 1);
+        if (slot < SVM_COMMIT_LOGS) s_logs[slot] = j;
         const T1Log& L = d.t1_log[j];
-        for (int i = 0; i < L.nvert; ++i) {
-            int v = L.vid[i];
-            if (L.vflag[i] & 1) {
-                d.px[pcur][v] = L.vpos[i][0];
-                d.py[pcur][v] = L.vpos[i][1];
-                d.pz[pcur][v] = L.vpos[i][2];
-            }
-            if (L.vflag[i] & 2) {
-                d.vn[v] = make_int4(L.vvn[i][0], L.vvn[i][1], L.vvn[i][2], d.vn[v].w);
-                d.vcn[v] = make_int4(L.vvcn[i][0], L.vvcn[i][1], L.vvcn[i][2], d.vcn[v].w);
-            }
-        }
-        for (int i = 0; i < L.ncell; ++i) {
-            int c = L.cid[i];
-            d.cnn[c] = L.ccnt[i];
-            for (int k = 0; k < L.ccnt[i]; ++k) ringref(d, c, k) = L.cring[i][k];
-        }
         atomicAdd(&s_flips, (unsigned long long)L.flips);
         atomicAdd(&s_skipped, (unsigned long long)L.skipped);
     }
-    if (use_group) apply_items(d, pcur, G.nvert, G.vid, G.vflag, G.vpos, G.vvn, G.vvcn, G.ncell, G.cid, G.ccnt, G.cring);
-    __threadfence_block();
     __syncthreads();
-    // vertices whose vln / director must be rebuilt
-    for (int j = t; j < total + (use_group ? 1 : 0); j += nt) {
-        bool grp = j == total;
-        if (!grp && !committed(j)) continue;
-        const T1Log* L = grp ? nullptr : &d.t1_log[j];
-        int ncell = grp ? G.ncell : L->ncell;
-        for (int i = 0; i < ncell; ++i) {
-            int c = grp ? G.cid[i] : L->cid[i];
-            int n = d.cnn[c];
-            int k0 = atomicAdd(&s_nverts, n);
-            for (int k = 0; k < n; ++k)
-                if (k0 + k < SVM_COMMIT_VERTS) s_verts[k0 + k] = ringref(d, c, k);
-        }
-        if (dir_mode) {
-            int nvert = grp ? G.nvert : L->nvert;
-            for (int i = 0; i < nvert; ++i) {
-                int fl = grp ? G.vflag[i] : L->vflag[i];
-                if (!(fl & 1)) continue;
-                int k = atomicAdd(&s_nmoved, 1);
-                if (k < SVM_COMMIT_VERTS) s_moved[k] = grp ? G.vid[i] : L->vid[i];
+    const int nlogs = s_nlogs;
+    unsigned* seen = dedup ? s_seen : nullptr;
+    if (nlogs <= SVM_COMMIT_LOGS) {
+        // items per log, then an exclusive scan: each thread owns a run of
+        // SVM_COMMIT_LOGS / SVM_TILE logs, and the run totals are scanned in LDS
+        constexpr int PER = SVM_COMMIT_LOGS / SVM_TILE;
+        int cnt[PER], run = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int li = t * PER + q;
+            cnt[q] = 0;
+            if (li < nlogs) {
+                const T1Log& L = d.t1_log[s_logs[li]];
+                cnt[q] = L.nvert + L.ncell;
             }
+            run += cnt[q];
         }
+        s_part[t] = run;
+        __syncthreads();
+        for (int off = 1; off < SVM_TILE; off <<= 1) {
+            const int add = t >= off ? s_part[t - off] : 0;
+            __syncthreads();
+            s_part[t] += add;
+            __syncthreads();
+        }
+        int acc = s_part[t] - run;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int li = t * PER + q;
+            if (li < nlogs) s_lstart[li] = acc;
+            acc += cnt[q];
+        }
+        if (t == nt - 1) s_lstart[nlogs] = s_part[t];
+        __syncthreads();
+        const int nitems = s_lstart[nlogs];
+        int li = 0;
+        for (int x = t; x < nitems; x += nt) {
+            while (s_lstart[li + 1] <= x) ++li;
+            const T1Log& L = d.t1_log[s_logs[li]];
+            apply_item<SVM_T1_RC>(d, pcur, x - s_lstart[li], L.nvert, L.vid, L.vflag, L.vpos, L.vvn, L.vvcn, L.cid,
+                                  L.ccnt, L.cring, dir_mode, vo, seen, seen_words, s_verts, &s_nverts, s_moved,
+                                  &s_nmoved, SVM_COMMIT_VERTS);
+        }
+    } else {
+        // more committed logs than the list holds (not seen in practice):
+        // one log per thread, and the whole sphere is rebuilt below
+        for (int j = t; j < total; j += nt) {
+            const int row = d.t1_list[j];
+            if (!(row >= vo && row < vo + nv && d.t1_logged[j] == gen && d.t1_flagged[j] != gen)) continue;
+            const T1Log& L = d.t1_log[j];
+            for (int x = 0; x < L.nvert + L.ncell; ++x)
+                apply_item<SVM_T1_RC>(d, pcur, x, L.nvert, L.vid, L.vflag, L.vpos, L.vvn, L.vvcn, L.cid, L.ccnt,
+                                      L.cring, dir_mode, vo, nullptr, 0, s_verts, &s_nverts, s_moved, &s_nmoved, 0);
+        }
+        if (t == 0) { s_nverts = SVM_COMMIT_VERTS + 1; s_nmoved = SVM_COMMIT_VERTS + 1; }
     }
+    if (use_group) {
+        const int ng = G.nvert, nitems = ng + G.ncell;
+        for (int x = t; x < nitems; x += nt)
+            apply_item<SVM_T1_RC>(d, pcur, x, ng, G.vid, G.vflag, G.vpos, G.vvn, G.vvcn, G.cid, G.ccnt, G.cring,
+                                  dir_mode, vo, seen, seen_words, s_verts, &s_nverts, s_moved, &s_nmoved,
+                                  SVM_COMMIT_VERTS);
+    }
+    __threadfence_block();
     __syncthreads();
     if (s_nverts > SVM_COMMIT_VERTS) {
         for (int x = vo + t; x < vo + nv; x += nt)
