@@ -112,6 +112,7 @@ struct Heap {
 // ------------------------------------------------------------ accessors ---
 // Direct access to the topology and positions in HBM.
 struct GlobalAcc {
+    static constexpr bool kCached = false;
     const Dev* d;
     int pcur;
     __device__ __attribute__((always_inline)) void pos(int v, double* p) { p[0] = d->px[pcur][v]; p[1] = d->py[pcur][v]; p[2] = d->pz[pcur][v]; }
@@ -150,6 +151,7 @@ struct WaveSync {
 
 template <int WV, int WC>
 struct CacheAcc {
+    static constexpr bool kCached = true;
     const Dev* d;
     int pcur;
     bool fail;
@@ -281,6 +283,19 @@ struct CacheAcc {
         cflag[h] = 1;
     }
     __device__ __attribute__((always_inline)) bool failed() { return fail; }
+    // slot handles: one lookup (which marks the slot touched), then direct access
+    __device__ __attribute__((always_inline)) int vh(int v) { return vslot(v); }
+    __device__ __attribute__((always_inline)) int ch(int c) { return cslot(c); }
+    __device__ __attribute__((always_inline)) double* vpos_ptr(int h) { return vp + 3 * h; }
+    __device__ __attribute__((always_inline)) int* vn_ptr(int h) { return vvn + 3 * h; }
+    __device__ __attribute__((always_inline)) int* vcn_ptr(int h) { return vvcn + 3 * h; }
+    __device__ __attribute__((always_inline)) int* ring_ptr(int h) { return cring + h * SVM_T1_RC; }
+    __device__ __attribute__((always_inline)) int cnn_h(int h) { return ccnt[h]; }
+    __device__ __attribute__((always_inline)) void mark_v(int h, int f) { vflag[h] |= f; }
+    __device__ __attribute__((always_inline)) void set_cnn_h(int h, int x) {
+        if (x > SVM_T1_RC - 1) fail = true;
+        else { ccnt[h] = x; cflag[h] = 1; }
+    }
 
     // ---- cooperative parts (every lane of the wavefront)
     // Insert `key` from many lanes at once.  New entries are limited to
@@ -489,21 +504,16 @@ __device__ __attribute__((always_inline)) bool cell_vertex_set(A& a, int vertex1
     return true;
 }
 
-// One T1 flip of edge (vertex1, vertex2): the body of
-// testAndPerformT1TransitionsCPU (src/model/sphericalVertexModel.cpp:496-598).
-// v1 is the caller's copy of vertex1's position.  Returns 0, or
-// SVM_ERR_RING_OVERFLOW (nothing modified) when a cell would outgrow `cap`.
-template <class A>
-__device__ __attribute__((always_inline)) int t1_flip(A& a, int vertex1, int vertex2, const int* cs, const int* vs, double* v1, double R,
-                       int cap, int* tmp) {
-    if (a.cnn(cs[1]) + 1 > cap || a.cnn(cs[3]) + 1 > cap) return 4;
-    double v2[3], vC[3];
-    a.pos(vertex2, v2);
+// The new positions of the flipped edge's vertices v1, v2 (in place): the
+// edge turned by a quarter turn about its normalised midpoint, its vertices
+// then put back on the sphere (sphericalVertexModel.cpp:506-527); vC is vertex
+// c, whose side fixes the sense of the turn.
+__device__ __attribute__((always_inline)) inline void flip_positions(double* v1, double* v2, const double* vC,
+                                                                     double R) {
     double mid[3] = {0.5 * (v1[0] + v2[0]), 0.5 * (v1[1] + v2[1]), 0.5 * (v1[2] + v2[2])};
     double mm = 0.0; mm += mid[0] * mid[0]; mm += mid[1] * mid[1]; mm += mid[2] * mid[2];
     double im = 1.0 / __builtin_sqrt(mm);
     mid[0] = im * mid[0]; mid[1] = im * mid[1]; mid[2] = im * mid[2];
-    a.pos(vs[2], vC);
     double determinant = vC[0] * (v1[1] * v2[2] - v1[2] * v2[1])
                        + vC[1] * (v1[2] * v2[0] - v1[0] * v2[2])
                        + vC[2] * (v1[0] * v2[1] - v1[1] * v2[0]);
@@ -516,6 +526,20 @@ __device__ __attribute__((always_inline)) int t1_flip(A& a, int vertex1, int ver
     double n2 = 0.0; n2 += v2[0] * v2[0]; n2 += v2[1] * v2[1]; n2 += v2[2] * v2[2];
     double i1 = 1.0 / __builtin_sqrt(n1), i2 = 1.0 / __builtin_sqrt(n2);
     for (int k = 0; k < 3; ++k) { v1[k] = R * (i1 * v1[k]); v2[k] = R * (i2 * v2[k]); }
+}
+
+// One T1 flip of edge (vertex1, vertex2): the body of
+// testAndPerformT1TransitionsCPU (src/model/sphericalVertexModel.cpp:496-598).
+// v1 is the caller's copy of vertex1's position.  Returns 0, or
+// SVM_ERR_RING_OVERFLOW (nothing modified) when a cell would outgrow `cap`.
+template <class A>
+__device__ __attribute__((always_inline)) int t1_flip(A& a, int vertex1, int vertex2, const int* cs, const int* vs, double* v1, double R,
+                       int cap, int* tmp) {
+    if (a.cnn(cs[1]) + 1 > cap || a.cnn(cs[3]) + 1 > cap) return 4;
+    double v2[3], vC[3];
+    a.pos(vertex2, v2);
+    a.pos(vs[2], vC);
+    flip_positions(v1, v2, vC, R);
     a.set_pos(vertex1, v1);
     a.set_pos(vertex2, v2);
     for (int vert = 0; vert < 3; ++vert) {
@@ -567,6 +591,176 @@ __device__ __attribute__((always_inline)) int t1_flip(A& a, int vertex1, int ver
     return 0;
 }
 
+// ---- cached fast paths (CacheAcc only) -----------------------------------
+// The walk runs on one lane, so every dependent LDS access is a full round
+// trip.  These versions of cell_vertex_set and t1_flip look each vertex and
+// cell up once, read what they need in one batch into registers (rings of up
+// to SVM_T1_RR entries) and only then write.  They return -1, having written
+// nothing, where that batched form is not provably the same as the in-place
+// sequence (repeated vertices or cells, longer rings, a vertex not exactly
+// once in a ring); the caller then runs the generic code.
+#define SVM_T1_RR 12
+
+// (vlast, vnext) around `target` in the ring r[0..n) (e: its first entries in
+// registers), 2 <= n <= SVM_T1_RR, as the loop of getCellVertexSetForT1 finds
+// them: the first match among r[n-1], r[0], ..., r[n-2]; without a match
+// (r[n-2], r[n-1]).  The search uses constant register indices (a dynamic
+// index would put e in scratch); the two results are read from r.
+__device__ __attribute__((always_inline)) inline void ring_nbrs(const int* r, const int (&e)[SVM_T1_RR], int n,
+                                                                int target, int& vlast, int& vnext) {
+    int qlast = -1, qfirst = -1;
+#pragma unroll
+    for (int j = 0; j < SVM_T1_RR; ++j) {
+        const bool m = j < n && e[j] == target;
+        qlast = (m && j == n - 1) ? j : qlast;
+        qfirst = (m && j < n - 1 && qfirst < 0) ? j : qfirst;
+    }
+    const int q = qlast >= 0 ? qlast : qfirst;
+    const int il = q < 0 ? n - 2 : (q == 0 ? n - 1 : q - 1);
+    const int in = q < 0 ? n - 1 : (q == n - 1 ? 0 : q + 1);
+    vlast = r[il];
+    vnext = r[in];
+}
+
+__device__ __attribute__((always_inline)) inline void ring_regs(const int* r, int (&e)[SVM_T1_RR]) {
+#pragma unroll
+    for (int j = 0; j < SVM_T1_RR; ++j) e[j] = r[j];      // a slot holds SVM_T1_RC >= SVM_T1_RR entries
+}
+
+// Position of the only occurrence of x in e[0..n), or -1 (none or several).
+__device__ __attribute__((always_inline)) inline int ring_only(const int (&e)[SVM_T1_RR], int n, int x) {
+    int p = -1, cnt = 0;
+#pragma unroll
+    for (int j = 0; j < SVM_T1_RR; ++j)
+        if (j < n && e[j] == x) { p = j; ++cnt; }
+    return cnt == 1 ? p : -1;
+}
+
+template <class A>
+__device__ __attribute__((always_inline)) int cell_vertex_set_cached(A& a, int vertex1, int vertex2, int* cs,
+                                                                     int* vs) {
+    const int h1 = a.vh(vertex1), h2 = a.vh(vertex2);
+    const int* c1 = a.vcn_ptr(h1);
+    const int* c2 = a.vcn_ptr(h2);
+    const int cells[3] = {c1[0], c1[1], c1[2]};
+    const int ct[3] = {c2[0], c2[1], c2[2]};
+    cs[0] = cs[1] = cs[2] = cs[3] = -1;
+#pragma unroll
+    for (int ff = 0; ff < 3; ++ff)
+        if (ct[ff] != cells[0] && ct[ff] != cells[1] && ct[ff] != cells[2]) cs[3] = ct[ff];
+    int hc[3], n[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        hc[t] = a.ch(cells[t]);
+        n[t] = a.cnn_h(hc[t]);
+    }
+    if (n[0] < 2 || n[0] > SVM_T1_RR || n[1] < 2 || n[1] > SVM_T1_RR || n[2] < 2 || n[2] > SVM_T1_RR) return -1;
+    int nl[3], nn[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        int e[SVM_T1_RR];
+        const int* r = a.ring_ptr(hc[t]);
+        ring_regs(r, e);
+        ring_nbrs(r, e, n[t], vertex1, nl[t], nn[t]);
+        if (nl[t] == vertex2) cs[0] = cells[t];
+        else if (nn[t] == vertex2) cs[2] = cells[t];
+        else cs[1] = cells[t];
+    }
+    if (cs[0] < 0 || cs[1] < 0 || cs[2] < 0 || cs[3] < 0) return 0;
+    // cs[1] is one of the three cells: its walk around vertex1 is done
+    const int t1 = cells[2] == cs[1] ? 2 : (cells[1] == cs[1] ? 1 : 0);
+    const int h3 = a.ch(cs[3]);
+    const int n3 = a.cnn_h(h3);
+    if (n3 < 2 || n3 > SVM_T1_RR) return -1;
+    int e3[SVM_T1_RR];
+    ring_regs(a.ring_ptr(h3), e3);
+    vs[0] = t1 == 2 ? nl[2] : (t1 == 1 ? nl[1] : nl[0]);
+    vs[1] = t1 == 2 ? nn[2] : (t1 == 1 ? nn[1] : nn[0]);
+    ring_nbrs(a.ring_ptr(h3), e3, n3, vertex2, vs[3], vs[2]);
+    return 1;
+}
+
+template <class A>
+__device__ __attribute__((always_inline)) int t1_flip_cached(A& a, int vertex1, int vertex2, const int* cs,
+                                                             const int* vs, double* v1, double R, int cap) {
+    const int hc1 = a.ch(cs[1]), hc3 = a.ch(cs[3]);
+    const int n1 = a.cnn_h(hc1), n3 = a.cnn_h(hc3);
+    if (n1 + 1 > cap || n3 + 1 > cap) return 4;
+    const int h2 = a.vh(vertex2), hC = a.vh(vs[2]), h1 = a.vh(vertex1), hb = a.vh(vs[1]);
+    const int hc0 = a.ch(cs[0]), hc2 = a.ch(cs[2]);
+    const int n0 = a.cnn_h(hc0), n2 = a.cnn_h(hc2);
+    if (h1 == h2 || h1 == hb || h1 == hC || h2 == hb || h2 == hC || hb == hC || hc0 == hc1 || hc0 == hc2 ||
+        hc0 == hc3 || hc1 == hc2 || hc1 == hc3 || hc2 == hc3)
+        return -1;
+    if (n0 < 1 || n0 > SVM_T1_RR || n2 < 1 || n2 > SVM_T1_RR || n1 + 1 > SVM_T1_RR || n3 + 1 > SVM_T1_RR) return -1;
+    int e0[SVM_T1_RR], e1[SVM_T1_RR], e2[SVM_T1_RR], e3[SVM_T1_RR];
+    ring_regs(a.ring_ptr(hc0), e0);
+    ring_regs(a.ring_ptr(hc1), e1);
+    ring_regs(a.ring_ptr(hc2), e2);
+    ring_regs(a.ring_ptr(hc3), e3);
+    const int p0 = ring_only(e0, n0, vertex2), p1 = ring_only(e1, n1, vertex1);
+    const int p2 = ring_only(e2, n2, vertex1), p3 = ring_only(e3, n3, vertex2);
+    if (p0 < 0 || p1 < 0 || p2 < 0 || p3 < 0) return -1;
+    int vc1[3], vc2[3], vnb[3], vnc[3], vn1[3], vn2[3];
+    {
+        const int* q1 = a.vcn_ptr(h1); const int* q2 = a.vcn_ptr(h2);
+        const int* qb = a.vn_ptr(hb); const int* qc = a.vn_ptr(hC);
+        const int* r1 = a.vn_ptr(h1); const int* r2 = a.vn_ptr(h2);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            vc1[k] = q1[k]; vc2[k] = q2[k]; vnb[k] = qb[k]; vnc[k] = qc[k]; vn1[k] = r1[k]; vn2[k] = r2[k];
+        }
+    }
+    double v2[3], vC[3];
+    {
+        const double* q = a.vpos_ptr(h2);
+        const double* r = a.vpos_ptr(hC);
+        v2[0] = q[0]; v2[1] = q[1]; v2[2] = q[2];
+        vC[0] = r[0]; vC[1] = r[1]; vC[2] = r[2];
+    }
+    flip_positions(v1, v2, vC, R);
+    {
+        double* q = a.vpos_ptr(h1);
+        double* r = a.vpos_ptr(h2);
+        q[0] = v1[0]; q[1] = v1[1]; q[2] = v1[2];
+        r[0] = v2[0]; r[1] = v2[1]; r[2] = v2[2];
+        a.mark_v(h1, 1);
+        a.mark_v(h2, 1);
+    }
+#pragma unroll
+    for (int vert = 0; vert < 3; ++vert) {
+        if (vc1[vert] == cs[2]) { a.vcn_ptr(h1)[vert] = cs[3]; a.mark_v(h1, 2); }
+        if (vc2[vert] == cs[0]) { a.vcn_ptr(h2)[vert] = cs[1]; a.mark_v(h2, 2); }
+        if (vnb[vert] == vertex1) { a.vn_ptr(hb)[vert] = vertex2; a.mark_v(hb, 2); }
+        if (vnc[vert] == vertex2) { a.vn_ptr(hC)[vert] = vertex1; a.mark_v(hC, 2); }
+        if (vn1[vert] == vs[1]) { a.vn_ptr(h1)[vert] = vs[2]; a.mark_v(h1, 2); }
+        if (vn2[vert] == vs[2]) { a.vn_ptr(h2)[vert] = vs[1]; a.mark_v(h2, 2); }
+    }
+    // cell i loses vertex2, cell j gains vertex2 after vertex1, cell k loses
+    // vertex1, cell l gains vertex1 after vertex2
+    int* r0 = a.ring_ptr(hc0);
+    int* r1 = a.ring_ptr(hc1);
+    int* r2 = a.ring_ptr(hc2);
+    int* r3 = a.ring_ptr(hc3);
+#pragma unroll
+    for (int j = 0; j + 1 < SVM_T1_RR; ++j) {
+        if (j >= p0 && j < n0 - 1) r0[j] = e0[j + 1];
+        if (j >= p2 && j < n2 - 1) r2[j] = e2[j + 1];
+    }
+#pragma unroll
+    for (int j = 1; j < SVM_T1_RR; ++j) {
+        if (j == p1 + 1) r1[j] = vertex2;
+        else if (j > p1 + 1 && j <= n1) r1[j] = e1[j - 1];
+        if (j == p3 + 1) r3[j] = vertex1;
+        else if (j > p3 + 1 && j <= n3) r3[j] = e3[j - 1];
+    }
+    a.set_cnn_h(hc0, n0 - 1);
+    a.set_cnn_h(hc1, n1 + 1);
+    a.set_cnn_h(hc2, n2 - 1);
+    a.set_cnn_h(hc3, n3 + 1);
+    return 0;
+}
+
 struct VisitStats {
     long long flips, skipped;
 };
@@ -587,7 +781,10 @@ __device__ __attribute__((always_inline)) int visit_vertex(A& a, int gu, int u, 
             a.pos(vertex2, q);
             if (short_edge(v1, q, pnorm(v1), pnorm(q), R, thr, cos_cut)) {
                 int cs[4], vs[4] = {0, 0, 0, 0};
-                if (!cell_vertex_set(a, gu, vertex2, cs, vs)) {
+                int got = -1;
+                if constexpr (A::kCached) got = cell_vertex_set_cached(a, gu, vertex2, cs, vs);
+                if (got < 0) got = cell_vertex_set(a, gu, vertex2, cs, vs) ? 1 : 0;
+                if (!got) {
                     ++st.skipped;
                     continue;
                 }
@@ -598,7 +795,9 @@ __device__ __attribute__((always_inline)) int visit_vertex(A& a, int gu, int u, 
                 }
                 if (!inside) return 5;
                 if (a.cnn(cs[0]) == 3 || a.cnn(cs[2]) == 3) continue;
-                int r = t1_flip(a, gu, vertex2, cs, vs, v1, R, cap, tmp);
+                int r = -1;
+                if constexpr (A::kCached) r = t1_flip_cached(a, gu, vertex2, cs, vs, v1, R, cap);
+                if (r < 0) r = t1_flip(a, gu, vertex2, cs, vs, v1, R, cap, tmp);
                 if (r) return r;
                 ++st.flips;
                 on_flip(gu, vertex2, cs);
