@@ -251,7 +251,7 @@ int launch_t1(svm_context* ctx, int pcur, int nin, int nout, int dir, int advanc
     }
     if (!ctx->t1_sequential) {
         Timer t(ctx, "k_t1_group");
-        hipLaunchKernelGGL(svm::k_t1_group, dim3(ctx->nsys), dim3(64), 0, ctx->stream, ctx->d, pcur);
+        hipLaunchKernelGGL(svm::k_t1_group, dim3(ctx->nsys), dim3(SVM_T1_GT), 0, ctx->stream, ctx->d, pcur);
         int r = last_launch(ctx);
         if (r) return r;
     }

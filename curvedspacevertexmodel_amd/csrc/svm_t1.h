@@ -774,12 +774,13 @@ __device__ __attribute__((always_inline)) int visit_vertex(A& a, int gu, int u, 
                             int* tmp, VisitStats& st, Push push, OnFlip on_flip) {
     double v1[3];
     a.pos(gu, v1);
+    double n1 = pnorm(v1);       // changes only with v1, in a flip
     for (int vv = 0; vv < 3; ++vv) {
         int vertex2 = a.vn(gu, vv);
         if (gu < vertex2) {
             double q[3];
             a.pos(vertex2, q);
-            if (short_edge(v1, q, pnorm(v1), pnorm(q), R, thr, cos_cut)) {
+            if (short_edge(v1, q, n1, pnorm(q), R, thr, cos_cut)) {
                 int cs[4], vs[4] = {0, 0, 0, 0};
                 int got = -1;
                 if constexpr (A::kCached) got = cell_vertex_set_cached(a, gu, vertex2, cs, vs);
@@ -799,6 +800,7 @@ __device__ __attribute__((always_inline)) int visit_vertex(A& a, int gu, int u, 
                 if constexpr (A::kCached) r = t1_flip_cached(a, gu, vertex2, cs, vs, v1, R, cap);
                 if (r < 0) r = t1_flip(a, gu, vertex2, cs, vs, v1, R, cap, tmp);
                 if (r) return r;
+                n1 = pnorm(v1);
                 ++st.flips;
                 on_flip(gu, vertex2, cs);
                 int dirty[6] = {u, vertex2 - vo, vs[0] - vo, vs[1] - vo, vs[2] - vo, vs[3] - vo};
@@ -1292,14 +1294,16 @@ struct BlockSync {
     __device__ void operator()() const { __syncthreads(); }
 };
 
-// Second round, block (one wavefront) per sphere whose single-candidate runs
+#define SVM_T1_GT 256         // threads of a group block
+
+// Second round, block per sphere whose single-candidate runs
 // overlapped: one dry run of the sequential loop over all the flagged
 // candidates together (in index order, with their disturbed vertices), in a
 // larger cache.  Touching an item of a run that was not flagged means that
 // run interacts: it is absorbed into the group and the group retries (the
 // cache keeps what it loaded).  Out of room, or too many retries, and the
 // sphere takes the sequential pass.
-__global__ __launch_bounds__(64) void k_t1_group(Dev d, int pcur) {
+__global__ __launch_bounds__(SVM_T1_GT) void k_t1_group(Dev d, int pcur) {
     __shared__ int s_vkey[SVM_T1_GV], s_vflag[SVM_T1_GV], s_vuse[SVM_T1_GV];
     __shared__ double s_vp[SVM_T1_GV * 3];
     __shared__ int s_vvn[SVM_T1_GV * 3], s_vvcn[SVM_T1_GV * 3];
@@ -1313,7 +1317,7 @@ __global__ __launch_bounds__(64) void k_t1_group(Dev d, int pcur) {
     __shared__ long long s_flips, s_skipped;
     const int s = blockIdx.x;
     SysParams* S = d.sys + s;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x;     // the walk runs on thread 0; the rest is shared by the block
     const BlockSync sync;
     T1GroupLog& G = d.t1_group[s];
     if (!S->t1_overlap || S->t1_overflow) {
@@ -1329,7 +1333,7 @@ __global__ __launch_bounds__(64) void k_t1_group(Dev d, int pcur) {
     const int total = min(*d.t1_count, SVM_T1_MAXCL);
     if (lane == 0) s_nmem = 0;
     __syncthreads();
-    for (int i = lane; i < total; i += 64) {
+    for (int i = lane; i < total; i += SVM_T1_GT) {
         int row = d.t1_list[i];
         if (row >= vo && row < vo + nv && d.t1_flagged[i] == gen) {
             int k = atomicAdd(&s_nmem, 1);
@@ -1337,7 +1341,7 @@ __global__ __launch_bounds__(64) void k_t1_group(Dev d, int pcur) {
         }
     }
     const int btotal = min(*d.t1_bcount, SVM_T1_MAXBL);
-    for (int i = lane; i < btotal; i += 64) {
+    for (int i = lane; i < btotal; i += SVM_T1_GT) {
         int row = d.t1_blist[i];
         if (row >= vo && row < vo + nv && d.t1_bflag[i] == gen) {
             int k = atomicAdd(&s_nmem, 1);
@@ -1352,12 +1356,12 @@ __global__ __launch_bounds__(64) void k_t1_group(Dev d, int pcur) {
     }
     CacheAcc<SVM_T1_GV, SVM_T1_GC> a{&d, pcur, false, s_vkey, s_vflag, s_vuse, s_vp, s_vvn, s_vvcn,
                                      s_ckey, s_cflag, s_cuse, s_ccnt, s_cring, &s_vfill, &s_cfill};
-    a.reset(lane, 64);
+    a.reset(lane, SVM_T1_GT);
     if (lane == 0) G.ok = 0;
     __syncthreads();
     int seeded = 0;
     for (int attempt = 0; attempt < 8; ++attempt) {
-        a.prefetch(s_mem + seeded, nmem - seeded, vo, lane, 64, SVM_T1_GV * 5 / 8, SVM_T1_GC * 5 / 8, sync);
+        a.prefetch(s_mem + seeded, nmem - seeded, vo, lane, SVM_T1_GT, SVM_T1_GV * 5 / 8, SVM_T1_GC * 5 / 8, sync);
         seeded = nmem;
         if (lane == 0) {
             for (int i = 1; i < nmem; ++i) {          // insertion sort
@@ -1404,12 +1408,12 @@ __global__ __launch_bounds__(64) void k_t1_group(Dev d, int pcur) {
             if (lane == 0) S->n_seq_reason[2] += 1;   // out of room (or a ring overflow)
             return;
         }
-        a.check_touched(gen, d.t1_flagged, s_absorb, &s_nabs, SVM_T1_GABS, lane, 64);
+        a.check_touched(gen, d.t1_flagged, s_absorb, &s_nabs, SVM_T1_GABS, lane, SVM_T1_GT);
         __syncthreads();
         const int nabs = s_nabs;
         if (nabs == 0) {
             const bool fit = a.template write_log<SVM_T1_GV, SVM_T1_GC>(G.vid, G.vflag, G.vpos, G.vvn, G.vvcn, G.cid,
-                                                                        G.ccnt, G.cring, &s_nlv, &s_nlc, lane, 64,
+                                                                        G.ccnt, G.cring, &s_nlv, &s_nlc, lane, SVM_T1_GT,
                                                                         sync);
             if (lane == 0 && fit) {
                 G.nvert = s_nlv;
@@ -1438,7 +1442,7 @@ __global__ __launch_bounds__(64) void k_t1_group(Dev d, int pcur) {
         __syncthreads();
         if (s_status) return;
         nmem = s_nmem;
-        a.restore(lane, 64);
+        a.restore(lane, SVM_T1_GT);
         __syncthreads();
     }
     if (lane == 0) S->n_seq_reason[2] += 1;
