@@ -68,6 +68,7 @@ struct Dev {
     int32_t* cnn;
     int32_t* ring;
     double *A, *P, *cx, *cy, *cz, *a0, *p0;
+    double* ctrig;                // cell centres' cosT, sinT, cosP, sinP: four planes of Cs
     // vertices
     double *px[2], *py[2], *pz[2];
     double *nx[2], *ny[2], *nz[2];

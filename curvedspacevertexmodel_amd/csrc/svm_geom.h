@@ -109,6 +109,79 @@ SVM_HD void grad_triangle(const PQT<T>& a, const PQT<T>& b, const PQT<T>& c, con
     }
 }
 
+// Terms of the pair (a, b) shared by the geodesic and triangle gradients at a
+// (sphericalDomain.h:248-273, 347-411): sin and cos of phi_a - phi_b, the
+// cosine s of the angle between a and b, and dd = cosT_a cos(phi_a - phi_b)
+// sinT_b - cosT_b sinT_a, in the reference's operation order.
+template <class T>
+struct PairT {
+    T sinPd, cosPd, s, dd;
+};
+template <class T>
+SVM_HD PairT<T> pair_terms(const PQT<T>& a, const PQT<T>& b) {
+    PairT<T> p;
+    p.sinPd = a.sinP * b.cosP - a.cosP * b.sinP;
+    p.cosPd = a.cosP * b.cosP + a.sinP * b.sinP;
+    p.s = a.cosT * b.cosT + p.cosPd * a.sinT * b.sinT;
+    p.dd = a.cosT * p.cosPd * b.sinT - b.cosT * a.sinT;
+    return p;
+}
+
+// grad_geodesic with the pair terms supplied.
+template <class T>
+SVM_HD void grad_geodesic_p(const PQT<T>& a, const PQT<T>& b, const PairT<T>& P, const T* tH, const T* pH, T* d) {
+    T denomInverse = T(1.) / vsqrt(T(1) - P.s * P.s);
+    T gradTheta = T(1.0) * (b.cosT * a.sinT - a.cosT * P.cosPd * b.sinT) * denomInverse;
+    T gradPhi = b.sinT * P.sinPd * denomInverse;
+    d[0] = gradTheta * tH[0] + gradPhi * pH[0];
+    d[1] = gradTheta * tH[1] + gradPhi * pH[1];
+    d[2] = gradTheta * tH[2] + gradPhi * pH[2];
+}
+
+// grad_triangle with the (a, b) and (a, c) pair terms supplied (pair_terms):
+// the same expression trees, each pair evaluated once per vertex.
+template <class T>
+SVM_HD void grad_triangle_p(const PQT<T>& a, const PQT<T>& b, const PQT<T>& c, const PairT<T>& P12,
+                            const PairT<T>& P13, const T* tH, const T* pH, T radius, T* d) {
+    T cosT1 = a.cosT, cosT2 = b.cosT, cosT3 = c.cosT;
+    T sinT1 = a.sinT, sinT2 = b.sinT, sinT3 = c.sinT;
+    T cosP1 = a.cosP, cosP2 = b.cosP, cosP3 = c.cosP;
+    T sinP1 = a.sinP, sinP2 = b.sinP, sinP3 = c.sinP;
+    T sinP1MinusP2 = P12.sinPd;
+    T sinP1MinusP3 = P13.sinPd;
+    T cosP2MinusP3 = cosP2 * cosP3 + sinP2 * sinP3;
+    T s12 = P12.s;
+    T s13 = P13.s;
+    T s23 = cosT2 * cosT3 + cosP2MinusP3 * sinT2 * sinT3;
+    T d12 = P12.dd;
+    T d13 = P13.dd;
+    T g = T(1) - s12 * s12 - s13 * s13 - s23 * s23 + T(2.0) * s12 * s13 * s23;
+    T denom1 = vsqrt((T(-1.) + s12 * s12) * (T(-1.) + s12 * s12) * g);
+    T denom2 = vsqrt((T(-1.) + s13 * s13) * (T(-1.) + s13 * s13) * g);
+    T denom3 = vsqrt((s12 * s12 - T(1.)) * (s13 * s13 - T(1.)) * (s12 * s12 - T(1.)) * (s13 * s13 - T(1.)) * g);
+    T gradTheta = (d13 * (s12 * s12 - T(1.0)) + d12 * (s23 - s12 * s13)) / denom1
+                + (d12 * (s13 * s13 - T(1.0)) + d13 * (s23 - s12 * s13)) / denom2
+                - (d12 * (s13 * s13 - T(1.)) * (s13 - s12 * s23) + d13 * (s12 * s12 - T(1.0)) * (s12 - s13 * s23)) /
+                      denom3;
+    gradTheta *= radius;
+    T gradPhi = ((s12 * s13 - s23) * sinP1MinusP2 * sinT2 - (T(-1) + s12 * s12) * sinP1MinusP3 * sinT3) / (denom1)
+              - ((T(-1) + s13 * s13) * sinP1MinusP2 * sinT2 + (-(s12 * s13) + s23) * sinP1MinusP3 * sinT3) / (denom2)
+              + ((T(-1) + s13 * s13) * (s13 - s12 * s23) * sinP1MinusP2 * sinT2 + (T(-1) + s12 * s12) * (s12 - s13 * s23) *
+                 sinP1MinusP3 * sinT3) / (denom3);
+    gradPhi *= radius;
+    T determinant = a.x * (b.y * c.z - b.z * c.y)
+                  + a.y * (b.z * c.x - b.x * c.z)
+                  + a.z * (b.x * c.y - b.y * c.x);
+    d[0] = gradTheta * tH[0] + gradPhi * pH[0];
+    d[1] = gradTheta * tH[1] + gradPhi * pH[1];
+    d[2] = gradTheta * tH[2] + gradPhi * pH[2];
+    if (determinant < 0) {
+        d[0] = T(-1.) * d[0];
+        d[1] = T(-1.) * d[1];
+        d[2] = T(-1.) * d[2];
+    }
+}
+
 // Force contribution of one (vertex, cell) pair: the body of the reference's
 // per-cell loop (sphericalVertexModel.cpp:386-413).  Returns false when the
 // contribution is NaN and must be skipped.

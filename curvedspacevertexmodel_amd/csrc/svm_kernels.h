@@ -125,6 +125,13 @@ __global__ __launch_bounds__(SVM_TILE) void k_geometry(Dev d, int pin, int skip_
         d.cx[c] = cxv;
         d.cy[c] = cyv;
         d.cz[c] = czv;
+        {   // the centre's spherical-coordinate terms, for every vertex of the cell
+            const PQ q = make_pq<double>(cxv, cyv, czv);
+            d.ctrig[c] = q.cosT;
+            d.ctrig[(size_t)Cs + c] = q.sinT;
+            d.ctrig[2 * (size_t)Cs + c] = q.cosP;
+            d.ctrig[3 * (size_t)Cs + c] = q.sinP;
+        }
 
         int il = rc[(size_t)(n - 3) * Cs], ic = rc[(size_t)(n - 2) * Cs], in = rc[(size_t)(n - 1) * Cs];
         double pl[3] = {X[il], Y[il], Z[il]};
@@ -248,6 +255,71 @@ __device__ inline int vertex_force(const Dev& d, const SysParams* S, const doubl
         }
     }
     f[0] = (double)f0; f[1] = (double)f1; f[2] = (double)f2;
+    return nan;
+}
+
+// The parity path (T = double) of vertex_force, arranged for fewer
+// operations with every value computed by the same expression as above: the
+// vertex's basis and its pair terms with each neighbour are evaluated once
+// (each neighbour's arc gradient serves two cells), and the cell centres'
+// spherical-coordinate terms come from the geometry pass.
+template <>
+__device__ inline int vertex_force<double>(const Dev& d, const SysParams* S, const double* X, const double* Y,
+                                           const double* Z, int v, double* f) {
+    const double R = S->R, Kr = S->kr;
+    const PQ qs = pq_at<double>(X, Y, Z, v);
+    const int4 nb = d.vn[v];
+    const int4 cl = d.vcn[v];
+    const int ln = d.vln[v];
+    const PQ q0 = pq_at<double>(X, Y, Z, nb.x);
+    const PQ q1 = pq_at<double>(X, Y, Z, nb.y);
+    const PQ q2 = pq_at<double>(X, Y, Z, nb.z);
+    double tH[3], pH[3];
+    basis(qs, tH, pH);
+    const PairT<double> w0 = pair_terms(qs, q0), w1 = pair_terms(qs, q1), w2 = pair_terms(qs, q2);
+    double g0[3], g1[3], g2[3];
+    grad_geodesic_p(qs, q0, w0, tH, pH, g0);
+    grad_geodesic_p(qs, q1, w1, tH, pH, g1);
+    grad_geodesic_p(qs, q2, w2, tH, pH, g2);
+    const size_t Cs = (size_t)d.Cs;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    int nan = 0;
+#pragma unroll
+    for (int sl = 0; sl < 3; ++sl) {
+        const int c = sl == 0 ? cl.x : (sl == 1 ? cl.y : cl.z);
+        const int ls = (ln >> (4 * sl)) & 3;
+        const int ns = (ln >> (4 * sl + 2)) & 3;
+        const double areaDifference = d.A[c] - d.a0[c];
+        const double perimeterDifference = d.P[c] - d.p0[c];
+        PQ qc;
+        qc.x = d.cx[c]; qc.y = d.cy[c]; qc.z = d.cz[c];
+        qc.cosT = d.ctrig[c]; qc.sinT = d.ctrig[Cs + c]; qc.cosP = d.ctrig[2 * Cs + c]; qc.sinP = d.ctrig[3 * Cs + c];
+        const PairT<double> wc = pair_terms(qs, qc);
+        const PQ& ql = pick3(ls, q0, q1, q2);
+        const PQ& qn = pick3(ns, q0, q1, q2);
+        const PairT<double>& wl = ls == 0 ? w0 : (ls == 1 ? w1 : w2);
+        const PairT<double>& wn = ns == 0 ? w0 : (ns == 1 ? w1 : w2);
+        const double* gl = ls == 0 ? g0 : (ls == 1 ? g1 : g2);
+        const double* gn = ns == 0 ? g0 : (ns == 1 ? g1 : g2);
+        // pair_force's sequence (sphericalVertexModel.cpp:386-413)
+        double fSet[3] = {0.0, 0.0, 0.0}, t[3];
+        const double cp = 2.0 * perimeterDifference;
+        const double ca = 2.0 * Kr * areaDifference;
+        fSet[0] -= cp * gl[0]; fSet[1] -= cp * gl[1]; fSet[2] -= cp * gl[2];
+        fSet[0] -= cp * gn[0]; fSet[1] -= cp * gn[1]; fSet[2] -= cp * gn[2];
+        grad_triangle_p(qs, ql, qc, wl, wc, tH, pH, R, t);
+        fSet[0] -= ca * t[0]; fSet[1] -= ca * t[1]; fSet[2] -= ca * t[2];
+        grad_triangle_p(qs, qc, qn, wc, wn, tH, pH, R, t);
+        fSet[0] -= ca * t[0]; fSet[1] -= ca * t[1]; fSet[2] -= ca * t[2];
+        if (!(fSet[0] != fSet[0])) {
+            f0 = f0 + fSet[0];
+            f1 = f1 + fSet[1];
+            f2 = f2 + fSet[2];
+        } else {
+            ++nan;
+        }
+    }
+    f[0] = f0; f[1] = f1; f[2] = f2;
     return nan;
 }
 
