@@ -37,6 +37,7 @@ EXPORTS = [
     "svm_cell_list", "svm_mlp_create", "svm_mlp_destroy", "svm_mlp_set_parameters", "svm_mlp_get_parameters",
     "svm_mlp_forward", "svm_mlp_gradient", "svm_mlp_train", "svm_mlp_time_forward", "svm_cell_descriptors",
     "svm_warm_start", "svm_fire_set_max_iterations", "svm_set_tuning", "svm_get_tuning", "svm_autotune",
+    "svm_adam_set_parameters", "svm_adam_minimize", "svm_adam_get_state", "svm_adam_set_max_iterations",
 ]
 
 
@@ -57,6 +58,16 @@ class FireState(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("n_since_negative_power", C.c_int32),
                 ("converged", C.c_int32), ("reserved", C.c_int32), ("dt", C.c_double),
                 ("alpha", C.c_double), ("force_max", C.c_double), ("power", C.c_double)]
+
+
+class AdamParams(C.Structure):
+    _fields_ = [("beta1", C.c_double), ("beta2", C.c_double), ("epsilon", C.c_double), ("dt", C.c_double),
+                ("force_cutoff", C.c_double), ("max_iterations", C.c_int32), ("reserved", C.c_int32)]
+
+
+class AdamState(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("force_max", C.c_double),
+                ("beta1t", C.c_double), ("beta2t", C.c_double)]
 
 
 _LIB = None
@@ -413,6 +424,24 @@ class Context:
         st = FireState()
         self._ck(lib().svm_fire_get_state(self.h, C.c_int32(sys), C.byref(st)), "svm_fire_get_state")
         return {k: getattr(st, k) for k, _ in FireState._fields_ if k != "reserved"}
+
+    def adam_set_parameters(self, sys, beta1=0.9, beta2=0.99, epsilon=1e-8, dt=1e-4, force_cutoff=1e-12,
+                            max_iterations=1000):
+        """setAdamParameters + setMaximumIterations (defaults as energyMinimizerAdam.h:19); resets the moments."""
+        p = AdamParams(beta1, beta2, epsilon, dt, force_cutoff, max_iterations, 0)
+        self._ck(lib().svm_adam_set_parameters(self.h, C.c_int32(sys), C.byref(p)), "svm_adam_set_parameters")
+
+    def adam_minimize(self, check_every=50):
+        self._ck(lib().svm_adam_minimize(self.h, C.c_int32(check_every)), "svm_adam_minimize")
+
+    def adam_set_max_iterations(self, sys, max_iterations):
+        self._ck(lib().svm_adam_set_max_iterations(self.h, C.c_int32(sys), C.c_int32(max_iterations)),
+                 "svm_adam_set_max_iterations")
+
+    def adam_state(self, sys):
+        st = AdamState()
+        self._ck(lib().svm_adam_get_state(self.h, C.c_int32(sys), C.byref(st)), "svm_adam_get_state")
+        return {k: getattr(st, k) for k, _ in AdamState._fields_}
 
     def synchronize(self):
         self._ck(lib().svm_synchronize(self.h), "svm_synchronize")

@@ -39,6 +39,7 @@ struct svm_context {
     std::vector<svm_system_desc> desc;
     std::vector<SysParams> layout;   // sizes/offsets (host copy of the constant part)
     std::vector<int> have_topology;
+    std::vector<int> have_adam;       // svm_adam_set_parameters called for the sphere
     int Cs = 0, Vs = 0, cap = 0, nct = 0, nvt = 0;
     int cluster_blocks = 256;        // T1 dry-run grid (persistent loop over candidates)
     int graph_steps = 10;            // steps per captured graph when the caller passes 0
@@ -299,6 +300,22 @@ int enqueue_fire_iteration(svm_context* ctx) {
     return last_launch(ctx);
 }
 
+// One Adam iteration: geometry, forces + moment update + move + reductions,
+// T1, then the per-sphere controller.
+int enqueue_adam_iteration(svm_context* ctx) {
+    int r = launch_geometry(ctx, 1);
+    if (r) return r;
+    const int pout = 1 - ctx->pcur;
+    r = launch_vertex<svm::VM_ADAM>(ctx, pout, 0, "k_vertex_adam");
+    if (r) return r;
+    r = launch_t1(ctx, pout, ctx->ncur, ctx->ncur, 0, 0, 1);
+    if (r) return r;
+    ctx->pcur = pout;
+    Timer t(ctx, "k_adam_control");
+    hipLaunchKernelGGL(svm::k_adam_control, dim3(ctx->nsys), dim3(SVM_TILE), 0, ctx->stream, ctx->d);
+    return last_launch(ctx);
+}
+
 int require_topology(svm_context* ctx) {
     for (int s = 0; s < ctx->nsys; ++s)
         if (!ctx->have_topology[s]) return fail(ctx, SVM_ERR_STATE, "sphere " + std::to_string(s) + " has no topology");
@@ -537,6 +554,7 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
     ctx->desc.assign(systems, systems + n_systems);
     ctx->layout.resize(n_systems);
     ctx->have_topology.assign(n_systems, 0);
+    ctx->have_adam.assign(n_systems, 0);
     ctx->triangles.resize(n_systems);
     int cap = 0;
     long long Cs = 0, Vs = 0;
@@ -1382,6 +1400,95 @@ int svm_fire_minimize(svm_context* ctx, int32_t check_every) {
         if (r) return r;
     }
     return svm_synchronize(ctx);
+}
+
+int svm_adam_set_parameters(svm_context* ctx, int32_t sys, const svm_adam_params* p) {
+    if (!valid_sys(ctx, sys) || !p || !(p->beta1 < 1.0) || !(p->beta2 < 1.0) || p->max_iterations < 0)
+        return fail(ctx, SVM_ERR_ARG, "bad argument");
+    Dev& d = ctx->d;
+    const size_t Vs = ctx->Vs;
+    if (!d.am) {
+        int r = dalloc(ctx, &d.am, 3 * Vs);
+        if (!r) r = dalloc(ctx, &d.av, 3 * Vs);
+        if (r) return r;
+    }
+    std::vector<SysParams> s;
+    int r = download_sys(ctx, s);
+    if (r) return r;
+    SysParams& S = s[sys];
+    S.a_beta1 = p->beta1;
+    S.a_beta2 = p->beta2;
+    S.a_eps = p->epsilon;
+    S.a_dt = p->dt;
+    S.a_cutoff = p->force_cutoff;
+    S.a_max_iter = p->max_iterations;
+    S.a_beta1t = p->beta1;
+    S.a_beta2t = p->beta2;
+    S.a_iter = 0;
+    S.a_force_max = 110.0;
+    const SysParams& L = ctx->layout[sys];
+    for (int k = 0; k < 3; ++k) {
+        HIPCHECK(ctx, hipMemsetAsync(d.am + k * Vs + L.vo, 0, sizeof(double) * L.nv, ctx->stream));
+        HIPCHECK(ctx, hipMemsetAsync(d.av + k * Vs + L.vo, 0, sizeof(double) * L.nv, ctx->stream));
+    }
+    HIPCHECK(ctx, hipMemcpyAsync(d.sys + sys, &S, sizeof(SysParams), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->have_adam[sys] = 1;
+    return SVM_OK;
+}
+
+int svm_adam_minimize(svm_context* ctx, int32_t check_every) {
+    if (!ctx) return SVM_ERR_ARG;
+    int r = require_topology(ctx);
+    if (r) return r;
+    for (int i = 0; i < ctx->nsys; ++i)
+        if (!ctx->have_adam[i]) return fail(ctx, SVM_ERR_STATE, "sphere " + std::to_string(i) + " has no Adam parameters");
+    if (check_every <= 0) check_every = 50;
+    // minimize(): forceMax = 110, then iterate while (iterations < max &&
+    // forceMax > cutoff) (energyMinimizerAdam.cpp:87-98)
+    std::vector<SysParams> s;
+    r = download_sys(ctx, s);
+    if (r) return r;
+    for (int i = 0; i < ctx->nsys; ++i) {
+        s[i].a_force_max = 110.0;
+        s[i].f_frozen = !((s[i].a_iter < s[i].a_max_iter) && (110.0 > s[i].a_cutoff));
+        r = set_field(ctx, i, offsetof(SysParams, a_force_max), s[i].a_force_max);
+        r = r ? r : set_field(ctx, i, offsetof(SysParams, f_frozen), s[i].f_frozen);
+        if (r) return r;
+    }
+    while (true) {
+        bool any = false;
+        for (int i = 0; i < ctx->nsys; ++i) any |= !s[i].f_frozen;
+        if (!any) break;
+        for (int k = 0; k < check_every; ++k) {
+            r = enqueue_adam_iteration(ctx);
+            if (r) return r;
+        }
+        r = download_sys(ctx, s);
+        if (r) return r;
+        r = check_device_errors(ctx);
+        if (r) return r;
+    }
+    return svm_synchronize(ctx);
+}
+
+int svm_adam_set_max_iterations(svm_context* ctx, int32_t sys, int32_t max_iterations) {
+    if (!valid_sys(ctx, sys) || max_iterations < 0) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    return set_field(ctx, sys, offsetof(SysParams, a_max_iter), (int32_t)max_iterations);
+}
+
+int svm_adam_get_state(svm_context* ctx, int32_t sys, svm_adam_state* st) {
+    if (!valid_sys(ctx, sys) || !st) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    std::vector<SysParams> s;
+    int r = download_sys(ctx, s);
+    if (r) return r;
+    const SysParams& S = s[sys];
+    st->iterations = S.a_iter;
+    st->converged = S.a_force_max <= S.a_cutoff;
+    st->force_max = S.a_force_max;
+    st->beta1t = S.a_beta1t;
+    st->beta2t = S.a_beta2t;
+    return SVM_OK;
 }
 
 int svm_fire_get_state(svm_context* ctx, int32_t sys, svm_fire_state* st) {

@@ -43,6 +43,10 @@ struct SysParams {
     // FIRE state
     double f_dt, f_alpha, f_force_max, f_power;
     int32_t f_iter, f_nsince, f_frozen, f_active;
+    // Adam (energyMinimizerAdam): parameters and state; f_frozen is the
+    // minimiser's freeze flag for either minimiser
+    double a_beta1, a_beta2, a_eps, a_dt, a_cutoff, a_beta1t, a_beta2t, a_force_max;
+    int32_t a_iter, a_max_iter;
     // deferred FIRE velocity mixing
     double mix_alpha, mix_scaling;
     int32_t mix_pending, mix_zero;
@@ -73,6 +77,7 @@ struct Dev {
     double *px[2], *py[2], *pz[2];
     double *nx[2], *ny[2], *nz[2];
     double *vx, *vy, *vz;
+    double *am, *av;              // Adam moment estimates: three planes of Vs each (null until Adam is set up)
     double *fx, *fy, *fz;
     int4* vcn;
     int4* vn;

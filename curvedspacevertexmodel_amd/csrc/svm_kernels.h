@@ -36,7 +36,7 @@ __device__ inline int xcd_tile(const Dev& d, int ntiles) {
 }
 
 
-enum VertexMode { VM_FORCE = 0, VM_BROWNIAN = 1, VM_ACTIVE = 2, VM_FIRE = 3, VM_FORCE_F32 = 4 };
+enum VertexMode { VM_FORCE = 0, VM_BROWNIAN = 1, VM_ACTIVE = 2, VM_FIRE = 3, VM_FORCE_F32 = 4, VM_ADAM = 5 };
 
 // ------------------------------------------------------------ reductions --
 // Halving tree over the 256 values of a block (s[i] += s[i+h], h = 128..1).
@@ -397,6 +397,10 @@ __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, i
     const double* X = d.px[pin];
     const double* Y = d.py[pin];
     const double* Z = d.pz[pin];
+    if (MODE == VM_ADAM && S->f_frozen) {   // a converged sphere keeps its positions
+        if (valid) { d.px[pout][v] = X[v]; d.py[pout][v] = Y[v]; d.pz[pout][v] = Z[v]; }
+        return;
+    }
     double f[3] = {0.0, 0.0, 0.0};
     double tp = 0.0, tf = 0.0, tv = 0.0;
     int nan = 0;
@@ -426,6 +430,30 @@ __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, i
             d.py[pout][v] = p[1];
             d.pz[pout][v] = p[2];
         }
+        if (MODE == VM_ADAM) {
+            // adamStepCPU (src/updaters/energyMinimizerAdam.cpp:59-80), f the negative gradient
+            const size_t Vs = (size_t)d.Vs;
+            const double beta1 = S->a_beta1, beta2 = S->a_beta2;
+            const double ic1 = 1.0 / (1.0 - S->a_beta1t), ic2 = 1.0 / (1.0 - S->a_beta2t);
+            double disp[3];
+            for (int k = 0; k < 3; ++k) {
+                const double mm = beta1 * d.am[k * Vs + v] + (beta1 - 1) * f[k];
+                const double vv = beta2 * d.av[k * Vs + v] + (1 - beta2) * (f[k] * f[k]);
+                d.am[k * Vs + v] = mm;
+                d.av[k * Vs + v] = vv;
+                const double mc = ic1 * mm;
+                const double vc = ic2 * vv;
+                double rootvc = __builtin_sqrt(vc);
+                if (rootvc == 0) rootvc = S->a_eps;
+                disp[k] = -S->a_dt * mc / (rootvc);
+            }
+            tf = dot0(f, f);
+            double p[3] = {X[v], Y[v], Z[v]};
+            move_on_sphere(p, disp, S->R);
+            d.px[pout][v] = p[0];
+            d.py[pout][v] = p[1];
+            d.pz[pout][v] = p[2];
+        }
         if (MODE == VM_FIRE) {
             double dt = S->f_dt;
             double h = (0.5 / 1.0) * dt;
@@ -439,6 +467,10 @@ __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, i
         }
     }
     if (nan) atomicAdd((unsigned long long*)&S->n_nan, (unsigned long long)nan);
+    if (MODE == VM_ADAM) {
+        double sf = block_tree256(tf, red);
+        if (threadIdx.x == 0) d.vpart[3 * tile + 1] = sf;
+    }
     if (MODE == VM_FIRE) {
         double sp = block_tree256(tp, red);
         double sf = block_tree256(tf, red);
@@ -492,6 +524,25 @@ __global__ __launch_bounds__(SVM_TILE) void k_fire_control(Dev d) {
         S->f_force_max = forceMax;
         S->f_power = Power;
         S->f_frozen = !((S->f_iter < S->f_max_iter) && (forceMax > S->f_cutoff));
+    }
+}
+
+// The end of an Adam iteration (energyMinimizerAdam.cpp:79-82, :90-95), one
+// block per sphere, after k_vertex<VM_ADAM> wrote the per-tile partial sums
+// of F.F: forceMax = sqrt(sum F^2) / N, the bias-correction powers, and the
+// loop condition (iterations < max and forceMax > cutoff).
+__global__ __launch_bounds__(SVM_TILE) void k_adam_control(Dev d) {
+    __shared__ double buf[SVM_MAX_TILES];
+    SysParams* S = d.sys + blockIdx.x;
+    if (S->f_frozen) return;
+    double forceNorm = fold_lds(d.vpart + 3 * S->vtile0 + 1, 3, S->nvtiles, buf);
+    if (threadIdx.x == 0) {
+        double forceMax = __builtin_sqrt(forceNorm) / (double)S->nv;
+        S->a_iter += 1;
+        S->a_force_max = forceMax;
+        S->a_beta1t *= S->a_beta1;
+        S->a_beta2t *= S->a_beta2;
+        S->f_frozen = !((S->a_iter < S->a_max_iter) && (forceMax > S->a_cutoff));
     }
 }
 

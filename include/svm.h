@@ -75,6 +75,21 @@ typedef struct {
     double dt, alpha, force_max, power;
 } svm_fire_state;
 
+/* Adam minimiser parameters (energyMinimizerAdam::setAdamParameters and
+ * setMaximumIterations, src/updaters/energyMinimizerAdam.h:19-31; the
+ * reference's defaults are beta1 0.9, beta2 0.99, epsilon 1e-8, dt 1e-4,
+ * force_cutoff 1e-12). */
+typedef struct {
+    double beta1, beta2, epsilon, dt, force_cutoff;
+    int32_t max_iterations, reserved;
+} svm_adam_params;
+
+/* Adam state of one sphere (energyMinimizerAdam.h:33-62). */
+typedef struct {
+    int32_t iterations, converged;
+    double force_max, beta1t, beta2t;
+} svm_adam_state;
+
 /* ---- lifetime ---------------------------------------------------------- */
 /* Allocate device storage for n_systems spheres on the current HIP device. */
 int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** out);
@@ -235,6 +250,22 @@ int svm_fire_minimize(svm_context* ctx, int32_t check_every);
  * velocities) is kept, so a later svm_fire_minimize continues. */
 int svm_fire_set_max_iterations(svm_context* ctx, int32_t sys, int32_t max_iterations);
 int svm_fire_get_state(svm_context* ctx, int32_t sys, svm_fire_state* state);
+
+/* Adam (energyMinimizerAdam, src/updaters/energyMinimizerAdam.cpp:7-103; the
+ * CPU step is the reference).  svm_adam_set_parameters is setAdamParameters +
+ * setMaximumIterations + initializeFromModel for sphere `sys`: moments
+ * zeroed, iterations 0, beta1t = beta1, beta2t = beta2.  svm_adam_minimize
+ * runs minimize() on every sphere: each iteration computes the forces, moves
+ * every vertex by -dt mc / sqrt(vc) per component and enforces the topology,
+ * until iterations reach the limit or forceMax = sqrt(sum F^2) / N drops to
+ * the cutoff; the host checks every `check_every` iterations.  Iterations and
+ * moments persist across calls, as in the reference.  FIRE and Adam share
+ * the minimiser's freeze flag: run one of them at a time. */
+int svm_adam_set_parameters(svm_context* ctx, int32_t sys, const svm_adam_params* params);
+int svm_adam_minimize(svm_context* ctx, int32_t check_every);
+/* setMaximumIterations: the limit only; moments and iterations are kept. */
+int svm_adam_set_max_iterations(svm_context* ctx, int32_t sys, int32_t max_iterations);
+int svm_adam_get_state(svm_context* ctx, int32_t sys, svm_adam_state* state);
 
 /* T1 path: 0 (default) resolves each step's flips by per-candidate dry runs,
  * a group run for overlapping candidates, and the sequential pass only when

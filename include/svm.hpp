@@ -238,6 +238,54 @@ class energyMinimizerFIRE : public updater {
     double forceMax = 0.0;
 };
 
+// energyMinimizerAdam (src/updaters/energyMinimizerAdam.h:8-62).  Parameters
+// take effect, with fresh moments and iteration count, at the first
+// minimize() after they change; setMaximumIterations keeps the state.
+class energyMinimizerAdam : public updater {
+   public:
+    explicit energyMinimizerAdam(std::shared_ptr<sphericalVertexModel> m) { setModel(std::move(m)); }
+    void setAdamParameters(double b1 = 0.9, double b2 = 0.99, double eps = 0.00000001, double dt = 0.0001,
+                           double fc = 1e-12) {
+        p_.beta1 = b1;
+        p_.beta2 = b2;
+        p_.epsilon = eps;
+        p_.dt = dt;
+        p_.force_cutoff = fc;
+        deltaT = dt;
+        dirty_ = true;
+    }
+    void setForceCutoff(double fc) {
+        p_.force_cutoff = fc;
+        dirty_ = true;
+    }
+    void setMaximumIterations(int maxIt) override {
+        maxIterations = maxIt;
+        p_.max_iterations = maxIt;
+        if (!dirty_) check(svm_adam_set_max_iterations(model->context(), 0, maxIt), model->context(),
+                           "svm_adam_set_max_iterations");
+    }
+    void minimize() {
+        svm_context* c = model->context();
+        if (dirty_) {
+            p_.max_iterations = maxIterations;
+            check(svm_adam_set_parameters(c, 0, &p_), c, "svm_adam_set_parameters");
+            dirty_ = false;
+        }
+        check(svm_adam_minimize(c, 100), c, "svm_adam_minimize");
+        svm_adam_state s;
+        check(svm_adam_get_state(c, 0, &s), c, "svm_adam_get_state");
+        iterations = s.iterations;
+        forceMax = s.force_max;
+    }
+    void performUpdate() override { minimize(); }
+    double getMaxForce() const { return forceMax; }
+
+   private:
+    svm_adam_params p_{0.9, 0.99, 1e-8, 1e-4, 1e-12, 1000, 0};
+    bool dirty_ = true;
+    double forceMax = 0.0;
+};
+
 // Simulation (src/simulation/simulation.cpp:102-174): updaters in order per step.
 class Simulation {
    public:

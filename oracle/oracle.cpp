@@ -879,6 +879,58 @@ void or_fire_minimize(void* h, const double* params, double* state, int reduce_m
     state[5] = Power;
 }
 
+// energyMinimizerAdam::minimize + adamStepCPU (src/updaters/energyMinimizerAdam.cpp:55-103).
+// params: [beta1, beta2, epsilon, deltaT, forceCutoff, maxIterations]
+// state (in/out): [iterations, beta1t, beta2t, forceMax]
+// mom, mom2 (in/out): the biased first and second moment estimates, [N][3].
+// Every iteration computes the forces, updates the moments, moves by
+// -deltaT mc / sqrt(vc) per component (epsilon where sqrt(vc) is 0) and
+// enforces the topology; forceMax = sqrt(sum F^2) / N of that iteration's
+// forces.
+void or_adam_minimize(void* h, const double* params, double* state, double* mom, double* mom2, int reduce_mode) {
+    Model* m = (Model*)h;
+    double beta1 = params[0], beta2 = params[1], epsilon = params[2], deltaT = params[3];
+    double forceCutoff = params[4];
+    int maxIterations = (int)params[5];
+    int iterations = (int)state[0];
+    double beta1t = state[1], beta2t = state[2];
+    int Ndof = m->N;
+    std::vector<dVec> disp(Ndof);
+    std::vector<double> tf(Ndof);
+    double forceMax = 110.0;
+    while ((iterations < maxIterations) && (forceMax > forceCutoff)) {
+        iterations += 1;
+        m->computeForces();
+        double forceNorm = 0.0;
+        for (int nn = 0; nn < Ndof; ++nn) {
+            const dVec& g = m->forces[nn];
+            double fdot = dot(g, g);
+            if (reduce_mode == 0) forceNorm += fdot;
+            else tf[nn] = fdot;
+            for (int dd = 0; dd < 3; ++dd) {
+                double mm = beta1 * mom[3 * nn + dd] + (beta1 - 1) * g[dd];
+                double vv = beta2 * mom2[3 * nn + dd] + (1 - beta2) * (g[dd] * g[dd]);
+                mom[3 * nn + dd] = mm;
+                mom2[3 * nn + dd] = vv;
+                double mc = (1.0 / (1.0 - beta1t)) * mm;
+                double vc = (1.0 / (1.0 - beta2t)) * vv;
+                double rootvc = std::sqrt(vc);
+                if (rootvc == 0) rootvc = epsilon;
+                disp[nn][dd] = -deltaT * mc / (rootvc);
+            }
+        }
+        if (reduce_mode) forceNorm = tree_sum(tf);
+        m->moveParticles(disp, 1.0);
+        forceMax = std::sqrt(forceNorm) / Ndof;
+        beta1t *= beta1;
+        beta2t *= beta2;
+    }
+    state[0] = iterations;
+    state[1] = beta1t;
+    state[2] = beta2t;
+    state[3] = forceMax;
+}
+
 // ---- accessors ----
 int or_num_vertices(void* h) { return ((Model*)h)->N; }
 int or_max_deg(void* h) { return ((Model*)h)->maxDeg; }

@@ -261,6 +261,16 @@ class OracleModel:
         return dict(iterations=int(st[0]), dt=st[1], alpha=st[2], n_since_negative_power=int(st[3]),
                     force_max=st[4], power=st[5], state=st)
 
+    def adam(self, beta1=0.9, beta2=0.99, epsilon=1e-8, dt=1e-4, force_cutoff=1e-12, max_iterations=1000,
+             state=None, moments=None, reduce_mode=1):
+        """Adam minimisation (oracle or_adam_minimize); returns the state and the moments."""
+        params = np.array([beta1, beta2, epsilon, dt, force_cutoff, max_iterations], np.float64)
+        st = np.array(state if state is not None else [0, beta1, beta2, 110.0], np.float64)
+        n = self.L.or_num_vertices(self.h)
+        m1, m2 = (np.zeros((n, 3)), np.zeros((n, 3))) if moments is None else (moments[0].copy(), moments[1].copy())
+        self.L.or_adam_minimize(self.h, _dp(params), _dp(st), _dp(m1), _dp(m2), C.c_int(reduce_mode))
+        return dict(iterations=int(st[0]), beta1t=st[1], beta2t=st[2], force_max=st[3], state=st, moments=(m1, m2))
+
     @property
     def n_t1(self):
         return self.L.or_num_t1(self.h)

@@ -387,6 +387,53 @@ def test_fire_one_dimensional_bitzek_rules():
     assert r["alpha"] == 0.99 * 0.9
 
 
+def move_np(P, D, R):
+    """sphericalDomain::move on arrays: tangent projection, add, back to radius R."""
+    D = D - (np.sum(D * P, 1) / np.sum(P * P, 1))[:, None] * P
+    Q = P + D
+    return R * Q / np.linalg.norm(Q, axis=1)[:, None]
+
+
+def test_adam_first_step_is_a_signed_unit_step():
+    """Adam's first step with zero moments moves every component by dt in the
+    direction of the force: m1/(1-b1) = -F and v1/(1-b2) = F^2 make the step
+    -dt m/sqrt(v) = dt sign(F) (energyMinimizerAdam.cpp:70-77); the move is then
+    projected on the tangent plane.  The reported force is sqrt(sum F^2)/N of
+    the forces before the move."""
+    st = svmtest.seeded_state(512, 1137)
+    om = svmtest.OracleModel(st)
+    om.compute_forces()
+    F, P0 = om.forces(), om.positions()
+    r = om.adam(dt=1e-3, max_iterations=1, reduce_mode=0)
+    assert r["iterations"] == 1 and om.n_t1 == 0
+    np.testing.assert_allclose(om.positions(), move_np(P0, 1e-3 * np.sign(F), st["R"]), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(r["force_max"], np.sqrt((F ** 2).sum()) / len(F), rtol=1e-14)
+    # the first moment holds -(1 - beta1) F, the second (1 - beta2) F^2
+    np.testing.assert_allclose(r["moments"][0], -(1 - 0.9) * F, rtol=1e-14)
+    np.testing.assert_allclose(r["moments"][1], (1 - 0.99) * F * F, rtol=1e-14)
+
+
+def test_adam_bias_powers_and_descent():
+    """After k iterations the bias-correction powers are beta^(k+1); small
+    Adam steps lower the energy, with T1s enforced after every move; the
+    sequential and tree-ordered force sums agree to rounding."""
+    st = svmtest.seeded_state(256, 1147)
+    runs = []
+    for mode in (0, 1):
+        om = svmtest.OracleModel(st, p0=3.7)
+        om.compute_forces()
+        e0 = om.energy(0)
+        r = om.adam(dt=2e-3, max_iterations=300, reduce_mode=mode)
+        om.compute_forces()
+        runs.append((r, om.energy(0), e0, om.positions()))
+    r, e1, e0, _ = runs[1]
+    assert r["iterations"] == 300
+    np.testing.assert_allclose(r["beta1t"], 0.9 ** 301, rtol=1e-12)
+    np.testing.assert_allclose(r["beta2t"], 0.99 ** 301, rtol=1e-12)
+    assert e1 < 0.9 * e0
+    np.testing.assert_allclose(runs[0][0]["force_max"], runs[1][0]["force_max"], rtol=1e-12)
+
+
 # -------------------------------------------------------------- C ABI ------
 def test_library_exports_every_declared_symbol():
     import re
