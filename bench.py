@@ -165,20 +165,40 @@ def cpu_baseline(ctx, params, budget_s=12.0, threads=None):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_batched-active-256.txt")
+FP64_FILE = os.path.join(ROOT, "profiles", "pmc_fp64_batched-active-256.txt")
+PEAK_FP64_TFLOPS = 78.6
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` on this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain copy
-    of known size); None if there is no such measurement."""
+def profile_figure(path, key, kernel):
+    """A per-launch figure of `kernel` on this workload from a committed
+    rocprofv3 summary line `key kernel value`; None if there is none."""
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(path):
             f = line.split()
-            if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            if len(f) == 3 and f[0] == key and f[1] == kernel:
                 return float(f[2])
     except OSError:
         pass
     return None
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated
+    against a plain copy of known size)."""
+    return profile_figure(TRAFFIC_FILE, "traffic_bytes_per_launch", kernel)
+
+
+def fp64_roofline(kernel, launch_ms):
+    """The FP64 side of the dominant kernel, which is VALU-bound in the
+    parity arithmetic: its FP64 FLOP per launch (rocprofv3
+    SQ_INSTS_VALU_{ADD,MUL,FMA}_F64 on this workload, committed summary) over
+    the launch time measured here, against the FP64 vector peak."""
+    flop = profile_figure(FP64_FILE, "fp64_flop_per_launch", kernel)
+    if flop is None or not launch_ms > 0:
+        return None
+    tf = flop / (launch_ms * 1e-3) / 1e12
+    return {"flop_per_launch": flop, "achieved": tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": tf / PEAK_FP64_TFLOPS}
 
 
 def copy_probe(svm, n_bytes=1 << 28):
@@ -300,7 +320,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(dom),
                      "bytes_per_launch": VERTEX_BYTES_PER_CELL * total_cells, "avg_launch_ms": dom_ms,
-                     "back_to_back_ms": b2b_ms},
+                     "back_to_back_ms": b2b_ms, "fp64": fp64_roofline(dom, dom_ms)},
         "kernel_ms": timing,
         "init": {"gpu_delaunay_topology_s": t_delaunay, "setup_s": t_setup,
                  "note": "256 spheres triangulated one call each (includes host<->device copies)"},
