@@ -38,6 +38,7 @@ EXPORTS = [
     "svm_mlp_forward", "svm_mlp_gradient", "svm_mlp_train", "svm_mlp_time_forward", "svm_cell_descriptors",
     "svm_warm_start", "svm_fire_set_max_iterations", "svm_set_tuning", "svm_get_tuning", "svm_autotune",
     "svm_adam_set_parameters", "svm_adam_minimize", "svm_adam_get_state", "svm_adam_set_max_iterations",
+    "svm_dyn_set_reference", "svm_dyn_measure", "svm_nc_open", "svm_nc_write", "svm_nc_num_records", "svm_nc_close",
 ]
 
 
@@ -443,6 +444,18 @@ class Context:
         self._ck(lib().svm_adam_get_state(self.h, C.c_int32(sys), C.byref(st)), "svm_adam_get_state")
         return {k: getattr(st, k) for k, _ in AdamState._fields_}
 
+    def dyn_set_reference(self):
+        """dynamicalFeatures(initialPos): snapshot every sphere's current cell positions."""
+        self._ck(lib().svm_dyn_set_reference(self.h), "svm_dyn_set_reference")
+
+    def dyn_measure(self, fraction=1.0, cutoff=0.5):
+        """(msd, overlap) per sphere against the snapshot: computeMSD / computeOverlapFunction."""
+        msd = np.zeros(self.n, np.float64)
+        ov = np.zeros(self.n, np.float64)
+        self._ck(lib().svm_dyn_measure(self.h, C.c_double(fraction), C.c_double(cutoff), _dp(msd), _dp(ov)),
+                 "svm_dyn_measure")
+        return msd, ov
+
     def synchronize(self):
         self._ck(lib().svm_synchronize(self.h), "svm_synchronize")
 
@@ -518,3 +531,48 @@ class Context:
         k = C.c_int32(0)
         self._ck(lib().svm_get_timing(self.h, names, _dp(ms), C.c_int32(n), C.byref(k)), "svm_get_timing")
         return {names[i].decode(): float(ms[i]) for i in range(k.value)}
+
+
+class VectorValueNetCDF:
+    """vectorValueNetCDF(fn, N, NcFile::Replace) (src/databases/vectorValueNetCDF.cpp:3-36):
+    writeState(vec, value) appends one record of a NetCDF classic file with
+    value(record, unit) and vector(record, N)."""
+
+    def __init__(self, path, n):
+        self.h = C.c_void_p()
+        r = lib().svm_nc_open(os.fsencode(path), C.c_int32(n), C.byref(self.h))
+        if r:
+            raise SVMError(f"svm_nc_open({path!r}, {n}) failed with status {r}")
+        self.n = n
+
+    def writeState(self, vec, value):
+        v = np.ascontiguousarray(vec, np.float64).ravel()
+        if v.size != self.n:
+            raise ValueError(f"vector of {v.size} values for N = {self.n}")
+        r = lib().svm_nc_write(self.h, _dp(v), C.c_double(value))
+        if r:
+            raise SVMError(f"svm_nc_write failed with status {r}")
+
+    @property
+    def num_records(self):
+        n = C.c_int32(0)
+        lib().svm_nc_num_records(self.h, C.byref(n))
+        return n.value
+
+    def close(self):
+        if self.h:
+            lib().svm_nc_close(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -27,6 +27,7 @@ static inline int xcd_grid(int ntiles) { return (ntiles + 7) / 8 * 8; }
 #include "svm_delaunay.h"
 #include "svm_voronoi.h"
 #include "svm_mlp.h"
+#include "svm_netcdf.h"
 
 using svm::Dev;
 using svm::SysParams;
@@ -2123,6 +2124,94 @@ int svm_warm_start(svm_context* ctx, svm_mlp* m, double scale, double max_step, 
     }
     if (reverted) *reverted = nrev;
     return svm_synchronize(ctx);
+}
+
+int svm_dyn_set_reference(svm_context* ctx) {
+    if (!ctx) return SVM_ERR_ARG;
+    int r = require_topology(ctx);
+    if (r) return r;
+    Dev& d = ctx->d;
+    const size_t Cs = ctx->Cs;
+    if (!d.dref) {
+        r = dalloc(ctx, &d.dref, 3 * Cs);
+        if (!r) r = dalloc(ctx, &d.dpart, 2 * (size_t)ctx->nct);
+        if (!r) r = dalloc(ctx, &d.dout, 2 * (size_t)ctx->nsys);
+        if (r) return r;
+    }
+    HIPCHECK(ctx, hipMemcpyAsync(d.dref, d.cx, sizeof(double) * Cs, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(d.dref + Cs, d.cy, sizeof(double) * Cs, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(d.dref + 2 * Cs, d.cz, sizeof(double) * Cs, hipMemcpyDeviceToDevice, ctx->stream));
+    return svm_synchronize(ctx);
+}
+
+int svm_dyn_measure(svm_context* ctx, double fraction, double cutoff, double* msd, double* overlap) {
+    if (!ctx || !msd || !overlap || !(fraction > 0.0)) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    if (!ctx->d.dref) return fail(ctx, SVM_ERR_STATE, "no reference positions (svm_dyn_set_reference)");
+    hipLaunchKernelGGL(svm::k_dyn_tiles, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, ctx->d, fraction, cutoff);
+    int r = last_launch(ctx);
+    if (r) return r;
+    hipLaunchKernelGGL(svm::k_dyn_fold, dim3(ctx->nsys), dim3(SVM_TILE), 0, ctx->stream, ctx->d, fraction);
+    r = last_launch(ctx);
+    if (r) return r;
+    std::vector<double> out(2 * (size_t)ctx->nsys);
+    HIPCHECK(ctx, hipMemcpyAsync(out.data(), ctx->d.dout, sizeof(double) * out.size(), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int s = 0; s < ctx->nsys; ++s) {
+        msd[s] = out[2 * s];
+        overlap[s] = out[2 * s + 1];
+    }
+    return SVM_OK;
+}
+
+int svm_nc_open(const char* path, int32_t n, svm_nc_file** out) {
+    if (!path || !out || n <= 0 || (int64_t)n * 8 > 0x7fffffffLL) return SVM_ERR_ARG;
+    *out = nullptr;
+    FILE* f = std::fopen(path, "wb+");
+    if (!f) return SVM_ERR_IO;
+    const std::string probe = svm_nc::header(n, 0, 0);
+    const uint32_t hsize = (uint32_t)probe.size();
+    const std::string h = svm_nc::header(n, hsize, hsize + 8);
+    if (std::fwrite(h.data(), 1, h.size(), f) != h.size() || std::fflush(f) != 0) {
+        std::fclose(f);
+        return SVM_ERR_IO;
+    }
+    svm_nc_file* nc = new svm_nc_file;
+    nc->f = f;
+    nc->n = n;
+    nc->header = (long)hsize;
+    nc->recsize = 8 + 8 * (long)n;
+    *out = nc;
+    return SVM_OK;
+}
+
+int svm_nc_write(svm_nc_file* nc, const double* vec, double value) {
+    if (!nc || !nc->f || !vec) return SVM_ERR_ARG;
+    std::vector<unsigned char> rec((size_t)nc->recsize);
+    svm_nc::be64d(rec.data(), value);
+    for (int32_t i = 0; i < nc->n; ++i) svm_nc::be64d(rec.data() + 8 + 8 * (size_t)i, vec[i]);
+    if (std::fseek(nc->f, nc->header + nc->recsize * (long)nc->numrecs, SEEK_SET) != 0 ||
+        std::fwrite(rec.data(), 1, rec.size(), nc->f) != rec.size())
+        return SVM_ERR_IO;
+    nc->numrecs += 1;
+    unsigned char cnt[4];
+    for (int k = 0; k < 4; ++k) cnt[k] = (unsigned char)(((uint32_t)nc->numrecs >> (8 * (3 - k))) & 0xff);
+    if (std::fseek(nc->f, 4, SEEK_SET) != 0 || std::fwrite(cnt, 1, 4, nc->f) != 4 || std::fflush(nc->f) != 0)
+        return SVM_ERR_IO;
+    return SVM_OK;
+}
+
+int svm_nc_num_records(svm_nc_file* nc, int32_t* n_records) {
+    if (!nc || !n_records) return SVM_ERR_ARG;
+    *n_records = nc->numrecs;
+    return SVM_OK;
+}
+
+int svm_nc_close(svm_nc_file* nc) {
+    if (!nc) return SVM_ERR_ARG;
+    int r = (nc->f && std::fclose(nc->f) != 0) ? SVM_ERR_IO : SVM_OK;
+    delete nc;
+    return r;
 }
 
 }  // extern "C"

@@ -78,6 +78,7 @@ struct Dev {
     double *nx[2], *ny[2], *nz[2];
     double *vx, *vy, *vz;
     double *am, *av;              // Adam moment estimates: three planes of Vs each (null until Adam is set up)
+    double *dref, *dpart, *dout;  // dynamical analysis: reference cell positions (3 planes of Cs), tile partials, results
     double *fx, *fy, *fz;
     int4* vcn;
     int4* vn;

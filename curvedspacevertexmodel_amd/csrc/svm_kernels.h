@@ -546,6 +546,48 @@ __global__ __launch_bounds__(SVM_TILE) void k_adam_control(Dev d) {
     }
 }
 
+// dynamicalFeatures::computeMSD / computeOverlapFunction
+// (src/analysis/dynamicalFeatures.cpp:17-53) on the cell positions against
+// the reference snapshot d.dref, for the first floor(N * fraction) cells of
+// each sphere (all of them when fraction >= 1): per-tile tree partials of the
+// squared geodesic displacement (NaN adds nothing) and of the count below the
+// cutoff, then one block per sphere folds them.
+__device__ inline int dyn_count(const SysParams* S, double fraction) {
+    return fraction < 1 ? (int)__builtin_floor(S->nc * fraction) : S->nc;
+}
+__global__ __launch_bounds__(SVM_TILE) void k_dyn_tiles(Dev d, double fraction, double cutoff) {
+    __shared__ double red[SVM_TILE + 1];
+    const int tile = blockIdx.x;
+    const SysParams* S = d.sys + d.ctile_sys[tile];
+    const int c = tile * SVM_TILE + threadIdx.x;
+    const size_t Cs = (size_t)d.Cs;
+    double sq = 0.0, ov = 0.0;
+    if (c - S->co < dyn_count(S, fraction)) {
+        const double init[3] = {d.dref[c], d.dref[Cs + c], d.dref[2 * Cs + c]};
+        const double cur[3] = {d.cx[c], d.cy[c], d.cz[c]};
+        const double disp = geodesic(init, cur, pnorm(init), pnorm(cur), S->R);
+        sq = disp != disp ? 0.0 : disp * disp;
+        ov = disp < cutoff ? 1.0 : 0.0;
+    }
+    const double a = block_tree256(sq, red);
+    const double b = block_tree256(ov, red);
+    if (threadIdx.x == 0) {
+        d.dpart[2 * tile] = a;
+        d.dpart[2 * tile + 1] = b;
+    }
+}
+__global__ __launch_bounds__(SVM_TILE) void k_dyn_fold(Dev d, double fraction) {
+    __shared__ double buf[SVM_MAX_TILES];
+    const SysParams* S = d.sys + blockIdx.x;
+    const double msd = fold_lds(d.dpart + 2 * S->ctile0, 2, S->nctiles, buf);
+    const double overlap = fold_lds(d.dpart + 2 * S->ctile0 + 1, 2, S->nctiles, buf);
+    if (threadIdx.x == 0) {
+        const int n = dyn_count(S, fraction);
+        d.dout[2 * blockIdx.x] = msd / n;
+        d.dout[2 * blockIdx.x + 1] = overlap / n;
+    }
+}
+
 // Pending FIRE velocity mixing (fireStepCPU :205-208, :227-231) of vertex v.
 __device__ inline void fire_mix(const Dev& d, const SysParams* S, int v, double* vel, const double* f) {
     if (S->mix_pending) {

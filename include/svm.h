@@ -40,7 +40,8 @@ enum svm_status {
     SVM_ERR_TOPOLOGY = 5,       /* inconsistent topology (ring/vertex lists disagree) */
     SVM_ERR_DELAUNAY = 6,       /* spherical Delaunay construction failed */
     SVM_ERR_STATE = 7,          /* call out of order (e.g. no topology yet) */
-    SVM_ERR_CAPACITY = 8        /* a fixed-size work buffer would overflow */
+    SVM_ERR_CAPACITY = 8,       /* a fixed-size work buffer would overflow */
+    SVM_ERR_IO = 9              /* a file could not be opened or written */
 };
 
 /* One sphere of the batch.  Mirrors the constructor arguments of
@@ -331,6 +332,29 @@ int svm_cell_descriptors(svm_context* ctx, int32_t sys, float* X, float* frames)
  * those spheres.
  * The warm start never leaves a sphere worse than plain FIRE would start. */
 int svm_warm_start(svm_context* ctx, svm_mlp* mlp, double scale, double max_step, int32_t* reverted);
+
+/* ---- trajectory analysis and output -------------------------------------
+ * dynamicalFeatures (src/analysis/dynamicalFeatures.cpp:7-53) on the GPU.
+ * svm_dyn_set_reference snapshots every sphere's current cell positions (the
+ * constructor's initialPos; the geometry pass computes them).
+ * svm_dyn_measure gives per sphere (arrays of n_systems) computeMSD and
+ * computeOverlapFunction of the current cell positions against it, over the
+ * first floor(N * fraction) cells (all for fraction >= 1): the mean squared
+ * geodesic displacement (NaN distances add nothing) and the fraction moved
+ * less than `cutoff` (the reference's default 0.5). */
+int svm_dyn_set_reference(svm_context* ctx);
+int svm_dyn_measure(svm_context* ctx, double fraction, double cutoff, double* msd, double* overlap);
+
+/* vectorValueNetCDF (src/databases/vectorValueNetCDF.cpp:3-36): a NetCDF
+ * classic file with dimensions record (unlimited), unit (1) and N (n), and
+ * double variables value(record, unit) and vector(record, N); opened with
+ * replace semantics.  svm_nc_write is writeState(vec[n], value): one record
+ * appended and the file flushed.  Host-only (no device is needed). */
+typedef struct svm_nc_file svm_nc_file;
+int svm_nc_open(const char* path, int32_t n, svm_nc_file** out);
+int svm_nc_write(svm_nc_file* f, const double* vec, double value);
+int svm_nc_num_records(svm_nc_file* f, int32_t* n_records);
+int svm_nc_close(svm_nc_file* f);
 
 /* ---- autotuning ----------------------------------------------------------
  * The launch schedule of a step: the grid of the T1 single-candidate dry runs

@@ -931,6 +931,37 @@ void or_adam_minimize(void* h, const double* params, double* state, double* mom,
     state[3] = forceMax;
 }
 
+// dynamicalFeatures::computeMSD and computeOverlapFunction
+// (src/analysis/dynamicalFeatures.cpp:17-53) on the cell positions, against
+// ref [nCells][3]: the first floor(nCells * fraction) cells (all when
+// fraction >= 1); a NaN distance adds nothing to the MSD but still counts in
+// N; the overlap counts distances below the cutoff.  reduce_mode 1 sums with
+// tree_sum (the device's order).  out: [msd, overlap].
+void or_msd_overlap(void* h, const double* ref, double fraction, double cutoff, int reduce_mode, double* out) {
+    Model* m = (Model*)h;
+    int N = m->nCells;
+    if (fraction < 1) N = (int)std::floor(N * fraction);
+    std::vector<double> sq(N), ov(N);
+    double msd = 0.0, overlap = 0.0;
+    for (int ii = 0; ii < N; ++ii) {
+        dVec init(0.0);
+        for (int d = 0; d < 3; ++d) init[d] = ref[3 * ii + d];
+        dVec cur = m->cellPositions[ii];
+        double disp;
+        m->sphere.geodesicDistance(init, cur, disp);
+        sq[ii] = std::isnan(disp) ? 0.0 : disp * disp;
+        ov[ii] = disp < cutoff ? 1.0 : 0.0;
+        msd += sq[ii];
+        overlap += ov[ii];
+    }
+    if (reduce_mode) {
+        msd = tree_sum(sq);
+        overlap = tree_sum(ov);
+    }
+    out[0] = msd / N;
+    out[1] = overlap / N;
+}
+
 // ---- accessors ----
 int or_num_vertices(void* h) { return ((Model*)h)->N; }
 int or_max_deg(void* h) { return ((Model*)h)->maxDeg; }

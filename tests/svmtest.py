@@ -271,6 +271,14 @@ class OracleModel:
         self.L.or_adam_minimize(self.h, _dp(params), _dp(st), _dp(m1), _dp(m2), C.c_int(reduce_mode))
         return dict(iterations=int(st[0]), beta1t=st[1], beta2t=st[2], force_max=st[3], state=st, moments=(m1, m2))
 
+    def msd_overlap(self, ref, fraction=1.0, cutoff=0.5, reduce_mode=1):
+        """dynamicalFeatures on the oracle's current cell positions against ref [N][3]."""
+        out = np.zeros(2)
+        r = np.ascontiguousarray(ref, np.float64)
+        self.L.or_msd_overlap(self.h, _dp(r), C.c_double(fraction), C.c_double(cutoff), C.c_int(reduce_mode),
+                              _dp(out))
+        return out[0], out[1]
+
     @property
     def n_t1(self):
         return self.L.or_num_t1(self.h)
