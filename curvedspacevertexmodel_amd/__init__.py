@@ -39,6 +39,7 @@ EXPORTS = [
     "svm_warm_start", "svm_fire_set_max_iterations", "svm_set_tuning", "svm_get_tuning", "svm_autotune",
     "svm_adam_set_parameters", "svm_adam_minimize", "svm_adam_get_state", "svm_adam_set_max_iterations",
     "svm_dyn_set_reference", "svm_dyn_measure", "svm_nc_open", "svm_nc_write", "svm_nc_num_records", "svm_nc_close",
+    "svm_set_vicsek", "svm_vicsek_steps",
 ]
 
 
@@ -399,6 +400,14 @@ class Context:
 
     def brownian_steps(self, n):
         self._ck(lib().svm_brownian_steps(self.h, C.c_int32(n)), "svm_brownian_steps")
+
+    def set_vicsek(self, sys, dt=0.01, mu=1.0, v0=0.01, tau=1.0, eta=1.0):
+        """sphericalVectorialVicsek parameters (defaults as sphericalVectorialVicsek.h:11, dt as the drivers')."""
+        self._ck(lib().svm_set_vicsek(self.h, C.c_int32(sys), C.c_double(dt), C.c_double(mu), C.c_double(v0),
+                                      C.c_double(tau), C.c_double(eta)), "svm_set_vicsek")
+
+    def vicsek_steps(self, n):
+        self._ck(lib().svm_vicsek_steps(self.h, C.c_int32(n)), "svm_vicsek_steps")
 
     def active_steps(self, n):
         self._ck(lib().svm_active_steps(self.h, C.c_int32(n)), "svm_active_steps")

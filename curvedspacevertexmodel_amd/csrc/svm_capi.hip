@@ -271,6 +271,21 @@ int enqueue_step(svm_context* ctx, int updater) {
     if (updater == 0) r = launch_vertex<svm::VM_BROWNIAN>(ctx, pout, 0, "k_vertex_brownian");
     else r = launch_vertex<svm::VM_ACTIVE>(ctx, pout, 0, "k_vertex_active");
     if (r) return r;
+    if (updater == 2) {
+        // Vicsek: the same move as the active updater, T1, then the alignment
+        // update of the directors on the new positions and topology
+        r = launch_t1(ctx, pout, ctx->ncur, ctx->ncur, 0, 1, 0);
+        if (r) return r;
+        ctx->pcur = pout;
+        const int nout = 1 - ctx->ncur;
+        Timer t(ctx, "k_vicsek_dir");
+        hipLaunchKernelGGL(svm::k_vicsek_dir, dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0, ctx->stream, ctx->d,
+                           ctx->pcur, ctx->ncur, nout);
+        r = last_launch(ctx);
+        if (r) return r;
+        ctx->ncur = nout;
+        return SVM_OK;
+    }
     int nout = updater == 1 ? 1 - ctx->ncur : ctx->ncur;
     r = launch_t1(ctx, pout, ctx->ncur, nout, updater == 1, 1, 0);
     if (r) return r;
@@ -1271,6 +1286,15 @@ int svm_set_active(svm_context* ctx, int32_t sys, double dt, double mu, double v
     return r ? r : set_field(ctx, sys, offsetof(SysParams, tau), tau);
 }
 
+int svm_set_vicsek(svm_context* ctx, int32_t sys, double dt, double mu, double v0, double tau, double eta) {
+    if (!valid_sys(ctx, sys) || !(tau != 0.0)) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    int r = set_field(ctx, sys, offsetof(SysParams, dt), dt);
+    r = r ? r : set_field(ctx, sys, offsetof(SysParams, mu), mu);
+    r = r ? r : set_field(ctx, sys, offsetof(SysParams, v0), v0);
+    r = r ? r : set_field(ctx, sys, offsetof(SysParams, tau), tau);
+    return r ? r : set_field(ctx, sys, offsetof(SysParams, eta), eta);
+}
+
 static int run_steps(svm_context* ctx, int updater, int n_steps) {
     if (!ctx || n_steps < 0) return SVM_ERR_ARG;
     int r = require_topology(ctx);
@@ -1284,10 +1308,11 @@ static int run_steps(svm_context* ctx, int updater, int n_steps) {
 
 int svm_brownian_steps(svm_context* ctx, int32_t n_steps) { return run_steps(ctx, 0, n_steps); }
 int svm_active_steps(svm_context* ctx, int32_t n_steps) { return run_steps(ctx, 1, n_steps); }
+int svm_vicsek_steps(svm_context* ctx, int32_t n_steps) { return run_steps(ctx, 2, n_steps); }
 
 int svm_run_graph(svm_context* ctx, int32_t updater, int32_t steps_per_graph, int32_t n_replays) {
     if (ctx && steps_per_graph == 0) steps_per_graph = ctx->graph_steps;
-    if (!ctx || (updater != 0 && updater != 1) || steps_per_graph <= 0 || (steps_per_graph & 1) || n_replays < 0)
+    if (!ctx || updater < 0 || updater > 2 || steps_per_graph <= 0 || (steps_per_graph & 1) || n_replays < 0)
         return fail(ctx, SVM_ERR_ARG, "steps_per_graph must be a positive even number");
     int r = require_topology(ctx);
     if (r) return r;

@@ -34,8 +34,9 @@ struct SysParams {
     double R, kr, thr;
     double thr_cos;   // t1_cos_cut(thr, R): edges with a smaller cosine are not short
     uint64_t seed;
-    // Brownian / active dynamics
+    // Brownian / active / Vicsek dynamics
     double dt, mu, T, v0, tau;
+    double eta;       // Vicsek noise strength
     uint64_t step;
     // FIRE parameters
     double f_dt0, f_alpha_start, f_dt_max, f_dt_inc, f_dt_dec, f_alpha_dec, f_cutoff, f_alpha_min, f_dt_min;

@@ -291,7 +291,8 @@ enum : uint32_t {
     STREAM_BROWNIAN    = 0x42524e31u,   // Brownian displacement noise
     STREAM_ACTIVE      = 0x41435431u,   // active director rotation angle
     STREAM_BATCH       = 0x42415431u,   // batch parameter draws
-    STREAM_MLP         = 0x4d4c5031u    // network initialisation / sampling
+    STREAM_MLP         = 0x4d4c5031u,   // network initialisation / sampling
+    STREAM_VICSEK      = 0x56494331u    // Vicsek random tangent direction
 };
 
 // Two uniforms on [0,1) for draw `idx` of `stream` at `step` under `seed`.

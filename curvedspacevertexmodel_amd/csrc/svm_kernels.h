@@ -679,6 +679,66 @@ __device__ inline void director_update(const SysParams* S, int local, const doub
     nout[0] = n[0]; nout[1] = n[1]; nout[2] = n[2];
 }
 
+// Director update of sphericalVectorialVicsek::integrateEOMCPU
+// (src/updaters/sphericalVectorialVicsek.cpp:24-64) on the vertex model, after
+// the move and the T1 pass: a random unit tangent direction (uniform on the
+// sphere, projected), the mean of the vertex's and its three neighbours'
+// directors plus eta times that direction, projected and normalised; then the
+// director relaxes towards it at rate dt / tau and is put back in the tangent
+// plane with unit length.  Reads directors nin, writes nout; the draws are
+// keyed by the step count after this step.
+__global__ __launch_bounds__(SVM_TILE) void k_vicsek_dir(Dev d, int pin, int nin, int nout) {
+    const int tile = xcd_tile(d, d.Vs / SVM_TILE);
+    if (tile >= d.Vs / SVM_TILE) return;
+    const SysParams* S = d.sys + d.vtile_sys[tile];
+    const int v = tile * SVM_TILE + threadIdx.x;
+    const int local = v - S->vo;
+    if (local >= S->nv) return;
+    const double p[3] = {d.px[pin][v], d.py[pin][v], d.pz[pin][v]};
+    double u, w;
+    uniform2(S->seed, STREAM_VICSEK, S->step, (uint32_t)local, &u, &w);
+    const double phi = 2.0 * SVM_PI * u;
+    const double theta = svm::acos(2.0 * w - 1);
+    double sp[3] = {1.0 * svm::sin(theta) * svm::cos(phi), 1.0 * svm::sin(theta) * svm::sin(phi),
+                    1.0 * svm::cos(theta)};
+    {   // projectToTangentPlane, then * (1 / norm)
+        double dvn = 0.0; dvn += sp[0] * p[0]; dvn += sp[1] * p[1]; dvn += sp[2] * p[2];
+        double dnn = 0.0; dnn += p[0] * p[0]; dnn += p[1] * p[1]; dnn += p[2] * p[2];
+        const double q = dvn / dnn;
+        sp[0] = sp[0] - q * p[0]; sp[1] = sp[1] - q * p[1]; sp[2] = sp[2] - q * p[2];
+        double ss = 0.0; ss += sp[0] * sp[0]; ss += sp[1] * sp[1]; ss += sp[2] * sp[2];
+        const double inv = 1.0 / __builtin_sqrt(ss);
+        sp[0] = inv * sp[0]; sp[1] = inv * sp[1]; sp[2] = inv * sp[2];
+    }
+    const double* NX = d.nx[nin];
+    const double* NY = d.ny[nin];
+    const double* NZ = d.nz[nin];
+    const int4 nb = d.vn[v];
+    const double n0[3] = {NX[v], NY[v], NZ[v]};
+    double nd[3] = {n0[0], n0[1], n0[2]};
+    const int nbs[3] = {nb.x, nb.y, nb.z};
+    for (int jj = 0; jj < 3; ++jj) {
+        nd[0] += NX[nbs[jj]]; nd[1] += NY[nbs[jj]]; nd[2] += NZ[nbs[jj]];
+    }
+    const double mi = 1.0 / 4;
+    const double eta = S->eta;
+    for (int k = 0; k < 3; ++k) nd[k] = mi * nd[k] + eta * sp[k];
+    project_normalize(nd, p);
+    const double rate = S->dt / S->tau;
+    double n[3];
+    for (int k = 0; k < 3; ++k) n[k] = n0[k] + rate * (nd[k] - n0[k]);
+    {   // projectToTangentPlane, then * (1 / norm)
+        double dvn = 0.0; dvn += n[0] * p[0]; dvn += n[1] * p[1]; dvn += n[2] * p[2];
+        double dnn = 0.0; dnn += p[0] * p[0]; dnn += p[1] * p[1]; dnn += p[2] * p[2];
+        const double q = dvn / dnn;
+        n[0] = n[0] - q * p[0]; n[1] = n[1] - q * p[1]; n[2] = n[2] - q * p[2];
+        double ss = 0.0; ss += n[0] * n[0]; ss += n[1] * n[1]; ss += n[2] * n[2];
+        const double inv = 1.0 / __builtin_sqrt(ss);
+        n[0] = inv * n[0]; n[1] = inv * n[1]; n[2] = inv * n[2];
+    }
+    d.nx[nout][v] = n[0]; d.ny[nout][v] = n[1]; d.nz[nout][v] = n[2];
+}
+
 // ------------------------------------------------------------- energies ---
 // Potential energy per sphere (computeEnergy, sphericalVertexModel.cpp:150-160)
 // and kinetic energy (simpleModel.cpp:63-73), tree-reduced.

@@ -234,8 +234,18 @@ int svm_brownian_steps(svm_context* ctx, int32_t n_steps);
  * parameters src/updaters/vectorialVicsek.h:188-191; tau = 1/Dr). */
 int svm_set_active(svm_context* ctx, int32_t sys, double dt, double mu, double v0, double tau);
 int svm_active_steps(svm_context* ctx, int32_t n_steps);
-/* Capture `steps_per_graph` steps of the given updater (0 Brownian, 1 active)
- * into a HIP graph and replay it n_replays times. */
+/* sphericalVectorialVicsek (src/updaters/sphericalVectorialVicsek.cpp:5-66;
+ * the reference's defaults mu 1, Eta 1, tau 1, v0 0.01, sphericalVectorialVicsek.h:11)
+ * on the vertex model: vertices move by dt (v0 n + mu F) and the topology is
+ * enforced; then each director relaxes at rate dt / tau towards the
+ * normalised tangent mean of itself and its three vertex neighbours plus eta
+ * times a random unit tangent direction.  The reference draws that direction
+ * from its own generator; here it is a Philox stream keyed by (seed, step,
+ * vertex). */
+int svm_set_vicsek(svm_context* ctx, int32_t sys, double dt, double mu, double v0, double tau, double eta);
+int svm_vicsek_steps(svm_context* ctx, int32_t n_steps);
+/* Capture `steps_per_graph` steps of the given updater (0 Brownian, 1 active,
+ * 2 Vicsek) into a HIP graph and replay it n_replays times. */
 int svm_run_graph(svm_context* ctx, int32_t updater, int32_t steps_per_graph, int32_t n_replays);
 
 /* energyMinimizerFIRE (src/updaters/energyMinimizerFIRE.cpp:181-259,

@@ -187,6 +187,24 @@ class sphericalSelfPropelledParticle : public updater {
     double mu = 1.0, v0 = 0.1, tau = 1.0;
 };
 
+// sphericalVectorialVicsek (src/updaters/sphericalVectorialVicsek.h:8-27,
+// setters of vectorialVicsek.h:31-34): alignment with the vertex neighbours.
+class sphericalVectorialVicsek : public updater {
+   public:
+    void setMu(double m) { mu = m; }
+    void setV0(double v) { v0 = v; }
+    void setTau(double t) { tau = t; }
+    void setEta(double e) { eta = e; }
+    void performUpdate() override {
+        svm_context* c = model->context();
+        check(svm_set_vicsek(c, 0, deltaT, mu, v0, tau, eta), c, "svm_set_vicsek");
+        check(svm_vicsek_steps(c, 1), c, "svm_vicsek_steps");
+    }
+
+   private:
+    double mu = 1.0, v0 = 0.01, tau = 1.0, eta = 1.0;
+};
+
 // energyMinimizerFIRE (energyMinimizerFIRE.cpp:58-273)
 class energyMinimizerFIRE : public updater {
    public:

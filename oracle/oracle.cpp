@@ -796,6 +796,48 @@ void or_active_steps(void* h, int nsteps, double deltaT, double mu, double v0, d
     }
 }
 
+// sphericalVectorialVicsek::integrateEOMCPU
+// (src/updaters/sphericalVectorialVicsek.cpp:5-66) on the vertex model, whose
+// degrees of freedom are the vertices and their neighbours the three vertex
+// neighbours.  The random direction of vertex ii in the step that takes the
+// step count from k to k + 1 uses uniform draws (STREAM_VICSEK, ii) at k + 1.
+void or_vicsek_steps(void* h, int nsteps, double deltaT, double mu, double v0, double tau, double Eta,
+                     uint64_t seed, uint64_t step0) {
+    Model* m = (Model*)h;
+    std::vector<dVec> disp(m->N), nDisp(m->N);
+    for (int s = 0; s < nsteps; ++s) {
+        uint64_t step = step0 + (uint64_t)s + 1;
+        m->computeForces();
+        for (int ii = 0; ii < m->N; ++ii)
+            disp[ii] = deltaT * (v0 * m->directors[ii] + mu * m->forces[ii]);
+        m->moveParticles(disp, 1.0);
+        for (int ii = 0; ii < m->N; ++ii) {
+            double u, w;
+            svm::uniform2(seed, svm::STREAM_VICSEK, step, (uint32_t)ii, &u, &w);
+            double phi = 2.0 * PI * u;
+            double theta = O_ACOS(2.0 * w - 1);
+            dVec spherePoint;
+            spherePoint[0] = 1.0 * O_SIN(theta) * O_COS(phi);
+            spherePoint[1] = 1.0 * O_SIN(theta) * O_SIN(phi);
+            spherePoint[2] = 1.0 * O_COS(theta);
+            m->sphere.projectToTangentPlane(spherePoint, m->positions[ii]);
+            spherePoint = spherePoint * (1.0 / norm(spherePoint));
+            int mm = 3;
+            nDisp[ii] = m->directors[ii];
+            for (int jj = 0; jj < mm; ++jj) nDisp[ii] += m->directors[m->neighbors[3 * ii + jj]];
+            mm += 1;
+            double mi = 1.0 / mm;
+            nDisp[ii] = nDisp[ii] * mi + spherePoint * Eta;
+            m->sphere.projectToTangentPlaneAndNormalize(nDisp[ii], m->positions[ii]);
+        }
+        for (int ii = 0; ii < m->N; ++ii) {
+            m->directors[ii] += (deltaT / tau) * (nDisp[ii] - m->directors[ii]);
+            m->sphere.projectToTangentPlane(m->directors[ii], m->positions[ii]);
+            m->directors[ii] = m->directors[ii] * (1.0 / norm(m->directors[ii]));
+        }
+    }
+}
+
 // energyMinimizerFIRE::minimize + velocityVerlet::integrateEOMCPU + fireStepCPU
 // (src/updaters/energyMinimizerFIRE.cpp:181-259, src/updaters/velocityVerlet.cpp:6-38).
 // params: [deltaT, alphaStart, deltaTMax, deltaTInc, deltaTDec, alphaDec,

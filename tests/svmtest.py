@@ -252,6 +252,10 @@ class OracleModel:
         self.L.or_active_steps(self.h, C.c_int(n), C.c_double(dt), C.c_double(mu), C.c_double(v0),
                                C.c_double(tau), C.c_uint64(seed), C.c_uint64(step0))
 
+    def vicsek(self, n, dt=0.01, mu=1.0, v0=0.01, tau=1.0, eta=1.0, seed=1137, step0=0):
+        self.L.or_vicsek_steps(self.h, C.c_int(n), C.c_double(dt), C.c_double(mu), C.c_double(v0),
+                               C.c_double(tau), C.c_double(eta), C.c_uint64(seed), C.c_uint64(step0))
+
     def fire(self, dt=0.01, alpha_start=0.99, dt_max=0.1, dt_inc=1.05, dt_dec=0.95, alpha_dec=0.9, n_min=5,
              force_cutoff=1e-7, alpha_min=0.0, max_iterations=1000, state=None, reduce_mode=1, stop_rms=0):
         params = np.array([dt, alpha_start, dt_max, dt_inc, dt_dec, alpha_dec, n_min, force_cutoff, alpha_min,

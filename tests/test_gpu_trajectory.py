@@ -58,3 +58,37 @@ def test_trajectory_files_hold_the_measurements(tmp_path):
     with netcdf_file(str(tmp_path / "t_cellPositions_s0.nc"), "r", mmap=False) as f:
         assert f.variables["vector"].shape == (len(saves), 3 * 800)
     assert np.all(np.diff(msd[:, 0]) > 0)          # active cells spread out
+
+
+def test_vicsek_matches_oracle():
+    """sphericalVectorialVicsek on the GPU: positions, directors and topology
+    bit-exact against the oracle over 120 steps with T1 flips."""
+    from test_gpu_parity import assert_topology_equal
+    st = svmtest.seeded_state(512, 1137)
+    om = svmtest.OracleModel(st)
+    ctx = make_ctx([st])
+    ctx.set_vicsek(0, dt=0.01, mu=1.0, v0=0.1, tau=1.0, eta=0.5)
+    for chunk in range(3):
+        om.vicsek(40, dt=0.01, mu=1.0, v0=0.1, tau=1.0, eta=0.5, seed=1137, step0=40 * chunk)
+        ctx.vicsek_steps(40)
+        np.testing.assert_array_equal(ctx.positions(0), om.positions())
+        np.testing.assert_array_equal(ctx.directors(0), om.directors())
+        assert_topology_equal(ctx, 0, om)
+    assert om.n_t1 > 0
+
+
+def test_vicsek_batch_graph_equals_eager():
+    states = [svmtest.seeded_state(n, 1137 + i) for i, n in enumerate((300, 900, 2000))]
+    runs = []
+    for graph in (False, True):
+        ctx = make_ctx(states)
+        for i in range(len(states)):
+            ctx.set_vicsek(i, dt=0.01, mu=1.0, v0=0.05 * (i + 1), tau=0.5 * (i + 1), eta=0.3 * i)
+        if graph:
+            ctx.run_graph(2, 10, 4)
+        else:
+            ctx.vicsek_steps(40)
+        runs.append([(ctx.positions(i), ctx.directors(i)) for i in range(len(states))])
+    for (pa, na), (pb, nb) in zip(*runs):
+        np.testing.assert_array_equal(pa, pb)
+        np.testing.assert_array_equal(na, nb)

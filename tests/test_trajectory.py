@@ -15,6 +15,7 @@ import pytest
 import svmtest
 import curvedspacevertexmodel_amd as svm
 from curvedspacevertexmodel_amd.trajectory import logSpacedIntegers
+from test_oracle import move_np
 
 
 def test_netcdf_records_read_back(tmp_path):
@@ -88,3 +89,44 @@ def test_oracle_msd_and_overlap_of_a_rigid_rotation():
     msd, ov = om.msd_overlap(C, 1.0, 0.5)
     assert msd < 1e-12
     assert 0.9 < ov <= 1.0
+
+
+# ------------------------------------------------ spherical vectorial Vicsek --
+def polar_order(n):
+    return np.linalg.norm(n.sum(0)) / len(n)
+
+
+def test_oracle_vicsek_alignment_noise_and_motion():
+    """sphericalVectorialVicsek on the vertex model (oracle): directors stay unit
+    tangents; without noise the alignment with the neighbours raises the polar
+    order; with overwhelming noise (eta 1e6, rate dt/tau = 1) each new director
+    is the random direction alone, isotropic in the tangent plane; with mu = 0
+    the first move is the sphere move by dt v0 n."""
+    st = svmtest.seeded_state(400, 1170)
+    R = st["R"]
+    om = svmtest.OracleModel(st)
+    P0, N0 = om.positions(), om.directors()
+    om.vicsek(1, dt=0.01, mu=0.0, v0=0.5, tau=1.0, eta=0.0)
+    # every vertex but the two of each T1 flip (which repositions them) made the plain move
+    moved = np.abs(om.positions() - move_np(P0, 0.01 * 0.5 * N0, R)).max(1) > 1e-12
+    assert moved.sum() <= 2 * om.n_t1
+    om.vicsek(60, dt=0.01, mu=1.0, v0=0.05, tau=0.05, eta=0.0, step0=1)
+    P, N = om.positions(), om.directors()
+    np.testing.assert_allclose(np.linalg.norm(N, axis=1), 1.0, atol=1e-12)
+    assert np.abs(np.sum(N * P, 1)).max() < 1e-9
+    # local alignment: neighbouring directors agree far more than at the start
+    vn = om.topology(svmtest.RING_CAP)[0]
+    def neighbour_agreement(n):
+        return np.mean(np.sum(n[:, None, :] * n[vn], 2))
+    assert neighbour_agreement(N) > 0.8 and neighbour_agreement(N0) < 0.3
+    # noise only: a random tangent direction per vertex
+    om2 = svmtest.OracleModel(st)
+    om2.vicsek(1, dt=0.01, mu=0.0, v0=0.0, tau=0.01, eta=1e6)
+    P2, N2 = om2.positions(), om2.directors()
+    e1 = np.cross(P2, [0.0, 0.0, 1.0])
+    e1 /= np.linalg.norm(e1, axis=1)[:, None]
+    e2 = np.cross(P2 / R, e1)
+    ang = np.arctan2(np.sum(N2 * e2, 1), np.sum(N2 * e1, 1))
+    from scipy.stats import kstest
+    assert kstest((ang + np.pi) / (2 * np.pi), "uniform").pvalue > 1e-3
+    assert polar_order(N2) < 0.15
