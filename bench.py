@@ -209,6 +209,44 @@ def copy_probe(svm, n_bytes=1 << 28):
           f"{2 * n_bytes / ms / 1e6:.0f} GB/s", flush=True)
 
 
+def single_sphere(svm, n=262144, steps=400):
+    """The single-system workload: one sphere of n seeded cells (p0 3.8),
+    stabilised, then Brownian dynamics (dt 0.01, mu 1, T 0.001, the reference
+    driver's temperature) replayed from HIP graphs with the FP64 parity forces
+    and with the FP32 forces.  Also the GPU triangulation + topology build of
+    the sphere and one more full retriangulation of its cell points (the
+    per-move work of a Voronoi-type model)."""
+    ctx = svm.Context([dict(n_cells=n, a0=1.0, p0=3.8, kr=1.0, t1_threshold=0.1, seed=1137)])
+    pts = svm.generate_cell_points(n, 1137)
+    pts = pts[svm.spatial_order(pts)]
+    t0 = time.perf_counter()
+    ctx.build_from_cell_points(0, pts)
+    t_build = time.perf_counter() - t0
+    svm.delaunay_triangles(pts)
+    t0 = time.perf_counter()
+    svm.delaunay_triangles(pts)
+    t_dl = time.perf_counter() - t0
+    ctx.set_directors(0, svm.generate_directors(ctx.positions(0), 1137))
+    ctx.set_preferred_parameters(0, 1.0, 1.0)
+    ctx.set_brownian(0, dt=0.01, mu=1.0, T=0.0)
+    ctx.run_graph(0, 10, 10)
+    ctx.set_preferred_parameters(0, 1.0, 3.8)
+    ctx.set_brownian(0, dt=0.01, mu=1.0, T=0.001)
+    rate = {}
+    for prec, name in ((0, "fp64"), (1, "fp32")):
+        ctx.set_force_precision(prec)
+        ctx.run_graph(0, 10, 5)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.run_graph(0, 10, steps // 10)
+        ctx.synchronize()
+        rate[name] = n * (steps // 10 * 10) / (time.perf_counter() - t0)
+    ctx.set_force_precision(0)
+    return {"workload": "single-262144", "cells": n, "updater": "brownian", "dt": 0.01, "T": 0.001, "p0": 3.8,
+            "steps": steps // 10 * 10, "cell_steps_per_s": rate, "topology_build_s": t_build,
+            "retriangulation_ms": 1e3 * t_dl}
+
+
 def fire_comparison(svm, n_sys=1024, cap=10000):
     """Batched FIRE from seeded uniform random starts (1024 spheres, N
     log-uniform in [500, 5000], p0 uniform in [3.6, 3.9]) to rms force
@@ -255,6 +293,7 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fire", action="store_true", help="skip the learned-vs-plain FIRE comparison")
+    ap.add_argument("--no-single", action="store_true", help="skip the single 262,144-cell sphere block")
     ap.add_argument("--autotune", type=int, default=1, help="1: pick the launch schedule with svm_autotune")
     ap.add_argument("--fire-spheres", type=int, default=1024)
     ap.add_argument("--fire-cap", type=int, default=10000)
@@ -329,6 +368,8 @@ def main():
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ctx, params)
     del ctx
+    if not args.no_single:
+        result["single"] = single_sphere(svm)
     if not args.no_fire:
         result["fire"] = fire_comparison(svm, args.fire_spheres, args.fire_cap)
     print(json.dumps(result), flush=True)

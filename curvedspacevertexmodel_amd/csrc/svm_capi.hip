@@ -626,6 +626,8 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
     r |= dalloc(ctx, &d.cy, Cs);
     r |= dalloc(ctx, &d.cz, Cs);
     r |= dalloc(ctx, &d.ctrig, (size_t)4 * Cs);
+    r |= dalloc(ctx, &d.cvlist, Vs);
+    r |= dalloc(ctx, &d.cvseen, (Vs + 31) / 32);
     r |= dalloc(ctx, &d.a0, Cs);
     r |= dalloc(ctx, &d.p0, Cs);
     for (int b = 0; b < 2; ++b) {

@@ -1086,7 +1086,7 @@ __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int n
     __shared__ int s_lstart[SVM_COMMIT_LOGS + 1];    // prefix sums of items per log
     __shared__ unsigned s_seen[SVM_COMMIT_SEEN];
     __shared__ int s_part[SVM_TILE];
-    __shared__ int s_nverts, s_nmoved, s_nlogs;
+    __shared__ int s_nverts, s_nmoved, s_nlogs, s_gcount;
     __shared__ unsigned long long s_flips, s_skipped;
     const int t = threadIdx.x, nt = blockDim.x;
     const int gen = S->t1_gen;
@@ -1169,7 +1169,6 @@ __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int n
                 apply_item<SVM_T1_RC>(d, pcur, x, L.nvert, L.vid, L.vflag, L.vpos, L.vvn, L.vvcn, L.cid, L.ccnt,
                                       L.cring, dir_mode, vo, nullptr, 0, s_verts, &s_nverts, s_moved, &s_nmoved, 0);
         }
-        if (t == 0) { s_nverts = SVM_COMMIT_VERTS + 1; s_nmoved = SVM_COMMIT_VERTS + 1; }
     }
     if (use_group) {
         const int ng = G.nvert, nitems = ng + G.ncell;
@@ -1180,22 +1179,71 @@ __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int n
     }
     __threadfence_block();
     __syncthreads();
-    if (s_nverts > SVM_COMMIT_VERTS) {
+    const bool whole = nlogs > SVM_COMMIT_LOGS;
+    // every item of the committed logs and of the group log, one per thread
+    // (for the paths where the LDS lists overflowed)
+    auto for_items = [&](auto&& fn) {
+        const int nitems = s_lstart[nlogs];
+        int li = 0;
+        for (int x = t; x < nitems; x += nt) {
+            while (s_lstart[li + 1] <= x) ++li;
+            const T1Log& L = d.t1_log[s_logs[li]];
+            fn(x - s_lstart[li], L.nvert, L.vid, L.vflag, L.ccnt, &L.cring[0][0]);
+        }
+        if (use_group) {
+            const int ng = G.nvert;
+            for (int x = t; x < ng + G.ncell; x += nt) fn(x, ng, G.vid, G.vflag, G.ccnt, &G.cring[0][0]);
+        }
+    };
+    if (whole) {
         for (int x = vo + t; x < vo + nv; x += nt)
             if (!build_vln(d, x)) atomicMax(&S->err, 5);
-    } else {
+    } else if (s_nverts <= SVM_COMMIT_VERTS) {
         for (int i = t; i < s_nverts; i += nt)
             if (!build_vln(d, s_verts[i])) atomicMax(&S->err, 5);
+    } else {
+        // more than the LDS list holds: the distinct vertices of the written
+        // cells go to this sphere's region of a global list, deduplicated by
+        // a global bitmap that is cleared again afterwards
+        if (t == 0) s_gcount = 0;
+        __syncthreads();
+        for_items([&](int item, int nvert, const int*, const int*, const int* ccnt, const int* cring) {
+            if (item < nvert) return;
+            const int i = item - nvert, cnt = ccnt[i];
+            for (int k = 0; k < cnt; ++k) {
+                const int x = cring[i * SVM_T1_RC + k];
+                if ((unsigned)(x - vo) >= (unsigned)nv) continue;
+                const unsigned bit = 1u << (x & 31);
+                if (atomicOr(&d.cvseen[x >> 5], bit) & bit) continue;
+                d.cvlist[vo + atomicAdd(&s_gcount, 1)] = x;   // distinct vertices of the sphere: fewer than nv
+            }
+        });
+        __threadfence_block();
+        __syncthreads();
+        const int gcount = s_gcount;
+        for (int i = t; i < gcount; i += nt)
+            if (!build_vln(d, d.cvlist[vo + i])) atomicMax(&S->err, 5);
+        for (int i = t; i < gcount; i += nt) {
+            const int x = d.cvlist[vo + i];
+            atomicAnd(&d.cvseen[x >> 5], ~(1u << (x & 31)));
+        }
     }
     if (dir_mode) {
-        const int all = s_nmoved > SVM_COMMIT_VERTS;
-        const int hi = all ? nv : s_nmoved;
-        for (int i = t; i < hi; i += nt) {
-            int x = all ? vo + i : s_moved[i];
+        auto redo = [&](int x) {
             double p[3] = {d.px[pcur][x], d.py[pcur][x], d.pz[pcur][x]};
             double nI[3] = {d.nx[nin][x], d.ny[nin][x], d.nz[nin][x]}, nO[3];
             director_update(S, x - vo, p, nI, nO);
             d.nx[nout][x] = nO[0]; d.ny[nout][x] = nO[1]; d.nz[nout][x] = nO[2];
+        };
+        if (whole) {
+            for (int i = t; i < nv; i += nt) redo(vo + i);
+        } else if (s_nmoved <= SVM_COMMIT_VERTS) {
+            for (int i = t; i < s_nmoved; i += nt) redo(s_moved[i]);
+        } else {
+            // moved vertices are distinct across the logs: straight from the items
+            for_items([&](int item, int nvert, const int* vid, const int* vflag, const int*, const int*) {
+                if (item < nvert && (vflag[item] & 1)) redo(vid[item]);
+            });
         }
     }
     if (t == 0) {

@@ -158,3 +158,42 @@ def test_large_batch_fp32_warm_start_fire_active():
         assert err[i, 5] < 1e-5, (i, err[i])
     c = ctx.counters()
     print("T1 flips", int(c[:, 0].sum()), "sequential sphere-steps", int(c[:, 4].sum()), flush=True)
+
+
+def test_t1_commit_overflow_matches_sequential():
+    """A 262,144-cell sphere right after stabilisation flips ~280 edges per
+    step, so the commit's vertices to rebuild outnumber its LDS list and go
+    through the global list; the trajectory equals the sequential pass's bit
+    for bit."""
+    n, seed = 262144, 1137
+    pts = svm.generate_cell_points(n, seed)
+    pts = pts[svm.spatial_order(pts)]
+    runs = []
+    for sequential in (False, True):
+        ctx = svm.Context([dict(n_cells=n, a0=1.0, p0=3.8, kr=1.0, t1_threshold=0.1, seed=seed)])
+        ctx.build_from_cell_points(0, pts)
+        ctx.set_t1_mode(sequential)
+        ctx.set_preferred_parameters(0, 1.0, 1.0)
+        ctx.set_brownian(0, dt=0.01, mu=1.0, T=0.0)
+        ctx.brownian_steps(100)
+        ctx.set_preferred_parameters(0, 1.0, 3.8)
+        ctx.set_brownian(0, dt=0.01, mu=1.0, T=0.001)
+        ctx.brownian_steps(50)              # the heaviest rearrangement, often resolved sequentially
+        c0 = ctx.counters()[0].copy()
+        t0 = time.perf_counter()
+        ctx.brownian_steps(20)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        c = ctx.counters()[0] - c0
+        flips, seq = int(c[0]), int(c[4])
+        print(f"{'sequential' if sequential else 'dry runs'}: {flips / 20:.0f} flips per step, "
+              f"{seq} sequential passes in 20 steps, {dt / 20 * 1e3:.2f} ms per step", flush=True)
+        runs.append((ctx.positions(0), ctx.get_topology(0, 32), flips, seq))
+    # ~280 flips a step: with no LDS deduplication above 131,072 vertices the
+    # commit's list of vertices to rebuild (about 20 ring entries per flip)
+    # exceeds its 4096 entries whenever the step is committed from dry runs
+    assert runs[0][2] == runs[1][2] > 200 * 20
+    assert runs[0][3] < 10 and runs[1][3] == 20
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    for a, b in zip(runs[0][1], runs[1][1]):
+        np.testing.assert_array_equal(a, b)
