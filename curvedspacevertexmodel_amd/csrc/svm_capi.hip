@@ -332,6 +332,44 @@ int enqueue_adam_iteration(svm_context* ctx) {
     return last_launch(ctx);
 }
 
+// `k` minimiser iterations (kind 3 FIRE, 4 Adam) replayed from a captured
+// HIP graph when k is even: the ping-pong position buffers then end where they
+// started, so the graph replays whenever the context is in that state, as in
+// svm_run_graph.  Odd k, or profiling, run eagerly.  Same kernels in the same
+// order either way.
+template <class F>
+int minimiser_iterations(svm_context* ctx, int kind, int k, F enqueue) {
+    if ((k & 1) || ctx->profiling) {
+        for (int i = 0; i < k; ++i) {
+            int r = enqueue(ctx);
+            if (r) return r;
+        }
+        return SVM_OK;
+    }
+    auto key = std::make_pair(kind, k * 4 + ctx->pcur * 2 + ctx->ncur);
+    auto it = ctx->graphs.find(key);
+    if (it == ctx->graphs.end()) {
+        hipGraph_t g;
+        HIPCHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < k; ++i) {
+            int r = enqueue(ctx);
+            if (r) {
+                hipGraph_t junk;
+                (void)hipStreamEndCapture(ctx->stream, &junk);
+                return r;
+            }
+        }
+        HIPCHECK(ctx, hipStreamEndCapture(ctx->stream, &g));
+        hipGraphExec_t ge;
+        HIPCHECK(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        ctx->graphs[key] = ge;
+        ctx->graph_defs[key] = g;
+        it = ctx->graphs.find(key);
+    }
+    HIPCHECK(ctx, hipGraphLaunch(it->second, ctx->stream));
+    return SVM_OK;
+}
+
 int require_topology(svm_context* ctx) {
     for (int s = 0; s < ctx->nsys; ++s)
         if (!ctx->have_topology[s]) return fail(ctx, SVM_ERR_STATE, "sphere " + std::to_string(s) + " has no topology");
@@ -1411,10 +1449,8 @@ int svm_fire_minimize(svm_context* ctx, int32_t check_every) {
         bool any = false;
         for (int i = 0; i < ctx->nsys; ++i) any |= !s[i].f_frozen;
         if (!any) break;
-        for (int k = 0; k < check_every; ++k) {
-            r = enqueue_fire_iteration(ctx);
-            if (r) return r;
-        }
+        r = minimiser_iterations(ctx, 3, check_every, enqueue_fire_iteration);
+        if (r) return r;
         r = download_sys(ctx, s);
         if (r) return r;
         r = check_device_errors(ctx);
@@ -1488,10 +1524,8 @@ int svm_adam_minimize(svm_context* ctx, int32_t check_every) {
         bool any = false;
         for (int i = 0; i < ctx->nsys; ++i) any |= !s[i].f_frozen;
         if (!any) break;
-        for (int k = 0; k < check_every; ++k) {
-            r = enqueue_adam_iteration(ctx);
-            if (r) return r;
-        }
+        r = minimiser_iterations(ctx, 4, check_every, enqueue_adam_iteration);
+        if (r) return r;
         r = download_sys(ctx, s);
         if (r) return r;
         r = check_device_errors(ctx);
