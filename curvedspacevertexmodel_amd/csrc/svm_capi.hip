@@ -1837,7 +1837,7 @@ int mlp_reserve(svm_mlp* m, size_t n) {
 
 // forward on device rows X[0..n) -> Y; with training buffers when `train`
 int mlp_forward_dev(svm_mlp* m, const float* X, int n, float* Y, bool train) {
-    const int blocks = std::max(1, std::min(4096, (n + 63) / 64));
+    const int blocks = std::max(1, std::min(1024, (n + 63) / 64));   // 512-2048 measure alike; weights load once per wave
     hipLaunchKernelGGL(svm::k_mlp_forward, dim3(blocks), dim3(256), 0, m->stream, mlp_params(m), X, n, Y,
                        train ? m->Z : nullptr, train ? m->H : nullptr);
     return hipGetLastError() == hipSuccess ? SVM_OK : SVM_ERR_HIP;

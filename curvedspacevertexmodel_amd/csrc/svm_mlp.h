@@ -67,9 +67,14 @@ __device__ __host__ inline float bf2f(unsigned short b) {
     __builtin_memcpy(&f, &u, 4);
     return f;
 }
-__device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative with the hardware exponential and reciprocal
+// (v_exp_f32, v_rcp_f32: about 1 ulp each); the activations are rounded to
+// bf16 right after, so an IEEE-exact divide would buy nothing and cost about
+// nine instructions per activation.
+__device__ inline float sigmoid_fast(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ inline float silu(float z) { return z * sigmoid_fast(z); }
 __device__ inline float dsilu(float z) {
-    float s = 1.0f / (1.0f + __expf(-z));
+    float s = sigmoid_fast(z);
     return s * (1.0f + z * (1.0f - s));
 }
 
@@ -80,16 +85,23 @@ __device__ inline mlp_v4s mlp_ldw(const unsigned short* W, int in_dim, int ob, i
     r[0] = (short)p[0]; r[1] = (short)p[1]; r[2] = (short)p[2]; r[3] = (short)p[3];
     return r;
 }
+// Four floats to the bf16 B operand of the MFMA, round to nearest even (as
+// f2bf for finite values), two per v_cvt_pk_bf16_f32.
+typedef float mlp_v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 mlp_v2bf __attribute__((ext_vector_type(2)));
 __device__ inline mlp_v4s mlp_pack(float a, float b, float c, float d) {
-    mlp_v4s r;
-    r[0] = (short)f2bf(a); r[1] = (short)f2bf(b); r[2] = (short)f2bf(c); r[3] = (short)f2bf(d);
-    return r;
+    const mlp_v2bf lo = __builtin_convertvector((mlp_v2f){a, b}, mlp_v2bf);
+    const mlp_v2bf hi = __builtin_convertvector((mlp_v2f){c, d}, mlp_v2bf);
+    struct {
+        mlp_v2bf lo, hi;
+    } pair = {lo, hi};
+    return __builtin_bit_cast(mlp_v4s, pair);
 }
 
 // Forward pass over n cells: X [n][16] raw descriptors -> Y [n][2].  With Z
 // (and H) non-null, also the hidden pre-activations Z [n][192] and the bf16
 // activations fed to layers 2..4 (as floats) H [n][192] for training.
-__global__ __launch_bounds__(256) void k_mlp_forward(MlpParams m, const float* X, int n, float* Y, float* Z,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_mlp_forward(MlpParams m, const float* X, int n, float* Y, float* Z,
                                                      float* H) {
     const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), nwaves = gridDim.x * 4;
