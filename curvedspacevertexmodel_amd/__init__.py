@@ -40,6 +40,7 @@ EXPORTS = [
     "svm_adam_set_parameters", "svm_adam_minimize", "svm_adam_get_state", "svm_adam_set_max_iterations",
     "svm_dyn_set_reference", "svm_dyn_measure", "svm_nc_open", "svm_nc_write", "svm_nc_num_records", "svm_nc_close",
     "svm_set_vicsek", "svm_vicsek_steps",
+    "svm_build_voronoi_from_cell_points", "svm_voronoi_forces", "svm_get_voronoi_forces",
 ]
 
 
@@ -305,6 +306,22 @@ class Context:
     def build_from_cell_points(self, sys, points):
         pts = np.ascontiguousarray(points, np.float64)
         self._ck(lib().svm_build_from_cell_points(self.h, C.c_int32(sys), _dp(pts)), "svm_build_from_cell_points")
+
+    def build_voronoi_from_cell_points(self, sys, points):
+        """Voronoi model: triangulate the generators, vertices at the circumcentres."""
+        pts = np.ascontiguousarray(points, np.float64)
+        self._ck(lib().svm_build_voronoi_from_cell_points(self.h, C.c_int32(sys), _dp(pts)),
+                 "svm_build_voronoi_from_cell_points")
+
+    def voronoi_forces(self):
+        """Generator forces of every sphere (built as Voronoi models); returns a list of [N][3] arrays."""
+        self._ck(lib().svm_voronoi_forces(self.h), "svm_voronoi_forces")
+        out = []
+        for s in range(self.n):
+            f = np.zeros((self.n_cells[s], 3), np.float64)
+            self._ck(lib().svm_get_voronoi_forces(self.h, C.c_int32(s), _dp(f)), "svm_get_voronoi_forces")
+            out.append(f)
+        return out
 
     def delaunay_triangles(self, sys):
         out = np.zeros((2 * self.n_cells[sys] - 4, 3), np.int32)

@@ -41,6 +41,7 @@ struct svm_context {
     std::vector<SysParams> layout;   // sizes/offsets (host copy of the constant part)
     std::vector<int> have_topology;
     std::vector<int> have_adam;       // svm_adam_set_parameters called for the sphere
+    std::vector<int> voronoi;         // sphere built by svm_build_voronoi_from_cell_points
     int Cs = 0, Vs = 0, cap = 0, nct = 0, nvt = 0;
     int cluster_blocks = 256;        // T1 dry-run grid (persistent loop over candidates)
     int graph_steps = 10;            // steps per captured graph when the caller passes 0
@@ -85,7 +86,12 @@ int dalloc(svm_context* ctx, T** p, size_t n) {
     size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
     hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess) return fail(ctx, SVM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    // hipMemset may return before the fill is done, and the null stream it
+    // runs on is not ordered with the context's non-blocking stream: wait, so
+    // that no later work on that stream can be overwritten by the zeros
+    // (buffers allocated lazily, mid-run, would otherwise race)
     e = hipMemset(q, 0, bytes);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) return fail(ctx, SVM_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
     ctx->allocs.push_back(q);
     *p = (T*)q;
@@ -609,6 +615,7 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
     ctx->layout.resize(n_systems);
     ctx->have_topology.assign(n_systems, 0);
     ctx->have_adam.assign(n_systems, 0);
+    ctx->voronoi.assign(n_systems, 0);
     ctx->triangles.resize(n_systems);
     int cap = 0;
     long long Cs = 0, Vs = 0;
@@ -876,6 +883,7 @@ int svm_set_topology(svm_context* ctx, int32_t sys, int32_t n_vertices, const do
     r = svm_synchronize(ctx);
     if (r) return r;
     ctx->have_topology[sys] = 1;
+    ctx->voronoi[sys] = 0;
     return SVM_OK;
 }
 
@@ -1013,7 +1021,10 @@ int svm_get_topology(svm_context* ctx, int32_t sys, int32_t* vertex_neighbors, i
     return SVM_OK;
 }
 
-int svm_build_from_cell_points(svm_context* ctx, int32_t sys, const double* cell_points) {
+// The topology of sphere `sys` from the GPU triangulation of its cell
+// points, vertices at the triangles' centroids (circum 0, the vertex model)
+// or circumcentres (circum 1, the Voronoi model, whose generators are kept).
+static int build_topology(svm_context* ctx, int32_t sys, const double* cell_points, int circum) {
     if (!valid_sys(ctx, sys) || !cell_points) return fail(ctx, SVM_ERR_ARG, "null argument");
     const SysParams& L = ctx->layout[sys];
     const int n = L.nc;
@@ -1027,7 +1038,16 @@ int svm_build_from_cell_points(svm_context* ctx, int32_t sys, const double* cell
     const int tb = 256;
     hipLaunchKernelGGL(svm::k_dl_vertices, dim3((o.ntri + tb - 1) / tb), dim3(tb), 0, ctx->stream, ctx->d, sys,
                        ctx->pcur, (const double*)o.P, o.ntri, ctx->cap, (const int*)o.T, (const int*)o.nbr,
-                       (const int*)o.deg, (const int*)o.tid, radius, o.err);
+                       (const int*)o.deg, (const int*)o.tid, radius, o.err, circum);
+    if (circum) {
+        if (!ctx->d.gen) {
+            r = dalloc(ctx, &ctx->d.gen, 3 * (size_t)ctx->Cs);
+            if (!r) r = dalloc(ctx, &ctx->d.cf, 3 * (size_t)ctx->Cs);
+            if (r) return r;
+        }
+        hipLaunchKernelGGL(svm::k_store_generators, dim3((n + tb - 1) / tb), dim3(tb), 0, ctx->stream, ctx->d, sys,
+                           (const double*)o.P, n);
+    }
     hipLaunchKernelGGL(svm::k_dl_rings, dim3((n + tb - 1) / tb), dim3(tb), 0, ctx->stream, ctx->d, sys, n, ctx->cap,
                        (const int*)o.deg, (const int*)o.tid);
     hipLaunchKernelGGL(svm::k_build_vln, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d, sys);
@@ -1043,7 +1063,40 @@ int svm_build_from_cell_points(svm_context* ctx, int32_t sys, const double* cell
     if (r) return r;
     ctx->triangles[sys] = std::move(T);
     ctx->have_topology[sys] = 1;
+    ctx->voronoi[sys] = circum;
     return SVM_OK;
+}
+
+int svm_build_from_cell_points(svm_context* ctx, int32_t sys, const double* cell_points) {
+    return build_topology(ctx, sys, cell_points, 0);
+}
+
+int svm_build_voronoi_from_cell_points(svm_context* ctx, int32_t sys, const double* cell_points) {
+    return build_topology(ctx, sys, cell_points, 1);
+}
+
+int svm_voronoi_forces(svm_context* ctx) {
+    if (!ctx) return SVM_ERR_ARG;
+    int r = require_topology(ctx);
+    if (r) return r;
+    for (int s = 0; s < ctx->nsys; ++s)
+        if (!ctx->voronoi[s])
+            return fail(ctx, SVM_ERR_STATE, "sphere " + std::to_string(s) + " was not built as a Voronoi model");
+    r = svm_compute_forces(ctx, ctx->force_fp32);
+    if (r) return r;
+    hipLaunchKernelGGL(svm::k_voronoi_chain, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, ctx->d);
+    r = last_launch(ctx);
+    if (r) return r;
+    return check_device_errors(ctx);
+}
+
+int svm_get_voronoi_forces(svm_context* ctx, int32_t sys, double* forces) {
+    if (!valid_sys(ctx, sys) || !forces) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    if (!ctx->voronoi[sys]) return fail(ctx, SVM_ERR_STATE, "sphere was not built as a Voronoi model");
+    const SysParams& L = ctx->layout[sys];
+    const double* cf = ctx->d.cf;
+    const size_t Cs = ctx->Cs;
+    return download_rows(ctx, L.nc, L.co, cf, cf + Cs, cf + 2 * Cs, forces);
 }
 
 int svm_cell_list(int32_t n_points, const double* points, double cell_size, int32_t* grid_dim, int32_t* bin_of,

@@ -81,7 +81,8 @@ struct Dev {
     double *am, *av;              // Adam moment estimates: three planes of Vs each (null until Adam is set up)
     int32_t* cvlist;              // T1 commit: per-sphere region of vertices to rebuild when the LDS list overflows
     uint32_t* cvseen;             // bitmap over all vertices for that list (all zero between commits)
-    double *dref, *dpart, *dout;  // dynamical analysis: reference cell positions (3 planes of Cs), tile partials, results
+    double *dref, *dpart, *dout;
+    double *gen, *cf;             // Voronoi model: generator points and their forces (3 planes of Cs each; lazy)  // dynamical analysis: reference cell positions (3 planes of Cs), tile partials, results
     double *fx, *fy, *fz;
     int4* vcn;
     int4* vn;

@@ -360,21 +360,36 @@ __global__ __launch_bounds__(256) void k_dl_tid(int n, int cap, const int* nbr, 
 // order as the host path), vertex->cell and vertex->vertex lists; per cell
 // its ring, starting at its smallest triangle and walking against the CCW
 // star order (clockwise seen from outside).
+// circum 0: the vertex of triangle t is R normalize(centroid), as the vertex
+// model places it (convexHullCGAL.cpp:67-80); circum 1: its circumcentre on
+// the sphere, R normalize((b - a) x (c - a)) for the CCW triangle (a, b, c),
+// the Voronoi vertex of the three generators.
 __global__ __launch_bounds__(256) void k_dl_vertices(Dev d, int sys, int pout, const double* P, int ntri, int cap,
                                                      const int* T, const int* nbr, const int* deg, const int* tid,
-                                                     double radius, int* err) {
+                                                     double radius, int* err, int circum) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntri) return;
     const SysParams& S = d.sys[sys];
     int c3[3] = {T[3 * (size_t)t], T[3 * (size_t)t + 1], T[3 * (size_t)t + 2]};
     double c[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < 3; ++k) {
-        const double* p = P + 3 * (size_t)c3[k];
-        c[0] += p[0];
-        c[1] += p[1];
-        c[2] += p[2];
+    if (circum) {
+        const double* a = P + 3 * (size_t)c3[0];
+        const double* b = P + 3 * (size_t)c3[1];
+        const double* e = P + 3 * (size_t)c3[2];
+        const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        const double w[3] = {e[0] - a[0], e[1] - a[1], e[2] - a[2]};
+        c[0] = u[1] * w[2] - u[2] * w[1];
+        c[1] = u[2] * w[0] - u[0] * w[2];
+        c[2] = u[0] * w[1] - u[1] * w[0];
+    } else {
+        for (int k = 0; k < 3; ++k) {
+            const double* p = P + 3 * (size_t)c3[k];
+            c[0] += p[0];
+            c[1] += p[1];
+            c[2] += p[2];
+        }
+        for (int k = 0; k < 3; ++k) c[k] = (1.0 / 3) * c[k];
     }
-    for (int k = 0; k < 3; ++k) c[k] = (1.0 / 3) * c[k];
     double nrm = __builtin_sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
     const int v = S.vo + t;
     d.px[pout][v] = (radius / nrm) * c[0];
@@ -410,6 +425,73 @@ __global__ __launch_bounds__(256) void k_dl_rings(Dev d, int sys, int n, int cap
         d.ring[(size_t)k * d.Cs + S.co + c] = x;
     }
     d.cnn[S.co + c] = dg;
+}
+
+}  // namespace svm
+
+
+namespace svm {
+
+// Generator points of sphere `sys` (AoS [n][3]) into the SoA planes d.gen.
+__global__ __launch_bounds__(256) void k_store_generators(Dev d, int sys, const double* P, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = d.sys[sys].co + i;
+    const size_t Cs = (size_t)d.Cs;
+    d.gen[c] = P[3 * (size_t)i];
+    d.gen[Cs + c] = P[3 * (size_t)i + 1];
+    d.gen[2 * Cs + c] = P[3 * (size_t)i + 2];
+}
+
+// Voronoi-model force on each generator: the chain rule from the vertex
+// forces F_v = -dE/dv (k_vertex<VM_FORCE>) through the circumcentre vertices
+// v = s R n / |n|, n = (b - a) x (c - a) of each Delaunay triangle (a, b, c)
+// (s = +1 for an outward n).  With dn/da = [c - b]x, dn/db = [a - c]x,
+// dn/dc = [b - a]x and y = s (R / |n|) (F_v - n^ (n^ . F_v)), generator a
+// receives (b - c) x y, b receives (c - a) x y and c receives (a - b) x y from
+// every vertex of its cell; the sum is projected on the tangent plane.
+__global__ __launch_bounds__(SVM_TILE) void k_voronoi_chain(Dev d) {
+    const int tile = blockIdx.x;
+    const SysParams* S = d.sys + d.ctile_sys[tile];
+    const int c = tile * SVM_TILE + threadIdx.x;
+    if (c - S->co >= S->nc) return;
+    const size_t Cs = (size_t)d.Cs;
+    const double R = S->R;
+    auto gen = [&](int i, double* p) { p[0] = d.gen[i]; p[1] = d.gen[Cs + i]; p[2] = d.gen[2 * Cs + i]; };
+    double F[3] = {0.0, 0.0, 0.0};
+    const int n = d.cnn[c];
+    for (int k = 0; k < n; ++k) {
+        const int v = d.ring[(size_t)k * Cs + c];
+        const int4 t = d.vcn[v];
+        double pa[3], pb[3], pc[3];
+        gen(t.x, pa);
+        gen(t.y, pb);
+        gen(t.z, pc);
+        const double u[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+        const double w[3] = {pc[0] - pa[0], pc[1] - pa[1], pc[2] - pa[2]};
+        const double nv[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+        const double nn = __builtin_sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+        const double nh[3] = {nv[0] / nn, nv[1] / nn, nv[2] / nn};
+        const double out = nv[0] * (pa[0] + pb[0] + pc[0]) + nv[1] * (pa[1] + pb[1] + pc[1]) +
+                           nv[2] * (pa[2] + pb[2] + pc[2]);
+        const double sc = (out < 0.0 ? -1.0 : 1.0) * R / nn;
+        const double Fv[3] = {d.fx[v], d.fy[v], d.fz[v]};
+        const double fn = nh[0] * Fv[0] + nh[1] * Fv[1] + nh[2] * Fv[2];
+        const double y[3] = {sc * (Fv[0] - nh[0] * fn), sc * (Fv[1] - nh[1] * fn), sc * (Fv[2] - nh[2] * fn)};
+        double q[3];
+        if (c == t.x) { q[0] = pb[0] - pc[0]; q[1] = pb[1] - pc[1]; q[2] = pb[2] - pc[2]; }
+        else if (c == t.y) { q[0] = pc[0] - pa[0]; q[1] = pc[1] - pa[1]; q[2] = pc[2] - pa[2]; }
+        else { q[0] = pa[0] - pb[0]; q[1] = pa[1] - pb[1]; q[2] = pa[2] - pb[2]; }
+        F[0] += q[1] * y[2] - q[2] * y[1];
+        F[1] += q[2] * y[0] - q[0] * y[2];
+        F[2] += q[0] * y[1] - q[1] * y[0];
+    }
+    double p[3];
+    gen(c, p);
+    const double pf = (p[0] * F[0] + p[1] * F[1] + p[2] * F[2]) / (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    d.cf[c] = F[0] - pf * p[0];
+    d.cf[Cs + c] = F[1] - pf * p[1];
+    d.cf[2 * Cs + c] = F[2] - pf * p[2];
 }
 
 }  // namespace svm

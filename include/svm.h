@@ -343,6 +343,23 @@ int svm_cell_descriptors(svm_context* ctx, int32_t sys, float* X, float* frames)
  * The warm start never leaves a sphere worse than plain FIRE would start. */
 int svm_warm_start(svm_context* ctx, svm_mlp* mlp, double scale, double max_step, int32_t* reverted);
 
+/* ---- Voronoi model -------------------------------------------------------
+ * The reference's sphericalVoronoi (src/model/sphericalVoronoi.cpp:8-12) has
+ * only the topology: the convex hull of the cell points after every move.
+ * This adds its forces.  svm_build_voronoi_from_cell_points triangulates the
+ * generators on the GPU as svm_build_from_cell_points does, but places each
+ * vertex at its triangle's circumcentre on the sphere, R n / |n| with
+ * n = (b - a) x (c - a), and keeps the generators; svm_voronoi_forces then
+ * computes, for every sphere (all must be built this way), the geometry, the
+ * vertex forces F_v = -dE/dv of E = sum Kr (A - A0)^2 + (P - P0)^2 (in the
+ * context's force precision) and their chain rule through the circumcentres
+ * to the generators, projected on the tangent plane; svm_get_voronoi_forces
+ * copies sphere `sys`'s generator forces out ([N][3]).  Call the builder
+ * again after moving the generators (the per-move retriangulation). */
+int svm_build_voronoi_from_cell_points(svm_context* ctx, int32_t sys, const double* cell_points);
+int svm_voronoi_forces(svm_context* ctx);
+int svm_get_voronoi_forces(svm_context* ctx, int32_t sys, double* forces);
+
 /* ---- trajectory analysis and output -------------------------------------
  * dynamicalFeatures (src/analysis/dynamicalFeatures.cpp:7-53) on the GPU.
  * svm_dyn_set_reference snapshots every sphere's current cell positions (the
