@@ -37,6 +37,7 @@ EXPORTS = [
     "svm_cell_list", "svm_mlp_create", "svm_mlp_destroy", "svm_mlp_set_parameters", "svm_mlp_get_parameters",
     "svm_mlp_forward", "svm_mlp_gradient", "svm_mlp_train", "svm_mlp_time_forward", "svm_cell_descriptors",
     "svm_warm_start", "svm_fire_set_max_iterations", "svm_set_tuning", "svm_get_tuning", "svm_autotune",
+    "svm_set_lanes", "svm_get_lanes",
     "svm_adam_set_parameters", "svm_adam_minimize", "svm_adam_get_state", "svm_adam_set_max_iterations",
     "svm_dyn_set_reference", "svm_dyn_measure", "svm_nc_open", "svm_nc_write", "svm_nc_num_records", "svm_nc_close",
     "svm_set_vicsek", "svm_vicsek_steps",
@@ -87,6 +88,9 @@ def lib():
         L.svm_last_error.argtypes = [C.c_void_p]
         _LIB = L
     return _LIB
+
+
+MAX_LANES = 4   # SVM_MAX_LANES of include/svm.h
 
 
 class SVMError(RuntimeError):
@@ -495,11 +499,22 @@ class Context:
     def tuning(self):
         a, b, c = C.c_int32(0), C.c_int32(0), C.c_int32(0)
         self._ck(lib().svm_get_tuning(self.h, C.byref(a), C.byref(b), C.byref(c)), "svm_get_tuning")
-        return dict(cluster_blocks=a.value, graph_steps=b.value, xcd_mapping=c.value)
+        return dict(cluster_blocks=a.value, graph_steps=b.value, xcd_mapping=c.value, lanes=self.lanes()[0])
+
+    def set_lanes(self, n_lanes):
+        """Cut the batch into n_lanes sphere ranges whose steps run on streams of their own."""
+        self._ck(lib().svm_set_lanes(self.h, C.c_int32(n_lanes)), "svm_set_lanes")
+
+    def lanes(self):
+        """(number of lanes, first sphere of each lane)."""
+        n = C.c_int32(0)
+        first = np.zeros(8, np.int32)
+        self._ck(lib().svm_get_lanes(self.h, C.byref(n), _ip(first)), "svm_get_lanes")
+        return n.value, first[:n.value].copy()
 
     def autotune(self, updater=1, steps_per_trial=60):
-        """Returns the trial table [(cluster_blocks, graph_steps, xcd_mapping, ms/step)]."""
-        tab = np.zeros((32, 4), np.float64)
+        """Returns the trial table [(lanes, cluster_blocks, graph_steps, xcd_mapping, ms/step)]."""
+        tab = np.zeros((32, 5), np.float64)
         n = C.c_int32(0)
         self._ck(lib().svm_autotune(self.h, C.c_int32(updater), C.c_int32(steps_per_trial), _dp(tab),
                                     C.c_int32(32), C.byref(n)), "svm_autotune")

@@ -58,6 +58,18 @@ struct svm_context {
     // force arithmetic of every force evaluation: 0 FP64 reference expressions
     // (parity), 1 FP32 vector form, 2 FP64 vector form
     int force_fp32 = 0;
+    // Lanes of the dynamics step (svm_set_lanes): contiguous sphere ranges
+    // whose steps are enqueued on streams of their own, forked from `stream`
+    // and joined back into it.  Lane 0 runs on `stream`; lane k > 0 on
+    // lane_streams[k-1] with its own T1 candidate lists lane_t1[k-1].
+    int nlanes = 1;
+    std::vector<int> lane_s0;                 // first sphere of each lane, then nsys
+    std::vector<hipStream_t> lane_streams;
+    std::vector<hipEvent_t> lane_joins;
+    hipEvent_t lane_fork = nullptr;
+    std::vector<Dev> lane_t1;                 // only the t1_* buffers are used
+    std::map<std::pair<int, int>, std::vector<hipGraphExec_t>> lane_graphs;
+    std::map<std::pair<int, int>, std::vector<hipGraph_t>> lane_graph_defs;
     // profiling
     bool profiling = false;
     std::map<std::string, std::pair<double, long>> timing;
@@ -86,12 +98,13 @@ int dalloc(svm_context* ctx, T** p, size_t n) {
     size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
     hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess) return fail(ctx, SVM_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-    // hipMemset may return before the fill is done, and the null stream it
-    // runs on is not ordered with the context's non-blocking stream: wait, so
-    // that no later work on that stream can be overwritten by the zeros
-    // (buffers allocated lazily, mid-run, would otherwise race)
-    e = hipMemset(q, 0, bytes);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
+    // the fill is complete before the buffer is handed out, so no later work
+    // on any stream can be overwritten by the zeros (buffers allocated
+    // lazily, mid-run, would otherwise race).  The library's own stream, not
+    // the null stream: the library leaves every hardware queue of the
+    // process (a handful) to its own streams and the lanes'.
+    e = hipMemsetAsync(q, 0, bytes, ctx->own_stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->own_stream);
     if (e != hipSuccess) return fail(ctx, SVM_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
     ctx->allocs.push_back(q);
     *p = (T*)q;
@@ -197,8 +210,14 @@ struct Timer {
 void drop_graphs(svm_context* ctx) {
     for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
     for (auto& g : ctx->graph_defs) (void)hipGraphDestroy(g.second);
+    for (auto& g : ctx->lane_graphs)
+        for (auto e : g.second) (void)hipGraphExecDestroy(e);
+    for (auto& g : ctx->lane_graph_defs)
+        for (auto e : g.second) (void)hipGraphDestroy(e);
     ctx->graphs.clear();
     ctx->graph_defs.clear();
+    ctx->lane_graphs.clear();
+    ctx->lane_graph_defs.clear();
 }
 
 int last_launch(svm_context* ctx) {
@@ -208,96 +227,188 @@ int last_launch(svm_context* ctx) {
 }
 
 // ---- step building blocks ----
-int launch_geometry(svm_context* ctx, int skip_frozen) {
+// What a launch covers and where it is enqueued: the whole batch on the
+// context's stream, or one lane of the dynamics step on the lane's stream.
+struct Target {
+    Dev d;
+    hipStream_t st;
+};
+Target whole(svm_context* ctx) { return Target{ctx->d, ctx->stream}; }
+
+int launch_geometry(svm_context* ctx, const Target& w, int skip_frozen) {
     {
         Timer t(ctx, "k_geometry");
-        hipLaunchKernelGGL(svm::k_geometry, dim3(xcd_grid(ctx->nct)), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur,
+        hipLaunchKernelGGL(svm::k_geometry, dim3(xcd_grid(w.d.ctn)), dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur,
                            skip_frozen);
         int r = last_launch(ctx);
         if (r) return r;
     }
     Timer t(ctx, "k_fan_fix");
-    hipLaunchKernelGGL(svm::k_fan_fix, dim3(xcd_grid(ctx->nct)), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, skip_frozen);
+    hipLaunchKernelGGL(svm::k_fan_fix, dim3(xcd_grid(w.d.ctn)), dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, skip_frozen);
     return last_launch(ctx);
 }
+int launch_geometry(svm_context* ctx, int skip_frozen) { return launch_geometry(ctx, whole(ctx), skip_frozen); }
 
 template <int MODE>
-int launch_vertex(svm_context* ctx, int pout, int store_forces, const char* name) {
+int launch_vertex(svm_context* ctx, const Target& w, int pout, int store_forces, const char* name) {
     Timer t(ctx, name);
+    const dim3 grid(xcd_grid(w.d.vtn));
     if (ctx->force_fp32 == 1)
-        hipLaunchKernelGGL((svm::k_vertex<MODE, float>), dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0, ctx->stream,
-                           ctx->d, ctx->pcur, pout, ctx->ncur, store_forces);
+        hipLaunchKernelGGL((svm::k_vertex<MODE, float>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
+                           ctx->ncur, store_forces);
     else if (ctx->force_fp32 == 2)
-        hipLaunchKernelGGL((svm::k_vertex<MODE, svm::VecF64>), dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0,
-                           ctx->stream, ctx->d, ctx->pcur, pout, ctx->ncur, store_forces);
+        hipLaunchKernelGGL((svm::k_vertex<MODE, svm::VecF64>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
+                           ctx->ncur, store_forces);
     else
-        hipLaunchKernelGGL((svm::k_vertex<MODE, double>), dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0, ctx->stream,
-                           ctx->d, ctx->pcur, pout, ctx->ncur, store_forces);
+        hipLaunchKernelGGL((svm::k_vertex<MODE, double>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
+                           ctx->ncur, store_forces);
     return last_launch(ctx);
 }
+template <int MODE>
+int launch_vertex(svm_context* ctx, int pout, int store_forces, const char* name) {
+    return launch_vertex<MODE>(ctx, whole(ctx), pout, store_forces, name);
+}
 
-int launch_t1(svm_context* ctx, int pcur, int nin, int nout, int dir, int advance_step, int fire_mode) {
+int launch_t1(svm_context* ctx, const Target& w, int pcur, int nin, int nout, int dir, int advance_step,
+              int fire_mode) {
     {
         Timer t(ctx, "k_t1_detect");
-        hipLaunchKernelGGL(svm::k_t1_detect, dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0, ctx->stream, ctx->d, pcur, nin, nout,
+        hipLaunchKernelGGL(svm::k_t1_detect, dim3(xcd_grid(w.d.vtn)), dim3(SVM_TILE), 0, w.st, w.d, pcur, nin, nout,
                            dir, fire_mode);
         int r = last_launch(ctx);
         if (r) return r;
     }
     if (!ctx->t1_sequential) {
         Timer t(ctx, "k_t1_cluster");
-        hipLaunchKernelGGL(svm::k_t1_cluster, dim3(ctx->cluster_blocks), dim3(64 * SVM_T1_WAVES), 0, ctx->stream,
-                           ctx->d, pcur);
+        hipLaunchKernelGGL(svm::k_t1_cluster, dim3(ctx->cluster_blocks), dim3(64 * SVM_T1_WAVES), 0, w.st, w.d,
+                           pcur);
         int r = last_launch(ctx);
         if (r) return r;
     }
     if (!ctx->t1_sequential) {
         Timer t(ctx, "k_t1_check");
-        hipLaunchKernelGGL(svm::k_t1_check, dim3(64), dim3(SVM_TILE), 0, ctx->stream, ctx->d, pcur);
+        hipLaunchKernelGGL(svm::k_t1_check, dim3(64), dim3(SVM_TILE), 0, w.st, w.d, pcur);
         int r = last_launch(ctx);
         if (r) return r;
     }
     if (!ctx->t1_sequential) {
         Timer t(ctx, "k_t1_group");
-        hipLaunchKernelGGL(svm::k_t1_group, dim3(ctx->nsys), dim3(SVM_T1_GT), 0, ctx->stream, ctx->d, pcur);
+        hipLaunchKernelGGL(svm::k_t1_group, dim3(w.d.ns), dim3(SVM_T1_GT), 0, w.st, w.d, pcur);
         int r = last_launch(ctx);
         if (r) return r;
     }
     Timer t(ctx, "k_t1_resolve");
-    hipLaunchKernelGGL(svm::k_t1_resolve, dim3(ctx->nsys), dim3(SVM_TILE), 0, ctx->stream, ctx->d, pcur, nin, nout,
-                       dir, advance_step, fire_mode, ctx->t1_sequential);
+    hipLaunchKernelGGL(svm::k_t1_resolve, dim3(w.d.ns), dim3(SVM_TILE), 0, w.st, w.d, pcur, nin, nout, dir,
+                       advance_step, fire_mode, ctx->t1_sequential);
     return last_launch(ctx);
 }
+int launch_t1(svm_context* ctx, int pcur, int nin, int nout, int dir, int advance_step, int fire_mode) {
+    return launch_t1(ctx, whole(ctx), pcur, nin, nout, dir, advance_step, fire_mode);
+}
 
-// One dynamics step: geometry, force+updater+move, T1.  updater 0 = Brownian, 1 = active.
-int enqueue_step(svm_context* ctx, int updater) {
-    int r = launch_geometry(ctx, 0);
+// One dynamics step of the target's spheres: geometry, force+updater+move,
+// T1.  updater 0 = Brownian, 1 = active, 2 = Vicsek.  The ping-pong buffer
+// indices advance in ctx->pcur / ctx->ncur.
+int enqueue_step(svm_context* ctx, const Target& w, int updater) {
+    int r = launch_geometry(ctx, w, 0);
     if (r) return r;
     int pout = 1 - ctx->pcur;
-    if (updater == 0) r = launch_vertex<svm::VM_BROWNIAN>(ctx, pout, 0, "k_vertex_brownian");
-    else r = launch_vertex<svm::VM_ACTIVE>(ctx, pout, 0, "k_vertex_active");
+    if (updater == 0) r = launch_vertex<svm::VM_BROWNIAN>(ctx, w, pout, 0, "k_vertex_brownian");
+    else r = launch_vertex<svm::VM_ACTIVE>(ctx, w, pout, 0, "k_vertex_active");
     if (r) return r;
     if (updater == 2) {
         // Vicsek: the same move as the active updater, T1, then the alignment
         // update of the directors on the new positions and topology
-        r = launch_t1(ctx, pout, ctx->ncur, ctx->ncur, 0, 1, 0);
+        r = launch_t1(ctx, w, pout, ctx->ncur, ctx->ncur, 0, 1, 0);
         if (r) return r;
         ctx->pcur = pout;
         const int nout = 1 - ctx->ncur;
         Timer t(ctx, "k_vicsek_dir");
-        hipLaunchKernelGGL(svm::k_vicsek_dir, dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0, ctx->stream, ctx->d,
-                           ctx->pcur, ctx->ncur, nout);
+        hipLaunchKernelGGL(svm::k_vicsek_dir, dim3(xcd_grid(w.d.vtn)), dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur,
+                           ctx->ncur, nout);
         r = last_launch(ctx);
         if (r) return r;
         ctx->ncur = nout;
         return SVM_OK;
     }
     int nout = updater == 1 ? 1 - ctx->ncur : ctx->ncur;
-    r = launch_t1(ctx, pout, ctx->ncur, nout, updater == 1, 1, 0);
+    r = launch_t1(ctx, w, pout, ctx->ncur, nout, updater == 1, 1, 0);
     if (r) return r;
     ctx->pcur = pout;
     ctx->ncur = nout;
     return SVM_OK;
+}
+int enqueue_step(svm_context* ctx, int updater) { return enqueue_step(ctx, whole(ctx), updater); }
+
+// Lane k of the batch: its spheres and tiles, its T1 candidate lists, its stream.
+Target lane_target(svm_context* ctx, int k) {
+    Dev d = ctx->d;
+    const int a = ctx->lane_s0[k], b = ctx->lane_s0[k + 1];
+    const SysParams& A = ctx->layout[a];
+    const SysParams& B = ctx->layout[b - 1];
+    d.s0 = a;
+    d.ns = b - a;
+    d.vt0 = A.vtile0;
+    d.vtn = B.vtile0 + B.nvtiles - A.vtile0;
+    d.ct0 = A.ctile0;
+    d.ctn = B.ctile0 + B.nctiles - A.ctile0;
+    if (k == 0) return Target{d, ctx->stream};
+    const Dev& L = ctx->lane_t1[k - 1];
+    d.t1_count = L.t1_count;
+    d.t1_list = L.t1_list;
+    d.t1_flagged = L.t1_flagged;
+    d.t1_logged = L.t1_logged;
+    d.t1_total = L.t1_total;
+    d.t1_bcount = L.t1_bcount;
+    d.t1_blist = L.t1_blist;
+    d.t1_bflag = L.t1_bflag;
+    d.t1_bskip = L.t1_bskip;
+    d.t1_log = L.t1_log;
+    return Target{d, ctx->lane_streams[k - 1]};
+}
+
+// Runs `body` (which enqueues work of every lane on the lanes' streams)
+// between a fork from the context's stream and a join back into it.  The
+// join is enqueued even when `body` fails, so that later work on the
+// context's stream stays ordered after whatever the lanes received.
+template <class F>
+int over_lanes(svm_context* ctx, F body) {
+    HIPCHECK(ctx, hipEventRecord(ctx->lane_fork, ctx->stream));
+    for (int k = 1; k < ctx->nlanes; ++k) HIPCHECK(ctx, hipStreamWaitEvent(ctx->lane_streams[k - 1], ctx->lane_fork, 0));
+    int r = body();
+    for (int k = 1; k < ctx->nlanes; ++k) {
+        if (hipEventRecord(ctx->lane_joins[k - 1], ctx->lane_streams[k - 1]) != hipSuccess ||
+            hipStreamWaitEvent(ctx->stream, ctx->lane_joins[k - 1], 0) != hipSuccess)
+            r = r ? r : fail(ctx, SVM_ERR_HIP, "lane join failed");
+    }
+    return r;
+}
+
+// n dynamics steps of every lane, interleaved step by step across the lanes'
+// streams.  All lanes share the ping-pong buffer indices, which advance
+// once per step.
+int enqueue_lane_steps(svm_context* ctx, int updater, int n) {
+    return over_lanes(ctx, [&]() -> int {
+        for (int i = 0; i < n; ++i) {
+            const int p = ctx->pcur, q = ctx->ncur;
+            int np = p, nq = q;
+            for (int k = 0; k < ctx->nlanes; ++k) {
+                ctx->pcur = p;
+                ctx->ncur = q;
+                int r = enqueue_step(ctx, lane_target(ctx, k), updater);
+                if (r) {
+                    ctx->pcur = p;
+                    ctx->ncur = q;
+                    return r;
+                }
+                np = ctx->pcur;
+                nq = ctx->ncur;
+            }
+            ctx->pcur = np;
+            ctx->ncur = nq;
+        }
+        return SVM_OK;
+    });
 }
 
 // One FIRE iteration: mixing + VV first half + move, T1, geometry, force +
@@ -656,6 +767,12 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
     d.nsys = n_systems;
     d.Cs = ctx->Cs;
     d.xcd = 1;
+    d.s0 = 0;
+    d.ns = n_systems;
+    d.vt0 = 0;
+    d.vtn = ctx->nvt;
+    d.ct0 = 0;
+    d.ctn = ctx->nct;
     d.Vs = ctx->Vs;
     d.cap = cap;
     int r = 0;
@@ -738,9 +855,11 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
         P.t1_gen = 1;   // stamps of generation 0 mean "never touched"
         hs[s] = P;
     }
-    if (hipMemcpy(cts, hct.data(), sizeof(int32_t) * ctx->nct, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(vts, hvt.data(), sizeof(int32_t) * ctx->nvt, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d.sys, hs.data(), sizeof(SysParams) * n_systems, hipMemcpyHostToDevice) != hipSuccess) {
+    hipStream_t st = ctx->own_stream;
+    if (hipMemcpyAsync(cts, hct.data(), sizeof(int32_t) * ctx->nct, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(vts, hvt.data(), sizeof(int32_t) * ctx->nvt, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(d.sys, hs.data(), sizeof(SysParams) * n_systems, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
         svm_destroy(ctx);
         return SVM_ERR_HIP;
     }
@@ -751,8 +870,9 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
             a0[ctx->layout[s].co + c] = systems[s].a0;
             p0[ctx->layout[s].co + c] = systems[s].p0;
         }
-    if (hipMemcpy(d.a0, a0.data(), sizeof(double) * Cs, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d.p0, p0.data(), sizeof(double) * Cs, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpyAsync(d.a0, a0.data(), sizeof(double) * Cs, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(d.p0, p0.data(), sizeof(double) * Cs, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
         svm_destroy(ctx);
         return SVM_ERR_HIP;
     }
@@ -763,8 +883,11 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
 int svm_destroy(svm_context* ctx) {
     if (!ctx) return SVM_ERR_ARG;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
-    for (auto& g : ctx->graph_defs) (void)hipGraphDestroy(g.second);
+    for (auto st : ctx->lane_streams) (void)hipStreamSynchronize(st);
+    drop_graphs(ctx);
+    for (auto st : ctx->lane_streams) (void)hipStreamDestroy(st);
+    for (auto ev : ctx->lane_joins) (void)hipEventDestroy(ev);
+    if (ctx->lane_fork) (void)hipEventDestroy(ctx->lane_fork);
     for (void* p : ctx->allocs) (void)hipFree(p);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
@@ -1392,6 +1515,8 @@ static int run_steps(svm_context* ctx, int updater, int n_steps) {
     if (!ctx || n_steps < 0) return SVM_ERR_ARG;
     int r = require_topology(ctx);
     if (r) return r;
+    // per-kernel timing records events on the context's stream: whole batch
+    if (ctx->nlanes > 1 && !ctx->profiling) return enqueue_lane_steps(ctx, updater, n_steps);
     for (int i = 0; i < n_steps; ++i) {
         r = enqueue_step(ctx, updater);
         if (r) return r;
@@ -1413,6 +1538,58 @@ int svm_run_graph(svm_context* ctx, int32_t updater, int32_t steps_per_graph, in
     // started, so a graph captured from buffer state (pcur, ncur) can be
     // replayed whenever the context is in that state again.
     auto key = std::make_pair((int)updater, (int)steps_per_graph * 4 + ctx->pcur * 2 + ctx->ncur);
+    if (ctx->nlanes > 1) {
+        // one graph per lane, each captured on the lane's stream
+        auto lt = ctx->lane_graphs.find(key);
+        if (lt == ctx->lane_graphs.end()) {
+            bool prof = ctx->profiling;
+            ctx->profiling = false;
+            const int p = ctx->pcur, q = ctx->ncur;
+            std::vector<hipGraphExec_t> execs;
+            std::vector<hipGraph_t> defs;
+            auto discard = [&]() {
+                for (auto e : execs) (void)hipGraphExecDestroy(e);
+                for (auto g : defs) (void)hipGraphDestroy(g);
+                ctx->pcur = p;
+                ctx->ncur = q;
+                ctx->profiling = prof;
+            };
+            for (int k = 0; k < ctx->nlanes; ++k) {
+                const Target w = lane_target(ctx, k);
+                ctx->pcur = p;
+                ctx->ncur = q;
+                if (hipStreamBeginCapture(w.st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                    discard();
+                    return fail(ctx, SVM_ERR_HIP, "lane graph capture failed");
+                }
+                for (int i = 0; i < steps_per_graph && !r; ++i) r = enqueue_step(ctx, w, updater);
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(w.st, &g);
+                if (g) defs.push_back(g);
+                hipGraphExec_t ge = nullptr;
+                if (!r && (e != hipSuccess || hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess))
+                    r = fail(ctx, SVM_ERR_HIP, "lane graph capture failed");
+                if (r) {
+                    discard();
+                    return r;
+                }
+                execs.push_back(ge);
+            }
+            ctx->pcur = p;   // an even number of steps: back where the capture started
+            ctx->ncur = q;
+            ctx->profiling = prof;
+            ctx->lane_graphs[key] = execs;
+            ctx->lane_graph_defs[key] = defs;
+            lt = ctx->lane_graphs.find(key);
+        }
+        const std::vector<hipGraphExec_t>& ge = lt->second;
+        return over_lanes(ctx, [&]() -> int {
+            for (int i = 0; i < n_replays; ++i)
+                for (int k = 0; k < ctx->nlanes; ++k)
+                    HIPCHECK(ctx, hipGraphLaunch(ge[k], k == 0 ? ctx->stream : ctx->lane_streams[k - 1]));
+            return SVM_OK;
+        });
+    }
     auto it = ctx->graphs.find(key);
     if (it == ctx->graphs.end()) {
         bool prof = ctx->profiling;
@@ -1718,6 +1895,60 @@ int svm_set_tuning(svm_context* ctx, int32_t cluster_blocks, int32_t graph_steps
     return SVM_OK;
 }
 
+int svm_set_lanes(svm_context* ctx, int32_t n_lanes) {
+    if (!ctx || n_lanes < 1 || n_lanes > SVM_MAX_LANES)
+        return fail(ctx, SVM_ERR_ARG, "n_lanes must be in [1, " + std::to_string(SVM_MAX_LANES) + "]");
+    const int n = std::min((int)n_lanes, ctx->nsys);
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (!ctx->lane_fork) HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->lane_fork, hipEventDisableTiming));
+    while ((int)ctx->lane_streams.size() < n - 1) {
+        // default priority: every stream of the library has the same one
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        HIPCHECK(ctx, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        ctx->lane_streams.push_back(st);
+        HIPCHECK(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        ctx->lane_joins.push_back(ev);
+        Dev L{};
+        int r = 0;
+        r |= dalloc(ctx, &L.t1_count, 1);
+        r |= dalloc(ctx, &L.t1_list, SVM_T1_MAXCL);
+        r |= dalloc(ctx, &L.t1_flagged, SVM_T1_MAXCL);
+        r |= dalloc(ctx, &L.t1_logged, SVM_T1_MAXCL);
+        r |= dalloc(ctx, &L.t1_total, 1);
+        r |= dalloc(ctx, &L.t1_bcount, 1);
+        r |= dalloc(ctx, &L.t1_blist, SVM_T1_MAXBL);
+        r |= dalloc(ctx, &L.t1_bflag, SVM_T1_MAXBL);
+        r |= dalloc(ctx, &L.t1_bskip, SVM_T1_MAXBL);
+        r |= dalloc(ctx, &L.t1_log, SVM_T1_MAXCL);
+        if (r) return fail(ctx, SVM_ERR_HIP, "lane buffers: out of device memory");
+        ctx->lane_t1.push_back(L);
+    }
+    // contiguous sphere ranges with about equal numbers of vertex tiles
+    std::vector<int> s0(1, 0);
+    long long acc = 0;
+    for (int s = 0; s < ctx->nsys && (int)s0.size() < n; ++s) {
+        acc += ctx->layout[s].nvtiles;
+        const int k = (int)s0.size();   // lane whose start is next
+        const bool room = ctx->nsys - (s + 1) >= n - k;
+        if (room && acc * n >= (long long)ctx->nvt * k) s0.push_back(s + 1);
+    }
+    while ((int)s0.size() < n) s0.push_back(ctx->nsys - (n - (int)s0.size()));
+    s0.push_back(ctx->nsys);
+    ctx->lane_s0 = s0;
+    ctx->nlanes = n;
+    drop_graphs(ctx);
+    return SVM_OK;
+}
+
+int svm_get_lanes(svm_context* ctx, int32_t* n_lanes, int32_t* first_sphere) {
+    if (!ctx || !n_lanes) return SVM_ERR_ARG;
+    *n_lanes = ctx->nlanes;
+    if (first_sphere)
+        for (int k = 0; k < ctx->nlanes; ++k) first_sphere[k] = ctx->nlanes == 1 ? 0 : ctx->lane_s0[k];
+    return SVM_OK;
+}
+
 int svm_get_tuning(svm_context* ctx, int32_t* cluster_blocks, int32_t* graph_steps, int32_t* xcd_mapping) {
     if (!ctx) return SVM_ERR_ARG;
     if (cluster_blocks) *cluster_blocks = ctx->cluster_blocks;
@@ -1756,26 +1987,41 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
         *ms = samples[1];
         return SVM_OK;
     };
-    auto record = [&](int cb, int G, int xcd, double ms) {
+    auto record = [&](int lanes, int cb, int G, int xcd, double ms) {
         if (table && nrows < table_rows) {
-            table[4 * nrows] = cb;
-            table[4 * nrows + 1] = G;
-            table[4 * nrows + 2] = xcd;
-            table[4 * nrows + 3] = ms;
+            table[5 * nrows] = lanes;
+            table[5 * nrows + 1] = cb;
+            table[5 * nrows + 2] = G;
+            table[5 * nrows + 3] = xcd;
+            table[5 * nrows + 4] = ms;
         }
         ++nrows;
     };
-    int best_cb = ctx->cluster_blocks, best_G = ctx->graph_steps, best_x = ctx->d.xcd;
+    int best_cb = ctx->cluster_blocks, best_G = ctx->graph_steps, best_x = ctx->d.xcd, best_l = ctx->nlanes;
     double best = 1e300;
-    // kernel schedules at the default graph length
+    // lanes x two dry-run grids, at 10 steps per graph with the XCD mapping
+    const int cb_pair[] = {256, 1024};
+    for (int l = 1; l <= std::min(ctx->nsys, (int)SVM_MAX_LANES) && !r; ++l)
+        for (int cb : cb_pair) {
+            r = svm_set_lanes(ctx, l);
+            if (!r) r = svm_set_tuning(ctx, cb, 10, 1);
+            double ms = 0.0;
+            if (!r) r = trial(10, &ms);
+            if (r) break;
+            record(l, cb, 10, 1, ms);
+            if (ms < best) { best = ms; best_l = l; best_cb = cb; best_x = 1; best_G = 10; }
+        }
+    if (!r) r = svm_set_lanes(ctx, best_l);
+    // the other kernel schedules at the default graph length
     const int cbs[] = {128, 256, 512, 1024};
     for (int xcd = 0; xcd <= 1 && !r; ++xcd)
         for (int cb : cbs) {
+            if (xcd == 1 && (cb == 256 || cb == 1024)) continue;   // measured above
             r = svm_set_tuning(ctx, cb, 10, xcd);
             double ms = 0.0;
             if (!r) r = trial(10, &ms);
             if (r) break;
-            record(cb, 10, xcd, ms);
+            record(best_l, cb, 10, xcd, ms);
             if (ms < best) { best = ms; best_cb = cb; best_x = xcd; best_G = 10; }
         }
     // then the graph length
@@ -1786,14 +2032,15 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
         double ms = 0.0;
         if (!r) r = trial(G, &ms);
         if (r) break;
-        record(best_cb, G, best_x, ms);
+        record(best_l, best_cb, G, best_x, ms);
         if (ms < best) { best = ms; best_G = G; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (r) return r;
     if (rows_written) *rows_written = nrows;
-    return svm_set_tuning(ctx, best_cb, best_G, best_x);
+    r = svm_set_lanes(ctx, best_l);
+    return r ? r : svm_set_tuning(ctx, best_cb, best_G, best_x);
 }
 
 int svm_copy_probe(int64_t n_bytes, int32_t reps, double* ms_per_copy) {

@@ -66,6 +66,11 @@ struct SysParams {
 struct Dev {
     int32_t nsys, Cs, Vs, cap;
     int32_t xcd;                  // 1: tiles dealt to XCDs in contiguous ranges (xcd_tile)
+    // The lane a launch covers: a contiguous run of spheres [s0, s0+ns) with
+    // vertex tiles [vt0, vt0+vtn) and cell tiles [ct0, ct0+ctn).  Launches of
+    // the dynamics step may cover one lane of the batch each, on a stream of
+    // their own; every other launch covers the whole batch.
+    int32_t s0, ns, vt0, vtn, ct0, ctn;
     SysParams* sys;
     const int32_t* ctile_sys;   // cell tile -> sphere
     const int32_t* vtile_sys;   // vertex tile -> sphere
@@ -94,7 +99,7 @@ struct Dev {
     double* vpart;                // 3 per vertex tile partial sums
     double* disp;                 // AoS displacement staging (3 per vertex row)
     // T1 dry runs
-    int32_t* t1_count;            // candidates appended this step (all spheres)
+    int32_t* t1_count;            // candidates appended this step (all spheres of the lane)
     int32_t* t1_list;             // candidate vertex rows
     int32_t* t1_flagged;          // per candidate: generation in which its run overlapped
     int32_t* t1_logged;           // per candidate: generation in which its run wrote a log

@@ -35,6 +35,17 @@ __device__ inline int xcd_tile(const Dev& d, int ntiles) {
     return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
 }
 
+// The vertex / cell tile of this workgroup within the launch's lane, or -1
+// for a spare workgroup of the XCD-padded grid.
+__device__ inline int lane_vtile(const Dev& d) {
+    const int t = xcd_tile(d, d.vtn);
+    return t < d.vtn ? d.vt0 + t : -1;
+}
+__device__ inline int lane_ctile(const Dev& d) {
+    const int t = xcd_tile(d, d.ctn);
+    return t < d.ctn ? d.ct0 + t : -1;
+}
+
 
 enum VertexMode { VM_FORCE = 0, VM_BROWNIAN = 1, VM_ACTIVE = 2, VM_FIRE = 3, VM_FORCE_F32 = 4, VM_ADAM = 5 };
 
@@ -96,8 +107,8 @@ __device__ inline double dot0(const double* a, const double* b) {
 // and recomputeAreasCPU (:162-207): one thread per cell.
 __global__ __launch_bounds__(SVM_TILE) void k_geometry(Dev d, int pin, int skip_frozen) {
     __shared__ double red[SVM_TILE + 1];
-    const int tile = xcd_tile(d, d.Cs / SVM_TILE);
-    if (tile >= d.Cs / SVM_TILE) return;
+    const int tile = lane_ctile(d);
+    if (tile < 0) return;
     const int s = d.ctile_sys[tile];
     SysParams* S = d.sys + s;
     if (skip_frozen && S->f_frozen) return;
@@ -175,8 +186,8 @@ __global__ __launch_bounds__(SVM_TILE) void k_geometry(Dev d, int pin, int skip_
 // decision without any cross-block synchronisation.
 __global__ __launch_bounds__(SVM_TILE) void k_fan_fix(Dev d, int pin, int skip_frozen) {
     __shared__ double buf[SVM_MAX_TILES];
-    const int tile = xcd_tile(d, d.Cs / SVM_TILE);
-    if (tile >= d.Cs / SVM_TILE) return;
+    const int tile = lane_ctile(d);
+    if (tile < 0) return;
     SysParams* S = d.sys + d.ctile_sys[tile];
     if (skip_frozen && S->f_frozen) return;
     const double R = S->R;
@@ -386,8 +397,8 @@ __device__ inline int vertex_force<VecF64>(const Dev& d, const SysParams* S, con
 template <int MODE, class FT>
 __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, int nin, int store_forces) {
     __shared__ double red[SVM_TILE + 1];
-    const int tile = xcd_tile(d, d.Vs / SVM_TILE);
-    if (tile >= d.Vs / SVM_TILE) return;
+    const int tile = lane_vtile(d);
+    if (tile < 0) return;
     const int s = d.vtile_sys[tile];
     SysParams* S = d.sys + s;
     if (MODE == VM_FIRE && S->f_frozen) return;
@@ -606,8 +617,8 @@ __device__ inline void fire_mix(const Dev& d, const SysParams* S, int v, double*
 // First half of velocityVerlet::integrateEOMCPU (src/updaters/velocityVerlet.cpp:14-23)
 // preceded by the previous iteration's FIRE mixing, then the move.
 __global__ __launch_bounds__(SVM_TILE) void k_fire_vv1(Dev d, int pin, int pout) {
-    const int tile = xcd_tile(d, d.Vs / SVM_TILE);
-    if (tile >= d.Vs / SVM_TILE) return;
+    const int tile = lane_vtile(d);
+    if (tile < 0) return;
     const int s = d.vtile_sys[tile];
     SysParams* S = d.sys + s;
     const int v = tile * SVM_TILE + threadIdx.x;
@@ -688,8 +699,8 @@ __device__ inline void director_update(const SysParams* S, int local, const doub
 // plane with unit length.  Reads directors nin, writes nout; the draws are
 // keyed by the step count after this step.
 __global__ __launch_bounds__(SVM_TILE) void k_vicsek_dir(Dev d, int pin, int nin, int nout) {
-    const int tile = xcd_tile(d, d.Vs / SVM_TILE);
-    if (tile >= d.Vs / SVM_TILE) return;
+    const int tile = lane_vtile(d);
+    if (tile < 0) return;
     const SysParams* S = d.sys + d.vtile_sys[tile];
     const int v = tile * SVM_TILE + threadIdx.x;
     const int local = v - S->vo;

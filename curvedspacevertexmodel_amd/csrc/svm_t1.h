@@ -877,8 +877,8 @@ __global__ __launch_bounds__(SVM_TILE) void k_build_vln(Dev d, int only) {
 // runs); the active updater's director step is fused here.
 __global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int nin, int nout, int dir_update,
                                                         int skip_frozen) {
-    const int tile = xcd_tile(d, d.Vs / SVM_TILE);
-    if (tile >= d.Vs / SVM_TILE) return;
+    const int tile = lane_vtile(d);
+    if (tile < 0) return;
     const int s = d.vtile_sys[tile];
     SysParams* S = d.sys + s;
     if (skip_frozen && S->f_frozen) return;
@@ -1091,7 +1091,7 @@ __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int n
     const int t = threadIdx.x, nt = blockDim.x;
     const int gen = S->t1_gen;
     const int vo = S->vo, nv = S->nv;
-    const T1GroupLog& G = d.t1_group[blockIdx.x];
+    const T1GroupLog& G = d.t1_group[S - d.sys];
     const int seen_words = (nv + 31) >> 5;
     const bool dedup = seen_words <= SVM_COMMIT_SEEN;
     if (t == 0) {
@@ -1363,7 +1363,7 @@ __global__ __launch_bounds__(SVM_T1_GT) void k_t1_group(Dev d, int pcur) {
     __shared__ int s_absorb[SVM_T1_GABS];
     __shared__ int s_nmem, s_nabs, s_vfill, s_cfill, s_status, s_nlv, s_nlc;
     __shared__ long long s_flips, s_skipped;
-    const int s = blockIdx.x;
+    const int s = d.s0 + (int)blockIdx.x;
     SysParams* S = d.sys + s;
     const int lane = threadIdx.x;     // the walk runs on thread 0; the rest is shared by the block
     const BlockSync sync;
@@ -1511,7 +1511,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_resolve(Dev d, int pcur, int ni
     __shared__ int scan[SVM_TILE];
     __shared__ int tmp[72];
     __shared__ int sh_run, sh_commit, sh_group, sh_nmoved, sh_ntouched;
-    const int s = blockIdx.x;
+    const int s = d.s0 + (int)blockIdx.x;
     SysParams* S = d.sys + s;
     const int t = threadIdx.x;
     if (t == 0) {
@@ -1527,8 +1527,8 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_resolve(Dev d, int pcur, int ni
         }
         sh_nmoved = 0;
         sh_ntouched = 0;
-        if (s == 0) {
-            *d.t1_count = 0;   // the candidate lists of this step are used up
+        if (blockIdx.x == 0) {
+            *d.t1_count = 0;   // the lane's candidate lists of this step are used up
             *d.t1_bcount = 0;
         }
     }

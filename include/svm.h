@@ -391,12 +391,30 @@ int svm_nc_close(svm_nc_file* f);
  * (xcd_mapping).  None of them changes any result, bit for bit. */
 int svm_set_tuning(svm_context* ctx, int32_t cluster_blocks, int32_t graph_steps, int32_t xcd_mapping);
 int svm_get_tuning(svm_context* ctx, int32_t* cluster_blocks, int32_t* graph_steps, int32_t* xcd_mapping);
+/* Lanes of the dynamics step (no counterpart in the reference, which steps
+ * one sphere at a time): the batch is cut into n_lanes contiguous runs of
+ * spheres with about equal numbers of vertices (at most SVM_MAX_LANES, at
+ * most one lane per sphere), and the steps of svm_brownian_steps,
+ * svm_active_steps, svm_vicsek_steps and svm_run_graph enqueue each lane's
+ * launches on a stream of its own, forked from the context's stream and
+ * joined back into it before those calls return.  One lane's T1 dry runs
+ * (long dependent chains on a few compute units) then overlap the other
+ * lanes' force passes.  Spheres are independent, so results are identical
+ * bit for bit for every n_lanes.  Each lane above the first holds its own
+ * streams and T1 candidate lists (about 65 MB of device memory); those are
+ * kept until svm_destroy.  svm_get_lanes returns the number of lanes and,
+ * when first_sphere is not null, the first sphere of each lane. */
+#define SVM_MAX_LANES 4
+int svm_set_lanes(svm_context* ctx, int32_t n_lanes);
+int svm_get_lanes(svm_context* ctx, int32_t* n_lanes, int32_t* first_sphere);
 /* Project autotuner, in the manner of kernelTuner's median-of-samples scan
  * (src/utilities/kernelTuner.cpp:35-126) but over the whole step: times
  * real steps of `updater` (0 Brownian, 1 active) under each candidate
- * schedule -- cluster_blocks in {128, 256, 512, 1024} x xcd_mapping in {0, 1}
- * at 10 steps per graph, then graph_steps in {4, 20, 40} -- keeps the fastest
- * and reports every trial in table (rows of 4 doubles: cluster_blocks,
+ * schedule -- first lanes in {1 .. min(SVM_MAX_LANES, spheres)} x
+ * cluster_blocks in {256, 1024}, then at the best lane count the rest of
+ * cluster_blocks in {128, 256, 512, 1024} x xcd_mapping in {0, 1}, all at 10
+ * steps per graph, then graph_steps in {4, 20, 40} -- keeps the fastest and
+ * reports every trial in table (rows of 5 doubles: lanes, cluster_blocks,
  * graph_steps, xcd_mapping, ms per step).  The steps advance the model
  * (about steps_per_trial per candidate), exactly as untuned steps would. */
 int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, double* table, int32_t table_rows,

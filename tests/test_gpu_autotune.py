@@ -39,13 +39,16 @@ def test_autotune_steps_are_ordinary_steps(states):
     for i in range(len(states)):
         ctx.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
     tab = ctx.autotune(updater=1, steps_per_trial=30)
-    assert len(tab) == 11 and (tab[:, 3] > 0).all()
+    # lanes 1, 2, 3 (capped at the 3 spheres) x 2 grids, 6 more kernel schedules, 3 graph lengths
+    assert len(tab) == 15 and (tab[:, 4] > 0).all()
+    np.testing.assert_array_equal(tab[:6, 0], [1, 1, 2, 2, 3, 3])
+    assert (tab[6:, 0] == tab[np.argmin(tab[:6, 4]), 0]).all()
     t = ctx.tuning()
-    best = tab[np.argmin(tab[:, 3])]
-    assert (t["cluster_blocks"], t["graph_steps"], t["xcd_mapping"]) == tuple(int(x) for x in best[:3])
+    best = tab[np.argmin(tab[:, 4])]
+    assert (t["lanes"], t["cluster_blocks"], t["graph_steps"], t["xcd_mapping"]) == tuple(int(x) for x in best[:4])
     # the number of steps the tuner took, from the trial table: capture warm-up + 3 samples per candidate
     steps = 0
-    for cb, G, x, ms in tab:
+    for lanes, cb, G, x, ms in tab:
         G = int(G)
         steps += G + 3 * max(1, 30 // (3 * G)) * G
     ref = make_ctx(states, p0=3.6)

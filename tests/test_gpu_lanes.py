@@ -1,0 +1,118 @@
+"""Lanes: the batch cut into sphere ranges whose steps run on streams of
+their own.  Spheres are independent, so every lane count must give the
+one-lane trajectory bit for bit -- positions, directors, topology and the T1
+counters -- for eager steps and for graph replays, with T1 flips, group
+runs and sphere sizes that differ between the lanes."""
+import numpy as np
+import pytest
+
+import svmtest
+import curvedspacevertexmodel_amd as svm
+
+pytestmark = pytest.mark.gpu
+
+
+def params(n_sys=14, seed=5151):
+    g = np.random.Generator(np.random.Philox(seed))
+    return [(int(round(np.exp(np.log(200) + u[0] * np.log(25)))), 3.7 + 0.25 * u[1], 0.05 + 0.4 * u[2])
+            for u in g.random((n_sys, 3))]
+
+
+def build(ps, lanes):
+    ctx = svm.Context([dict(n_cells=n, a0=1.0, p0=p0, kr=1.0, t1_threshold=0.1, seed=3100 + i)
+                       for i, (n, p0, v0) in enumerate(ps)])
+    for i, (n, p0, v0) in enumerate(ps):
+        pts = svm.generate_cell_points(n, 3100 + i)
+        ctx.build_from_cell_points(i, pts[svm.spatial_order(pts)])
+        ctx.set_directors(i, svm.generate_directors(ctx.positions(i), 3100 + i))
+        ctx.set_preferred_parameters(i, 1.0, 1.0)
+        ctx.set_brownian(i, dt=0.01, mu=1.0, T=0.0)
+    ctx.set_lanes(lanes)
+    return ctx
+
+
+def state(ctx, n):
+    return ([ctx.positions(i) for i in range(n)], [ctx.directors(i) for i in range(n)],
+            [ctx.get_topology(i, svmtest.RING_CAP) for i in range(n)], ctx.counters())
+
+
+def assert_same(a, b):
+    for x, y in zip(a[0], b[0]):
+        np.testing.assert_array_equal(x, y)
+    for x, y in zip(a[1], b[1]):
+        np.testing.assert_array_equal(x, y)
+    for tx, ty in zip(a[2], b[2]):
+        for x, y in zip(tx, ty):
+            np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(a[3], b[3])
+
+
+def trajectory(ps, lanes, updater):
+    ctx = build(ps, lanes)
+    ctx.brownian_steps(30)          # eager, then a graph of the Brownian updater
+    ctx.run_graph(0, 10, 2)
+    for i, (n, p0, v0) in enumerate(ps):
+        ctx.set_preferred_parameters(i, 1.0, p0)
+        if updater == 2:
+            ctx.set_vicsek(i, dt=0.01, mu=1.0, v0=v0, tau=1.0, eta=0.2)
+        else:
+            ctx.set_active(i, dt=0.01, mu=1.0, v0=v0, tau=1.0)
+    steps = {1: ctx.active_steps, 2: ctx.vicsek_steps}[updater]
+    steps(25)                       # odd: the graphs below start from the other buffers
+    ctx.run_graph(updater, 6, 5)
+    steps(7)
+    ctx.run_graph(updater, 4, 3)
+    ctx.synchronize()
+    return ctx, state(ctx, len(ps))
+
+
+@pytest.mark.parametrize("updater", [1, 2])
+def test_lanes_give_the_one_lane_trajectory(updater):
+    ps = params()
+    _, ref = trajectory(ps, 1, updater)
+    c = ref[3]
+    print(f"updater {updater}: flips {int(c[:, 0].sum())}, group sphere-steps {int(c[:, 5].sum())}")
+    assert c[:, 0].sum() > 200      # T1 flips in the run
+    for lanes in (2, 3, 4):
+        ctx, got = trajectory(ps, lanes, updater)
+        n, first = ctx.lanes()
+        assert n == lanes and first[0] == 0 and (np.diff(first) > 0).all()
+        assert_same(got, ref)
+
+
+def test_lane_partition_and_limits():
+    ps = params(5, 77)
+    ctx = build(ps, 1)
+    assert ctx.lanes()[0] == 1
+    ctx.set_lanes(4)
+    n, first = ctx.lanes()
+    assert n == 4 and first[0] == 0 and (np.diff(first) > 0).all() and first[-1] < 5
+    small = build(ps[:3], 1)
+    small.set_lanes(4)             # capped at one lane per sphere
+    assert small.lanes()[0] == 3 and list(small.lanes()[1]) == [0, 1, 2]
+    with pytest.raises(svm.SVMError):
+        ctx.set_lanes(0)
+    with pytest.raises(svm.SVMError):
+        ctx.set_lanes(svm.MAX_LANES + 1)
+    # about equal vertex counts per lane
+    ctx.set_lanes(2)
+    n, first = ctx.lanes()
+    nv = np.array([2 * p[0] - 4 for p in ps])
+    a, b = nv[:first[1]].sum(), nv[first[1]:].sum()
+    assert max(a, b) <= nv.sum() - min(a, b)
+
+
+def test_lanes_then_whole_batch_calls():
+    """Calls that cover the whole batch (forces, FIRE) see the lanes' steps."""
+    ps = params(6, 91)
+    a, b = build(ps, 1), build(ps, 3)
+    for ctx in (a, b):
+        for i, (n, p0, v0) in enumerate(ps):
+            ctx.set_preferred_parameters(i, 1.0, p0)
+            ctx.set_active(i, dt=0.01, mu=1.0, v0=v0, tau=1.0)
+        ctx.active_steps(12)
+        ctx.run_graph(1, 4, 2)
+        ctx.compute_forces()
+    for i in range(len(ps)):
+        np.testing.assert_array_equal(a.forces(i), b.forces(i))
+        np.testing.assert_array_equal(a.area_perimeter(i), b.area_perimeter(i))

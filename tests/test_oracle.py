@@ -447,6 +447,7 @@ def test_library_exports_every_declared_symbol():
     assert not missing
     import curvedspacevertexmodel_amd as svm
     assert sorted(svm.EXPORTS) == names
+    assert int(re.search(r"#define SVM_MAX_LANES (\d+)", decl).group(1)) == svm.MAX_LANES
 
 
 def test_library_reports_no_device_without_gpu():
