@@ -28,9 +28,9 @@ METRIC = "cell-steps/s (batched spheres)"   # the FIRE learned-vs-plain part is 
 PEAK_HBM_GBS = 8000.0
 # Algorithmic HBM bytes per cell per step of the vertex pass (k_vertex_active):
 # two vertices per cell, each reading its position (24 B) and director (24 B),
-# writing its new position (24 B) and reading its vertex-cell and
-# vertex-vertex lists (2 x 12 B): 2 x 96 = 192 B.
-VERTEX_BYTES_PER_CELL = 192.0
+# writing its new position (24 B) and rotated director (24 B) and reading its
+# vertex-cell and vertex-vertex lists (2 x 12 B): 2 x 120 = 240 B.
+VERTEX_BYTES_PER_CELL = 240.0
 STEP_BYTES_PER_CELL = 244.0   # whole active step (positions, directors r/w, topology)
 
 

@@ -249,19 +249,22 @@ int launch_geometry(svm_context* ctx, const Target& w, int skip_frozen) {
 }
 int launch_geometry(svm_context* ctx, int skip_frozen) { return launch_geometry(ctx, whole(ctx), skip_frozen); }
 
+// The active updater also writes the rotated directors, into the other
+// director buffer (1 - ncur); no other mode touches directors.
 template <int MODE>
 int launch_vertex(svm_context* ctx, const Target& w, int pout, int store_forces, const char* name) {
     Timer t(ctx, name);
     const dim3 grid(xcd_grid(w.d.vtn));
+    const int nout = 1 - ctx->ncur;
     if (ctx->force_fp32 == 1)
         hipLaunchKernelGGL((svm::k_vertex<MODE, float>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
-                           ctx->ncur, store_forces);
+                           ctx->ncur, nout, store_forces);
     else if (ctx->force_fp32 == 2)
         hipLaunchKernelGGL((svm::k_vertex<MODE, svm::VecF64>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
-                           ctx->ncur, store_forces);
+                           ctx->ncur, nout, store_forces);
     else
         hipLaunchKernelGGL((svm::k_vertex<MODE, double>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
-                           ctx->ncur, store_forces);
+                           ctx->ncur, nout, store_forces);
     return last_launch(ctx);
 }
 template <int MODE>
@@ -273,8 +276,7 @@ int launch_t1(svm_context* ctx, const Target& w, int pcur, int nin, int nout, in
               int fire_mode) {
     {
         Timer t(ctx, "k_t1_detect");
-        hipLaunchKernelGGL(svm::k_t1_detect, dim3(xcd_grid(w.d.vtn)), dim3(SVM_TILE), 0, w.st, w.d, pcur, nin, nout,
-                           dir, fire_mode);
+        hipLaunchKernelGGL(svm::k_t1_detect, dim3(xcd_grid(w.d.vtn)), dim3(SVM_TILE), 0, w.st, w.d, pcur, fire_mode);
         int r = last_launch(ctx);
         if (r) return r;
     }

@@ -1,19 +1,19 @@
 // svm_kernels.h -- HIP kernels of the per-timestep hot path (gfx950, wave64).
 //
-// One dynamics step of a batch of spheres is four launches over the whole
-// batch (every sphere's tiles in one grid):
+// One dynamics step of a batch of spheres (or of one lane of it) is a few
+// launches, each over every sphere's tiles in one grid:
 //   k_geometry  (thread per cell)   cell centre, area, perimeter; per-sphere
 //                                   total-area check with the fan fallback
 //   k_vertex    (thread per vertex) chain-rule force over the 3 cells of the
 //                                   vertex, fused with the updater epilogue
-//                                   (Brownian / active displacement and the
-//                                   move on the sphere, or the FIRE
-//                                   half-kick, power/norm reductions and the
-//                                   FIRE controller)
-//   k_t1_detect (thread per vertex) short-edge test -> candidate bit mask;
-//                                   active-director update
-//   k_t1_resolve (block per sphere) applies T1 flips in exactly the order of
-//                                   the reference's sequential CPU loop
+//                                   (Brownian / active displacement, the move
+//                                   on the sphere and the active-director
+//                                   update, or the FIRE half-kick and its
+//                                   power/norm reductions)
+//   k_t1_detect (thread per vertex) short-edge test -> candidate lists
+//   k_t1_cluster .. k_t1_resolve    the T1 dry runs and their commit, in
+//                                   exactly the order of the reference's
+//                                   sequential CPU loop (svm_t1.h)
 // Per-sphere reductions are deterministic: 256-row tiles folded by a fixed
 // halving tree, then the tile partials folded the same way by the last block
 // of that sphere to arrive.
@@ -218,6 +218,20 @@ __global__ __launch_bounds__(SVM_TILE) void k_fan_fix(Dev d, int pin, int skip_f
     d.A[c] = a;
 }
 
+// Active-director update of sphericalSelfPropelledParticle::integrateEOMCPU
+// (src/updaters/sphericalSelfPropelledParticle.cpp:37-45) at position p.
+__device__ inline void director_update(const SysParams* S, int local, const double* p, const double* nin,
+                                       double* nout) {
+    double u, unused;
+    uniform2(S->seed, STREAM_ACTIVE, S->step, (uint32_t)local, &u, &unused);
+    double randomNumber = u * (.5 - (-.5)) + (-.5);
+    double theta = randomNumber * __builtin_sqrt(2.0 * S->dt * S->tau);
+    double n[3] = {nin[0], nin[1], nin[2]};
+    project_normalize(n, p);
+    rodrigues(n, p, theta);
+    nout[0] = n[0]; nout[1] = n[1]; nout[2] = n[2];
+}
+
 // --------------------------------------------------------------- vertices --
 template <class T>
 __device__ inline PQT<T> pq_at(const double* X, const double* Y, const double* Z, int v) {
@@ -395,7 +409,7 @@ __device__ inline int vertex_force<VecF64>(const Dev& d, const SysParams* S, con
 }
 
 template <int MODE, class FT>
-__global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, int nin, int store_forces) {
+__global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, int nin, int nout, int store_forces) {
     __shared__ double red[SVM_TILE + 1];
     const int tile = lane_vtile(d);
     if (tile < 0) return;
@@ -421,7 +435,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, i
             d.fx[v] = f[0]; d.fy[v] = f[1]; d.fz[v] = f[2];
         }
         if (MODE == VM_BROWNIAN || MODE == VM_ACTIVE) {
-            double disp[3];
+            double disp[3], nI[3];
             if (MODE == VM_BROWNIAN) {
                 double forcePrefactor = S->dt * S->mu;
                 double noisePrefactor = __builtin_sqrt(2.0 * forcePrefactor * S->T);
@@ -429,17 +443,27 @@ __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, i
                 if (S->T != 0.0) normal3(S->seed, S->step, (uint32_t)local, xi);
                 for (int k = 0; k < 3; ++k) disp[k] = xi[k] * noisePrefactor + f[k] * forcePrefactor;
             } else {
-                double n0 = d.nx[nin][v], n1 = d.ny[nin][v], n2 = d.nz[nin][v];
+                nI[0] = d.nx[nin][v]; nI[1] = d.ny[nin][v]; nI[2] = d.nz[nin][v];
                 double dt = S->dt, v0 = S->v0, mu = S->mu;
-                disp[0] = dt * (v0 * n0 + mu * f[0]);
-                disp[1] = dt * (v0 * n1 + mu * f[1]);
-                disp[2] = dt * (v0 * n2 + mu * f[2]);
+                disp[0] = dt * (v0 * nI[0] + mu * f[0]);
+                disp[1] = dt * (v0 * nI[1] + mu * f[1]);
+                disp[2] = dt * (v0 * nI[2] + mu * f[2]);
             }
             double p[3] = {X[v], Y[v], Z[v]};
             move_on_sphere(p, disp, S->R);
             d.px[pout][v] = p[0];
             d.py[pout][v] = p[1];
             d.pz[pout][v] = p[2];
+            if (MODE == VM_ACTIVE) {
+                // the director's rotational-diffusion step at the moved
+                // position; a vertex that a T1 flip then moves gets it redone
+                // at its final position by the T1 commit
+                double nO[3];
+                director_update(S, local, p, nI, nO);
+                d.nx[nout][v] = nO[0];
+                d.ny[nout][v] = nO[1];
+                d.nz[nout][v] = nO[2];
+            }
         }
         if (MODE == VM_ADAM) {
             // adamStepCPU (src/updaters/energyMinimizerAdam.cpp:59-80), f the negative gradient
@@ -676,19 +700,6 @@ __global__ __launch_bounds__(SVM_TILE) void k_move_ext(Dev d, int pin, int pout,
     d.pz[pout][v] = p[2];
 }
 
-// Active-director update of sphericalSelfPropelledParticle::integrateEOMCPU
-// (src/updaters/sphericalSelfPropelledParticle.cpp:37-45) at position p.
-__device__ inline void director_update(const SysParams* S, int local, const double* p, const double* nin,
-                                       double* nout) {
-    double u, unused;
-    uniform2(S->seed, STREAM_ACTIVE, S->step, (uint32_t)local, &u, &unused);
-    double randomNumber = u * (.5 - (-.5)) + (-.5);
-    double theta = randomNumber * __builtin_sqrt(2.0 * S->dt * S->tau);
-    double n[3] = {nin[0], nin[1], nin[2]};
-    project_normalize(n, p);
-    rodrigues(n, p, theta);
-    nout[0] = n[0]; nout[1] = n[1]; nout[2] = n[2];
-}
 
 // Director update of sphericalVectorialVicsek::integrateEOMCPU
 // (src/updaters/sphericalVectorialVicsek.cpp:24-64) on the vertex model, after

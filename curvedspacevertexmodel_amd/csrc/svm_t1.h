@@ -875,8 +875,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_build_vln(Dev d, int only) {
 // it has an edge (v, w), v < w, shorter than the threshold.  Candidates go to
 // the bit mask (for the sequential path) and to the compact list (for the dry
 // runs); the active updater's director step is fused here.
-__global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int nin, int nout, int dir_update,
-                                                        int skip_frozen) {
+__global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int skip_frozen) {
     const int tile = lane_vtile(d);
     if (tile < 0) return;
     const int s = d.vtile_sys[tile];
@@ -909,14 +908,6 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int nin
                     }
                 }
             }
-        }
-        if (dir_update) {
-            double nI[3] = {d.nx[nin][v], d.ny[nin][v], d.nz[nin][v]};
-            double nO[3];
-            director_update(S, local, p, nI, nO);
-            d.nx[nout][v] = nO[0];
-            d.ny[nout][v] = nO[1];
-            d.nz[nout][v] = nO[2];
         }
     }
     unsigned long long m = __ballot(flag);
