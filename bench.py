@@ -247,14 +247,14 @@ def single_sphere(svm, n=262144, steps=400):
             "retriangulation_ms": 1e3 * t_dl}
 
 
-def fire_comparison(svm, n_sys=1024, cap=10000):
+def fire_comparison(svm, n_sys=1024, cap=10000, lanes=1):
     """Batched FIRE from seeded uniform random starts (1024 spheres, N
     log-uniform in [500, 5000], p0 uniform in [3.6, 3.9]) to rms force
     < 1e-10, plain and after the learned warm start (weights trained on
     held-out seeds, curvedspacevertexmodel_amd/warmstart_weights.npz).  FIRE
     parameters of the reference's minimisation driver; iterations capped at
     `cap` per sphere (spheres with p0 above ~3.81 are floppy and keep flipping
-    without reaching 1e-10)."""
+    without reaching 1e-10).  Both runs use `lanes` stream lanes."""
     from curvedspacevertexmodel_amd import warmstart as ws
     params = ws.batch_params(n_sys)
     net, scale = ws.load()
@@ -262,6 +262,7 @@ def fire_comparison(svm, n_sys=1024, cap=10000):
     energies = {}
     for mode in ("plain", "learned"):
         ctx = ws.fire_batch(params, seed0=1137, cutoff=1e-10, max_iterations=cap)
+        ctx.set_lanes(lanes)
         wall, its, conv, fmax = ws.run_fire(ctx, n_sys, net if mode == "learned" else None, scale)
         ctx.compute_geometry()
         energies[mode] = (np.array(ctx.energies()[0]), conv)
@@ -276,7 +277,7 @@ def fire_comparison(svm, n_sys=1024, cap=10000):
     total = int(sum(n for n, _ in params))
     ms = net.time_forward(total, reps=10)
     return {"workload": "batched-fire-1024", "spheres": n_sys, "total_cells": total, "N_range": [500, 5000],
-            "p0_range": [3.6, 3.9], "stop": "rms force < 1e-10", "iteration_cap": cap, **res,
+            "p0_range": [3.6, 3.9], "stop": "rms force < 1e-10", "iteration_cap": cap, "lanes": lanes, **res,
             "speedup_wall": res["plain"]["wall_s"] / res["learned"]["wall_s"],
             "final_energy_rel_diff_median": float(np.median(rel)) if both.any() else None,
             "final_energy_within_1e-9": float(np.mean(rel <= 1e-9)) if both.any() else None,
@@ -367,11 +368,12 @@ def main():
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ctx, params)
+    lanes = ctx.tuning()["lanes"]
     del ctx
     if not args.no_single:
         result["single"] = single_sphere(svm)
     if not args.no_fire:
-        result["fire"] = fire_comparison(svm, args.fire_spheres, args.fire_cap)
+        result["fire"] = fire_comparison(svm, args.fire_spheres, args.fire_cap, lanes=lanes)
     print(json.dumps(result), flush=True)
     if args.copy_probe:
         copy_probe(svm)

@@ -340,7 +340,6 @@ int enqueue_step(svm_context* ctx, const Target& w, int updater) {
     ctx->ncur = nout;
     return SVM_OK;
 }
-int enqueue_step(svm_context* ctx, int updater) { return enqueue_step(ctx, whole(ctx), updater); }
 
 // Lane k of the batch: its spheres and tiles, its T1 candidate lists, its stream.
 Target lane_target(svm_context* ctx, int k) {
@@ -386,10 +385,12 @@ int over_lanes(svm_context* ctx, F body) {
     return r;
 }
 
-// n dynamics steps of every lane, interleaved step by step across the lanes'
-// streams.  All lanes share the ping-pong buffer indices, which advance
-// once per step.
-int enqueue_lane_steps(svm_context* ctx, int updater, int n) {
+// n iterations of `enqueue` (one step or minimiser iteration of a target)
+// for every lane, interleaved iteration by iteration across the lanes'
+// streams.  All lanes share the ping-pong buffer indices, which advance once
+// per iteration.
+template <class F>
+int enqueue_lane_iterations(svm_context* ctx, int n, F enqueue) {
     return over_lanes(ctx, [&]() -> int {
         for (int i = 0; i < n; ++i) {
             const int p = ctx->pcur, q = ctx->ncur;
@@ -397,7 +398,7 @@ int enqueue_lane_steps(svm_context* ctx, int updater, int n) {
             for (int k = 0; k < ctx->nlanes; ++k) {
                 ctx->pcur = p;
                 ctx->ncur = q;
-                int r = enqueue_step(ctx, lane_target(ctx, k), updater);
+                int r = enqueue(lane_target(ctx, k));
                 if (r) {
                     ctx->pcur = p;
                     ctx->ncur = q;
@@ -415,78 +416,134 @@ int enqueue_lane_steps(svm_context* ctx, int updater, int n) {
 
 // One FIRE iteration: mixing + VV first half + move, T1, geometry, force +
 // VV second half + reductions + controller.
-int enqueue_fire_iteration(svm_context* ctx) {
+int enqueue_fire_iteration(svm_context* ctx, const Target& w) {
     int pout = 1 - ctx->pcur;
     {
         Timer t(ctx, "k_fire_vv1");
-        hipLaunchKernelGGL(svm::k_fire_vv1, dim3(xcd_grid(ctx->nvt)), dim3(SVM_TILE), 0, ctx->stream, ctx->d, ctx->pcur, pout);
+        hipLaunchKernelGGL(svm::k_fire_vv1, dim3(xcd_grid(w.d.vtn)), dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout);
         int r = last_launch(ctx);
         if (r) return r;
     }
-    int r = launch_t1(ctx, pout, ctx->ncur, ctx->ncur, 0, 0, 1);
+    int r = launch_t1(ctx, w, pout, ctx->ncur, ctx->ncur, 0, 0, 1);
     if (r) return r;
     ctx->pcur = pout;
-    r = launch_geometry(ctx, 1);
+    r = launch_geometry(ctx, w, 1);
     if (r) return r;
-    r = launch_vertex<svm::VM_FIRE>(ctx, ctx->pcur, 1, "k_vertex_fire");
+    r = launch_vertex<svm::VM_FIRE>(ctx, w, ctx->pcur, 1, "k_vertex_fire");
     if (r) return r;
     Timer t(ctx, "k_fire_control");
-    hipLaunchKernelGGL(svm::k_fire_control, dim3(ctx->nsys), dim3(SVM_TILE), 0, ctx->stream, ctx->d);
+    hipLaunchKernelGGL(svm::k_fire_control, dim3(w.d.ns), dim3(SVM_TILE), 0, w.st, w.d);
     return last_launch(ctx);
 }
 
 // One Adam iteration: geometry, forces + moment update + move + reductions,
 // T1, then the per-sphere controller.
-int enqueue_adam_iteration(svm_context* ctx) {
-    int r = launch_geometry(ctx, 1);
+int enqueue_adam_iteration(svm_context* ctx, const Target& w) {
+    int r = launch_geometry(ctx, w, 1);
     if (r) return r;
     const int pout = 1 - ctx->pcur;
-    r = launch_vertex<svm::VM_ADAM>(ctx, pout, 0, "k_vertex_adam");
+    r = launch_vertex<svm::VM_ADAM>(ctx, w, pout, 0, "k_vertex_adam");
     if (r) return r;
-    r = launch_t1(ctx, pout, ctx->ncur, ctx->ncur, 0, 0, 1);
+    r = launch_t1(ctx, w, pout, ctx->ncur, ctx->ncur, 0, 0, 1);
     if (r) return r;
     ctx->pcur = pout;
     Timer t(ctx, "k_adam_control");
-    hipLaunchKernelGGL(svm::k_adam_control, dim3(ctx->nsys), dim3(SVM_TILE), 0, ctx->stream, ctx->d);
+    hipLaunchKernelGGL(svm::k_adam_control, dim3(w.d.ns), dim3(SVM_TILE), 0, w.st, w.d);
     return last_launch(ctx);
 }
 
-// `k` minimiser iterations (kind 3 FIRE, 4 Adam) replayed from a captured
-// HIP graph when k is even: the ping-pong position buffers then end where they
-// started, so the graph replays whenever the context is in that state, as in
-// svm_run_graph.  Odd k, or profiling, run eagerly.  Same kernels in the same
-// order either way.
+// `replays` x `k` iterations of `enqueue(target)` -- dynamics steps (kind =
+// updater 0-2), FIRE (3) or Adam (4) iterations -- over the whole batch, or
+// over every lane when the context has lanes.  With `graph` and k even they
+// are replayed from captured HIP graphs: the ping-pong buffers then end where
+// they started, so a graph captured from buffer state (pcur, ncur) replays
+// whenever the context is in that state again.  Otherwise, or when
+// profiling, they run eagerly.  Same kernels in the same order either way.
 template <class F>
-int minimiser_iterations(svm_context* ctx, int kind, int k, F enqueue) {
-    if ((k & 1) || ctx->profiling) {
-        for (int i = 0; i < k; ++i) {
-            int r = enqueue(ctx);
-            if (r) return r;
+int iterate(svm_context* ctx, int kind, int k, int replays, bool graph, F enqueue) {
+    const bool lanes = ctx->nlanes > 1 && !ctx->profiling;   // per-kernel timing is on the context's stream
+    if (!graph || (k & 1) || ctx->profiling) {
+        for (int rep = 0; rep < replays; ++rep) {
+            if (lanes) {
+                int r = enqueue_lane_iterations(ctx, k, enqueue);
+                if (r) return r;
+                continue;
+            }
+            for (int i = 0; i < k; ++i) {
+                int r = enqueue(whole(ctx));
+                if (r) return r;
+            }
         }
         return SVM_OK;
     }
     auto key = std::make_pair(kind, k * 4 + ctx->pcur * 2 + ctx->ncur);
-    auto it = ctx->graphs.find(key);
-    if (it == ctx->graphs.end()) {
-        hipGraph_t g;
-        HIPCHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < k; ++i) {
-            int r = enqueue(ctx);
+    const int nl = lanes ? ctx->nlanes : 1;
+    std::vector<hipGraphExec_t>* execs = nullptr;
+    hipGraphExec_t whole_exec = nullptr;
+    if (lanes) {
+        auto lt = ctx->lane_graphs.find(key);
+        if (lt != ctx->lane_graphs.end()) execs = &lt->second;
+    } else {
+        auto it = ctx->graphs.find(key);
+        if (it != ctx->graphs.end()) whole_exec = it->second;
+    }
+    if (!execs && !whole_exec) {
+        // capture: one graph per lane, each on the lane's stream (the whole
+        // batch is the one lane of the context's stream)
+        const int p = ctx->pcur, q = ctx->ncur;
+        std::vector<hipGraphExec_t> ex;
+        std::vector<hipGraph_t> defs;
+        auto discard = [&]() {
+            for (auto e : ex) (void)hipGraphExecDestroy(e);
+            for (auto g : defs) (void)hipGraphDestroy(g);
+            ctx->pcur = p;
+            ctx->ncur = q;
+        };
+        for (int l = 0; l < nl; ++l) {
+            const Target w = lanes ? lane_target(ctx, l) : whole(ctx);
+            ctx->pcur = p;
+            ctx->ncur = q;
+            if (hipStreamBeginCapture(w.st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                discard();
+                return fail(ctx, SVM_ERR_HIP, "graph capture failed");
+            }
+            int r = SVM_OK;
+            for (int i = 0; i < k && !r; ++i) r = enqueue(w);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(w.st, &g);
+            if (g) defs.push_back(g);
+            hipGraphExec_t ge = nullptr;
+            if (!r && (e != hipSuccess || hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess))
+                r = fail(ctx, SVM_ERR_HIP, "graph capture failed");
             if (r) {
-                hipGraph_t junk;
-                (void)hipStreamEndCapture(ctx->stream, &junk);
+                discard();
                 return r;
             }
+            ex.push_back(ge);
         }
-        HIPCHECK(ctx, hipStreamEndCapture(ctx->stream, &g));
-        hipGraphExec_t ge;
-        HIPCHECK(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-        ctx->graphs[key] = ge;
-        ctx->graph_defs[key] = g;
-        it = ctx->graphs.find(key);
+        ctx->pcur = p;
+        ctx->ncur = q;
+        if (lanes) {
+            ctx->lane_graphs[key] = ex;
+            ctx->lane_graph_defs[key] = defs;
+            execs = &ctx->lane_graphs[key];
+        } else {
+            ctx->graphs[key] = ex[0];
+            ctx->graph_defs[key] = defs[0];
+            whole_exec = ex[0];
+        }
     }
-    HIPCHECK(ctx, hipGraphLaunch(it->second, ctx->stream));
-    return SVM_OK;
+    if (!lanes) {
+        for (int i = 0; i < replays; ++i) HIPCHECK(ctx, hipGraphLaunch(whole_exec, ctx->stream));
+        return SVM_OK;
+    }
+    const std::vector<hipGraphExec_t>& ge = *execs;
+    return over_lanes(ctx, [&]() -> int {
+        for (int i = 0; i < replays; ++i)
+            for (int l = 0; l < ctx->nlanes; ++l)
+                HIPCHECK(ctx, hipGraphLaunch(ge[l], l == 0 ? ctx->stream : ctx->lane_streams[l - 1]));
+        return SVM_OK;
+    });
 }
 
 int require_topology(svm_context* ctx) {
@@ -1517,13 +1574,7 @@ static int run_steps(svm_context* ctx, int updater, int n_steps) {
     if (!ctx || n_steps < 0) return SVM_ERR_ARG;
     int r = require_topology(ctx);
     if (r) return r;
-    // per-kernel timing records events on the context's stream: whole batch
-    if (ctx->nlanes > 1 && !ctx->profiling) return enqueue_lane_steps(ctx, updater, n_steps);
-    for (int i = 0; i < n_steps; ++i) {
-        r = enqueue_step(ctx, updater);
-        if (r) return r;
-    }
-    return SVM_OK;
+    return iterate(ctx, updater, n_steps, 1, false, [&](const Target& w) { return enqueue_step(ctx, w, updater); });
 }
 
 int svm_brownian_steps(svm_context* ctx, int32_t n_steps) { return run_steps(ctx, 0, n_steps); }
@@ -1536,87 +1587,8 @@ int svm_run_graph(svm_context* ctx, int32_t updater, int32_t steps_per_graph, in
         return fail(ctx, SVM_ERR_ARG, "steps_per_graph must be a positive even number");
     int r = require_topology(ctx);
     if (r) return r;
-    // An even number of steps returns the ping-pong buffers to where they
-    // started, so a graph captured from buffer state (pcur, ncur) can be
-    // replayed whenever the context is in that state again.
-    auto key = std::make_pair((int)updater, (int)steps_per_graph * 4 + ctx->pcur * 2 + ctx->ncur);
-    if (ctx->nlanes > 1) {
-        // one graph per lane, each captured on the lane's stream
-        auto lt = ctx->lane_graphs.find(key);
-        if (lt == ctx->lane_graphs.end()) {
-            bool prof = ctx->profiling;
-            ctx->profiling = false;
-            const int p = ctx->pcur, q = ctx->ncur;
-            std::vector<hipGraphExec_t> execs;
-            std::vector<hipGraph_t> defs;
-            auto discard = [&]() {
-                for (auto e : execs) (void)hipGraphExecDestroy(e);
-                for (auto g : defs) (void)hipGraphDestroy(g);
-                ctx->pcur = p;
-                ctx->ncur = q;
-                ctx->profiling = prof;
-            };
-            for (int k = 0; k < ctx->nlanes; ++k) {
-                const Target w = lane_target(ctx, k);
-                ctx->pcur = p;
-                ctx->ncur = q;
-                if (hipStreamBeginCapture(w.st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-                    discard();
-                    return fail(ctx, SVM_ERR_HIP, "lane graph capture failed");
-                }
-                for (int i = 0; i < steps_per_graph && !r; ++i) r = enqueue_step(ctx, w, updater);
-                hipGraph_t g = nullptr;
-                const hipError_t e = hipStreamEndCapture(w.st, &g);
-                if (g) defs.push_back(g);
-                hipGraphExec_t ge = nullptr;
-                if (!r && (e != hipSuccess || hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess))
-                    r = fail(ctx, SVM_ERR_HIP, "lane graph capture failed");
-                if (r) {
-                    discard();
-                    return r;
-                }
-                execs.push_back(ge);
-            }
-            ctx->pcur = p;   // an even number of steps: back where the capture started
-            ctx->ncur = q;
-            ctx->profiling = prof;
-            ctx->lane_graphs[key] = execs;
-            ctx->lane_graph_defs[key] = defs;
-            lt = ctx->lane_graphs.find(key);
-        }
-        const std::vector<hipGraphExec_t>& ge = lt->second;
-        return over_lanes(ctx, [&]() -> int {
-            for (int i = 0; i < n_replays; ++i)
-                for (int k = 0; k < ctx->nlanes; ++k)
-                    HIPCHECK(ctx, hipGraphLaunch(ge[k], k == 0 ? ctx->stream : ctx->lane_streams[k - 1]));
-            return SVM_OK;
-        });
-    }
-    auto it = ctx->graphs.find(key);
-    if (it == ctx->graphs.end()) {
-        bool prof = ctx->profiling;
-        ctx->profiling = false;
-        hipGraph_t g;
-        HIPCHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < steps_per_graph; ++i) {
-            r = enqueue_step(ctx, updater);
-            if (r) {
-                hipGraph_t junk;
-                (void)hipStreamEndCapture(ctx->stream, &junk);
-                ctx->profiling = prof;
-                return r;
-            }
-        }
-        HIPCHECK(ctx, hipStreamEndCapture(ctx->stream, &g));
-        ctx->profiling = prof;
-        hipGraphExec_t ge;
-        HIPCHECK(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-        ctx->graphs[key] = ge;
-        ctx->graph_defs[key] = g;
-        it = ctx->graphs.find(key);
-    }
-    for (int i = 0; i < n_replays; ++i) HIPCHECK(ctx, hipGraphLaunch(it->second, ctx->stream));
-    return SVM_OK;
+    return iterate(ctx, updater, steps_per_graph, n_replays, true,
+                   [&](const Target& w) { return enqueue_step(ctx, w, updater); });
 }
 
 int svm_fire_set_parameters(svm_context* ctx, int32_t sys, const svm_fire_params* p) {
@@ -1681,7 +1653,7 @@ int svm_fire_minimize(svm_context* ctx, int32_t check_every) {
         bool any = false;
         for (int i = 0; i < ctx->nsys; ++i) any |= !s[i].f_frozen;
         if (!any) break;
-        r = minimiser_iterations(ctx, 3, check_every, enqueue_fire_iteration);
+        r = iterate(ctx, 3, check_every, 1, true, [&](const Target& w) { return enqueue_fire_iteration(ctx, w); });
         if (r) return r;
         r = download_sys(ctx, s);
         if (r) return r;
@@ -1756,7 +1728,7 @@ int svm_adam_minimize(svm_context* ctx, int32_t check_every) {
         bool any = false;
         for (int i = 0; i < ctx->nsys; ++i) any |= !s[i].f_frozen;
         if (!any) break;
-        r = minimiser_iterations(ctx, 4, check_every, enqueue_adam_iteration);
+        r = iterate(ctx, 4, check_every, 1, true, [&](const Target& w) { return enqueue_adam_iteration(ctx, w); });
         if (r) return r;
         r = download_sys(ctx, s);
         if (r) return r;

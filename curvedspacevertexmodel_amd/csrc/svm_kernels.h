@@ -523,7 +523,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, i
 // k_vertex<VM_FIRE> wrote the per-tile partial sums of F.v, F.F and v.v.
 __global__ __launch_bounds__(SVM_TILE) void k_fire_control(Dev d) {
     __shared__ double buf[SVM_MAX_TILES];
-    SysParams* S = d.sys + blockIdx.x;
+    SysParams* S = d.sys + d.s0 + blockIdx.x;
     if (S->f_frozen) return;
     double Power = fold_lds(d.vpart + 3 * S->vtile0 + 0, 3, S->nvtiles, buf);
     double forceNorm = fold_lds(d.vpart + 3 * S->vtile0 + 1, 3, S->nvtiles, buf);
@@ -568,7 +568,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_fire_control(Dev d) {
 // loop condition (iterations < max and forceMax > cutoff).
 __global__ __launch_bounds__(SVM_TILE) void k_adam_control(Dev d) {
     __shared__ double buf[SVM_MAX_TILES];
-    SysParams* S = d.sys + blockIdx.x;
+    SysParams* S = d.sys + d.s0 + blockIdx.x;
     if (S->f_frozen) return;
     double forceNorm = fold_lds(d.vpart + 3 * S->vtile0 + 1, 3, S->nvtiles, buf);
     if (threadIdx.x == 0) {

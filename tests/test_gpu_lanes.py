@@ -116,3 +116,32 @@ def test_lanes_then_whole_batch_calls():
     for i in range(len(ps)):
         np.testing.assert_array_equal(a.forces(i), b.forces(i))
         np.testing.assert_array_equal(a.area_perimeter(i), b.area_perimeter(i))
+
+
+@pytest.mark.parametrize("check_every", [40, 7])
+def test_minimisers_with_lanes(check_every):
+    """FIRE and Adam over lanes (graph replays for an even check interval,
+    eager iterations for an odd one): the one-lane result bit for bit, with
+    spheres converging at different iterations."""
+    ps = params(9, 313)
+    out = {}
+    for lanes in (1, 3):
+        ctx = build(ps, lanes)
+        ctx.brownian_steps(20)
+        for i, (n, p0, v0) in enumerate(ps):
+            ctx.set_preferred_parameters(i, 1.0, p0)
+            ctx.fire_set_parameters(i, dt=0.01, max_iterations=400 + 50 * i, force_cutoff=1e-9)
+        ctx.fire_minimize(check_every)
+        fire = ([ctx.positions(i) for i in range(len(ps))], [ctx.fire_state(i) for i in range(len(ps))])
+        for i in range(len(ps)):
+            ctx.adam_set_parameters(i, dt=1e-3, max_iterations=150 + 20 * i)
+        ctx.adam_minimize(check_every)
+        adam = ([ctx.positions(i) for i in range(len(ps))], [ctx.adam_state(i) for i in range(len(ps))])
+        out[lanes] = (fire, adam, ctx.counters())
+    for k in (0, 1):
+        for a, b in zip(out[1][k][0], out[3][k][0]):
+            np.testing.assert_array_equal(a, b)
+        assert out[1][k][1] == out[3][k][1]
+    its = [st["iterations"] for st in out[1][0][1]]
+    assert len(set(its)) > 1        # the spheres stop at different iterations
+    np.testing.assert_array_equal(out[1][2], out[3][2])
