@@ -6,7 +6,8 @@
 // Launches per step:
 //   k_t1_detect   thread per vertex: short-edge test on the moved positions
 //                 (bit mask + compact lists of flippable and of blocked
-//                 candidates), active-director update.
+//                 candidates).  The lists and their counters belong to the
+//                 launch's lane of spheres.
 //   k_t1_cluster  one wavefront per candidate vertex u: a dry run of the
 //                 sequential loop restricted to u -- u's visit, then the
 //                 visits of every later vertex its flips disturbed, in index
@@ -874,7 +875,8 @@ __global__ __launch_bounds__(SVM_TILE) void k_build_vln(Dev d, int only) {
 // (src/model/sphericalVertexModel.cpp:469-481): vertex v is a candidate when
 // it has an edge (v, w), v < w, shorter than the threshold.  Candidates go to
 // the bit mask (for the sequential path) and to the compact list (for the dry
-// runs); the active updater's director step is fused here.
+// runs).  The active updater's director step is in the vertex pass; the T1
+// commit redoes it for every vertex a flip moved.
 __global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int skip_frozen) {
     const int tile = lane_vtile(d);
     if (tile < 0) return;
