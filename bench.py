@@ -323,7 +323,7 @@ def main():
     ctx.synchronize()
     tune_table = None
     if args.autotune:
-        tune_table = ctx.autotune(updater=1, steps_per_trial=90)
+        tune_table = ctx.autotune(updater=1, steps_per_trial=200)
         G = ctx.tuning()["graph_steps"]
         steps = max(G, args.steps - args.steps % G)
         print(f"autotuned: {ctx.tuning()}", flush=True)
