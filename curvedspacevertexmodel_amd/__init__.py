@@ -41,7 +41,7 @@ EXPORTS = [
     "svm_adam_set_parameters", "svm_adam_minimize", "svm_adam_get_state", "svm_adam_set_max_iterations",
     "svm_dyn_set_reference", "svm_dyn_measure", "svm_nc_open", "svm_nc_write", "svm_nc_num_records", "svm_nc_close",
     "svm_set_vicsek", "svm_vicsek_steps",
-    "svm_build_voronoi_from_cell_points", "svm_voronoi_forces", "svm_get_voronoi_forces",
+    "svm_build_voronoi_from_cell_points", "svm_voronoi_forces", "svm_get_voronoi_forces", "svm_voronoi_brownian_steps", "svm_get_generators",
 ]
 
 
@@ -326,6 +326,14 @@ class Context:
             self._ck(lib().svm_get_voronoi_forces(self.h, C.c_int32(s), _dp(f)), "svm_get_voronoi_forces")
             out.append(f)
         return out
+
+    def voronoi_brownian_steps(self, n):
+        """n Brownian steps of every (Voronoi-model) sphere's generators on the GPU,
+        each followed by the GPU retriangulation; dt, mu, T from set_brownian."""
+        self._ck(lib().svm_voronoi_brownian_steps(self.h, C.c_int32(n)), "svm_voronoi_brownian_steps")
+
+    def generators(self, sys):
+        return self._get3("svm_get_generators", sys, self.n_cells[sys])
 
     def delaunay_triangles(self, sys):
         out = np.zeros((2 * self.n_cells[sys] - 4, 3), np.int32)

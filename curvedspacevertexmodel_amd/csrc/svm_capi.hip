@@ -558,16 +558,33 @@ int require_topology(svm_context* ctx) {
 namespace {
 
 // Device scratch of one triangulation, freed when the holder goes.
+// Device buffers handed out in call order.  rewind() starts the order again
+// and later gets reuse the buffers of the same position when they are large
+// enough, so a loop that repeats the same sequence of gets allocates only on
+// its first pass (or when a size grows).
 struct DevScratch {
     std::vector<void*> ptrs;
+    std::vector<size_t> sizes;
+    size_t cursor = 0;
     ~DevScratch() {
         for (void* p : ptrs) (void)hipFree(p);
     }
+    void rewind() { cursor = 0; }
     template <class T>
     T* get(size_t n) {
+        const size_t bytes = sizeof(T) * std::max<size_t>(n, 1);
+        if (cursor < ptrs.size() && sizes[cursor] >= bytes) return (T*)ptrs[cursor++];
         void* p = nullptr;
-        if (hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) return nullptr;
-        ptrs.push_back(p);
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        if (cursor < ptrs.size()) {
+            (void)hipFree(ptrs[cursor]);   // too small: replaced (hipFree waits for the device)
+            ptrs[cursor] = p;
+            sizes[cursor] = bytes;
+        } else {
+            ptrs.push_back(p);
+            sizes.push_back(bytes);
+        }
+        ++cursor;
         return (T*)p;
     }
 };
@@ -634,22 +651,15 @@ struct DlOut {
 
 // Spherical Delaunay triangulation of n points (host, AoS) on stream `st`,
 // exact (see svm_voronoi.h).  Returns SVM_OK or an error code with `msg`.
-int dl_triangulate(hipStream_t st, DevScratch& m, const double* pts, int n, int cap, DlOut& o, std::string& msg) {
+// Triangulation of n device points P (AoS) whose largest radius is at most
+// Rs (used to size the cell list and the search radii; the result does not
+// depend on it).
+int dl_triangulate_dev(hipStream_t st, DevScratch& m, double* P, int n, double Rs, int cap, DlOut& o,
+                       std::string& msg) {
     if (n < 4) { msg = "a spherical triangulation needs at least 4 points"; return SVM_ERR_ARG; }
-    double Rs = 0.0;
-    for (int i = 0; i < n; ++i) {
-        const double* p = pts + 3 * (size_t)i;
-        double r = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
-        if (!(r > 0.0) || !std::isfinite(r)) { msg = "points must be finite and away from the origin"; return SVM_ERR_ARG; }
-        Rs = std::max(Rs, r);
-    }
     const double spacing = std::sqrt(4.0 * SVM_PI * Rs * Rs / n);   // mean distance between neighbours
     const double r0 = 3.5 * spacing;
-    o.P = m.get<double>(3 * (size_t)n);
-    if (!o.P || hipMemcpyAsync(o.P, pts, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess) {
-        msg = "device copy failed";
-        return SVM_ERR_HIP;
-    }
+    o.P = P;
     DlCells cl;
     if (!dl_cell_list(st, m, o.P, n, Rs, r0, cl)) { msg = "cell list failed"; return SVM_ERR_HIP; }
     const svm::DlGrid g = cl.g;
@@ -734,6 +744,25 @@ int dl_triangulate(hipStream_t st, DevScratch& m, const double* pts, int n, int 
     }
     if (herr) { msg = "stars of neighbouring points disagree"; return SVM_ERR_TOPOLOGY; }
     return SVM_OK;
+}
+
+// Triangulation of n host points (AoS), checked to be finite and away from
+// the origin.
+int dl_triangulate(hipStream_t st, DevScratch& m, const double* pts, int n, int cap, DlOut& o, std::string& msg) {
+    if (n < 4) { msg = "a spherical triangulation needs at least 4 points"; return SVM_ERR_ARG; }
+    double Rs = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double* p = pts + 3 * (size_t)i;
+        double r = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+        if (!(r > 0.0) || !std::isfinite(r)) { msg = "points must be finite and away from the origin"; return SVM_ERR_ARG; }
+        Rs = std::max(Rs, r);
+    }
+    double* P = m.get<double>(3 * (size_t)n);
+    if (!P || hipMemcpyAsync(P, pts, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess) {
+        msg = "device copy failed";
+        return SVM_ERR_HIP;
+    }
+    return dl_triangulate_dev(st, m, P, n, Rs, cap, o, msg);
 }
 
 }  // namespace
@@ -1270,6 +1299,86 @@ int svm_voronoi_forces(svm_context* ctx) {
     r = last_launch(ctx);
     if (r) return r;
     return check_device_errors(ctx);
+}
+
+// Retriangulation of a Voronoi-model sphere from its device-resident
+// generators: the topology, circumcentre vertices and vertex lists of
+// build_topology(..., circum = 1), without host copies of the points.  The
+// circumcentres use the radius of the sphere's first generator, as the
+// host-points build does; `keep_triangles` also copies the triangles to the
+// host (for svm_get_delaunay_triangles).
+static int retopologise_voronoi(svm_context* ctx, int32_t sys, DevScratch& m, bool keep_triangles) {
+    const SysParams& L = ctx->layout[sys];
+    const int n = L.nc;
+    const int tb = 256;
+    m.rewind();
+    double* P = m.get<double>(3 * (size_t)n);
+    if (!P) return fail(ctx, SVM_ERR_HIP, "device allocation failed");
+    hipLaunchKernelGGL(svm::k_gen_aos, dim3((n + tb - 1) / tb), dim3(tb), 0, ctx->stream, ctx->d, sys, P);
+    double p0[3];
+    for (int k = 0; k < 3; ++k)
+        HIPCHECK(ctx, hipMemcpyAsync(&p0[k], ctx->d.gen + (size_t)k * ctx->Cs + L.co, sizeof(double),
+                                     hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const double radius = std::sqrt(p0[0] * p0[0] + p0[1] * p0[1] + p0[2] * p0[2]);
+    if (!(radius > 0.0) || !std::isfinite(radius)) return fail(ctx, SVM_ERR_STATE, "generators are not finite");
+    DlOut o;
+    std::string msg;
+    // every generator lies on the sphere of the model's radius (to rounding)
+    const double R_model = std::sqrt(ctx->desc[sys].n_cells / (4.0 * SVM_PI));
+    int r = dl_triangulate_dev(ctx->stream, m, P, n, std::max(radius, R_model) * (1.0 + 1e-12), ctx->cap, o, msg);
+    if (r) return fail(ctx, r, msg);
+    hipLaunchKernelGGL(svm::k_dl_vertices, dim3((o.ntri + tb - 1) / tb), dim3(tb), 0, ctx->stream, ctx->d, sys,
+                       ctx->pcur, (const double*)o.P, o.ntri, ctx->cap, (const int*)o.T, (const int*)o.nbr,
+                       (const int*)o.deg, (const int*)o.tid, radius, o.err, 1);
+    hipLaunchKernelGGL(svm::k_dl_rings, dim3((n + tb - 1) / tb), dim3(tb), 0, ctx->stream, ctx->d, sys, n, ctx->cap,
+                       (const int*)o.deg, (const int*)o.tid);
+    hipLaunchKernelGGL(svm::k_build_vln, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, ctx->d, sys);
+    r = last_launch(ctx);
+    if (r) return r;
+    int herr = 0;
+    HIPCHECK(ctx, hipMemcpyAsync(&herr, o.err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    if (keep_triangles) {
+        ctx->triangles[sys].resize(3 * (size_t)o.ntri);
+        HIPCHECK(ctx, hipMemcpyAsync(ctx->triangles[sys].data(), o.T, sizeof(int32_t) * 3 * (size_t)o.ntri,
+                                     hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (herr) return fail(ctx, SVM_ERR_TOPOLOGY, "triangulation is not closed");
+    return SVM_OK;
+}
+
+int svm_voronoi_brownian_steps(svm_context* ctx, int32_t n_steps) {
+    if (!ctx || n_steps < 0) return SVM_ERR_ARG;
+    int r = require_topology(ctx);
+    if (r) return r;
+    for (int s = 0; s < ctx->nsys; ++s)
+        if (!ctx->voronoi[s])
+            return fail(ctx, SVM_ERR_STATE, "sphere " + std::to_string(s) + " was not built as a Voronoi model");
+    DevScratch m;   // reused by every retriangulation of the call
+    for (int i = 0; i < n_steps; ++i) {
+        r = svm_compute_forces(ctx, ctx->force_fp32);
+        if (r) return r;
+        hipLaunchKernelGGL(svm::k_voronoi_chain, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, ctx->d);
+        hipLaunchKernelGGL(svm::k_voronoi_move, dim3(ctx->nct), dim3(SVM_TILE), 0, ctx->stream, ctx->d);
+        hipLaunchKernelGGL(svm::k_advance_step, dim3(ctx->nsys), dim3(64), 0, ctx->stream, ctx->d);
+        r = last_launch(ctx);
+        if (r) return r;
+        for (int s = 0; s < ctx->nsys; ++s) {
+            r = retopologise_voronoi(ctx, s, m, i == n_steps - 1);
+            if (r) return r;
+        }
+    }
+    return check_device_errors(ctx);
+}
+
+int svm_get_generators(svm_context* ctx, int32_t sys, double* points) {
+    if (!valid_sys(ctx, sys) || !points) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    if (!ctx->voronoi[sys]) return fail(ctx, SVM_ERR_STATE, "sphere was not built as a Voronoi model");
+    const SysParams& L = ctx->layout[sys];
+    const double* g = ctx->d.gen;
+    const size_t Cs = ctx->Cs;
+    return download_rows(ctx, L.nc, L.co, g, g + Cs, g + 2 * Cs, points);
 }
 
 int svm_get_voronoi_forces(svm_context* ctx, int32_t sys, double* forces) {

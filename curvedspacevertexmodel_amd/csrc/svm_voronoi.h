@@ -29,6 +29,8 @@
 //    triangles around a cell starting from its smallest triangle.
 #pragma once
 #include "svm_exact.h"
+#include "svm_device.h"
+#include "svm_geom.h"
 
 namespace svm {
 
@@ -441,6 +443,50 @@ __global__ __launch_bounds__(256) void k_store_generators(Dev d, int sys, const 
     d.gen[c] = P[3 * (size_t)i];
     d.gen[Cs + c] = P[3 * (size_t)i + 1];
     d.gen[2 * Cs + c] = P[3 * (size_t)i + 2];
+}
+
+// Generators of sphere `sys` (the three planes of d.gen) as AoS points for
+// the triangulation.
+__global__ __launch_bounds__(256) void k_gen_aos(Dev d, int sys, double* P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const SysParams& S = d.sys[sys];
+    if (i >= S.nc) return;
+    const int c = S.co + i;
+    const size_t Cs = (size_t)d.Cs;
+    P[3 * (size_t)i] = d.gen[c];
+    P[3 * (size_t)i + 1] = d.gen[Cs + c];
+    P[3 * (size_t)i + 2] = d.gen[2 * Cs + c];
+}
+
+// Brownian move of the generators of a Voronoi-model sphere
+// (brownianDynamics::integrateEOMCPU, src/updaters/brownianDynamics.cpp:29-48,
+// on the cell degrees of freedom, then sphericalDomain::move,
+// inc/sphericalDomain.h:203-209): d = sqrt(2 dt mu T) xi + dt mu F with F the
+// generator force (k_voronoi_chain) and xi the step's Philox normals of the
+// cell, moved on the sphere of the model's radius.  Thread per cell.
+__global__ __launch_bounds__(SVM_TILE) void k_voronoi_move(Dev d) {
+    const int tile = blockIdx.x;
+    const SysParams* S = d.sys + d.ctile_sys[tile];
+    const int c = tile * SVM_TILE + threadIdx.x;
+    const int local = c - S->co;
+    if (local >= S->nc) return;
+    const size_t Cs = (size_t)d.Cs;
+    const double forcePrefactor = S->dt * S->mu;
+    const double noisePrefactor = __builtin_sqrt(2.0 * forcePrefactor * S->T);
+    double xi[3] = {0.0, 0.0, 0.0};
+    if (S->T != 0.0) normal3(S->seed, S->step, (uint32_t)local, xi);
+    double disp[3];
+    for (int k = 0; k < 3; ++k) disp[k] = xi[k] * noisePrefactor + d.cf[k * Cs + c] * forcePrefactor;
+    double p[3] = {d.gen[c], d.gen[Cs + c], d.gen[2 * Cs + c]};
+    move_on_sphere(p, disp, S->R);
+    d.gen[c] = p[0];
+    d.gen[Cs + c] = p[1];
+    d.gen[2 * Cs + c] = p[2];
+}
+
+// One block per sphere: the step counter that keys the noise advances.
+__global__ void k_advance_step(Dev d) {
+    if (threadIdx.x == 0) d.sys[blockIdx.x].step += 1;
 }
 
 // Voronoi-model force on each generator: the chain rule from the vertex

@@ -359,6 +359,20 @@ int svm_warm_start(svm_context* ctx, svm_mlp* mlp, double scale, double max_step
 int svm_build_voronoi_from_cell_points(svm_context* ctx, int32_t sys, const double* cell_points);
 int svm_voronoi_forces(svm_context* ctx);
 int svm_get_voronoi_forces(svm_context* ctx, int32_t sys, double* forces);
+/* n_steps Brownian steps of the Voronoi model on the GPU, the generators
+ * never leaving the device: brownianDynamics::integrateEOMCPU
+ * (src/updaters/brownianDynamics.cpp:29-48) on the cells, d = sqrt(2 dt mu T)
+ * xi + dt mu F with F the generator forces of svm_voronoi_forces and xi
+ * Philox normals keyed by (sphere seed, step, cell), the move of
+ * sphericalDomain::move (inc/sphericalDomain.h:203-209) on the model's
+ * radius, then the retriangulation of every sphere that
+ * sphericalModel::moveParticles does after each move
+ * (src/model/sphericalModel.cpp:96-121), on the GPU.  dt, mu and T are the
+ * sphere's svm_set_brownian values.  Every sphere must have been built with
+ * svm_build_voronoi_from_cell_points.  svm_get_generators copies sphere
+ * `sys`'s generators out ([N][3]). */
+int svm_voronoi_brownian_steps(svm_context* ctx, int32_t n_steps);
+int svm_get_generators(svm_context* ctx, int32_t sys, double* points);
 
 /* ---- trajectory analysis and output -------------------------------------
  * dynamicalFeatures (src/analysis/dynamicalFeatures.cpp:7-53) on the GPU.

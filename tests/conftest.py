@@ -3,6 +3,15 @@ import sys
 
 import pytest
 
+# torch (whose ROCm build ships its own HIP runtime) is loaded before the
+# library's first call, as a caller that shares streams with torch must do:
+# both then use one runtime in the process, and torch stream handles are
+# valid for the library.
+try:
+    import torch  # noqa: F401
+except Exception:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))

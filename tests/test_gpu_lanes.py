@@ -145,3 +145,27 @@ def test_minimisers_with_lanes(check_every):
     its = [st["iterations"] for st in out[1][0][1]]
     assert len(set(its)) > 1        # the spheres stop at different iterations
     np.testing.assert_array_equal(out[1][2], out[3][2])
+
+
+def test_lanes_on_a_caller_stream():
+    """Lanes forked from and joined into a stream the caller owns (torch's):
+    the trajectory is the one-lane one, and a read that synchronises only the
+    caller's stream already sees every lane's last step (the join)."""
+    import torch
+    ps = params(8, 4040)
+    ref = build(ps, 1)
+    got = build(ps, 3)
+    st = torch.cuda.Stream()
+    got.set_stream(st.cuda_stream)
+    for ctx in (ref, got):
+        for i, (n, p0, v0) in enumerate(ps):
+            ctx.set_preferred_parameters(i, 1.0, p0)
+            ctx.set_active(i, dt=0.01, mu=1.0, v0=v0, tau=1.0)
+    ref.active_steps(9)
+    ref.run_graph(1, 10, 3)
+    got.active_steps(9)
+    got.run_graph(1, 10, 3)
+    for i in range(len(ps)):
+        np.testing.assert_array_equal(got.positions(i), ref.positions(i))
+        np.testing.assert_array_equal(got.directors(i), ref.directors(i))
+    got.set_stream(0)

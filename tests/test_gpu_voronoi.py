@@ -47,3 +47,84 @@ def test_voronoi_relaxation_lowers_the_energy():
     _, E = overdamped_steps(ctx, pts, 40, dt=0.02)
     assert (np.diff(E, axis=0) < 0).all(), E[:5]
     assert (E[-1] < 0.5 * E[0]).all()
+
+
+def oracle_voronoi_steps(p, steps, dt, mu, a0=1.0, p0=3.8, kr=1.0):
+    """The host loop the device steps restate: forces from the current
+    triangulation, the move on the model's radius, the retriangulation."""
+    from test_oracle import move_np
+    n = len(p)
+    R_model = np.sqrt(n / (4.0 * np.pi))
+    for _ in range(steps):
+        tris = svmtest.qhull_triangles(p)
+        topo = svmtest.topology_from_triangles(p, tris)
+        R0 = np.sqrt(p[0, 0] * p[0, 0] + p[0, 1] * p[0, 1] + p[0, 2] * p[0, 2])
+        topo["vpos"] = vo.circumcentres(p, tris, R0)
+        om = svmtest.OracleModel(topo, a0=a0, p0=p0, kr=kr)
+        om.compute_forces()
+        F = vo.generator_forces(p, tris, om.forces(), R_model)
+        p = move_np(p, dt * mu * F, R_model)
+    return p, svmtest.qhull_triangles(p)
+
+
+def test_device_voronoi_steps_follow_the_host_loop():
+    """Overdamped steps (T = 0) with the generators kept on the GPU and a GPU
+    retriangulation after every move, against the host loop on the oracle,
+    step by step from the GPU's current generators: each step's displacement
+    agrees within 1e-11 of its size (the chain-rule sums differ from the
+    oracle's at 1e-12 relative) and the triangulation after every step equals
+    Qhull's.  (Over several steps from a random start the differences grow
+    with the dynamics: 1e-9 relative after 6 steps of up to 0.9.)"""
+    sizes = (300, 1500)
+    pts = [svmtest.cell_points(n, 1210 + i) for i, n in enumerate(sizes)]
+    ctx = svm.Context([dict(n_cells=n, a0=1.0, p0=3.75, kr=1.0, t1_threshold=0.1, seed=1210 + i)
+                       for i, n in enumerate(sizes)])
+    for i, p in enumerate(pts):
+        ctx.build_voronoi_from_cell_points(i, p)
+        ctx.set_brownian(i, dt=0.02, mu=1.0, T=0.0)
+    for step in range(4):
+        prev = [ctx.generators(i) for i in range(len(sizes))]
+        ctx.voronoi_brownian_steps(1)
+        for i, g0 in enumerate(prev):
+            q, tris = oracle_voronoi_steps(g0, 1, 0.02, 1.0, p0=3.75)
+            g = ctx.generators(i)
+            move = np.abs(q - g0).max()
+            err = np.abs(g - q).max()
+            print(f"step {step} sphere {i}: displacement {move:.3e}, difference {err:.2e}")
+            assert move > 1e-4
+            assert err <= 1e-11 * move
+            np.testing.assert_array_equal(ctx.delaunay_triangles(i), tris)
+            np.testing.assert_allclose(np.linalg.norm(g, axis=1), np.sqrt(sizes[i] / (4 * np.pi)), rtol=1e-14)
+
+
+def test_device_voronoi_noise():
+    """Brownian noise of the device Voronoi steps: keyed by (seed, step,
+    cell), so 3 + 4 steps equal 7 steps and a sphere steps alike alone or in
+    a batch; its amplitude is sqrt(2 mu T dt) per component before the
+    tangent projection (2 of 3 components survive it)."""
+    n, T, dt = 2000, 5.0, 1e-4
+    p = svmtest.cell_points(n, 1220)
+    q = svmtest.cell_points(700, 1221)
+
+    def run(parts, batch):
+        descs = [dict(n_cells=n, a0=1.0, p0=3.8, kr=1.0, t1_threshold=0.1, seed=77)]
+        if batch:
+            descs.append(dict(n_cells=700, a0=1.0, p0=3.8, kr=1.0, t1_threshold=0.1, seed=78))
+        ctx = svm.Context(descs)
+        ctx.build_voronoi_from_cell_points(0, p)
+        if batch:
+            ctx.build_voronoi_from_cell_points(1, q)
+        for i in range(len(descs)):
+            ctx.set_brownian(i, dt=dt, mu=1.0, T=T)
+        for k in parts:
+            ctx.voronoi_brownian_steps(k)
+        return ctx.generators(0)
+
+    a = run([7], False)
+    np.testing.assert_array_equal(run([3, 4], False), a)
+    np.testing.assert_array_equal(run([7], True), a)
+    one = run([1], False)
+    msd = np.mean(np.sum((one - p) ** 2, 1))
+    expect = 2 * (2 * 1.0 * T * dt)               # two tangent components
+    print(f"one-step mean square displacement {msd:.4e}, expected {expect:.4e}")
+    assert abs(msd / expect - 1) < 0.1
