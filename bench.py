@@ -242,9 +242,24 @@ def single_sphere(svm, n=262144, steps=400):
         ctx.synchronize()
         rate[name] = n * (steps // 10 * 10) / (time.perf_counter() - t0)
     ctx.set_force_precision(0)
+    del ctx
+    # the same cell points as a Voronoi model: every step moves the generators
+    # (forces through the circumcentres, T 0.001) and retriangulates on the GPU
+    vctx = svm.Context([dict(n_cells=n, a0=1.0, p0=3.8, kr=1.0, t1_threshold=0.1, seed=1137)])
+    vctx.build_voronoi_from_cell_points(0, pts)
+    vctx.set_brownian(0, dt=0.01, mu=1.0, T=0.001)
+    vctx.voronoi_brownian_steps(2)
+    vctx.synchronize()
+    vsteps = 20
+    t0 = time.perf_counter()
+    vctx.voronoi_brownian_steps(vsteps)
+    vctx.synchronize()
+    t_vor = (time.perf_counter() - t0) / vsteps
     return {"workload": "single-262144", "cells": n, "updater": "brownian", "dt": 0.01, "T": 0.001, "p0": 3.8,
             "steps": steps // 10 * 10, "cell_steps_per_s": rate, "topology_build_s": t_build,
-            "retriangulation_ms": 1e3 * t_dl}
+            "retriangulation_ms": 1e3 * t_dl,
+            "voronoi_model": {"steps": vsteps, "ms_per_step": 1e3 * t_vor, "cell_steps_per_s": n / t_vor,
+                              "note": "FP64 forces on the generators, Brownian move, GPU retriangulation every step"}}
 
 
 def fire_comparison(svm, n_sys=1024, cap=10000, lanes=1):
