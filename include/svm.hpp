@@ -2,8 +2,12 @@
 // class API, so a driver written against the reference's classes keeps its
 // shape:
 //
-//   sphericalVertexModel           src/model/sphericalVertexModel.h:10-85,
-//                                  src/model/simpleModel.h:33-145
+//   simpleModel                    src/model/simpleModel.h:33-145
+//   sphericalVertexModel           src/model/sphericalVertexModel.h:10-85
+//   sphericalVoronoi               src/model/sphericalVoronoi.h (cell points as
+//                                  the particles, retriangulated after every
+//                                  move), with forces of the vertex-model
+//                                  energy of its Voronoi cells
 //   brownianDynamics               src/updaters/brownianDynamics.h:8-28
 //   sphericalSelfPropelledParticle src/updaters/sphericalSelfPropelledParticle.h:13-22,
 //                                  src/updaters/vectorialVicsek.h:31-42
@@ -39,11 +43,26 @@ inline void check(int rc, svm_context* ctx, const char* what) {
     }
 }
 
+// What the updaters and the Simulation see of a model (simpleModel,
+// src/model/simpleModel.h:33-145).
+class simpleModel {
+   public:
+    virtual ~simpleModel() = default;
+    virtual svm_context* context() = 0;
+    // true for sphericalVoronoi: the cell points are the degrees of freedom
+    virtual bool voronoi() const { return false; }
+    virtual int getNumberOfParticles() const = 0;
+    virtual void computeGeometry() = 0;
+    virtual void computeForces(bool zeroOutForces = false) = 0;
+    virtual double computeEnergy() = 0;
+    virtual double computeKineticEnergy() = 0;
+};
+
 // One sphere.  sphericalVertexModel(n, noise, area, perimeter, useGPU, neverGPU)
 // (src/model/sphericalVertexModel.cpp:5-82): seeded uniform cell points on a
 // sphere of mean cell area 1, their Delaunay topology (built on the GPU),
 // seeded unit tangent directors.
-class sphericalVertexModel {
+class sphericalVertexModel : public simpleModel {
    public:
     sphericalVertexModel(int nCells, uint64_t seed, double area = 1.0, double perimeter = 3.8, double kr = 1.0,
                          double t1Threshold = 0.1)
@@ -75,10 +94,10 @@ class sphericalVertexModel {
     sphericalVertexModel& operator=(const sphericalVertexModel&) = delete;
 
     // simpleModel::getNumberOfParticles: the vertices are the particles.
-    int getNumberOfParticles() const { return N_; }
+    int getNumberOfParticles() const override { return N_; }
     int getNumberOfCells() const { return nCells_; }
-    void computeGeometry() { check(svm_compute_geometry(ctx_), ctx_, "svm_compute_geometry"); }
-    void computeForces(bool zeroOutForces = false) {
+    void computeGeometry() override { check(svm_compute_geometry(ctx_), ctx_, "svm_compute_geometry"); }
+    void computeForces(bool zeroOutForces = false) override {
         (void)zeroOutForces;   // the self force always overwrites
         check(svm_compute_forces(ctx_, forcePrecision_), ctx_, "svm_compute_forces");
     }
@@ -88,12 +107,12 @@ class sphericalVertexModel {
     void enforceTopology() { check(svm_enforce_topology(ctx_), ctx_, "svm_enforce_topology"); }
     // sphericalVertexModel::computeEnergy: sum Kr (A - A0)^2 + (P - P0)^2 of
     // the last computed geometry.
-    double computeEnergy() {
+    double computeEnergy() override {
         double pe = 0.0, ke = 0.0;
         check(svm_get_energies(ctx_, &pe, &ke), ctx_, "svm_get_energies");
         return pe;
     }
-    double computeKineticEnergy() {
+    double computeKineticEnergy() override {
         double pe = 0.0, ke = 0.0;
         check(svm_get_energies(ctx_, &pe, &ke), ctx_, "svm_get_energies");
         return ke;
@@ -123,7 +142,7 @@ class sphericalVertexModel {
         check(svm_get_area_perimeter(ctx_, 0, &out[0][0]), ctx_, "svm_get_area_perimeter");
         return out;
     }
-    svm_context* context() { return ctx_; }
+    svm_context* context() override { return ctx_; }
 
    private:
     template <class F>
@@ -137,25 +156,93 @@ class sphericalVertexModel {
     int forcePrecision_ = 0;
 };
 
+// One sphere of the Voronoi model (sphericalVoronoi, src/model/sphericalVoronoi.h;
+// sphericalModel::setParticlePositionsRandomly, src/model/sphericalModel.cpp:37-65):
+// seeded uniform cell points, the particles, with their Delaunay topology
+// built on the GPU and rebuilt after every move.  The reference model has no
+// forces; here they are those of E = sum Kr (A - A0)^2 + (P - P0)^2 over the
+// Voronoi cells (circumcentre vertices), carried to the cell points.
+class sphericalVoronoi : public simpleModel {
+   public:
+    sphericalVoronoi(int nCells, uint64_t seed, double area = 1.0, double perimeter = 3.8, double kr = 1.0)
+        : N_(nCells) {
+        svm_system_desc d{};
+        d.n_cells = nCells;
+        d.ring_capacity = 32;
+        d.a0 = area;
+        d.p0 = perimeter;
+        d.kr = kr;
+        d.t1_threshold = 0.1;
+        d.seed = seed;
+        check(svm_create(&d, 1, &ctx_), nullptr, "svm_create");
+        std::vector<double> pts(3 * (size_t)nCells);
+        check(svm_generate_cell_points(nCells, seed, pts.data()), ctx_, "svm_generate_cell_points");
+        check(svm_build_voronoi_from_cell_points(ctx_, 0, pts.data()), ctx_, "svm_build_voronoi_from_cell_points");
+    }
+    ~sphericalVoronoi() {
+        if (ctx_) svm_destroy(ctx_);
+    }
+    sphericalVoronoi(const sphericalVoronoi&) = delete;
+    sphericalVoronoi& operator=(const sphericalVoronoi&) = delete;
+
+    bool voronoi() const override { return true; }
+    int getNumberOfParticles() const override { return N_; }
+    void computeGeometry() override { check(svm_compute_geometry(ctx_), ctx_, "svm_compute_geometry"); }
+    void computeForces(bool zeroOutForces = false) override {
+        (void)zeroOutForces;
+        check(svm_voronoi_forces(ctx_), ctx_, "svm_voronoi_forces");
+    }
+    double computeEnergy() override {
+        double pe = 0.0, ke = 0.0;
+        check(svm_get_energies(ctx_, &pe, &ke), ctx_, "svm_get_energies");
+        return pe;
+    }
+    double computeKineticEnergy() override { return 0.0; }
+    void setPreferredParameters(double a0, double p0) {
+        check(svm_set_preferred_parameters(ctx_, 0, a0, p0), ctx_, "svm_set_preferred_parameters");
+    }
+    // the cell points, and the forces on them of the last computeForces
+    std::vector<dVec> returnPositions() { return get3(svm_get_generators, "svm_get_generators"); }
+    std::vector<dVec> returnForces() { return get3(svm_get_voronoi_forces, "svm_get_voronoi_forces"); }
+    svm_context* context() override { return ctx_; }
+
+   private:
+    template <class F>
+    std::vector<dVec> get3(F f, const char* what) {
+        std::vector<dVec> out((size_t)N_);
+        check(f(ctx_, 0, &out[0][0]), ctx_, what);
+        return out;
+    }
+    svm_context* ctx_ = nullptr;
+    int N_ = 0;
+};
+
 // updater (src/updaters/baseUpdater.h:19-102)
 class updater {
    public:
     virtual ~updater() = default;
-    virtual void setModel(std::shared_ptr<sphericalVertexModel> m) { model = std::move(m); }
+    virtual void setModel(std::shared_ptr<simpleModel> m) { model = std::move(m); }
     virtual void setDeltaT(double dt) { deltaT = dt; }
     virtual void setMaximumIterations(int maxIt) { maxIterations = maxIt; }
     virtual int getCurrentIterations() { return iterations; }
     virtual void performUpdate() = 0;
 
    protected:
-    std::shared_ptr<sphericalVertexModel> model;
+    // the updaters below other than brownianDynamics move vertex models only
+    svm_context* vertexModelContext(const char* who) {
+        if (model->voronoi()) throw std::runtime_error(std::string(who) + " needs a sphericalVertexModel");
+        return model->context();
+    }
+    std::shared_ptr<simpleModel> model;
     double deltaT = 0.01;
     int maxIterations = 1000;
     int iterations = 0;
 };
 typedef std::shared_ptr<updater> UpdaterPtr;
 
-// brownianDynamics: d = sqrt(2 dt mu T) xi + dt mu F (brownianDynamics.cpp:29-48)
+// brownianDynamics: d = sqrt(2 dt mu T) xi + dt mu F (brownianDynamics.cpp:29-48),
+// on the vertices of a vertex model or the cell points of a Voronoi model
+// (which are then retriangulated).
 class brownianDynamics : public updater {
    public:
     explicit brownianDynamics(bool reproducible = true) { (void)reproducible; }
@@ -164,7 +251,8 @@ class brownianDynamics : public updater {
     void performUpdate() override {
         svm_context* c = model->context();
         check(svm_set_brownian(c, 0, deltaT, mu, temperature), c, "svm_set_brownian");
-        check(svm_brownian_steps(c, 1), c, "svm_brownian_steps");
+        if (model->voronoi()) check(svm_voronoi_brownian_steps(c, 1), c, "svm_voronoi_brownian_steps");
+        else check(svm_brownian_steps(c, 1), c, "svm_brownian_steps");
     }
 
    private:
@@ -178,7 +266,7 @@ class sphericalSelfPropelledParticle : public updater {
     void setV0(double v) { v0 = v; }
     void setTau(double t) { tau = t; }
     void performUpdate() override {
-        svm_context* c = model->context();
+        svm_context* c = vertexModelContext("sphericalSelfPropelledParticle");
         check(svm_set_active(c, 0, deltaT, mu, v0, tau), c, "svm_set_active");
         check(svm_active_steps(c, 1), c, "svm_active_steps");
     }
@@ -196,7 +284,7 @@ class sphericalVectorialVicsek : public updater {
     void setTau(double t) { tau = t; }
     void setEta(double e) { eta = e; }
     void performUpdate() override {
-        svm_context* c = model->context();
+        svm_context* c = vertexModelContext("sphericalVectorialVicsek");
         check(svm_set_vicsek(c, 0, deltaT, mu, v0, tau, eta), c, "svm_set_vicsek");
         check(svm_vicsek_steps(c, 1), c, "svm_vicsek_steps");
     }
@@ -235,7 +323,7 @@ class energyMinimizerFIRE : public updater {
                            "svm_fire_set_max_iterations");
     }
     void minimize() {
-        svm_context* c = model->context();
+        svm_context* c = vertexModelContext("energyMinimizerFIRE");
         if (dirty_) {
             p_.max_iterations = maxIterations;
             check(svm_fire_set_parameters(c, 0, &p_), c, "svm_fire_set_parameters");
@@ -283,7 +371,7 @@ class energyMinimizerAdam : public updater {
                            "svm_adam_set_max_iterations");
     }
     void minimize() {
-        svm_context* c = model->context();
+        svm_context* c = vertexModelContext("energyMinimizerAdam");
         if (dirty_) {
             p_.max_iterations = maxIterations;
             check(svm_adam_set_parameters(c, 0, &p_), c, "svm_adam_set_parameters");
@@ -307,9 +395,9 @@ class energyMinimizerAdam : public updater {
 // Simulation (src/simulation/simulation.cpp:102-174): updaters in order per step.
 class Simulation {
    public:
-    void setConfiguration(std::shared_ptr<sphericalVertexModel> c) { config = std::move(c); }
+    void setConfiguration(std::shared_ptr<simpleModel> c) { config = std::move(c); }
     void addUpdater(UpdaterPtr u) { updaters.push_back(std::move(u)); }
-    void addUpdater(UpdaterPtr u, std::shared_ptr<sphericalVertexModel> c) {
+    void addUpdater(UpdaterPtr u, std::shared_ptr<simpleModel> c) {
         u->setModel(std::move(c));
         updaters.push_back(std::move(u));
     }
@@ -331,7 +419,7 @@ class Simulation {
     int integerTimestep = 0;
 
    private:
-    std::shared_ptr<sphericalVertexModel> config;
+    std::shared_ptr<simpleModel> config;
     std::vector<UpdaterPtr> updaters;
 };
 

@@ -1,7 +1,8 @@
 // A reference-style driver over the C++ adapter (include/svm.hpp), in the
 // shape of the reference's sphericalVertex.cpp / minimization.cpp drivers:
 // build a model, stabilise with Brownian dynamics at T = 0 and (A0, P0) =
-// (1, 1), run active dynamics at the target p0, then minimise with FIRE.
+// (1, 1), run active dynamics at the target p0, then minimise with FIRE; then
+// a few overdamped steps of the Voronoi model of the same cell points.
 //
 //   driver N seed p0 steps
 //
@@ -49,6 +50,20 @@ int main(int argc, char** argv) {
         double cs = 0.0;
         for (const auto& p : model->returnPositions()) cs += p[0] + 2.0 * p[1] + 3.0 * p[2];
         std::printf("position_checksum %.17g\n", cs);
+        // the Voronoi model of the same cell points: overdamped Brownian steps
+        // of the cell points, each followed by the retriangulation
+        auto vor = std::make_shared<svm::sphericalVoronoi>(N, seed, 1.0, p0);
+        auto vbd = std::make_shared<svm::brownianDynamics>(true);
+        vbd->setT(0.0);
+        vbd->setDeltaT(0.01);
+        svm::Simulation vsim;
+        vsim.setConfiguration(vor);
+        vsim.addUpdater(vbd, vor);
+        for (int i = 0; i < 5; ++i) vsim.performTimestep();
+        std::printf("voronoi_energy %.17g\n", vsim.computePotentialEnergy());
+        double vcs = 0.0;
+        for (const auto& p : vor->returnPositions()) vcs += p[0] + 2.0 * p[1] + 3.0 * p[2];
+        std::printf("voronoi_checksum %.17g\n", vcs);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

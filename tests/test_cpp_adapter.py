@@ -71,3 +71,13 @@ def test_driver_matches_python_binding(tmp_path):
     for x, y, z in ctx.positions(0).tolist():   # the driver's summation order
         cs += x + 2.0 * y + 3.0 * z
     assert float(out["position_checksum"]) == cs
+    vctx = svm.Context([dict(n_cells=n, a0=1.0, p0=p0, kr=1.0, t1_threshold=0.1, seed=seed)])
+    vctx.build_voronoi_from_cell_points(0, svm.generate_cell_points(n, seed))
+    vctx.set_brownian(0, dt=0.01, mu=1.0, T=0.0)
+    vctx.voronoi_brownian_steps(5)
+    vctx.compute_geometry()
+    assert float(out["voronoi_energy"]) == vctx.energies()[0][0]
+    cs = 0.0
+    for x, y, z in vctx.generators(0).tolist():
+        cs += x + 2.0 * y + 3.0 * z
+    assert float(out["voronoi_checksum"]) == cs
