@@ -42,6 +42,8 @@ EXPORTS = [
     "svm_dyn_set_reference", "svm_dyn_measure", "svm_nc_open", "svm_nc_write", "svm_nc_num_records", "svm_nc_close",
     "svm_set_vicsek", "svm_vicsek_steps",
     "svm_build_voronoi_from_cell_points", "svm_voronoi_forces", "svm_get_voronoi_forces", "svm_voronoi_brownian_steps", "svm_get_generators",
+    "svm_set_positions_device", "svm_get_positions_device", "svm_set_directors_device", "svm_get_directors_device",
+    "svm_set_velocities_device", "svm_get_velocities_device", "svm_get_forces_device",
 ]
 
 
@@ -326,6 +328,34 @@ class Context:
             self._ck(lib().svm_get_voronoi_forces(self.h, C.c_int32(s), _dp(f)), "svm_get_voronoi_forces")
             out.append(f)
         return out
+
+    # ---- state through caller-owned device buffers (e.g. torch tensors) ----
+    def _dev(self, fn, sys, buf):
+        ptr = buf.data_ptr() if hasattr(buf, "data_ptr") else int(buf)
+        self._ck(getattr(lib(), fn)(self.h, C.c_int32(sys), C.c_void_p(ptr)), fn)
+
+    def get_positions_device(self, sys, out):
+        """Copy sphere sys's vertex positions into device buffer `out` ([V][3] float64,
+        e.g. a contiguous torch tensor on the GPU), on the context's stream."""
+        self._dev("svm_get_positions_device", sys, out)
+
+    def set_positions_device(self, sys, src):
+        self._dev("svm_set_positions_device", sys, src)
+
+    def get_directors_device(self, sys, out):
+        self._dev("svm_get_directors_device", sys, out)
+
+    def set_directors_device(self, sys, src):
+        self._dev("svm_set_directors_device", sys, src)
+
+    def get_velocities_device(self, sys, out):
+        self._dev("svm_get_velocities_device", sys, out)
+
+    def set_velocities_device(self, sys, src):
+        self._dev("svm_set_velocities_device", sys, src)
+
+    def get_forces_device(self, sys, out):
+        self._dev("svm_get_forces_device", sys, out)
 
     def voronoi_brownian_steps(self, n):
         """n Brownian steps of every (Voronoi-model) sphere's generators on the GPU,

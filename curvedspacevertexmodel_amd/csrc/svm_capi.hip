@@ -1465,6 +1465,81 @@ STATE_IO(positions, ctx->d.px[ctx->pcur], ctx->d.py[ctx->pcur], ctx->d.pz[ctx->p
 STATE_IO(directors, ctx->d.nx[ctx->ncur], ctx->d.ny[ctx->ncur], ctx->d.nz[ctx->ncur])
 STATE_IO(velocities, ctx->d.vx, ctx->d.vy, ctx->d.vz)
 
+// Caller-owned device buffers: [n][3] rows copied to or from the state's
+// planes by a kernel on the context's stream.  The buffer must be device
+// memory of this runtime covering `bytes` from `p`; anything else is refused
+// before a kernel could touch it.
+int check_device_buffer(svm_context* ctx, const void* p, size_t bytes) {
+    if (!p) return fail(ctx, SVM_ERR_ARG, "null device buffer");
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, SVM_ERR_ARG, "not a device buffer of this process's HIP runtime");
+    }
+    if (attr.type != hipMemoryTypeDevice) return fail(ctx, SVM_ERR_ARG, "not device memory");
+    if (attr.device != ctx->device) return fail(ctx, SVM_ERR_ARG, "device buffer on another device");
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ctx, SVM_ERR_ARG, "device buffer of unknown extent");
+    }
+    const char* b = (const char*)base;
+    const char* q = (const char*)p;
+    if (q < b || (size_t)(q - b) + bytes > size) return fail(ctx, SVM_ERR_ARG, "device buffer too small");
+    return SVM_OK;
+}
+
+__global__ __launch_bounds__(256) void k_rows_to_aos(const double* X, const double* Y, const double* Z, int off, int n,
+                                                    double* out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    out[3 * (size_t)i] = X[off + i];
+    out[3 * (size_t)i + 1] = Y[off + i];
+    out[3 * (size_t)i + 2] = Z[off + i];
+}
+
+__global__ __launch_bounds__(256) void k_aos_to_rows(const double* in, int off, int n, double* X, double* Y,
+                                                    double* Z) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    X[off + i] = in[3 * (size_t)i];
+    Y[off + i] = in[3 * (size_t)i + 1];
+    Z[off + i] = in[3 * (size_t)i + 2];
+}
+
+#define STATE_IO_DEVICE(NAME, X, Y, Z)                                                                           \
+    int svm_set_##NAME##_device(svm_context* ctx, int32_t sys, const double* src) {                                \
+        if (!valid_sys(ctx, sys)) return fail(ctx, SVM_ERR_ARG, "bad sphere index");                               \
+        const int n = ctx->layout[sys].nv, off = ctx->layout[sys].vo;                                              \
+        int r = check_device_buffer(ctx, src, sizeof(double) * 3 * (size_t)n);                                     \
+        if (r) return r;                                                                                           \
+        hipLaunchKernelGGL(k_aos_to_rows, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, src, off, n, X, Y, Z);  \
+        return last_launch(ctx);                                                                                   \
+    }                                                                                                              \
+    int svm_get_##NAME##_device(svm_context* ctx, int32_t sys, double* dst) {                                      \
+        if (!valid_sys(ctx, sys)) return fail(ctx, SVM_ERR_ARG, "bad sphere index");                               \
+        const int n = ctx->layout[sys].nv, off = ctx->layout[sys].vo;                                              \
+        int r = check_device_buffer(ctx, dst, sizeof(double) * 3 * (size_t)n);                                     \
+        if (r) return r;                                                                                           \
+        hipLaunchKernelGGL(k_rows_to_aos, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, X, Y, Z, off, n, dst);  \
+        return last_launch(ctx);                                                                                   \
+    }
+
+STATE_IO_DEVICE(positions, ctx->d.px[ctx->pcur], ctx->d.py[ctx->pcur], ctx->d.pz[ctx->pcur])
+STATE_IO_DEVICE(directors, ctx->d.nx[ctx->ncur], ctx->d.ny[ctx->ncur], ctx->d.nz[ctx->ncur])
+STATE_IO_DEVICE(velocities, ctx->d.vx, ctx->d.vy, ctx->d.vz)
+
+int svm_get_forces_device(svm_context* ctx, int32_t sys, double* dst) {
+    if (!valid_sys(ctx, sys)) return fail(ctx, SVM_ERR_ARG, "bad sphere index");
+    const int n = ctx->layout[sys].nv, off = ctx->layout[sys].vo;
+    int r = check_device_buffer(ctx, dst, sizeof(double) * 3 * (size_t)n);
+    if (r) return r;
+    hipLaunchKernelGGL(k_rows_to_aos, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, ctx->d.fx, ctx->d.fy,
+                       ctx->d.fz, off, n, dst);
+    return last_launch(ctx);
+}
+
 int svm_get_forces(svm_context* ctx, int32_t sys, double* forces) {
     if (!valid_sys(ctx, sys) || !forces) return fail(ctx, SVM_ERR_ARG, "null argument");
     return download_rows(ctx, ctx->layout[sys].nv, ctx->layout[sys].vo, ctx->d.fx, ctx->d.fy, ctx->d.fz, forces);

@@ -171,6 +171,24 @@ int svm_get_directors(svm_context* ctx, int32_t sys, double* directors);
 int svm_set_velocities(svm_context* ctx, int32_t sys, const double* velocities);
 int svm_get_velocities(svm_context* ctx, int32_t sys, double* velocities);
 int svm_get_forces(svm_context* ctx, int32_t sys, double* forces);
+/* The same state through caller-owned device buffers (double[3*V], device
+ * memory of this process's HIP runtime, e.g. a torch tensor's data_ptr()
+ * when torch was loaded first): a copy kernel enqueued on the context's
+ * stream, ordered with the context's other work and not synchronised --
+ * synchronise that stream (svm_synchronize) or work on it before the buffer
+ * is read elsewhere.  The reference's kernel callers take device pointers of
+ * GPUArray buffers (src/model/sphericalVertexModel.cuh:14-92); these let a
+ * caller keep the state on the device.  A pointer that is not device memory
+ * of this device, or whose allocation ends before 3*V doubles, is refused
+ * with SVM_ERR_ARG; a sub-buffer of a larger allocation (a tensor of a
+ * caching allocator) is the caller's to size. */
+int svm_set_positions_device(svm_context* ctx, int32_t sys, const double* positions);
+int svm_get_positions_device(svm_context* ctx, int32_t sys, double* positions);
+int svm_set_directors_device(svm_context* ctx, int32_t sys, const double* directors);
+int svm_get_directors_device(svm_context* ctx, int32_t sys, double* directors);
+int svm_set_velocities_device(svm_context* ctx, int32_t sys, const double* velocities);
+int svm_get_velocities_device(svm_context* ctx, int32_t sys, double* velocities);
+int svm_get_forces_device(svm_context* ctx, int32_t sys, double* forces);
 int svm_get_cell_positions(svm_context* ctx, int32_t sys, double* cell_positions);
 int svm_get_area_perimeter(svm_context* ctx, int32_t sys, double* area_perimeter);
 /* sphericalVertexModel::setPreferredParameters (src/model/sphericalVertexModel.cpp:103-109). */
