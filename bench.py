@@ -102,6 +102,24 @@ def build_batch(svm, params):
     return ctx, t_dl
 
 
+def host_cpu():
+    """The host's CPU model and the logical CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"model": model, "logical_cpus_usable": usable, "logical_cpus_total": os.cpu_count()}
+
+
 def cpu_baseline(ctx, params, budget_s=12.0, threads=None):
     """The CPU oracle (a restatement of the reference's CPU path, which is
     single-threaded) on the host cores: independent spheres of the same batch,
@@ -157,7 +175,7 @@ def cpu_baseline(ctx, params, budget_s=12.0, threads=None):
         for h, *_ in g:
             L.or_destroy(h)
     ncells = sum(m[1] for g in groups for m in g)
-    return dict(value=multi, unit="cell-steps/s", cores=len(groups), kind="port",
+    return dict(value=multi, unit="cell-steps/s", cores=len(groups), kind="port", host_cpu=host_cpu(),
                 single_thread_value=single_work / single_t,
                 sample=f"{sum(len(g) for g in groups)} spheres of the batch ({ncells} cells) in {len(groups)} "
                        f"threads, each advancing its own spheres with the oracle's active step for "
