@@ -280,6 +280,55 @@ def single_sphere(svm, n=262144, steps=400):
                               "note": "FP64 forces on the generators, Brownian move, GPU retriangulation every step"}}
 
 
+def large_batch(svm, steps=400):
+    """The large batched workload: 32 spheres, N log-uniform in [50,000,
+    500,000] (one Philox generator, seed 13377), p0 uniform in [3.7, 3.85],
+    FP32 forces on FP64 positions with FP64 predicates.  GPU Delaunay set-up,
+    the reference driver's stabilisation (100 Brownian steps at (A0, P0) =
+    (1, 1)), the learned warm start and 200 FIRE iterations, then active
+    dynamics (v0 0.05) replayed from HIP graphs over autotuned lanes; the
+    rate is taken over `steps` active steps after 40 warm-up steps."""
+    from curvedspacevertexmodel_amd import warmstart as ws
+    params = ws.batch_params(32, seed=13377, nmin=50000, nmax=500000, p0_range=(3.7, 3.85))
+    total = int(sum(n for n, _ in params))
+    ctx = svm.Context([dict(n_cells=n, a0=1.0, p0=p0, kr=1.0, t1_threshold=0.1, seed=1137 + i)
+                       for i, (n, p0) in enumerate(params)])
+    t0 = time.perf_counter()
+    for i, (n, _) in enumerate(params):
+        pts = svm.generate_cell_points(n, 1137 + i)
+        ctx.build_from_cell_points(i, pts[svm.spatial_order(pts)])
+        ctx.set_directors(i, svm.generate_directors(ctx.positions(i), 1137 + i))
+    t_topo = time.perf_counter() - t0
+    ctx.set_force_precision(1)
+    for i in range(len(params)):
+        ctx.set_preferred_parameters(i, 1.0, 1.0)
+        ctx.set_brownian(i, dt=0.01, mu=1.0, T=0.0)
+    ctx.run_graph(0, 20, 5)
+    for i, (n, p0) in enumerate(params):
+        ctx.set_preferred_parameters(i, 1.0, p0)
+    net, scale = ws.load()
+    ctx.warm_start(net, scale)
+    for i in range(len(params)):
+        ctx.fire_set_parameters(i, max_iterations=200, force_cutoff=1e-10, stop_rms=1, **ws.FIRE)
+    ctx.fire_minimize(100)
+    for i in range(len(params)):
+        ctx.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
+    ctx.autotune(updater=1, steps_per_trial=40)
+    G = ctx.tuning()["graph_steps"]
+    ctx.run_graph(1, G, max(1, 40 // G))
+    ctx.synchronize()
+    reps = max(1, steps // G)
+    t0 = time.perf_counter()
+    ctx.run_graph(1, G, reps)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    err = ctx.force_error()
+    return {"workload": "batched-large-32", "spheres": len(params), "cells": total, "N_range": [50000, 500000],
+            "forces": "fp32", "topology_build_s": t_topo, "steps": reps * G, "ms_per_step": 1e3 * dt / (reps * G),
+            "cell_steps_per_s": total * reps * G / dt, "schedule": ctx.tuning(),
+            "fp32_force_rms_rel_error_max": float(err[:, 1].max())}
+
+
 def fire_comparison(svm, n_sys=1024, cap=10000, lanes=1):
     """Batched FIRE from seeded uniform random starts (1024 spheres, N
     log-uniform in [500, 5000], p0 uniform in [3.6, 3.9]) to rms force
@@ -328,6 +377,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fire", action="store_true", help="skip the learned-vs-plain FIRE comparison")
     ap.add_argument("--no-single", action="store_true", help="skip the single 262,144-cell sphere block")
+    ap.add_argument("--large", action="store_true",
+                    help="also run the 32-sphere, 50k-500k-cell FP32 batch (about 6 M cells)")
     ap.add_argument("--autotune", type=int, default=1, help="1: pick the launch schedule with svm_autotune")
     ap.add_argument("--fire-spheres", type=int, default=1024)
     ap.add_argument("--fire-cap", type=int, default=10000)
@@ -405,6 +456,8 @@ def main():
     del ctx
     if not args.no_single:
         result["single"] = single_sphere(svm)
+    if args.large:
+        result["large"] = large_batch(svm)
     if not args.no_fire:
         result["fire"] = fire_comparison(svm, args.fire_spheres, args.fire_cap, lanes=lanes)
     print(json.dumps(result), flush=True)
