@@ -299,6 +299,15 @@ int launch_t1(svm_context* ctx, const Target& w, int pcur, int nin, int nout, in
         int r = last_launch(ctx);
         if (r) return r;
     }
+    bool large = false;
+    for (int s = w.d.s0; s < w.d.s0 + w.d.ns; ++s) large |= ctx->layout[s].nv >= SVM_COMMIT_LARGE;
+    if (large && !ctx->t1_sequential) {
+        Timer t(ctx, "k_t1_commit_large");
+        hipLaunchKernelGGL(svm::k_t1_commit_large, dim3(w.d.ns), dim3(1024), 0, w.st, w.d, pcur, nin, nout, dir,
+                           fire_mode, ctx->t1_sequential);
+        int r = last_launch(ctx);
+        if (r) return r;
+    }
     Timer t(ctx, "k_t1_resolve");
     hipLaunchKernelGGL(svm::k_t1_resolve, dim3(w.d.ns), dim3(SVM_TILE), 0, w.st, w.d, pcur, nin, nout, dir,
                        advance_step, fire_mode, ctx->t1_sequential);

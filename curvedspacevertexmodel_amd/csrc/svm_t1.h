@@ -1071,6 +1071,7 @@ __device__ __attribute__((always_inline)) inline void apply_item(
 #define SVM_COMMIT_VERTS 4096
 #define SVM_COMMIT_LOGS 1024
 #define SVM_COMMIT_SEEN 4096      // bitmap words: spheres of up to 131072 vertices are deduplicated
+template <int NT>
 __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int nin, int nout, int dir_mode,
                                      int use_group) {
     __shared__ int s_verts[SVM_COMMIT_VERTS];
@@ -1078,7 +1079,7 @@ __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int n
     __shared__ int s_logs[SVM_COMMIT_LOGS];
     __shared__ int s_lstart[SVM_COMMIT_LOGS + 1];    // prefix sums of items per log
     __shared__ unsigned s_seen[SVM_COMMIT_SEEN];
-    __shared__ int s_part[SVM_TILE];
+    __shared__ int s_part[NT];
     __shared__ int s_nverts, s_nmoved, s_nlogs, s_gcount;
     __shared__ unsigned long long s_flips, s_skipped;
     const int t = threadIdx.x, nt = blockDim.x;
@@ -1112,8 +1113,9 @@ __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int n
     unsigned* seen = dedup ? s_seen : nullptr;
     if (nlogs <= SVM_COMMIT_LOGS) {
         // items per log, then an exclusive scan: each thread owns a run of
-        // SVM_COMMIT_LOGS / SVM_TILE logs, and the run totals are scanned in LDS
-        constexpr int PER = SVM_COMMIT_LOGS / SVM_TILE;
+        // SVM_COMMIT_LOGS / NT logs, and the run totals are scanned in LDS
+        constexpr int PER = SVM_COMMIT_LOGS / NT;
+        static_assert(PER * NT == SVM_COMMIT_LOGS, "the block size divides the log list");
         int cnt[PER], run = 0;
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
@@ -1127,7 +1129,7 @@ __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int n
         }
         s_part[t] = run;
         __syncthreads();
-        for (int off = 1; off < SVM_TILE; off <<= 1) {
+        for (int off = 1; off < NT; off <<= 1) {
             const int add = t >= off ? s_part[t - off] : 0;
             __syncthreads();
             s_part[t] += add;
@@ -1496,6 +1498,45 @@ __global__ __launch_bounds__(SVM_T1_GT) void k_t1_group(Dev d, int pcur) {
 // of the bit mask plus every later vertex a flip disturbed (a min-heap).  If
 // the heap fills up, every remaining vertex is visited in order (still
 // exact).
+// How a sphere's T1 step ends, from the state the detection and dry runs
+// left: the sequential pass (run), a commit of the dry-run logs (commit,
+// with or without the group log), or nothing.  Read-only, so the large-sphere
+// commit and k_t1_resolve reach the same decision.
+struct T1Ending {
+    int run, commit, group;
+};
+__device__ inline T1Ending t1_ending(const Dev& d, const SysParams* S, int s, int fire_mode, int force_sequential) {
+    const bool frozen = fire_mode && S->f_frozen;
+    const int any = S->t1_flag;
+    const int group_ok = !force_sequential && S->t1_overlap && !S->t1_overflow && d.t1_group[s].ok;
+    T1Ending e;
+    e.run = !frozen && any && (force_sequential || S->t1_overflow || (S->t1_overlap && !group_ok));
+    e.commit = !frozen && any && !e.run;
+    e.group = group_ok;
+    return e;
+}
+
+// Spheres of at least SVM_COMMIT_LARGE vertices commit with a 1024-thread
+// block (their logs and the vertex lists to rebuild grow with the sphere),
+// before k_t1_resolve, which then only closes their step.
+#define SVM_COMMIT_LARGE 65536
+__global__ __launch_bounds__(1024) void k_t1_commit_large(Dev d, int pcur, int nin, int nout, int dir_mode,
+                                                          int fire_mode, int force_sequential) {
+    __shared__ int sh_commit, sh_group;
+    const int s = d.s0 + (int)blockIdx.x;
+    SysParams* S = d.sys + s;
+    if (S->nv < SVM_COMMIT_LARGE) return;
+    if (threadIdx.x == 0) {
+        const T1Ending e = t1_ending(d, S, s, fire_mode, force_sequential);
+        sh_commit = e.commit;
+        sh_group = e.group;
+    }
+    __syncthreads();
+    if (!sh_commit) return;
+    commit_sphere<1024>(d, S, pcur, nin, nout, dir_mode, sh_group);
+    if (threadIdx.x == 0 && sh_group) S->n_t1_group += 1;
+}
+
 __global__ __launch_bounds__(SVM_TILE) void k_t1_resolve(Dev d, int pcur, int nin, int nout, int dir_mode,
                                                          int advance_step, int fire_mode, int force_sequential) {
     __shared__ int heapbuf[SVM_HEAP_CAP];
@@ -1508,12 +1549,11 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_resolve(Dev d, int pcur, int ni
     SysParams* S = d.sys + s;
     const int t = threadIdx.x;
     if (t == 0) {
-        bool frozen = fire_mode && S->f_frozen;
-        int any = S->t1_flag;
-        int group_ok = !force_sequential && S->t1_overlap && !S->t1_overflow && d.t1_group[s].ok;
-        sh_run = !frozen && any && (force_sequential || S->t1_overflow || (S->t1_overlap && !group_ok));
-        sh_commit = !frozen && any && !sh_run;
-        sh_group = group_ok;
+        const bool frozen = fire_mode && S->f_frozen;
+        const T1Ending e = t1_ending(d, S, s, fire_mode, force_sequential);
+        sh_run = e.run;
+        sh_commit = e.commit && S->nv < SVM_COMMIT_LARGE;   // large spheres committed by k_t1_commit_large
+        sh_group = e.group;
         if (!frozen) {
             S->t1_flag = 0;
             if (fire_mode) S->mix_pending = 0;
@@ -1528,7 +1568,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_resolve(Dev d, int pcur, int ni
     __syncthreads();
     if (!sh_run) {
         if (sh_commit) {
-            commit_sphere(d, S, pcur, nin, nout, dir_mode, sh_group);
+            commit_sphere<SVM_TILE>(d, S, pcur, nin, nout, dir_mode, sh_group);
             if (t == 0 && sh_group) S->n_t1_group += 1;
         }
         if (t == 0) {
