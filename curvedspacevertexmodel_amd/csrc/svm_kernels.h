@@ -203,16 +203,45 @@ __global__ __launch_bounds__(SVM_TILE) void k_fan_fix(Dev d, int pin, int skip_f
     const int Cs = d.Cs;
     const int n = d.cnn[c];
     const int32_t* rc = d.ring + c;
-    double cp[3] = {d.cx[c], d.cy[c], d.cz[c]};
-    int il = rc[(size_t)(n - 2) * Cs], ic = rc[(size_t)(n - 1) * Cs];
-    double pl[3] = {X[il], Y[il], Z[il]};
+    // the fan's triangles (centre, ring[k-2], ring[k-1]) in ring order, each
+    // triangle_area(cp, pl, pc, R) term for term; the centre's unit vector,
+    // and the previous triangle's unit pc, its dot with the centre and
+    // sqrt(1 - dot^2), are the same expressions on the same inputs, carried
+    // instead of recomputed
+    const double cp[3] = {d.cx[c], d.cy[c], d.cz[c]};
+    auto unit = [](const double* p, double* u) {
+        double dn = 0.0; dn += p[0] * p[0]; dn += p[1] * p[1]; dn += p[2] * p[2];
+        const double inv = 1.0 / __builtin_sqrt(dn);
+        for (int k = 0; k < 3; ++k) u[k] = inv * p[k];
+    };
+    double ua[3], ub[3], uc[3];
+    unit(cp, ua);
+    const int il = rc[(size_t)(n - 2) * Cs], ic = rc[(size_t)(n - 1) * Cs];
+    {
+        const double pl[3] = {X[il], Y[il], Z[il]};
+        unit(pl, ub);
+    }
+    double d12 = 0.0; d12 += ua[0] * ub[0]; d12 += ua[1] * ub[1]; d12 += ua[2] * ub[2];
+    double s12 = __builtin_sqrt(1 - d12 * d12);
     double pc[3] = {X[ic], Y[ic], Z[ic]};
     double a = 0.0;
     for (int k = 0; k < n; ++k) {
-        int ip = rc[(size_t)k * Cs];
-        double pn[3] = {X[ip], Y[ip], Z[ip]};
-        a += triangle_area(cp, pl, pc, R);
-        pl[0] = pc[0]; pl[1] = pc[1]; pl[2] = pc[2];
+        const int ip = rc[(size_t)k * Cs];
+        const double pn[3] = {X[ip], Y[ip], Z[ip]};
+        unit(pc, uc);
+        double d13 = 0.0; d13 += ua[0] * uc[0]; d13 += ua[1] * uc[1]; d13 += ua[2] * uc[2];
+        double d23 = 0.0; d23 += ub[0] * uc[0]; d23 += ub[1] * uc[1]; d23 += ub[2] * uc[2];
+        const double s13 = __builtin_sqrt(1 - d13 * d13);
+        const double s23 = __builtin_sqrt(1 - d23 * d23);
+        double area = -SVM_PI;
+        area += svm::acos((d23 - d12 * d13) / (s12 * s13));
+        area += svm::acos((d12 - d13 * d23) / (s13 * s23));
+        area += svm::acos((d13 - d12 * d23) / (s12 * s23));
+        area *= (R * R);
+        a += area;
+        ub[0] = uc[0]; ub[1] = uc[1]; ub[2] = uc[2];
+        d12 = d13;
+        s12 = s13;
         pc[0] = pn[0]; pc[1] = pn[1]; pc[2] = pn[2];
     }
     d.A[c] = a;
