@@ -445,7 +445,9 @@ class Context:
         return pe, ke
 
     def counters(self):
-        out = np.zeros((self.n, 12), np.int64)
+        """Per sphere (rows): T1 flips, fan-area fallbacks, skipped NaN force contributions,
+        flips skipped for want of a fourth cell, sequential-pass T1 steps, group-run T1 steps."""
+        out = np.zeros((self.n, 6), np.int64)
         self._ck(lib().svm_get_counters(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "svm_get_counters")
         return out
 

@@ -169,3 +169,30 @@ def test_lanes_on_a_caller_stream():
         np.testing.assert_array_equal(got.positions(i), ref.positions(i))
         np.testing.assert_array_equal(got.directors(i), ref.directors(i))
     got.set_stream(0)
+
+
+def test_lanes_at_the_bench_scale():
+    """The bench's 256-sphere batch (982,401 cells): 3 lanes give the
+    one-lane state of every sphere, bit for bit, after graph-replayed
+    Brownian and active steps with T1 flips, and the same flips, fan
+    fallbacks and skips.  Which T1 path a sphere's step took may differ: the
+    candidate lists hold SVM_T1_MAXCL entries per lane, and in the first
+    steps after a random start one lane's list overflows (those spheres take
+    the sequential pass, which is exact too)."""
+    import bench
+    ps = bench.batch_params(256)
+    states = []
+    for lanes in (1, 3):
+        ctx, _ = bench.build_batch(svm, ps)
+        ctx.set_lanes(lanes)
+        bench.stabilise(ctx, ps, 20)
+        ctx.run_graph(1, 10, 4)
+        ctx.synchronize()
+        states.append(([ctx.positions(i) for i in range(len(ps))], [ctx.directors(i) for i in range(len(ps))],
+                       ctx.counters()))
+    (p1, d1, c1), (p3, d3, c3) = states
+    for i in range(len(ps)):
+        np.testing.assert_array_equal(p1[i], p3[i])
+        np.testing.assert_array_equal(d1[i], d3[i])
+    np.testing.assert_array_equal(c1[:, :4], c3[:, :4])
+    assert c1[:, 0].sum() > 1000     # T1 flips in the run
