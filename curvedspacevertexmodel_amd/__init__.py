@@ -93,6 +93,9 @@ def lib():
 
 
 MAX_LANES = 4   # SVM_MAX_LANES of include/svm.h
+COUNTER_COLS = 6      # SVM_COUNTER_COLS: per-sphere counters
+FORCE_ERR_COLS = 7    # SVM_FORCE_ERR_COLS: per-sphere force-error statistics
+T1_DIAG_COLS = 12     # SVM_T1_DIAG_COLS: per-sphere T1 diagnostics
 
 
 class SVMError(RuntimeError):
@@ -447,7 +450,7 @@ class Context:
     def counters(self):
         """Per sphere (rows): T1 flips, fan-area fallbacks, skipped NaN force contributions,
         flips skipped for want of a fourth cell, sequential-pass T1 steps, group-run T1 steps."""
-        out = np.zeros((self.n, 6), np.int64)
+        out = np.zeros((self.n, COUNTER_COLS), np.int64)
         self._ck(lib().svm_get_counters(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "svm_get_counters")
         return out
 
@@ -583,7 +586,7 @@ class Context:
     def force_error(self):
         """Per sphere: FP32 vs FP64 reference expressions (max|dF|/max|F|, rms|dF|/rms|F|, max|dF|,
         rms|F|), FP32 vs FP64 vector form (max, rms), FP64 vector form vs reference (rms)."""
-        out = np.zeros((self.n, 7), np.float64)
+        out = np.zeros((self.n, FORCE_ERR_COLS), np.float64)
         self._ck(lib().svm_get_force_error(self.h, _dp(out)), "svm_get_force_error")
         return out
 
@@ -595,7 +598,7 @@ class Context:
         return ms.value
 
     def t1_diagnostics(self):
-        out = np.zeros((self.n, 12), np.int64)
+        out = np.zeros((self.n, T1_DIAG_COLS), np.int64)
         self._ck(lib().svm_get_t1_diagnostics(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "diagnostics")
         return out
 

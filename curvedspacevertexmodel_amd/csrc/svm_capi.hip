@@ -1729,12 +1729,13 @@ int svm_get_counters(svm_context* ctx, int64_t* counters) {
     int r = download_sys(ctx, s);
     if (r) return r;
     for (int i = 0; i < ctx->nsys; ++i) {
-        counters[6 * i] = s[i].n_t1;
-        counters[6 * i + 1] = s[i].n_fan;
-        counters[6 * i + 2] = s[i].n_nan;
-        counters[6 * i + 3] = s[i].n_t1_skipped;
-        counters[6 * i + 4] = s[i].n_t1_sequential;
-        counters[6 * i + 5] = s[i].n_t1_group;
+        int64_t* c = counters + SVM_COUNTER_COLS * (size_t)i;
+        c[0] = s[i].n_t1;
+        c[1] = s[i].n_fan;
+        c[2] = s[i].n_nan;
+        c[3] = s[i].n_t1_skipped;
+        c[4] = s[i].n_t1_sequential;
+        c[5] = s[i].n_t1_group;
     }
     return SVM_OK;
 }

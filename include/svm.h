@@ -236,11 +236,12 @@ int svm_enforce_topology(svm_context* ctx);
  * (computeEnergy, sphericalVertexModel.cpp:150-160) and kinetic
  * sum m v^2/2 (simpleModel.cpp:63-73); double[n_systems] each (may be NULL). */
 int svm_get_energies(svm_context* ctx, double* potential, double* kinetic);
-/* Counters since creation, int64[6*n_systems]: T1 flips, fan-area
+/* Counters since creation, int64[SVM_COUNTER_COLS*n_systems]: T1 flips, fan-area
  * fallbacks, skipped NaN force contributions, T1 flips skipped because the
  * edge's two vertices share all three cells (no fourth cell exists), steps
  * whose T1 flips were applied by the sequential pass, and steps whose
  * overlapping candidates were resolved together by one group dry run. */
+#define SVM_COUNTER_COLS 6
 int svm_get_counters(svm_context* ctx, int64_t* counters);
 
 /* ---- updaters ---------------------------------------------------------- */

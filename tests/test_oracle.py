@@ -447,7 +447,10 @@ def test_library_exports_every_declared_symbol():
     assert not missing
     import curvedspacevertexmodel_amd as svm
     assert sorted(svm.EXPORTS) == names
-    assert int(re.search(r"#define SVM_MAX_LANES (\d+)", decl).group(1)) == svm.MAX_LANES
+    # the binding's array widths are the header's (a width mismatch silently mixes spheres' rows)
+    for name, value in (("SVM_MAX_LANES", svm.MAX_LANES), ("SVM_COUNTER_COLS", svm.COUNTER_COLS),
+                        ("SVM_FORCE_ERR_COLS", svm.FORCE_ERR_COLS), ("SVM_T1_DIAG_COLS", svm.T1_DIAG_COLS)):
+        assert int(re.search(r"#define %s (\d+)" % name, decl).group(1)) == value, name
 
 
 def test_library_reports_no_device_without_gpu():
