@@ -91,6 +91,31 @@ SVM_HD int exact_det3_sum(const double* const* X, const double* const* Y, const 
     return expansion_sign(len, buf);
 }
 
+// The exact evaluations take their points by value and are kept out of line:
+// the expansion code indexes arrays of point pointers, which would otherwise
+// force the callers' coordinates out of registers into private memory on
+// every call, the common filtered case included.
+SVM_HD_OUTLINE int orient_origin_exact(double a0, double a1, double a2, double b0, double b1, double b2,
+                                       double c0, double c1, double c2) {
+    const double a[3] = {a0, a1, a2}, b[3] = {b0, b1, b2}, c[3] = {c0, c1, c2};
+    const double* X[1] = {a};
+    const double* Y[1] = {b};
+    const double* Z[1] = {c};
+    double sg[1] = {1.0};
+    return exact_det3_sum(X, Y, Z, sg, 1);
+}
+
+// det[b-a, c-a, d-a] = det3(b,c,d) - det3(a,c,d) + det3(a,b,d) - det3(a,b,c)
+SVM_HD_OUTLINE int orient3d_exact(double a0, double a1, double a2, double b0, double b1, double b2, double c0,
+                                  double c1, double c2, double d0, double d1, double d2) {
+    const double a[3] = {a0, a1, a2}, b[3] = {b0, b1, b2}, c[3] = {c0, c1, c2}, d[3] = {d0, d1, d2};
+    const double* X[4] = {b, a, a, a};
+    const double* Y[4] = {c, c, b, b};
+    const double* Z[4] = {d, d, d, c};
+    double sg[4] = {1.0, -1.0, 1.0, -1.0};
+    return exact_det3_sum(X, Y, Z, sg, 4);
+}
+
 // Sign of det[a, b, c] = a . (b x c): +1 when (a, b, c) is counter-clockwise
 // seen from outside a sphere centred at the origin.
 SVM_HD int orient_origin_sign(const double* a, const double* b, const double* c) {
@@ -104,11 +129,7 @@ SVM_HD int orient_origin_sign(const double* a, const double* b, const double* c)
     double bound = 1e-14 * perm;
     if (det > bound) return 1;
     if (det < -bound) return -1;
-    const double* X[1] = {a};
-    const double* Y[1] = {b};
-    const double* Z[1] = {c};
-    double sg[1] = {1.0};
-    return exact_det3_sum(X, Y, Z, sg, 1);
+    return orient_origin_exact(a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2]);
 }
 
 // Sign of det[b-a, c-a, d-a].  For (a, b, c) counter-clockwise seen from
@@ -126,12 +147,7 @@ SVM_HD int orient3d_sign(const double* a, const double* b, const double* c, cons
     double bound = 1e-14 * perm;
     if (det > bound) return 1;
     if (det < -bound) return -1;
-    // det[b-a, c-a, d-a] = det3(b,c,d) - det3(a,c,d) + det3(a,b,d) - det3(a,b,c)
-    const double* X[4] = {b, a, a, a};
-    const double* Y[4] = {c, c, b, b};
-    const double* Z[4] = {d, d, d, c};
-    double sg[4] = {1.0, -1.0, 1.0, -1.0};
-    return exact_det3_sum(X, Y, Z, sg, 4);
+    return orient3d_exact(a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2], d[0], d[1], d[2]);
 }
 
 }  // namespace svm

@@ -26,8 +26,10 @@
 
 #if defined(__HIPCC__)
 #define SVM_HD __host__ __device__ inline __attribute__((always_inline))
+#define SVM_HD_OUTLINE __host__ __device__ inline __attribute__((noinline))
 #else
 #define SVM_HD static inline __attribute__((always_inline))
+#define SVM_HD_OUTLINE static inline __attribute__((noinline))
 #endif
 
 namespace svm {

@@ -131,12 +131,21 @@ __device__ inline bool dl_beats(const double* pi, const double* pj, const double
 
 // Star of every point i with status[i] == DL_PENDING or DL_RETRY.  Writes the
 // CCW ring of Delaunay neighbours (nbr[i*cap ..], deg[i]) and the status.
+//
+// Latency is what bounds a star (a handful of FP64 predicates between
+// dependent memory round trips), so the round trips are kept few: the bins
+// read are those that meet the box [p - r, p + r], their entries are dealt
+// evenly over the lanes (a wave scan of the bin sizes, compaction by ballot,
+// so the candidates come in bin order), each lane keeps its first candidate
+// in registers for the tournament and the checks, and lane k keeps ring
+// point k.
 __global__ __launch_bounds__(64 * SVM_DL_WAVES) void k_dl_star(const double* P, int n, DlGrid g, const int* start,
                                                                const int* list, double r, double R, int cap,
                                                                int* nbr, int* deg, int* status) {
     __shared__ int s_idx[SVM_DL_WAVES][SVM_DL_MAXC];
     __shared__ int s_ring[SVM_DL_WAVES][64];
-    __shared__ int s_n[SVM_DL_WAVES];
+    __shared__ int s_bstart[SVM_DL_WAVES][64];   // first entry of each bin of a chunk
+    __shared__ int s_bpre[SVM_DL_WAVES][64];     // entries of the chunk's earlier bins
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = blockIdx.x * SVM_DL_WAVES + w;
     if (i >= n) return;
@@ -144,43 +153,94 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES) void k_dl_star(const double* P, 
     if (st0 != DL_PENDING && st0 != DL_RETRY) return;
     double pi[3];
     dl_load(P, i, pi);
-    // ---- gather the candidates within chord distance r
-    if (lane == 0) s_n[w] = 0;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const int wr = (int)__builtin_ceil(r / g.h);
-    const int gx = dl_axis(pi[0], g), gy = dl_axis(pi[1], g), gz = dl_axis(pi[2], g);
-    const int span = 2 * wr + 1, nb = span * span * span;
+    // ---- gather the candidates within chord distance r.  Binning is monotone
+    // per axis and so is rounding, so every point within r of p lies in a bin
+    // between those of p - r and p + r on each axis; the 1e-9 pad covers the
+    // rounding of the distance test itself.
+    const double rb = r * (1.0 + 1e-9);
+    const int x0 = dl_axis(pi[0] - rb, g), y0 = dl_axis(pi[1] - rb, g), z0 = dl_axis(pi[2] - rb, g);
+    const int nx = dl_axis(pi[0] + rb, g) - x0 + 1, ny = dl_axis(pi[1] + rb, g) - y0 + 1,
+              nz = dl_axis(pi[2] + rb, g) - z0 + 1;
+    const int nbin = nx * ny * nz;
     const double r2 = r * r;
-    for (int b = lane; b < nb; b += 64) {
-        int x = gx + b % span - wr, y = gy + (b / span) % span - wr, z = gz + b / (span * span) - wr;
-        if (x < 0 || y < 0 || z < 0 || x >= g.G || y >= g.G || z >= g.G) continue;
-        int bin = (x * g.G + y) * g.G + z;
-        for (int e = start[bin]; e < start[bin + 1]; ++e) {
-            int q = list[e];
-            if (q == i) continue;
-            double dx = P[3 * (size_t)q] - pi[0], dy = P[3 * (size_t)q + 1] - pi[1], dz = P[3 * (size_t)q + 2] - pi[2];
-            if (dx * dx + dy * dy + dz * dz < r2) {
-                int k = atomicAdd(&s_n[w], 1);
+    int* bs = s_bstart[w];
+    int* bp = s_bpre[w];
+    int nc = 0;   // wave-uniform: candidates found (also those beyond the capacity)
+    for (int c0 = 0; c0 < nbin; c0 += 64) {
+        const int b = c0 + lane;
+        int s0 = 0, cnt = 0;
+        if (b < nbin) {
+            const int x = x0 + b % nx, y = y0 + (b / nx) % ny, z = z0 + b / (nx * ny);
+            const int bin = (x * g.G + y) * g.G + z;
+            s0 = start[bin];
+            cnt = start[bin + 1] - s0;
+        }
+        int inc = cnt;   // inclusive scan of the bin sizes over the lanes
+        for (int m = 1; m < 64; m <<= 1) {
+            const int o = __shfl_up(inc, m, 64);
+            if (lane >= m) inc += o;
+        }
+        const int total = __shfl(inc, 63, 64);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        bs[lane] = s0;
+        bp[lane] = inc - cnt;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (int t0 = 0; t0 < total; t0 += 64) {
+            const int t = t0 + lane;
+            bool hit = false;
+            int q = -1;
+            if (t < total) {
+                // the last bin whose prefix is <= t holds entry t (it has entries:
+                // the next bin's prefix exceeds t)
+                int lo = 0, hi = 63;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (bp[mid] <= t) lo = mid;
+                    else hi = mid - 1;
+                }
+                q = list[bs[lo] + (t - bp[lo])];
+                if (q != i) {
+                    const double dx = P[3 * (size_t)q] - pi[0], dy = P[3 * (size_t)q + 1] - pi[1],
+                                 dz = P[3 * (size_t)q + 2] - pi[2];
+                    hit = dx * dx + dy * dy + dz * dz < r2;
+                }
+            }
+            const unsigned long long hits = __ballot(hit);
+            if (hit) {
+                const int k = nc + __popcll(hits & ((1ull << lane) - 1ull));
                 if (k < SVM_DL_MAXC) s_idx[w][k] = q;
             }
+            nc += __popcll(hits);
         }
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const int nc = s_n[w];
     if (nc > SVM_DL_MAXC || nc < 3) {
         if (lane == 0) status[i] = nc < 3 ? (r < 2.0 * R ? DL_RETRY : DL_INVALID) : DL_CROWDED;
         return;
     }
     const int* cand = s_idx[w];
+    // candidate c's point: lane c holds it for c < 64, later ones are loaded
+    double pm[3] = {0.0, 0.0, 0.0};
+    if (lane < nc) dl_load(P, cand[lane], pm);
+    auto cand_point = [&](int c, double* pq) {
+        if (c < 64) {
+            pq[0] = pm[0]; pq[1] = pm[1]; pq[2] = pm[2];
+        } else {
+            dl_load(P, cand[c], pq);
+        }
+    };
     // ---- nearest candidate (squared distance, then index)
     double bd = 1e300;
     int bj = 0x7fffffff;
     for (int c = lane; c < nc; c += 64) {
-        int q = cand[c];
-        double dx = P[3 * (size_t)q] - pi[0], dy = P[3 * (size_t)q + 1] - pi[1], dz = P[3 * (size_t)q + 2] - pi[2];
-        double d2 = dx * dx + dy * dy + dz * dz;
+        const int q = cand[c];
+        double pq[3];
+        cand_point(c, pq);
+        const double dx = pq[0] - pi[0], dy = pq[1] - pi[1], dz = pq[2] - pi[2];
+        const double d2 = dx * dx + dy * dy + dz * dz;
         if (d2 < bd || (d2 == bd && q < bj)) { bd = d2; bj = q; }
     }
     for (int m = 32; m >= 1; m >>= 1) {
@@ -196,9 +256,11 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES) void k_dl_star(const double* P, 
     int* ring = s_ring[w];   // cap <= 64
     double pc[3];
     dl_load(P, cur, pc);
+    double rp[3] = {0.0, 0.0, 0.0};   // lane k: ring point k
     while (true) {
         if (d >= cap) { status_out = DL_OPEN; break; }
         if (lane == 0) ring[d] = cur;
+        if (lane == d) { rp[0] = pc[0]; rp[1] = pc[1]; rp[2] = pc[2]; }
         ++d;
         // tournament: each lane its candidates, then across lanes
         int bk = -1;
@@ -207,7 +269,7 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES) void k_dl_star(const double* P, 
             int q = cand[c];
             if (q == cur) continue;
             double pq[3];
-            dl_load(P, q, pq);
+            cand_point(c, pq);
             if (bk < 0 || dl_beats(pi, pc, pk, bk, pq, q, degenerate)) {
                 bk = q;
                 pk[0] = pq[0]; pk[1] = pq[1]; pk[2] = pq[2];
@@ -242,10 +304,13 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES) void k_dl_star(const double* P, 
     bool bad = false, far = false;
     if (status_out == DL_OK) {
         for (int s = 0; s < d; ++s) {
-            int a = ring[s], b = ring[(s + 1) % d];
+            const int s1 = (s + 1) % d;
+            int a = ring[s], b = ring[s1];
             double pa[3], pb[3];
-            dl_load(P, a, pa);
-            dl_load(P, b, pb);
+            for (int k = 0; k < 3; ++k) {
+                pa[k] = __shfl(rp[k], s, 64);
+                pb[k] = __shfl(rp[k], s1, 64);
+            }
             if (lane == 0) {
                 if (orient_origin_sign(pi, pa, pb) <= 0) bad = true;
                 // circumradius of (pi, pa, pb); the cap beyond the facet lies within 2.25 rho of pi
@@ -263,7 +328,7 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES) void k_dl_star(const double* P, 
                 int q = cand[c];
                 if (q == a || q == b) continue;
                 double pq[3];
-                dl_load(P, q, pq);
+                cand_point(c, pq);
                 int o = orient3d_sign(pi, pa, pb, pq);
                 if (o > 0) bad = true;
                 if (o == 0) degenerate = true;
