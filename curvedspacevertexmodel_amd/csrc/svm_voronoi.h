@@ -17,8 +17,11 @@
 //  * star of point i, one wavefront per point: the candidates within chord
 //    distance r; the nearest is a Delaunay neighbour; then gift wrapping
 //    around i: the next neighbour after j is the candidate k with no other
-//    candidate beyond the plane (i, j, k), found by a tournament across the
-//    lanes whose comparison is the exact orient3d predicate (svm_exact.h).
+//    candidate beyond the plane (i, j, k).  It is found by a floating-point
+//    angle key about the axis (i, j) and confirmed with the exact orient3d
+//    predicate (svm_exact.h) against every candidate; when the confirmation
+//    fails (or meets an exact zero), by a tournament across the lanes whose
+//    comparison is the exact predicate.
 //    Every facet is then checked against every candidate (exactly), and its
 //    circumcircle must lie well inside the search ball, so no point outside
 //    the candidates can lie in it; otherwise the point is redone with a
@@ -138,8 +141,9 @@ __device__ inline bool dl_beats(const double* pi, const double* pj, const double
 // evenly over the lanes (a wave scan of the bin sizes, compaction by ballot,
 // so the candidates come in bin order), each lane keeps its first candidate
 // in registers for the tournament and the checks, and lane k keeps ring
-// point k.
-__global__ __launch_bounds__(64 * SVM_DL_WAVES) void k_dl_star(const double* P, int n, DlGrid g, const int* start,
+// point k.  At least 4 waves per SIMD (128 VGPRs): measured faster than the
+// compiler's own choice of 132 registers and 3 waves.
+__global__ __launch_bounds__(64 * SVM_DL_WAVES, 4) void k_dl_star(const double* P, int n, DlGrid g, const int* start,
                                                                const int* list, double r, double R, int cap,
                                                                int* nbr, int* deg, int* status) {
     __shared__ int s_idx[SVM_DL_WAVES][SVM_DL_MAXC];
@@ -257,14 +261,83 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES) void k_dl_star(const double* P, 
     double pc[3];
     dl_load(P, cur, pc);
     double rp[3] = {0.0, 0.0, 0.0};   // lane k: ring point k
+    unsigned long long confirmed = 0;   // facets whose candidate checks were done during the wrap
     while (true) {
         if (d >= cap) { status_out = DL_OPEN; break; }
         if (lane == 0) ring[d] = cur;
         if (lane == d) { rp[0] = pc[0]; rp[1] = pc[1]; rp[2] = pc[2]; }
         ++d;
-        // tournament: each lane its candidates, then across lanes
+        // Fast path: the next neighbour is the candidate of largest angle
+        // about the axis u = pc - pi (angle measured from e1, the direction to
+        // the sphere's centre normal to u, towards e2 = u x e1), found with a
+        // floating-point key and then confirmed exactly: no other candidate
+        // on or beyond the plane (pi, pc, pk).  A confirmed winner beats every
+        // candidate, so the exact tournament below would return it too; and
+        // the confirmation is that facet's candidate check.
         int bk = -1;
         double pk[3] = {0.0, 0.0, 0.0};
+        {
+            const double ux = pc[0] - pi[0], uy = pc[1] - pi[1], uz = pc[2] - pi[2];
+            const double mx = -(pi[0] + pc[0]), my = -(pi[1] + pc[1]), mz = -(pi[2] + pc[2]);
+            const double t = (mx * ux + my * uy + mz * uz) / (ux * ux + uy * uy + uz * uz);
+            const double e1x = mx - t * ux, e1y = my - t * uy, e1z = mz - t * uz;
+            const double e2x = uy * e1z - uz * e1y, e2y = uz * e1x - ux * e1z, e2z = ux * e1y - uy * e1x;
+            double key = -1e300;
+            int kq = 0x7fffffff;
+            for (int c = lane; c < nc; c += 64) {
+                const int q = cand[c];
+                if (q == cur) continue;
+                double pq[3];
+                cand_point(c, pq);
+                const double dx = pq[0] - pi[0], dy = pq[1] - pi[1], dz = pq[2] - pi[2];
+                const double a = dx * e1x + dy * e1y + dz * e1z, b = dx * e2x + dy * e2y + dz * e2z;
+                const double sab = __builtin_fabs(a) + __builtin_fabs(b);
+                if (!(sab > 0.0)) continue;   // no direction (or not finite): left to the exact path
+                const double ps = b / sab;    // pseudo-angle, increasing with atan2(b, a)
+                const double kx = a >= 0.0 ? ps : (b >= 0.0 ? 2.0 - ps : -2.0 - ps);
+                if (kx > key || (kx == key && q < kq)) {
+                    key = kx;
+                    kq = q;
+                    pk[0] = pq[0]; pk[1] = pq[1]; pk[2] = pq[2];
+                }
+            }
+            for (int m = 1; m < 64; m <<= 1) {
+                const double ok = __shfl_xor(key, m, 64);
+                const int oq = __shfl_xor(kq, m, 64);
+                const double o0 = __shfl_xor(pk[0], m, 64), o1 = __shfl_xor(pk[1], m, 64),
+                             o2 = __shfl_xor(pk[2], m, 64);
+                if (ok > key || (ok == key && oq < kq)) {
+                    key = ok;
+                    kq = oq;
+                    pk[0] = o0; pk[1] = o1; pk[2] = o2;
+                }
+            }
+            if (kq != 0x7fffffff) {
+                bool fail = false;
+                for (int c = lane; c < nc; c += 64) {
+                    const int q = cand[c];
+                    if (q == cur || q == kq) continue;
+                    double pq[3];
+                    cand_point(c, pq);
+                    if (orient3d_sign(pi, pc, pk, pq) >= 0) fail = true;
+                }
+                if (__ballot(fail) == 0ull) {
+                    bk = kq;
+                    confirmed |= 1ull << (d - 1);
+                }
+            }
+        }
+        if (bk >= 0) {
+            if (bk == j0) {
+                if (d < 3) status_out = DL_INVALID;
+                break;
+            }
+            cur = bk;
+            pc[0] = pk[0]; pc[1] = pk[1]; pc[2] = pk[2];
+            continue;
+        }
+        // exact tournament: each lane its candidates, then across lanes
+        pk[0] = 0.0; pk[1] = 0.0; pk[2] = 0.0;
         for (int c = lane; c < nc; c += 64) {
             int q = cand[c];
             if (q == cur) continue;
@@ -324,6 +397,7 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES) void k_dl_star(const double* P, 
                              __builtin_sqrt(wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2]) / (2.0 * area2);
                 if (!(2.25 * rho * (1.0 + 1e-6) < r) && r < 2.0 * R) far = true;
             }
+            if ((confirmed >> s) & 1ull) continue;
             for (int c = lane; c < nc; c += 64) {
                 int q = cand[c];
                 if (q == a || q == b) continue;
