@@ -261,23 +261,27 @@ def single_sphere(svm, n=262144, steps=400):
         rate[name] = n * (steps // 10 * 10) / (time.perf_counter() - t0)
     ctx.set_force_precision(0)
     del ctx
-    # the same cell points as a Voronoi model: every step moves the generators
-    # (forces through the circumcentres, T 0.001) and retriangulates on the GPU
+    return {"workload": "single-262144", "cells": n, "updater": "brownian", "dt": 0.01, "T": 0.001, "p0": 3.8,
+            "steps": steps // 10 * 10, "cell_steps_per_s": rate, "topology_build_s": t_build,
+            "retriangulation_ms": 1e3 * t_dl, "voronoi_model": voronoi_model(svm, pts)}
+
+
+def voronoi_model(svm, pts, vsteps=20):
+    """The cell points as a Voronoi model: every step moves the generators
+    (FP64 forces through the circumcentres, Brownian, T 0.001) and
+    retriangulates them on the GPU."""
+    n = len(pts)
     vctx = svm.Context([dict(n_cells=n, a0=1.0, p0=3.8, kr=1.0, t1_threshold=0.1, seed=1137)])
     vctx.build_voronoi_from_cell_points(0, pts)
     vctx.set_brownian(0, dt=0.01, mu=1.0, T=0.001)
     vctx.voronoi_brownian_steps(2)
     vctx.synchronize()
-    vsteps = 20
     t0 = time.perf_counter()
     vctx.voronoi_brownian_steps(vsteps)
     vctx.synchronize()
     t_vor = (time.perf_counter() - t0) / vsteps
-    return {"workload": "single-262144", "cells": n, "updater": "brownian", "dt": 0.01, "T": 0.001, "p0": 3.8,
-            "steps": steps // 10 * 10, "cell_steps_per_s": rate, "topology_build_s": t_build,
-            "retriangulation_ms": 1e3 * t_dl,
-            "voronoi_model": {"steps": vsteps, "ms_per_step": 1e3 * t_vor, "cell_steps_per_s": n / t_vor,
-                              "note": "FP64 forces on the generators, Brownian move, GPU retriangulation every step"}}
+    return {"steps": vsteps, "ms_per_step": 1e3 * t_vor, "cell_steps_per_s": n / t_vor,
+            "note": "FP64 forces on the generators, Brownian move, GPU retriangulation every step"}
 
 
 def large_batch(svm, steps=400):
@@ -377,6 +381,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fire", action="store_true", help="skip the learned-vs-plain FIRE comparison")
     ap.add_argument("--no-single", action="store_true", help="skip the single 262,144-cell sphere block")
+    ap.add_argument("--voronoi", action="store_true",
+                    help="only the Voronoi model of 262,144 cells (no graphs): one JSON line, then exit")
     ap.add_argument("--large", action="store_true",
                     help="also run the 32-sphere, 50k-500k-cell FP32 batch (about 6 M cells)")
     ap.add_argument("--autotune", type=int, default=1, help="1: pick the launch schedule with svm_autotune")
@@ -392,6 +398,12 @@ def main():
         sys.exit(2)
     import curvedspacevertexmodel_amd as svm
 
+    if args.voronoi:
+        n = 262144
+        pts = svm.generate_cell_points(n, 1137)
+        pts = pts[svm.spatial_order(pts)]
+        print(json.dumps({"workload": "voronoi-262144", "cells": n, **voronoi_model(svm, pts)}), flush=True)
+        return
     params = batch_params(args.spheres)
     total_cells = int(sum(p[0] for p in params))
     t_setup = time.perf_counter()
