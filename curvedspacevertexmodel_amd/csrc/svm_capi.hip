@@ -681,7 +681,8 @@ int dl_triangulate_dev(hipStream_t st, DevScratch& m, double* P, int n, double R
     o.off = m.get<int>(n + 1);
     o.tid = m.get<int>((size_t)n * cap);
     o.err = m.get<int>(1);
-    if (!o.nbr || !o.deg || !status || !cnt || !o.off || !o.tid || !o.err) {
+    int* outcome = m.get<int>(2);
+    if (!o.nbr || !o.deg || !status || !cnt || !o.off || !o.tid || !o.err || !outcome) {
         msg = "device allocation failed";
         return SVM_ERR_HIP;
     }
@@ -694,7 +695,6 @@ int dl_triangulate_dev(hipStream_t st, DevScratch& m, double* P, int n, double R
     }
     hipLaunchKernelGGL(svm::k_fill_int, blocks(n), dim3(tb), 0, st, status, n, (int)svm::DL_PENDING);
     // stars, widening the search for the points that need it
-    std::vector<int> hs(n);
     // search radii r0, 2 r0, 3 r0 (about 40, 150 and 350 candidates at uniform
     // density), then the whole sphere when it holds few enough points
     for (int pass = 0; pass < 4; ++pass) {
@@ -703,16 +703,27 @@ int dl_triangulate_dev(hipStream_t st, DevScratch& m, double* P, int n, double R
         hipLaunchKernelGGL(svm::k_dl_star, dim3((n + SVM_DL_WAVES - 1) / SVM_DL_WAVES), dim3(64 * SVM_DL_WAVES), 0, st,
                            (const double*)o.P, n, g, (const int*)start, (const int*)list, r, Rs, cap, o.nbr, o.deg,
                            status);
+        // the pass's outcome, reduced on the device: {stars to redo, first failed point}
+        const int init[2] = {0, n};
+        int ho[2] = {0, n};
         if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(hs.data(), status, sizeof(int) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(outcome, init, sizeof(init), hipMemcpyHostToDevice, st) != hipSuccess) {
+            msg = "star kernel failed";
+            return SVM_ERR_HIP;
+        }
+        hipLaunchKernelGGL(svm::k_dl_status, blocks(n), dim3(tb), 0, st, (const int*)status, n, outcome);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(ho, outcome, sizeof(ho), hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess) {
             msg = "star kernel failed";
             return SVM_ERR_HIP;
         }
-        int retry = 0, worst = 0, where = -1;
-        for (int i = 0; i < n; ++i) {
-            if (hs[i] == svm::DL_RETRY) ++retry;
-            else if (hs[i] != svm::DL_OK && where < 0) { worst = hs[i]; where = i; }
+        const int retry = ho[0], where = ho[1] < n ? ho[1] : -1;
+        int worst = 0;
+        if (where >= 0 && (hipMemcpyAsync(&worst, status + where, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                           hipStreamSynchronize(st) != hipSuccess)) {
+            msg = "device copy failed";
+            return SVM_ERR_HIP;
         }
         if (worst) {
             msg = worst == svm::DL_DEGENERATE ? "degenerate input: four co-circular points"

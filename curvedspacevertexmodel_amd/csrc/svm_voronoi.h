@@ -376,20 +376,17 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES, 4) void k_dl_star(const double* 
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     bool bad = false, far = false;
     if (status_out == DL_OK) {
-        for (int s = 0; s < d; ++s) {
-            const int s1 = (s + 1) % d;
-            int a = ring[s], b = ring[s1];
-            double pa[3], pb[3];
-            for (int k = 0; k < 3; ++k) {
-                pa[k] = __shfl(rp[k], s, 64);
-                pb[k] = __shfl(rp[k], s1, 64);
-            }
-            if (lane == 0) {
-                if (orient_origin_sign(pi, pa, pb) <= 0) bad = true;
+        // lane s: facet s's orientation and circumcircle size (d >= 3 here)
+        {
+            const int s1 = (lane + 1) % d;
+            double pb[3];
+            for (int k = 0; k < 3; ++k) pb[k] = __shfl(rp[k], s1, 64);
+            if (lane < d) {
+                if (orient_origin_sign(pi, rp, pb) <= 0) bad = true;
                 // circumradius of (pi, pa, pb); the cap beyond the facet lies within 2.25 rho of pi
-                double u[3] = {pa[0] - pi[0], pa[1] - pi[1], pa[2] - pi[2]};
+                double u[3] = {rp[0] - pi[0], rp[1] - pi[1], rp[2] - pi[2]};
                 double v[3] = {pb[0] - pi[0], pb[1] - pi[1], pb[2] - pi[2]};
-                double wv[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+                double wv[3] = {pb[0] - rp[0], pb[1] - rp[1], pb[2] - rp[2]};
                 double cx = u[1] * v[2] - u[2] * v[1], cy = u[2] * v[0] - u[0] * v[2], cz = u[0] * v[1] - u[1] * v[0];
                 double area2 = __builtin_sqrt(cx * cx + cy * cy + cz * cz);
                 double rho = __builtin_sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) *
@@ -397,7 +394,17 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES, 4) void k_dl_star(const double* 
                              __builtin_sqrt(wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2]) / (2.0 * area2);
                 if (!(2.25 * rho * (1.0 + 1e-6) < r) && r < 2.0 * R) far = true;
             }
+        }
+        // every candidate against the facets the wrap did not confirm
+        for (int s = 0; s < d; ++s) {
             if ((confirmed >> s) & 1ull) continue;
+            const int s1 = (s + 1) % d;
+            int a = ring[s], b = ring[s1];
+            double pa[3], pb[3];
+            for (int k = 0; k < 3; ++k) {
+                pa[k] = __shfl(rp[k], s, 64);
+                pb[k] = __shfl(rp[k], s1, 64);
+            }
             for (int c = lane; c < nc; c += 64) {
                 int q = cand[c];
                 if (q == a || q == b) continue;
@@ -428,6 +435,16 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES, 4) void k_dl_star(const double* 
         deg[i] = d;
     }
     for (int s = lane; s < d; s += 64) nbr[(size_t)i * cap + s] = ring[s];
+}
+
+// Outcome of a star pass for the host: out[0] = stars to redo wider,
+// out[1] = smallest point whose star failed (n if none).  out = {0, n} on entry.
+__global__ __launch_bounds__(256) void k_dl_status(const int* status, int n, int* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int st = i < n ? status[i] : DL_OK;
+    const unsigned long long retry = __ballot(st == DL_RETRY);
+    if ((threadIdx.x & 63) == 0 && retry) atomicAdd(out, (int)__popcll(retry));
+    if (st != DL_OK && st != DL_RETRY) atomicMin(out + 1, i);
 }
 
 __global__ __launch_bounds__(256) void k_fill_int(int* a, int n, int value) {
