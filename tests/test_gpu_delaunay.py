@@ -111,3 +111,33 @@ def test_many_random_spheres_against_qhull():
         if i % 2:
             pts = pts[svm.spatial_order(pts)]
         np.testing.assert_array_equal(svm.delaunay_triangles(pts), svmtest.qhull_triangles(pts), err_msg=f"sphere {i}")
+
+
+def test_cocircular_points_are_rejected():
+    """The eight vertices of a cube: each face's four vertices lie on one empty
+    circle, so the Delaunay triangulation is not unique and the library reports
+    degenerate input (SVM_ERR_TOPOLOGY) rather than pick a diagonal."""
+    s = 2.0 / np.sqrt(3.0)
+    cube = np.array([[x, y, z] for x in (-s, s) for y in (-s, s) for z in (-s, s)])
+    with pytest.raises(svm.SVMError) as e:
+        svm.delaunay_triangles(cube)
+    assert e.value.code == 5
+
+
+@pytest.mark.parametrize("seed,eps", [(1, 1e-13), (2, 3e-15), (3, 3e-15)])
+def test_nearly_cocircular_points_are_triangulated_exactly(seed, eps):
+    """A latitude-longitude grid (every grid quadrilateral is an isosceles
+    trapezoid, so its corners are co-circular) moved off degeneracy by relative
+    perturbations of 1e-13 or 3e-15 (tens of ulps): the floating-point filters leave many
+    orientation tests to the exact arithmetic, and the angle keys of the wrap
+    are nearly tied.  The triangulation must be closed and pass the
+    empty-circumcircle test on every edge in exact rational arithmetic."""
+    rng = np.random.default_rng(seed)
+    lat = np.linspace(-80.0, 80.0, 17) * np.pi / 180.0
+    lon = np.arange(36) * 2.0 * np.pi / 36.0
+    pts = [[np.cos(a) * np.cos(b), np.cos(a) * np.sin(b), np.sin(a)] for a in lat for b in lon]
+    pts = np.array(pts + [[0.0, 0.0, 1.0], [0.0, 0.0, -1.0]]) * 7.0
+    pts *= 1.0 + eps * rng.standard_normal(pts.shape)
+    tris = svm.delaunay_triangles(pts)
+    assert tris.shape == (2 * len(pts) - 4, 3)
+    assert svmtest.locally_delaunay_violations(pts, tris, exact=True) == []
