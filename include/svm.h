@@ -116,8 +116,10 @@ int svm_build_from_cell_points(svm_context* ctx, int32_t sys, const double* cell
  * (src/model/sphericalVoronoi.cpp:8-12, src/model/convexHullCGAL.cpp:15-65)
  * and its neighbour search (hyperrectangularCellList,
  * src/utilities/hyperrectangularCellList.cpp:14-51, 165-342).  Exact
- * predicates; four co-circular points are rejected (SVM_ERR_TOPOLOGY).
- * triangles: int32[3*(2N-4)], counter-clockwise seen from outside, smallest
+ * predicates.  Input whose triangulation is not unique (four co-circular
+ * points on an empty circle) is rejected (SVM_ERR_TOPOLOGY); so may be input
+ * with other exactly co-circular quadruples near a point, when the exact
+ * tournament compares them.  triangles: int32[3*(2N-4)], counter-clockwise seen from outside, smallest
  * index first, sorted lexicographically. */
 int svm_delaunay_triangles(int32_t n_points, const double* points, int32_t* triangles);
 /* Cell list of points on the GPU (the binning of hyperrectangularCellList,
