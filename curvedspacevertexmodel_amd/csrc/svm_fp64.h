@@ -21,6 +21,17 @@
 //               easy as 1, 2, 3", SC'11.
 // Accuracy against glibc is checked by the oracle tests (<= 1 ulp for acos,
 // sin, cos, log on the ranges used).
+//
+// The acos, sin, cos and log code below follows fdlibm's e_acos.c, k_sin.c,
+// k_cos.c and e_log.c (their coefficients and evaluation steps), which carry
+// this notice:
+//
+//   Copyright (C) 1993 by Sun Microsystems, Inc. All rights reserved.
+//
+//   Developed at SunSoft, a Sun Microsystems, Inc. business.
+//   Permission to use, copy, modify, and distribute this
+//   software is freely granted, provided that this notice
+//   is preserved.
 #pragma once
 #include <stdint.h>
 

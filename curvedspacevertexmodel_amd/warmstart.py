@@ -93,15 +93,31 @@ def training_set(n_sys=48, seed0=TRAIN_SEED0, cutoff=1e-8, max_iterations=20000)
     return np.concatenate(X), np.concatenate(T)
 
 
+# Offsets of the weight blocks in the parameter vector (the layout documented
+# with SVM_MLP_NPARAM in include/svm.h): W1 [64][16], b1 [64], W2 [64][64], b2,
+# W3 [64][64], b3, W4 [16][64], b4 [16].
+_W1, _W2, _W3, _W4 = 0, 1088, 5248, 9408
+
+
+def init_parameters(seed):
+    """He-style initial weights (normal, variance 2/fan_in; 1/fan_in for the
+    linear output layer), biases zero; the padding inputs 12..15 of W1 and the
+    padding outputs 2..15 of W4 stay zero."""
+    rng = np.random.default_rng(seed)
+    th = np.zeros(svm.MLP_NPARAM, np.float32)
+    th[_W1:_W1 + 1024].reshape(64, 16)[:, :12] = rng.normal(0, np.sqrt(2.0 / 12), (64, 12))
+    th[_W2:_W2 + 4096] = rng.normal(0, np.sqrt(2.0 / 64), 4096)
+    th[_W3:_W3 + 4096] = rng.normal(0, np.sqrt(2.0 / 64), 4096)
+    th[_W4:_W4 + 1024].reshape(16, 64)[:2] = rng.normal(0, np.sqrt(1.0 / 64), (2, 64))
+    return th
+
+
 def train(X, T, epochs=40, batch=4096, lr=1e-3, seed=1):
-    import sys
-    sys.path.insert(0, os.path.join(HERE, "..", "oracle"))
-    from mlp_oracle import init_theta   # initialiser only
     mu = X[:, :12].mean(0)
     sg = X[:, :12].std(0) + 1e-6
     scale = float(T.std()) + 1e-12
     net = svm.MLP()
-    net.set_parameters(init_theta(seed), mu, sg, scale)
+    net.set_parameters(init_parameters(seed), mu, sg, scale)
     losses = net.train(X, T, epochs=epochs, batch=batch, lr=lr, seed=seed)
     return net, losses
 
