@@ -1062,5 +1062,10 @@ void or_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
     svm::u32x4 r = svm::philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
     for (int i = 0; i < 4; ++i) out[i] = r.v[i];
 }
+// the draws the updaters consume, for the tests that pin the updaters
+void or_uniform2(uint64_t seed, uint32_t stream, uint64_t step, uint32_t idx, double* out) {
+    svm::uniform2(seed, stream, step, idx, &out[0], &out[1]);
+}
+void or_normal3(uint64_t seed, uint64_t step, uint32_t v, double* out) { svm::normal3(seed, step, v, out); }
 
 }  // extern "C"
