@@ -44,7 +44,10 @@ EXPORTS = [
     "svm_build_voronoi_from_cell_points", "svm_voronoi_forces", "svm_get_voronoi_forces", "svm_voronoi_brownian_steps", "svm_get_generators",
     "svm_set_positions_device", "svm_get_positions_device", "svm_set_directors_device", "svm_get_directors_device",
     "svm_set_velocities_device", "svm_get_velocities_device", "svm_get_forces_device",
+    "svm_delaunay_triangles_noalloc",
 ]
+
+NOALLOC_MAX_POINTS = 65536
 
 
 class SystemDesc(C.Structure):
@@ -147,6 +150,19 @@ def delaunay_triangles(points):
     rc = lib().svm_delaunay_triangles(C.c_int32(len(pts)), _dp(pts), _ip(out))
     if rc != 0:
         raise SVMError(rc, "svm_delaunay_triangles")
+    return out
+
+
+def delaunay_triangles_noalloc(points, stream=None):
+    """delaunay_triangles through svm_delaunay_triangles_noalloc: no device
+    allocation (the library's static work array), on `stream` (a hipStream_t
+    handle as an int, e.g. torch.cuda.current_stream().cuda_stream; None = the
+    null stream).  At most NOALLOC_MAX_POINTS points."""
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    out = np.empty((max(2 * len(pts) - 4, 0), 3), np.int32)
+    rc = lib().svm_delaunay_triangles_noalloc(C.c_int32(len(pts)), _dp(pts), _ip(out), C.c_void_p(stream))
+    if rc != 0:
+        raise SVMError(rc, "svm_delaunay_triangles_noalloc")
     return out
 
 

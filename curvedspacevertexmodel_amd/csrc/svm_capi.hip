@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -571,10 +572,22 @@ namespace {
 // and later gets reuse the buffers of the same position when they are large
 // enough, so a loop that repeats the same sequence of gets allocates only on
 // its first pass (or when a size grows).
+// Device work buffers of one call.  By default each get() is a hipMalloc,
+// kept (and reused by a rewound cursor) until the object ends.  In arena mode
+// (constructed over caller-provided device memory) get() hands out 256-byte
+// aligned ranges of that memory and never allocates; a request beyond its end
+// returns nullptr and sets `overflow`.
 struct DevScratch {
     std::vector<void*> ptrs;
     std::vector<size_t> sizes;
     size_t cursor = 0;
+    char* arena = nullptr;
+    size_t arena_bytes = 0;
+    bool overflow = false;
+    DevScratch() = default;
+    DevScratch(void* base, size_t bytes) : arena((char*)base), arena_bytes(bytes) {}
+    DevScratch(const DevScratch&) = delete;
+    DevScratch& operator=(const DevScratch&) = delete;
     ~DevScratch() {
         for (void* p : ptrs) (void)hipFree(p);
     }
@@ -582,6 +595,15 @@ struct DevScratch {
     template <class T>
     T* get(size_t n) {
         const size_t bytes = sizeof(T) * std::max<size_t>(n, 1);
+        if (arena) {
+            const size_t at = (cursor + 255) & ~(size_t)255;
+            if (at > arena_bytes || bytes > arena_bytes - at) {
+                overflow = true;
+                return nullptr;
+            }
+            cursor = at + bytes;
+            return (T*)(arena + at);
+        }
         if (cursor < ptrs.size() && sizes[cursor] >= bytes) return (T*)ptrs[cursor++];
         void* p = nullptr;
         if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
@@ -634,10 +656,9 @@ bool dl_cell_list(hipStream_t st, DevScratch& m, const double* P, int n, double 
     int* fill = m.get<int>(c.nbins);
     c.list = m.get<int>(n);
     if (!c.bin_of || !count || !c.start || !fill || !c.list) return false;
-    if (hipMemsetAsync(count, 0, sizeof(int) * (c.nbins + 1), st) != hipSuccess ||
-        hipMemsetAsync(fill, 0, sizeof(int) * c.nbins, st) != hipSuccess)
-        return false;
     const int tb = 256;
+    hipLaunchKernelGGL(svm::k_fill_int, dim3((c.nbins + 1 + tb - 1) / tb), dim3(tb), 0, st, count, c.nbins + 1, 0);
+    hipLaunchKernelGGL(svm::k_fill_int, dim3((c.nbins + tb - 1) / tb), dim3(tb), 0, st, fill, c.nbins, 0);
     hipLaunchKernelGGL(svm::k_dl_bin, dim3((n + tb - 1) / tb), dim3(tb), 0, st, P, n, c.g, c.bin_of, count);
     if (!dl_scan(st, m, count, c.nbins + 1, c.start)) return false;
     hipLaunchKernelGGL(svm::k_dl_fill, dim3((n + tb - 1) / tb), dim3(tb), 0, st, n, (const int*)c.bin_of,
@@ -688,11 +709,8 @@ int dl_triangulate_dev(hipStream_t st, DevScratch& m, double* P, int n, double R
     }
     const int tb = 256;
     auto blocks = [&](long long x) { return dim3((unsigned)((x + tb - 1) / tb)); };
-    if (hipMemsetAsync(o.err, 0, sizeof(int), st) != hipSuccess ||
-        hipMemsetAsync(cnt, 0, sizeof(int) * (n + 1), st) != hipSuccess) {
-        msg = "device copy failed";
-        return SVM_ERR_HIP;
-    }
+    hipLaunchKernelGGL(svm::k_fill_int, dim3(1), dim3(tb), 0, st, o.err, 1, 0);
+    hipLaunchKernelGGL(svm::k_fill_int, blocks(n + 1), dim3(tb), 0, st, cnt, n + 1, 0);
     hipLaunchKernelGGL(svm::k_fill_int, blocks(n), dim3(tb), 0, st, status, n, (int)svm::DL_PENDING);
     // stars, widening the search for the points that need it
     // search radii r0, 2 r0, 3 r0 (about 40, 150 and 350 candidates at uniform
@@ -1461,6 +1479,58 @@ int svm_delaunay_triangles(int32_t n_points, const double* points, int32_t* tria
             r = SVM_ERR_HIP;
     }
     (void)hipStreamDestroy(st);
+    return r;
+}
+
+}  // extern "C"
+
+// Device memory of svm_delaunay_triangles_noalloc: one array per GPU, held
+// until the process ends.  Host code never uses the array's name as an
+// address (in host code it is a stand-in, not device memory): the device
+// address of each GPU's copy comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[SVM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of the current device's scratch array.
+int scratch_on_current_device(void** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return SVM_ERR_HIP;
+    if (dev < 0 || dev >= kMaxScratchDevices) return SVM_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SVM_ERR_HIP;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SVM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int svm_delaunay_triangles_noalloc(int32_t n_points, const double* points, int32_t* triangles, void* stream) {
+    if (n_points < 4 || !points || !triangles) return SVM_ERR_ARG;
+    if (n_points > SVM_NOALLOC_MAX_POINTS) return SVM_ERR_CAPACITY;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SVM_ERR_NO_DEVICE;
+    void* base = nullptr;
+    int r = scratch_on_current_device(&base);
+    if (r) return r;
+    hipStream_t st = (hipStream_t)stream;
+    DevScratch m(base, SVM_NOALLOC_SCRATCH_BYTES);
+    DlOut o;
+    std::string msg;
+    r = dl_triangulate(st, m, points, n_points, 64, o, msg);
+    if (m.overflow) return SVM_ERR_CAPACITY;
+    if (!r && (hipMemcpyAsync(triangles, o.T, sizeof(int32_t) * 3 * (size_t)o.ntri, hipMemcpyDeviceToHost, st) !=
+                   hipSuccess ||
+               hipStreamSynchronize(st) != hipSuccess))
+        r = SVM_ERR_HIP;
     return r;
 }
 

@@ -122,6 +122,19 @@ int svm_build_from_cell_points(svm_context* ctx, int32_t sys, const double* cell
  * tournament compares them.  triangles: int32[3*(2N-4)], counter-clockwise seen from outside, smallest
  * index first, sorted lexicographically. */
 int svm_delaunay_triangles(int32_t n_points, const double* points, int32_t* triangles);
+/* svm_delaunay_triangles without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, same triangles and the same status codes as
+ * svm_delaunay_triangles, plus `stream` (a hipStream_t; NULL = the null
+ * stream), on which the work runs; the call synchronises it.  Its device work
+ * memory is one static array of SVM_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends, so n_points is limited to SVM_NOALLOC_MAX_POINTS (larger
+ * input: SVM_ERR_CAPACITY).  Every call on a GPU uses that GPU's one array:
+ * calls that overlap on one GPU (from several threads or streams) race. */
+#define SVM_NOALLOC_MAX_POINTS 65536
+#define SVM_NOALLOC_SCRATCH_BYTES (40u << 20)
+int svm_delaunay_triangles_noalloc(int32_t n_points, const double* points, int32_t* triangles, void* stream);
 /* Cell list of points on the GPU (the binning of hyperrectangularCellList,
  * src/utilities/hyperrectangularCellList.cpp:165-342, on a cube grid
  * [-L, L]^3 with L = (1 + 1e-9) * max |coordinate|): grid_dim G bins per axis,
