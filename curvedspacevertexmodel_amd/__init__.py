@@ -44,7 +44,7 @@ EXPORTS = [
     "svm_build_voronoi_from_cell_points", "svm_voronoi_forces", "svm_get_voronoi_forces", "svm_voronoi_brownian_steps", "svm_get_generators",
     "svm_set_positions_device", "svm_get_positions_device", "svm_set_directors_device", "svm_get_directors_device",
     "svm_set_velocities_device", "svm_get_velocities_device", "svm_get_forces_device",
-    "svm_delaunay_triangles_noalloc",
+    "svm_delaunay_triangles_noalloc", "svm_set_vertex_form", "svm_get_vertex_form",
 ]
 
 NOALLOC_MAX_POINTS = 65536
@@ -559,6 +559,16 @@ class Context:
         a, b, c = C.c_int32(0), C.c_int32(0), C.c_int32(0)
         self._ck(lib().svm_get_tuning(self.h, C.byref(a), C.byref(b), C.byref(c)), "svm_get_tuning")
         return dict(cluster_blocks=a.value, graph_steps=b.value, xcd_mapping=c.value, lanes=self.lanes()[0])
+
+    def set_vertex_form(self, form):
+        """0: a thread per vertex; 1, 2: corner form (three wavefronts per 64 vertices) compiled for
+        4 or 5 wavefronts per SIMD.  Same results, bit for bit."""
+        self._ck(lib().svm_set_vertex_form(self.h, C.c_int32(form)), "svm_set_vertex_form")
+
+    def vertex_form(self):
+        f = C.c_int32(0)
+        self._ck(lib().svm_get_vertex_form(self.h, C.byref(f)), "svm_get_vertex_form")
+        return f.value
 
     def set_lanes(self, n_lanes):
         """Cut the batch into n_lanes sphere ranges whose steps run on streams of their own."""

@@ -47,6 +47,7 @@ struct svm_context {
     int cluster_blocks = 256;        // T1 dry-run grid (persistent loop over candidates)
     int graph_steps = 10;            // steps per captured graph when the caller passes 0
     int t1_sequential = 0;           // 1: skip the dry runs, always use the sequential pass
+    int vertex_corner = 0;           // Brownian/active vertex pass: 0 k_vertex; 1, 2 corner form at 4, 5 waves per SIMD
     Dev d{};
     int pcur = 0, ncur = 0;          // current position / director buffer
     std::string err;
@@ -263,6 +264,16 @@ int launch_vertex(svm_context* ctx, const Target& w, int pout, int store_forces,
     else if (ctx->force_fp32 == 2)
         hipLaunchKernelGGL((svm::k_vertex<MODE, svm::VecF64>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
                            ctx->ncur, nout, store_forces);
+    else if ((MODE == svm::VM_BROWNIAN || MODE == svm::VM_ACTIVE) && ctx->vertex_corner && !store_forces) {
+        constexpr int M = MODE == svm::VM_ACTIVE ? svm::VM_ACTIVE : svm::VM_BROWNIAN;
+        const dim3 cgrid(xcd_grid(w.d.vtn * (SVM_TILE / SVM_CORNER_V)));
+        if (ctx->vertex_corner == 2)
+            hipLaunchKernelGGL((svm::k_vertex_corner<M, 5>), cgrid, dim3(3 * SVM_CORNER_V), 0, w.st, w.d, ctx->pcur,
+                               pout, ctx->ncur, nout);
+        else
+            hipLaunchKernelGGL((svm::k_vertex_corner<M, 4>), cgrid, dim3(3 * SVM_CORNER_V), 0, w.st, w.d, ctx->pcur,
+                               pout, ctx->ncur, nout);
+    }
     else
         hipLaunchKernelGGL((svm::k_vertex<MODE, double>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
                            ctx->ncur, nout, store_forces);
@@ -2141,6 +2152,19 @@ int svm_set_tuning(svm_context* ctx, int32_t cluster_blocks, int32_t graph_steps
     ctx->graph_steps = graph_steps;
     ctx->d.xcd = xcd_mapping;
     drop_graphs(ctx);
+    return SVM_OK;
+}
+
+int svm_set_vertex_form(svm_context* ctx, int32_t form) {
+    if (!ctx || form < 0 || form > 2) return fail(ctx, SVM_ERR_ARG, "vertex form must be 0, 1 or 2");
+    ctx->vertex_corner = form;
+    drop_graphs(ctx);
+    return SVM_OK;
+}
+
+int svm_get_vertex_form(svm_context* ctx, int32_t* form) {
+    if (!ctx || !form) return SVM_ERR_ARG;
+    *form = ctx->vertex_corner;
     return SVM_OK;
 }
 

@@ -317,6 +317,15 @@ __device__ inline int vertex_force(const Dev& d, const SysParams* S, const doubl
 // vertex's basis and its pair terms with each neighbour are evaluated once
 // (each neighbour's arc gradient serves two cells), and the cell centres'
 // spherical-coordinate terms come from the geometry pass.
+//
+// Which two neighbours bound a cell is data (vln), but around a vertex the
+// map from a cell's previous ring neighbour to its next one is a cyclic
+// permutation of the three neighbour slots: 0->1->2 or 0->2->1.  Naming the
+// neighbours so that it is always 0->1->2 (swapping slots 1 and 2 for the
+// other orientation, an integer select of rows), cell j of the loop below is
+// the one whose previous neighbour is j and next j+1 (mod 3): its operands
+// are fixed registers, not per-cell selections of every neighbour term.  The
+// contributions are then added in the reference's cell-slot order.
 template <>
 __device__ inline int vertex_force<double>(const Dev& d, const SysParams* S, const double* X, const double* Y,
                                            const double* Z, int v, double* f) {
@@ -325,9 +334,15 @@ __device__ inline int vertex_force<double>(const Dev& d, const SysParams* S, con
     const int4 nb = d.vn[v];
     const int4 cl = d.vcn[v];
     const int ln = d.vln[v];
+    const int ls0 = ln & 3, ns0 = (ln >> 2) & 3, ls1 = (ln >> 4) & 3, ls2 = (ln >> 8) & 3;
+    const int o = ns0 == (ls0 == 2 ? 0 : ls0 + 1) ? 0 : 1;   // 1: the cycle runs 0->2->1
+    auto rn = [o](int k) { return o && k ? 3 - k : k; };
+    // the renamed previous-neighbour slot of cell slots 0, 1, 2
+    const int L0 = rn(ls0), L1 = rn(ls1), L2 = rn(ls2);
+    const int r1 = o ? nb.z : nb.y, r2 = o ? nb.y : nb.z;
     const PQ q0 = pq_at<double>(X, Y, Z, nb.x);
-    const PQ q1 = pq_at<double>(X, Y, Z, nb.y);
-    const PQ q2 = pq_at<double>(X, Y, Z, nb.z);
+    const PQ q1 = pq_at<double>(X, Y, Z, r1);
+    const PQ q2 = pq_at<double>(X, Y, Z, r2);
     double tH[3], pH[3];
     basis(qs, tH, pH);
     const PairT<double> w0 = pair_terms(qs, q0), w1 = pair_terms(qs, q1), w2 = pair_terms(qs, q2);
@@ -336,27 +351,18 @@ __device__ inline int vertex_force<double>(const Dev& d, const SysParams* S, con
     grad_geodesic_p(qs, q1, w1, tH, pH, g1);
     grad_geodesic_p(qs, q2, w2, tH, pH, g2);
     const size_t Cs = (size_t)d.Cs;
-    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-    int nan = 0;
-#pragma unroll
-    for (int sl = 0; sl < 3; ++sl) {
-        const int c = sl == 0 ? cl.x : (sl == 1 ? cl.y : cl.z);
-        const int ls = (ln >> (4 * sl)) & 3;
-        const int ns = (ln >> (4 * sl + 2)) & 3;
+    // pair_force's sequence (sphericalVertexModel.cpp:386-413) for the cell
+    // whose previous and next ring neighbours are (ql, qn)
+    auto cell_force = [&](int c, const PQ& ql, const PQ& qn, const PairT<double>& wl, const PairT<double>& wn,
+                          const double* gl, const double* gn, double* fSet) {
         const double areaDifference = d.A[c] - d.a0[c];
         const double perimeterDifference = d.P[c] - d.p0[c];
         PQ qc;
         qc.x = d.cx[c]; qc.y = d.cy[c]; qc.z = d.cz[c];
         qc.cosT = d.ctrig[c]; qc.sinT = d.ctrig[Cs + c]; qc.cosP = d.ctrig[2 * Cs + c]; qc.sinP = d.ctrig[3 * Cs + c];
         const PairT<double> wc = pair_terms(qs, qc);
-        const PQ& ql = pick3(ls, q0, q1, q2);
-        const PQ& qn = pick3(ns, q0, q1, q2);
-        const PairT<double>& wl = ls == 0 ? w0 : (ls == 1 ? w1 : w2);
-        const PairT<double>& wn = ns == 0 ? w0 : (ns == 1 ? w1 : w2);
-        const double* gl = ls == 0 ? g0 : (ls == 1 ? g1 : g2);
-        const double* gn = ns == 0 ? g0 : (ns == 1 ? g1 : g2);
-        // pair_force's sequence (sphericalVertexModel.cpp:386-413)
-        double fSet[3] = {0.0, 0.0, 0.0}, t[3];
+        double t[3];
+        fSet[0] = 0.0; fSet[1] = 0.0; fSet[2] = 0.0;
         const double cp = 2.0 * perimeterDifference;
         const double ca = 2.0 * Kr * areaDifference;
         fSet[0] -= cp * gl[0]; fSet[1] -= cp * gl[1]; fSet[2] -= cp * gl[2];
@@ -365,6 +371,21 @@ __device__ inline int vertex_force<double>(const Dev& d, const SysParams* S, con
         fSet[0] -= ca * t[0]; fSet[1] -= ca * t[1]; fSet[2] -= ca * t[2];
         grad_triangle_p(qs, qc, qn, wc, wn, tH, pH, R, t);
         fSet[0] -= ca * t[0]; fSet[1] -= ca * t[1]; fSet[2] -= ca * t[2];
+    };
+    // cell rows of loop cells 0, 1, 2 (previous neighbour 0, 1, 2)
+    const int c0 = L0 == 0 ? cl.x : (L1 == 0 ? cl.y : cl.z);
+    const int c1 = L0 == 1 ? cl.x : (L1 == 1 ? cl.y : cl.z);
+    const int c2 = L0 == 2 ? cl.x : (L1 == 2 ? cl.y : cl.z);
+    double fa[3], fb[3], fc[3];
+    cell_force(c0, q0, q1, w0, w1, g0, g1, fa);
+    cell_force(c1, q1, q2, w1, w2, g1, g2, fb);
+    cell_force(c2, q2, q0, w2, w0, g2, g0, fc);
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    int nan = 0;
+#pragma unroll
+    for (int sl = 0; sl < 3; ++sl) {
+        const int j = sl == 0 ? L0 : (sl == 1 ? L1 : L2);
+        const double* fSet = j == 0 ? fa : (j == 1 ? fb : fc);
         if (!(fSet[0] != fSet[0])) {
             f0 = f0 + fSet[0];
             f1 = f1 + fSet[1];
@@ -394,25 +415,35 @@ __device__ inline int vertex_force_vec(const Dev& d, const SysParams* S, const d
     int4 nb = d.vn[v];
     int4 cl = d.vcn[v];
     int ln = d.vln[v];
+    // neighbours named so that each cell's previous -> next neighbour is
+    // j -> j+1 (mod 3), as in vertex_force<double>
+    const int ls0 = ln & 3, ns0 = (ln >> 2) & 3, ls1 = (ln >> 4) & 3, ls2 = (ln >> 8) & 3;
+    const int o = ns0 == (ls0 == 2 ? 0 : ls0 + 1) ? 0 : 1;
+    auto rn = [o](int k) { return o && k ? 3 - k : k; };
+    const int L0 = rn(ls0), L1 = rn(ls1), L2 = rn(ls2);
+    const int r1 = o ? nb.z : nb.y, r2 = o ? nb.y : nb.z;
     const V3<T> e0 = rel(X[nb.x], Y[nb.x], Z[nb.x]);
-    const V3<T> e1 = rel(X[nb.y], Y[nb.y], Z[nb.y]);
-    const V3<T> e2 = rel(X[nb.z], Y[nb.z], Z[nb.z]);
+    const V3<T> e1 = rel(X[r1], Y[r1], Z[r1]);
+    const V3<T> e2 = rel(X[r2], Y[r2], Z[r2]);
     const V3<T> g0 = grad_arc_v(a, e0), g1 = grad_arc_v(a, e1), g2 = grad_arc_v(a, e2);
+    auto cell_force = [&](int c, const V3<T>& el, const V3<T>& en, const V3<T>& gl, const V3<T>& gn) {
+        const T cp = T(2) * (T)(d.P[c] - d.p0[c]);
+        const T ca = T(2) * Kr * (T)(d.A[c] - d.a0[c]);
+        const V3<T> ec = rel(d.cx[c], d.cy[c], d.cz[c]);
+        return (-cp) * (gl + gn) - ca * (grad_tri_v(a, el, ec, Rt) + grad_tri_v(a, ec, en, Rt));
+    };
+    const int c0 = L0 == 0 ? cl.x : (L1 == 0 ? cl.y : cl.z);
+    const int c1 = L0 == 1 ? cl.x : (L1 == 1 ? cl.y : cl.z);
+    const int c2 = L0 == 2 ? cl.x : (L1 == 2 ? cl.y : cl.z);
+    const V3<T> fa = cell_force(c0, e0, e1, g0, g1);
+    const V3<T> fb = cell_force(c1, e1, e2, g1, g2);
+    const V3<T> fc = cell_force(c2, e2, e0, g2, g0);
     V3<T> F{T(0), T(0), T(0)};
     int nan = 0;
 #pragma unroll
     for (int sl = 0; sl < 3; ++sl) {
-        int c = sl == 0 ? cl.x : (sl == 1 ? cl.y : cl.z);
-        int ls = (ln >> (4 * sl)) & 3;
-        int ns = (ln >> (4 * sl + 2)) & 3;
-        const T cp = T(2) * (T)(d.P[c] - d.p0[c]);
-        const T ca = T(2) * Kr * (T)(d.A[c] - d.a0[c]);
-        const V3<T> ec = rel(d.cx[c], d.cy[c], d.cz[c]);
-        const V3<T> el = ls == 0 ? e0 : (ls == 1 ? e1 : e2);
-        const V3<T> en = ns == 0 ? e0 : (ns == 1 ? e1 : e2);
-        const V3<T> gl = ls == 0 ? g0 : (ls == 1 ? g1 : g2);
-        const V3<T> gn = ns == 0 ? g0 : (ns == 1 ? g1 : g2);
-        V3<T> fs = (-cp) * (gl + gn) - ca * (grad_tri_v(a, el, ec, Rt) + grad_tri_v(a, ec, en, Rt));
+        const int j = sl == 0 ? L0 : (sl == 1 ? L1 : L2);
+        const V3<T> fs = j == 0 ? fa : (j == 1 ? fb : fc);
         // skipped like the reference's NaN contributions; here a degenerate
         // edge (coincident vertices) gives +-inf, which x - x turns into NaN
         if (!(fs.x - fs.x == T(0) && fs.y - fs.y == T(0) && fs.z - fs.z == T(0))) ++nan;
@@ -545,6 +576,175 @@ __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, i
             d.vpart[3 * tile + 2] = sv;
         }
     }
+}
+
+// Corner form of k_vertex for the Brownian and active updaters in the parity
+// arithmetic.  A block takes 64 vertices (a quarter of a 256-row tile) with
+// three wavefronts and works in three phases:
+//   1. wavefront k evaluates, for every vertex, the terms of its neighbour
+//      slot k that vertex_force<double> shares between two cells (the
+//      neighbour's spherical coordinates, the pair terms with the vertex and
+//      the arc-length gradient) and leaves them in LDS;
+//   2. wavefront k evaluates the contribution of cell slot k (the body of the
+//      reference's per-cell loop, sphericalVertexModel.cpp:386-413) from them;
+//   3. wavefront 0 adds the three contributions in slot order, skipping NaN
+//      ones as the reference does, and applies the updater.
+// Every value is computed by the same operations on the same inputs as in
+// k_vertex, so the results are identical bit for bit.  A thread carries the
+// terms of one neighbour and one cell instead of three of each, which leaves
+// room for four or five wavefronts per SIMD (WAVES: at most 128 or 102
+// VGPRs) to hide the gathers' latency.
+#define SVM_CORNER_V 64
+__device__ inline int corner_block(const Dev& d) {
+    const int nq = d.vtn * (SVM_TILE / SVM_CORNER_V);
+    int q = (int)blockIdx.x;
+    if (d.xcd) {
+        const int per = (nq + 7) >> 3;
+        q = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    }
+    return q < nq ? q : -1;
+}
+
+template <int MODE, int WAVES>
+__global__ __launch_bounds__(3 * SVM_CORNER_V) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_vertex_corner(
+    Dev d, int pin, int pout, int nin, int nout) {
+    // per neighbour slot: cosT, sinT, cosP, sinP, the pair terms sinPd,
+    // cosPd, s, dd, and the arc gradient (3); the neighbour's coordinates are
+    // read again from memory (cache-resident after phase 1)
+    __shared__ double nsh[3][11][SVM_CORNER_V];
+    __shared__ double fsh[3][3][SVM_CORNER_V];
+    __shared__ int oksh[3][SVM_CORNER_V];
+    const int q = corner_block(d);
+    if (q < 0) return;
+    const int tile = d.vt0 + q / (SVM_TILE / SVM_CORNER_V);
+    const int lane = threadIdx.x & (SVM_CORNER_V - 1);
+    const int slot = threadIdx.x / SVM_CORNER_V;   // the same for a whole wavefront
+    SysParams* S = d.sys + d.vtile_sys[tile];
+    const int v = tile * SVM_TILE + (q % (SVM_TILE / SVM_CORNER_V)) * SVM_CORNER_V + lane;
+    const int local = v - S->vo;
+    const bool valid = local < S->nv;
+    const double* X = d.px[pin];
+    const double* Y = d.py[pin];
+    const double* Z = d.pz[pin];
+    PQ qs{};
+    double tH[3] = {0.0, 0.0, 0.0}, pH[3] = {0.0, 0.0, 0.0};
+    if (valid) {
+        qs = pq_at<double>(X, Y, Z, v);
+        basis(qs, tH, pH);
+        const int4 nb = d.vn[v];
+        const int j = slot == 0 ? nb.x : (slot == 1 ? nb.y : nb.z);
+        const PQ qj = pq_at<double>(X, Y, Z, j);
+        const PairT<double> w = pair_terms(qs, qj);
+        double g[3];
+        grad_geodesic_p(qs, qj, w, tH, pH, g);
+        double* o = &nsh[slot][0][lane];
+        o[0 * SVM_CORNER_V] = qj.cosT; o[1 * SVM_CORNER_V] = qj.sinT;
+        o[2 * SVM_CORNER_V] = qj.cosP; o[3 * SVM_CORNER_V] = qj.sinP;
+        o[4 * SVM_CORNER_V] = w.sinPd; o[5 * SVM_CORNER_V] = w.cosPd; o[6 * SVM_CORNER_V] = w.s;
+        o[7 * SVM_CORNER_V] = w.dd;
+        o[8 * SVM_CORNER_V] = g[0]; o[9 * SVM_CORNER_V] = g[1]; o[10 * SVM_CORNER_V] = g[2];
+    }
+    __syncthreads();
+    if (valid) {
+        const double R = S->R, Kr = S->kr;
+        const int4 cl = d.vcn[v];
+        const int4 nb = d.vn[v];
+        const int ln = d.vln[v];
+        const int c = slot == 0 ? cl.x : (slot == 1 ? cl.y : cl.z);
+        // neighbour slots of the cell's previous and next ring vertices; a
+        // field value of 3 selects slot 2, as pick3 does in k_vertex
+        const int ls = min((ln >> (4 * slot)) & 3, 2);
+        const int ns = min((ln >> (4 * slot + 2)) & 3, 2);
+        const int jl = ls == 0 ? nb.x : (ls == 1 ? nb.y : nb.z);
+        const int jn = ns == 0 ? nb.x : (ns == 1 ? nb.y : nb.z);
+        const size_t Cs = (size_t)d.Cs;
+        const double areaDifference = d.A[c] - d.a0[c];
+        const double perimeterDifference = d.P[c] - d.p0[c];
+        PQ qc;
+        qc.x = d.cx[c]; qc.y = d.cy[c]; qc.z = d.cz[c];
+        qc.cosT = d.ctrig[c]; qc.sinT = d.ctrig[Cs + c]; qc.cosP = d.ctrig[2 * Cs + c]; qc.sinP = d.ctrig[3 * Cs + c];
+        auto load_pq = [&](int k, int j) {
+            const double* o = &nsh[k][0][lane];
+            PQ r;
+            r.x = X[j]; r.y = Y[j]; r.z = Z[j];
+            r.cosT = o[0 * SVM_CORNER_V]; r.sinT = o[1 * SVM_CORNER_V];
+            r.cosP = o[2 * SVM_CORNER_V]; r.sinP = o[3 * SVM_CORNER_V];
+            return r;
+        };
+        auto load_pair = [&](int k) {
+            const double* o = &nsh[k][0][lane];
+            PairT<double> r;
+            r.sinPd = o[4 * SVM_CORNER_V]; r.cosPd = o[5 * SVM_CORNER_V]; r.s = o[6 * SVM_CORNER_V];
+            r.dd = o[7 * SVM_CORNER_V];
+            return r;
+        };
+        const PairT<double> wc = pair_terms(qs, qc);
+        double fSet[3] = {0.0, 0.0, 0.0}, t[3];
+        const double cp = 2.0 * perimeterDifference;
+        const double ca = 2.0 * Kr * areaDifference;
+        {
+            const double* gl = &nsh[ls][8][lane];
+            const double* gn = &nsh[ns][8][lane];
+            fSet[0] -= cp * gl[0]; fSet[1] -= cp * gl[SVM_CORNER_V]; fSet[2] -= cp * gl[2 * SVM_CORNER_V];
+            fSet[0] -= cp * gn[0]; fSet[1] -= cp * gn[SVM_CORNER_V]; fSet[2] -= cp * gn[2 * SVM_CORNER_V];
+        }
+        {
+            const PQ ql = load_pq(ls, jl);
+            grad_triangle_p(qs, ql, qc, load_pair(ls), wc, tH, pH, R, t);
+        }
+        fSet[0] -= ca * t[0]; fSet[1] -= ca * t[1]; fSet[2] -= ca * t[2];
+        {
+            const PQ qn = load_pq(ns, jn);
+            grad_triangle_p(qs, qc, qn, wc, load_pair(ns), tH, pH, R, t);
+        }
+        fSet[0] -= ca * t[0]; fSet[1] -= ca * t[1]; fSet[2] -= ca * t[2];
+        fsh[slot][0][lane] = fSet[0];
+        fsh[slot][1][lane] = fSet[1];
+        fsh[slot][2][lane] = fSet[2];
+        oksh[slot][lane] = !(fSet[0] != fSet[0]);
+    }
+    __syncthreads();
+    if (slot != 0 || !valid) return;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    int nan = 0;
+#pragma unroll
+    for (int sl = 0; sl < 3; ++sl) {
+        if (oksh[sl][lane]) {
+            f0 = f0 + fsh[sl][0][lane];
+            f1 = f1 + fsh[sl][1][lane];
+            f2 = f2 + fsh[sl][2][lane];
+        } else {
+            ++nan;
+        }
+    }
+    const double f[3] = {f0, f1, f2};
+    double disp[3], nI[3];
+    if (MODE == VM_BROWNIAN) {
+        double forcePrefactor = S->dt * S->mu;
+        double noisePrefactor = __builtin_sqrt(2.0 * forcePrefactor * S->T);
+        double xi[3] = {0.0, 0.0, 0.0};
+        if (S->T != 0.0) normal3(S->seed, S->step, (uint32_t)local, xi);
+        for (int k = 0; k < 3; ++k) disp[k] = xi[k] * noisePrefactor + f[k] * forcePrefactor;
+    } else {
+        nI[0] = d.nx[nin][v]; nI[1] = d.ny[nin][v]; nI[2] = d.nz[nin][v];
+        double dt = S->dt, v0 = S->v0, mu = S->mu;
+        disp[0] = dt * (v0 * nI[0] + mu * f[0]);
+        disp[1] = dt * (v0 * nI[1] + mu * f[1]);
+        disp[2] = dt * (v0 * nI[2] + mu * f[2]);
+    }
+    double p[3] = {X[v], Y[v], Z[v]};
+    move_on_sphere(p, disp, S->R);
+    d.px[pout][v] = p[0];
+    d.py[pout][v] = p[1];
+    d.pz[pout][v] = p[2];
+    if (MODE == VM_ACTIVE) {
+        double nO[3];
+        director_update(S, local, p, nI, nO);
+        d.nx[nout][v] = nO[0];
+        d.ny[nout][v] = nO[1];
+        d.nz[nout][v] = nO[2];
+    }
+    if (nan) atomicAdd((unsigned long long*)&S->n_nan, (unsigned long long)nan);
 }
 
 // The FIRE step's reductions and controller (energyMinimizerFIRE::fireStepCPU,

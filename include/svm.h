@@ -439,6 +439,17 @@ int svm_nc_close(svm_nc_file* f);
  * (xcd_mapping).  None of them changes any result, bit for bit. */
 int svm_set_tuning(svm_context* ctx, int32_t cluster_blocks, int32_t graph_steps, int32_t xcd_mapping);
 int svm_get_tuning(svm_context* ctx, int32_t* cluster_blocks, int32_t* graph_steps, int32_t* xcd_mapping);
+/* Kernel form of the vertex pass of Brownian and active steps in the FP64
+ * parity arithmetic (no counterpart in the reference): 0 = a thread per
+ * vertex evaluating its three cells (k_vertex, 190 VGPRs: two wavefronts
+ * per SIMD); 1 and 2 = corner form, three wavefronts per 64 vertices,
+ * wavefront k evaluating the vertex's neighbour slot k and then its cell slot
+ * k, the contributions added in slot order (k_vertex_corner), compiled for
+ * four (1) or five (2) wavefronts per SIMD.  All give identical results, bit
+ * for bit; they differ in register use and so in how many wavefronts a SIMD
+ * keeps in flight to hide memory latency. */
+int svm_set_vertex_form(svm_context* ctx, int32_t form);
+int svm_get_vertex_form(svm_context* ctx, int32_t* form);
 /* Lanes of the dynamics step (no counterpart in the reference, which steps
  * one sphere at a time): the batch is cut into n_lanes contiguous runs of
  * spheres with about equal numbers of vertices (at most SVM_MAX_LANES, at
