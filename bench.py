@@ -182,6 +182,17 @@ def cpu_baseline(ctx, params, budget_s=12.0, threads=None):
                        f"{budget_s:.0f} s; single-thread figure: the first thread's spheres alone")
 
 
+SCHEDULES = os.path.join(ROOT, "curvedspacevertexmodel_amd", "schedules.txt")
+
+
+def schedule_table(path):
+    """The persisted schedule table's lines (svm_tuning_save format), or None."""
+    try:
+        return [l.strip() for l in open(path) if l.strip()]
+    except OSError:
+        return None
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_batched-active-256.txt")
 FP64_FILE = os.path.join(ROOT, "profiles", "pmc_fp64_batched-active-256.txt")
 PEAK_FP64_TFLOPS = 78.6
@@ -284,7 +295,7 @@ def voronoi_model(svm, pts, vsteps=20):
             "note": "FP64 forces on the generators, Brownian move, GPU retriangulation every step"}
 
 
-def large_batch(svm, steps=400):
+def large_batch(svm, steps=400, schedules=None):
     """The large batched workload: 32 spheres, N log-uniform in [50,000,
     500,000] (one Philox generator, seed 13377), p0 uniform in [3.7, 3.85],
     FP32 forces on FP64 positions with FP64 predicates.  GPU Delaunay set-up,
@@ -317,7 +328,10 @@ def large_batch(svm, steps=400):
     ctx.fire_minimize(100)
     for i in range(len(params)):
         ctx.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
-    ctx.autotune(updater=1, steps_per_trial=40)
+    schedules = schedules or SCHEDULES
+    if not ctx.load_tuning(schedules, updater=1):
+        ctx.autotune(updater=1, steps_per_trial=40)
+        ctx.save_tuning(schedules, updater=1)
     G = ctx.tuning()["graph_steps"]
     ctx.run_graph(1, G, max(1, 40 // G))
     ctx.synchronize()
@@ -385,7 +399,11 @@ def main():
                     help="only the Voronoi model of 262,144 cells (no graphs): one JSON line, then exit")
     ap.add_argument("--large", action="store_true",
                     help="also run the 32-sphere, 50k-500k-cell FP32 batch (about 6 M cells)")
-    ap.add_argument("--autotune", type=int, default=1, help="1: pick the launch schedule with svm_autotune")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="0: default schedule; 1: the schedule stored for this batch in --schedules, else "
+                         "svm_autotune (and store its choice); 2: always svm_autotune, then store")
+    ap.add_argument("--schedules", default=SCHEDULES,
+                    help="persisted schedule table (svm_tuning_save / svm_tuning_load)")
     ap.add_argument("--fire-spheres", type=int, default=1024)
     ap.add_argument("--fire-cap", type=int, default=10000)
     ap.add_argument("--copy-probe", action="store_true",
@@ -409,27 +427,43 @@ def main():
     t_setup = time.perf_counter()
     ctx, t_delaunay = build_batch(svm, params)
     t_setup = time.perf_counter() - t_setup
-    G = max(2, args.graph_steps - args.graph_steps % 2)
-    warm = max(G, args.warmup - args.warmup % G)
-    steps = max(G, args.steps - args.steps % G)
+    K, W = max(1, args.steps), max(0, args.warmup)
     print(f"setup {t_setup:.1f} s: {len(params)} spheres, {total_cells} cells", flush=True)
     stabilise(ctx, params, 200)
-    print(f"stabilised (200 Brownian steps); warmup {warm} active steps", flush=True)
-    ctx.run_graph(1, G, warm // G)
-    ctx.synchronize()
-    tune_table = None
-    if args.autotune:
+    print("stabilised (200 Brownian steps)", flush=True)
+    tune_table, schedule_source, tune_steps = None, "default", 0
+    if args.autotune == 1 and ctx.load_tuning(args.schedules, updater=1):
+        schedule_source = "table"
+    elif args.autotune:
         tune_table = ctx.autotune(updater=1, steps_per_trial=200)
-        G = ctx.tuning()["graph_steps"]
-        steps = max(G, args.steps - args.steps % G)
-        print(f"autotuned: {ctx.tuning()}", flush=True)
-    print("warmup done", flush=True)
+        ctx.save_tuning(args.schedules, updater=1)
+        schedule_source = "autotuned"
+        tune_steps = int(sum(r[2] + 3 * max(1, 200 // (3 * r[2])) * r[2] for r in tune_table))
+    if not args.autotune:
+        ctx.set_tuning(graph_steps=max(2, args.graph_steps - args.graph_steps % 2))
+    # graph length: the schedule's, cut to an even divisor of the timed steps
+    # so that exactly K steps are timed (an odd K times one eager step more)
+    Kg = K - K % 2
+    G = max([g for g in range(2, ctx.tuning()["graph_steps"] + 1, 2) if Kg % g == 0] or [2])
+    print(f"schedule ({schedule_source}): {ctx.tuning()}; {G} steps per timed graph", flush=True)
+    # exactly W warm-up steps, then the timed graph captured (no steps)
+    if W >= G:
+        ctx.run_graph(1, G, W // G)
+    if W % G:
+        ctx.active_steps(W % G)
+    ctx.run_graph(1, G, 0)
+    ctx.synchronize()
+    print(f"warmup done ({W} steps)", flush=True)
     base = (ctx.counters(), ctx.t1_diagnostics())
     t0 = time.perf_counter()
-    ctx.run_graph(1, G, steps // G)
+    if Kg:
+        ctx.run_graph(1, G, Kg // G)
+    if K % 2:
+        ctx.active_steps(1)
     ctx.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
+    steps, warm = K, W
     value = total_cells * steps / elapsed
     print(f"timed {steps} steps in {elapsed:.3f} s", flush=True)
 
@@ -452,6 +486,8 @@ def main():
         "config": {"workload": "batched-active-256", "spheres": len(params), "total_cells": total_cells,
                    "N_range": [200, 20000], "p0_range": [3.5, 4.0], "v0_range": [0.01, 0.5],
                    "Dr": [0.1, 1, 10], "dt": 0.01, "steps_per_graph": G, "schedule": ctx.tuning(),
+                   "schedule_source": schedule_source, "schedule_table": schedule_table(args.schedules),
+                   "untimed_steps_before_warmup": {"stabilisation_brownian": 200, "autotune_active": tune_steps},
                    "autotune_table": tune_table.tolist() if tune_table is not None else None},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(dom),
@@ -469,7 +505,7 @@ def main():
     if not args.no_single:
         result["single"] = single_sphere(svm)
     if args.large:
-        result["large"] = large_batch(svm)
+        result["large"] = large_batch(svm, schedules=args.schedules)
     if not args.no_fire:
         result["fire"] = fire_comparison(svm, args.fire_spheres, args.fire_cap, lanes=lanes)
     print(json.dumps(result), flush=True)

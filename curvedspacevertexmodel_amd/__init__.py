@@ -45,6 +45,7 @@ EXPORTS = [
     "svm_set_positions_device", "svm_get_positions_device", "svm_set_directors_device", "svm_get_directors_device",
     "svm_set_velocities_device", "svm_get_velocities_device", "svm_get_forces_device",
     "svm_delaunay_triangles_noalloc", "svm_set_vertex_form", "svm_get_vertex_form",
+    "svm_tuning_save", "svm_tuning_load",
 ]
 
 NOALLOC_MAX_POINTS = 65536
@@ -99,6 +100,7 @@ MAX_LANES = 4   # SVM_MAX_LANES of include/svm.h
 COUNTER_COLS = 6      # SVM_COUNTER_COLS: per-sphere counters
 FORCE_ERR_COLS = 7    # SVM_FORCE_ERR_COLS: per-sphere force-error statistics
 T1_DIAG_COLS = 12     # SVM_T1_DIAG_COLS: per-sphere T1 diagnostics
+TUNE_COLS = 6         # SVM_TUNE_COLS: autotuner trial-table width
 
 
 class SVMError(RuntimeError):
@@ -558,7 +560,8 @@ class Context:
     def tuning(self):
         a, b, c = C.c_int32(0), C.c_int32(0), C.c_int32(0)
         self._ck(lib().svm_get_tuning(self.h, C.byref(a), C.byref(b), C.byref(c)), "svm_get_tuning")
-        return dict(cluster_blocks=a.value, graph_steps=b.value, xcd_mapping=c.value, lanes=self.lanes()[0])
+        return dict(cluster_blocks=a.value, graph_steps=b.value, xcd_mapping=c.value, lanes=self.lanes()[0],
+                    vertex_form=self.vertex_form())
 
     def set_vertex_form(self, form):
         """0: a thread per vertex; 1, 2: corner form (three wavefronts per 64 vertices) compiled for
@@ -582,12 +585,23 @@ class Context:
         return n.value, first[:n.value].copy()
 
     def autotune(self, updater=1, steps_per_trial=60):
-        """Returns the trial table [(lanes, cluster_blocks, graph_steps, xcd_mapping, ms/step)]."""
-        tab = np.zeros((32, 5), np.float64)
+        """Returns the trial table [(lanes, cluster_blocks, graph_steps, xcd_mapping, vertex_form, ms/step)]."""
+        tab = np.zeros((32, TUNE_COLS), np.float64)
         n = C.c_int32(0)
         self._ck(lib().svm_autotune(self.h, C.c_int32(updater), C.c_int32(steps_per_trial), _dp(tab),
                                     C.c_int32(32), C.byref(n)), "svm_autotune")
         return tab[:n.value]
+
+    def save_tuning(self, path, updater=1):
+        """Store the current schedule under this batch's key (svm_tuning_save)."""
+        self._ck(lib().svm_tuning_save(self.h, C.c_int32(updater), os.fsencode(path)), "svm_tuning_save")
+
+    def load_tuning(self, path, updater=1):
+        """Apply the schedule stored under this batch's key; returns whether one was found."""
+        found = C.c_int32(0)
+        self._ck(lib().svm_tuning_load(self.h, C.c_int32(updater), os.fsencode(path), C.byref(found)),
+                 "svm_tuning_load")
+        return bool(found.value)
 
     def cell_descriptors(self, sys):
         """(X float32 [N][16], frames float32 [N][6]) of sphere sys."""

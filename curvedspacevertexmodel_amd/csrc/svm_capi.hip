@@ -47,6 +47,7 @@ struct svm_context {
     int cluster_blocks = 256;        // T1 dry-run grid (persistent loop over candidates)
     int graph_steps = 10;            // steps per captured graph when the caller passes 0
     int t1_sequential = 0;           // 1: skip the dry runs, always use the sequential pass
+    double tuned_ms = 0.0;           // ms per step of the schedule the autotuner kept (0: not tuned)
     int vertex_corner = 0;           // Brownian/active vertex pass: 0 k_vertex; 1, 2 corner form at 4, 5 waves per SIMD
     Dev d{};
     int pcur = 0, ncur = 0;          // current position / director buffer
@@ -2260,18 +2261,22 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
         *ms = samples[1];
         return SVM_OK;
     };
-    auto record = [&](int lanes, int cb, int G, int xcd, double ms) {
+    auto record = [&](int lanes, int cb, int G, int xcd, int form, double ms) {
         if (table && nrows < table_rows) {
-            table[5 * nrows] = lanes;
-            table[5 * nrows + 1] = cb;
-            table[5 * nrows + 2] = G;
-            table[5 * nrows + 3] = xcd;
-            table[5 * nrows + 4] = ms;
+            double* row = table + (size_t)SVM_TUNE_COLS * nrows;
+            row[0] = lanes;
+            row[1] = cb;
+            row[2] = G;
+            row[3] = xcd;
+            row[4] = form;
+            row[5] = ms;
         }
         ++nrows;
     };
     int best_cb = ctx->cluster_blocks, best_G = ctx->graph_steps, best_x = ctx->d.xcd, best_l = ctx->nlanes;
+    int best_f = 0;
     double best = 1e300;
+    r = svm_set_vertex_form(ctx, 0);
     // lanes x two dry-run grids, at 10 steps per graph with the XCD mapping
     const int cb_pair[] = {256, 1024};
     for (int l = 1; l <= std::min(ctx->nsys, (int)SVM_MAX_LANES) && !r; ++l)
@@ -2281,10 +2286,21 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
             double ms = 0.0;
             if (!r) r = trial(10, &ms);
             if (r) break;
-            record(l, cb, 10, 1, ms);
+            record(l, cb, 10, 1, 0, ms);
             if (ms < best) { best = ms; best_l = l; best_cb = cb; best_x = 1; best_G = 10; }
         }
     if (!r) r = svm_set_lanes(ctx, best_l);
+    // the corner forms of the vertex pass (the FP64 parity arithmetic only)
+    for (int form = 1; form <= 2 && !r && ctx->force_fp32 == 0; ++form) {
+        r = svm_set_tuning(ctx, best_cb, 10, 1);
+        if (!r) r = svm_set_vertex_form(ctx, form);
+        double ms = 0.0;
+        if (!r) r = trial(10, &ms);
+        if (r) break;
+        record(best_l, best_cb, 10, 1, form, ms);
+        if (ms < best) { best = ms; best_f = form; best_x = 1; best_G = 10; }
+    }
+    if (!r) r = svm_set_vertex_form(ctx, best_f);
     // the other kernel schedules at the default graph length
     const int cbs[] = {128, 256, 512, 1024};
     for (int xcd = 0; xcd <= 1 && !r; ++xcd)
@@ -2294,7 +2310,7 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
             double ms = 0.0;
             if (!r) r = trial(10, &ms);
             if (r) break;
-            record(best_l, cb, 10, xcd, ms);
+            record(best_l, cb, 10, xcd, best_f, ms);
             if (ms < best) { best = ms; best_cb = cb; best_x = xcd; best_G = 10; }
         }
     // then the graph length
@@ -2305,7 +2321,7 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
         double ms = 0.0;
         if (!r) r = trial(G, &ms);
         if (r) break;
-        record(best_l, best_cb, G, best_x, ms);
+        record(best_l, best_cb, G, best_x, best_f, ms);
         if (ms < best) { best = ms; best_G = G; }
     }
     (void)hipEventDestroy(e0);
@@ -2313,7 +2329,94 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
     if (r) return r;
     if (rows_written) *rows_written = nrows;
     r = svm_set_lanes(ctx, best_l);
-    return r ? r : svm_set_tuning(ctx, best_cb, best_G, best_x);
+    r = r ? r : svm_set_vertex_form(ctx, best_f);
+    r = r ? r : svm_set_tuning(ctx, best_cb, best_G, best_x);
+    if (!r) ctx->tuned_ms = best;
+    return r;
+}
+
+// ---- persisted schedules ----
+namespace {
+int floor_log2(long long x) {
+    int k = -1;
+    while (x > 0) { x >>= 1; ++k; }
+    return k;
+}
+// The key of a batch's schedule: updater, force precision, and the binary
+// orders of magnitude of its sphere count and of its mean sphere size.
+std::array<int, 4> schedule_key(svm_context* ctx, int updater) {
+    long long cells = 0;
+    for (int s = 0; s < ctx->nsys; ++s) cells += ctx->layout[s].nc;
+    return {updater, ctx->force_fp32, floor_log2(ctx->nsys), floor_log2(cells / std::max(1, ctx->nsys))};
+}
+const char* kScheduleTag = "svm-schedule-v1";
+}  // namespace
+
+int svm_tuning_save(svm_context* ctx, int32_t updater, const char* path) {
+    if (!ctx || !path || updater < 0 || updater > 2) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    const auto key = schedule_key(ctx, updater);
+    std::vector<std::string> keep;
+    if (FILE* f = std::fopen(path, "r")) {
+        char line[512];
+        while (std::fgets(line, sizeof line, f)) {
+            char tag[64];
+            int k[4];
+            if (std::sscanf(line, "%63s %d %d %d %d", tag, &k[0], &k[1], &k[2], &k[3]) == 5 &&
+                std::strcmp(tag, kScheduleTag) == 0 && k[0] == key[0] && k[1] == key[1] && k[2] == key[2] &&
+                k[3] == key[3])
+                continue;   // replaced below
+            keep.push_back(line);
+        }
+        std::fclose(f);
+    }
+    char row[512];
+    std::snprintf(row, sizeof row, "%s %d %d %d %d %d %d %d %d %d %.6f\n", kScheduleTag, key[0], key[1], key[2],
+                  key[3], ctx->nlanes, ctx->cluster_blocks, ctx->graph_steps, ctx->d.xcd, ctx->vertex_corner,
+                  ctx->tuned_ms);
+    keep.push_back(row);
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "w");
+    if (!f) return fail(ctx, SVM_ERR_IO, "cannot write " + tmp);
+    bool ok = true;
+    for (const auto& l : keep) ok &= std::fputs(l.c_str(), f) >= 0;
+    ok &= std::fclose(f) == 0;
+    if (!ok || std::rename(tmp.c_str(), path) != 0) return fail(ctx, SVM_ERR_IO, std::string("cannot write ") + path);
+    return SVM_OK;
+}
+
+int svm_tuning_load(svm_context* ctx, int32_t updater, const char* path, int32_t* found) {
+    if (!ctx || !path || !found || updater < 0 || updater > 2) return fail(ctx, SVM_ERR_ARG, "bad argument");
+    *found = 0;
+    const auto key = schedule_key(ctx, updater);
+    FILE* f = std::fopen(path, "r");
+    if (!f) return SVM_OK;
+    char line[512];
+    int sched[5];
+    double ms = 0.0;
+    while (std::fgets(line, sizeof line, f)) {
+        char tag[64];
+        int k[4], v[5];
+        double t = 0.0;
+        if (std::sscanf(line, "%63s %d %d %d %d %d %d %d %d %d %lf", tag, &k[0], &k[1], &k[2], &k[3], &v[0], &v[1],
+                        &v[2], &v[3], &v[4], &t) == 11 &&
+            std::strcmp(tag, kScheduleTag) == 0 && k[0] == key[0] && k[1] == key[1] && k[2] == key[2] &&
+            k[3] == key[3]) {
+            std::memcpy(sched, v, sizeof sched);
+            ms = t;
+            *found = 1;
+        }
+    }
+    std::fclose(f);
+    if (!*found) return SVM_OK;
+    int r = svm_set_lanes(ctx, sched[0]);
+    r = r ? r : svm_set_tuning(ctx, sched[1], sched[2], sched[3]);
+    r = r ? r : svm_set_vertex_form(ctx, sched[4]);
+    if (r) {
+        *found = 0;
+        return r;
+    }
+    ctx->tuned_ms = ms;
+    return SVM_OK;
 }
 
 int svm_copy_probe(int64_t n_bytes, int32_t reps, double* ms_per_copy) {

@@ -470,14 +470,30 @@ int svm_get_lanes(svm_context* ctx, int32_t* n_lanes, int32_t* first_sphere);
  * (src/utilities/kernelTuner.cpp:35-126) but over the whole step: times
  * real steps of `updater` (0 Brownian, 1 active) under each candidate
  * schedule -- first lanes in {1 .. min(SVM_MAX_LANES, spheres)} x
- * cluster_blocks in {256, 1024}, then at the best lane count the rest of
- * cluster_blocks in {128, 256, 512, 1024} x xcd_mapping in {0, 1}, all at 10
- * steps per graph, then graph_steps in {4, 20, 40} -- keeps the fastest and
- * reports every trial in table (rows of 5 doubles: lanes, cluster_blocks,
- * graph_steps, xcd_mapping, ms per step).  The steps advance the model
- * (about steps_per_trial per candidate), exactly as untuned steps would. */
+ * cluster_blocks in {256, 1024} with the thread-per-vertex form; at the best
+ * lane count the corner vertex forms 1 and 2 (FP64 parity forces only); then
+ * the rest of cluster_blocks in {128, 256, 512, 1024} x xcd_mapping in
+ * {0, 1}, all at 10 steps per graph; then graph_steps in {4, 20, 40} -- keeps
+ * the fastest and reports every trial in table (rows of SVM_TUNE_COLS
+ * doubles: lanes, cluster_blocks, graph_steps, xcd_mapping, vertex_form, ms
+ * per step).  The steps advance the model (about steps_per_trial per
+ * candidate), exactly as untuned steps would. */
+#define SVM_TUNE_COLS 6
 int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, double* table, int32_t table_rows,
                  int32_t* rows_written);
+/* Persisted schedules (no counterpart in the reference, whose kernelTuner
+ * keeps its choice in memory for one run): a text file with one line per
+ * key, "svm-schedule-v1 updater precision sphere_bucket size_bucket lanes
+ * cluster_blocks graph_steps xcd_mapping vertex_form ms_per_step", where the
+ * key is the updater, the force precision, floor(log2(number of spheres)) and
+ * floor(log2(mean cells per sphere)) of the context's batch.
+ * svm_tuning_save writes the context's current schedule (and the ms per step
+ * its last svm_autotune measured) under the batch's key, replacing an older
+ * line with that key.  svm_tuning_load applies the schedule stored under the
+ * batch's key, if any (found = 1), before any graph is captured with it; a
+ * missing file or key leaves the schedule unchanged (found = 0, SVM_OK). */
+int svm_tuning_save(svm_context* ctx, int32_t updater, const char* path);
+int svm_tuning_load(svm_context* ctx, int32_t updater, const char* path, int32_t* found);
 
 /* Kernel timing without launch gaps (no counterpart in the reference): the
  * fused vertex pass of `updater` (0 Brownian, 1 active; force + update +

@@ -39,16 +39,20 @@ def test_autotune_steps_are_ordinary_steps(states):
     for i in range(len(states)):
         ctx.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
     tab = ctx.autotune(updater=1, steps_per_trial=30)
-    # lanes 1, 2, 3 (capped at the 3 spheres) x 2 grids, 6 more kernel schedules, 3 graph lengths
-    assert len(tab) == 15 and (tab[:, 4] > 0).all()
+    # lanes 1, 2, 3 (capped at the 3 spheres) x 2 grids, 2 corner vertex forms, 6 more kernel schedules,
+    # 3 graph lengths
+    assert len(tab) == 17 and (tab[:, 5] > 0).all()
     np.testing.assert_array_equal(tab[:6, 0], [1, 1, 2, 2, 3, 3])
-    assert (tab[6:, 0] == tab[np.argmin(tab[:6, 4]), 0]).all()
+    np.testing.assert_array_equal(tab[:6, 4], 0)
+    np.testing.assert_array_equal(tab[6:8, 4], [1, 2])
+    assert (tab[6:, 0] == tab[np.argmin(tab[:6, 5]), 0]).all()
     t = ctx.tuning()
-    best = tab[np.argmin(tab[:, 4])]
-    assert (t["lanes"], t["cluster_blocks"], t["graph_steps"], t["xcd_mapping"]) == tuple(int(x) for x in best[:4])
+    best = tab[np.argmin(tab[:, 5])]
+    assert (t["lanes"], t["cluster_blocks"], t["graph_steps"], t["xcd_mapping"], t["vertex_form"]) == \
+        tuple(int(x) for x in best[:5])
     # the number of steps the tuner took, from the trial table: capture warm-up + 3 samples per candidate
     steps = 0
-    for lanes, cb, G, x, ms in tab:
+    for lanes, cb, G, x, form, ms in tab:
         G = int(G)
         steps += G + 3 * max(1, 30 // (3 * G)) * G
     ref = make_ctx(states, p0=3.6)
@@ -57,3 +61,34 @@ def test_autotune_steps_are_ordinary_steps(states):
     ref.active_steps(steps)
     for i in range(len(states)):
         np.testing.assert_array_equal(ctx.positions(i), ref.positions(i))
+
+
+def test_persisted_schedule_round_trip(states, tmp_path):
+    """A schedule saved under a batch's key is found and applied for a batch
+    of the same key, not for another key, and graphs captured after it run
+    the same trajectory."""
+    path = str(tmp_path / "schedules.txt")
+    a = make_ctx(states, p0=3.6)
+    for i in range(len(states)):
+        a.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
+    assert not a.load_tuning(path)                 # no file yet
+    a.set_lanes(2)
+    a.set_tuning(cluster_blocks=512, graph_steps=4, xcd_mapping=0)
+    a.set_vertex_form(2)
+    a.save_tuning(path)
+    a.save_tuning(path, updater=0)                 # a second key in the same file
+    a.save_tuning(path)                            # replaces its own line
+    lines = open(path).read().splitlines()
+    assert len(lines) == 2 and all(l.startswith("svm-schedule-v1 ") for l in lines)
+    b = make_ctx(states, p0=3.6)
+    for i in range(len(states)):
+        b.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
+    assert b.load_tuning(path)
+    assert b.tuning() == dict(cluster_blocks=512, graph_steps=4, xcd_mapping=0, lanes=2, vertex_form=2)
+    b.set_force_precision(1)                       # another key: FP32 forces
+    assert not b.load_tuning(path)
+    b.set_force_precision(0)
+    b.run_graph(1, 0, 10)
+    _, ref = run(states, 40)
+    for i in range(len(states)):
+        np.testing.assert_array_equal(b.positions(i), ref[i])

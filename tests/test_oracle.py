@@ -449,7 +449,8 @@ def test_library_exports_every_declared_symbol():
     assert sorted(svm.EXPORTS) == names
     # the binding's array widths are the header's (a width mismatch silently mixes spheres' rows)
     for name, value in (("SVM_MAX_LANES", svm.MAX_LANES), ("SVM_COUNTER_COLS", svm.COUNTER_COLS),
-                        ("SVM_FORCE_ERR_COLS", svm.FORCE_ERR_COLS), ("SVM_T1_DIAG_COLS", svm.T1_DIAG_COLS)):
+                        ("SVM_FORCE_ERR_COLS", svm.FORCE_ERR_COLS), ("SVM_T1_DIAG_COLS", svm.T1_DIAG_COLS),
+                        ("SVM_TUNE_COLS", svm.TUNE_COLS), ("SVM_NOALLOC_MAX_POINTS", svm.NOALLOC_MAX_POINTS)):
         assert int(re.search(r"#define %s (\d+)" % name, decl).group(1)) == value, name
 
 
