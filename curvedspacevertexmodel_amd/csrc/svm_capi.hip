@@ -285,11 +285,14 @@ int launch_vertex(svm_context* ctx, int pout, int store_forces, const char* name
     return launch_vertex<MODE>(ctx, whole(ctx), pout, store_forces, name);
 }
 
+// detect_skip: spheres the detection leaves out (k_t1_detect's `skip`; -1:
+// as fire_mode, i.e. the spheres FIRE has stopped when in FIRE mode).
 int launch_t1(svm_context* ctx, const Target& w, int pcur, int nin, int nout, int dir, int advance_step,
-              int fire_mode) {
+              int fire_mode, int detect_skip = -1) {
     {
         Timer t(ctx, "k_t1_detect");
-        hipLaunchKernelGGL(svm::k_t1_detect, dim3(xcd_grid(w.d.vtn)), dim3(SVM_TILE), 0, w.st, w.d, pcur, fire_mode);
+        hipLaunchKernelGGL(svm::k_t1_detect, dim3(xcd_grid(w.d.vtn)), dim3(SVM_TILE), 0, w.st, w.d, pcur,
+                           detect_skip < 0 ? fire_mode : detect_skip);
         int r = last_launch(ctx);
         if (r) return r;
     }
@@ -326,8 +329,9 @@ int launch_t1(svm_context* ctx, const Target& w, int pcur, int nin, int nout, in
                        advance_step, fire_mode, ctx->t1_sequential);
     return last_launch(ctx);
 }
-int launch_t1(svm_context* ctx, int pcur, int nin, int nout, int dir, int advance_step, int fire_mode) {
-    return launch_t1(ctx, whole(ctx), pcur, nin, nout, dir, advance_step, fire_mode);
+int launch_t1(svm_context* ctx, int pcur, int nin, int nout, int dir, int advance_step, int fire_mode,
+              int detect_skip = -1) {
+    return launch_t1(ctx, whole(ctx), pcur, nin, nout, dir, advance_step, fire_mode, detect_skip);
 }
 
 // One dynamics step of the target's spheres: geometry, force+updater+move,
@@ -1903,6 +1907,7 @@ int svm_fire_set_parameters(svm_context* ctx, int32_t sys, const svm_fire_params
     S.f_power = 0;
     S.f_force_max = 100.;
     S.mix_pending = 0;
+    S.f_frozen = 0;   // a minimiser set up afresh has not stopped
     HIPCHECK(ctx, hipMemcpyAsync(ctx->d.sys + sys, &S, sizeof(SysParams), hipMemcpyHostToDevice, ctx->stream));
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     return SVM_OK;
@@ -1980,6 +1985,7 @@ int svm_adam_set_parameters(svm_context* ctx, int32_t sys, const svm_adam_params
     S.a_beta1t = p->beta1;
     S.a_beta2t = p->beta2;
     S.a_iter = 0;
+    S.f_frozen = 0;   // a minimiser set up afresh has not stopped
     S.a_force_max = 110.0;
     const SysParams& L = ctx->layout[sys];
     for (int k = 0; k < 3; ++k) {
@@ -2827,9 +2833,10 @@ int svm_warm_start(svm_context* ctx, svm_mlp* m, double scale, double max_step, 
     hipLaunchKernelGGL(svm::k_warm_disp, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, d, p0, (const float*)dY,
                        (const float*)dF, scale, max_step > 0.0 ? max_step : 0.0);
     const int pout = 1 - p0;
-    hipLaunchKernelGGL(svm::k_move_ext, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, d, p0, pout, -1, 1.0);
+    // converged spheres keep their positions and topology
+    hipLaunchKernelGGL(svm::k_move_ext, dim3(ctx->nvt), dim3(SVM_TILE), 0, ctx->stream, d, p0, pout, -2, 1.0);
     r = last_launch(ctx);
-    if (!r) r = launch_t1(ctx, pout, ctx->ncur, ctx->ncur, 0, 0, 0);
+    if (!r) r = launch_t1(ctx, pout, ctx->ncur, ctx->ncur, 0, 0, 0, 2);
     if (r) return r;
     ctx->pcur = pout;
     // a sphere whose angle-sum areas tile it worse than before (a cell

@@ -49,6 +49,12 @@ __device__ inline int lane_ctile(const Dev& d) {
 
 enum VertexMode { VM_FORCE = 0, VM_BROWNIAN = 1, VM_ACTIVE = 2, VM_FIRE = 3, VM_FORCE_F32 = 4, VM_ADAM = 5 };
 
+// A sphere whose minimiser (FIRE or Adam) has converged: its force statistic
+// at or below the cutoff after at least one iteration.
+__device__ inline bool minimiser_converged(const SysParams* S) {
+    return (S->f_iter > 0 && !(S->f_force_max > S->f_cutoff)) || (S->a_iter > 0 && !(S->a_force_max > S->a_cutoff));
+}
+
 // ------------------------------------------------------------ reductions --
 // Halving tree over the 256 values of a block (s[i] += s[i+h], h = 128..1).
 // The result is valid in every thread.
@@ -911,7 +917,9 @@ __global__ __launch_bounds__(SVM_TILE) void k_fire_finalize(Dev d) {
 }
 
 // External displacement (moveParticles(displacements, scale),
-// src/model/sphericalVertexModel.cpp:111-131) for sphere `only` (-1: all).
+// src/model/sphericalVertexModel.cpp:111-131) for sphere `only` (-1: all;
+// -2: all whose minimiser has not converged).  Other spheres keep their
+// positions bit for bit.
 __global__ __launch_bounds__(SVM_TILE) void k_move_ext(Dev d, int pin, int pout, int only, double scale) {
     const int tile = blockIdx.x;
     const int s = d.vtile_sys[tile];
@@ -919,7 +927,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_move_ext(Dev d, int pin, int pout,
     const int v = tile * SVM_TILE + threadIdx.x;
     if (v - S->vo >= S->nv) return;
     double p[3] = {d.px[pin][v], d.py[pin][v], d.pz[pin][v]};
-    if (only < 0 || only == s) {
+    if (only == -1 || only == s || (only == -2 && !minimiser_converged(S))) {
         double disp[3] = {d.disp[3 * (size_t)v], d.disp[3 * (size_t)v + 1], d.disp[3 * (size_t)v + 2]};
         if (scale != 1.) { disp[0] = scale * disp[0]; disp[1] = scale * disp[1]; disp[2] = scale * disp[2]; }
         move_on_sphere(p, disp, S->R);

@@ -435,7 +435,8 @@ __global__ __launch_bounds__(SVM_TILE) void k_warm_disp(Dev d, int pin, const fl
     if (v >= d.Vs) return;
     const SysParams* S = d.sys + d.vtile_sys[v / SVM_TILE];
     double out[3] = {0.0, 0.0, 0.0};
-    if (v - S->vo < S->nv) {
+    // a sphere whose minimiser has converged keeps its positions
+    if (v - S->vo < S->nv && !minimiser_converged(S)) {
         int4 cl = d.vcn[v];
         int c3[3] = {cl.x, cl.y, cl.z};
         for (int k = 0; k < 3; ++k) {

@@ -877,12 +877,14 @@ __global__ __launch_bounds__(SVM_TILE) void k_build_vln(Dev d, int only) {
 // the bit mask (for the sequential path) and to the compact list (for the dry
 // runs).  The active updater's director step is in the vertex pass; the T1
 // commit redoes it for every vertex a flip moved.
-__global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int skip_frozen) {
+// skip: 1 = spheres a minimiser has stopped (f_frozen), 2 = spheres whose
+// minimiser has converged; 0 = none.
+__global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int skip) {
     const int tile = lane_vtile(d);
     if (tile < 0) return;
     const int s = d.vtile_sys[tile];
     SysParams* S = d.sys + s;
-    if (skip_frozen && S->f_frozen) return;
+    if ((skip == 1 && S->f_frozen) || (skip == 2 && minimiser_converged(S))) return;
     const int v = tile * SVM_TILE + threadIdx.x;
     const int local = v - S->vo;
     const double* X = d.px[pcur];

@@ -374,7 +374,10 @@ int svm_cell_descriptors(svm_context* ctx, int32_t sys, float* X, float* frames)
  * worse afterwards (|sum A / 4 pi R^2 - 1| grown by more than 1e-8: a cell
  * folded) gets its positions and topology back; `reverted` (optional) counts
  * those spheres.
- * The warm start never leaves a sphere worse than plain FIRE would start. */
+ * The warm start never leaves a sphere worse than plain FIRE would start.
+ * Spheres whose minimiser has converged (FIRE or Adam force statistic at or
+ * below its cutoff after at least one iteration) are not moved, so the call
+ * can also correct the unconverged spheres between minimiser calls. */
 int svm_warm_start(svm_context* ctx, svm_mlp* mlp, double scale, double max_step, int32_t* reverted);
 
 /* ---- Voronoi model -------------------------------------------------------

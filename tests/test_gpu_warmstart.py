@@ -67,3 +67,23 @@ def test_warm_started_fire_converges_like_plain_fire():
     both = conv_p & conv_l
     print(f"spheres converged: plain {conv_p.sum()}, learned {conv_l.sum()}; iterations on spheres both converge: "
           f"plain {its_p[both].sum()}, learned {its_l[both].sum()}, ratio {its_p[both].sum() / its_l[both].sum():.3f}")
+
+
+def test_warm_start_leaves_converged_spheres_alone():
+    """Between minimiser calls the warm start moves only the spheres whose
+    FIRE has not converged; a converged sphere keeps its positions and
+    topology, bit for bit."""
+    net, scale = ws.load()
+    params = [(n, p0) for n, p0 in ws.batch_params(24, seed=HELD_OUT_SEED + 2) if p0 < 3.7][:4]
+    ctx = ws.fire_batch(params, seed0=HELD_OUT_SEED + 200, cutoff=1e-10, max_iterations=40000)
+    ctx.fire_set_max_iterations(0, 40000)
+    for i in range(1, len(params)):
+        ctx.fire_set_max_iterations(i, 50)        # stopped early: not converged
+    ctx.fire_minimize(100)
+    assert ctx.fire_state(0)["converged"] and not ctx.fire_state(1)["converged"]
+    before = [(ctx.positions(i), ctx.get_topology(i)) for i in range(len(params))]
+    ctx.warm_start(net, scale)
+    np.testing.assert_array_equal(ctx.positions(0), before[0][0])
+    for a, b in zip(ctx.get_topology(0), before[0][1]):
+        np.testing.assert_array_equal(a, b)
+    assert any(not np.array_equal(ctx.positions(i), before[i][0]) for i in range(1, len(params)))
