@@ -719,7 +719,10 @@ int dl_triangulate_dev(hipStream_t st, DevScratch& m, double* P, int n, double R
     o.tid = m.get<int>((size_t)n * cap);
     o.err = m.get<int>(1);
     int* outcome = m.get<int>(2);
-    if (!o.nbr || !o.deg || !status || !cnt || !o.off || !o.tid || !o.err || !outcome) {
+    double* rad = m.get<double>(n);    // per-star search radius and its bisection bounds
+    double* rlo = m.get<double>(n);
+    double* rhi = m.get<double>(n);
+    if (!o.nbr || !o.deg || !status || !cnt || !o.off || !o.tid || !o.err || !outcome || !rad || !rlo || !rhi) {
         msg = "device allocation failed";
         return SVM_ERR_HIP;
     }
@@ -728,15 +731,19 @@ int dl_triangulate_dev(hipStream_t st, DevScratch& m, double* P, int n, double R
     hipLaunchKernelGGL(svm::k_fill_int, dim3(1), dim3(tb), 0, st, o.err, 1, 0);
     hipLaunchKernelGGL(svm::k_fill_int, blocks(n + 1), dim3(tb), 0, st, cnt, n + 1, 0);
     hipLaunchKernelGGL(svm::k_fill_int, blocks(n), dim3(tb), 0, st, status, n, (int)svm::DL_PENDING);
-    // stars, widening the search for the points that need it
-    // search radii r0, 2 r0, 3 r0 (about 40, 150 and 350 candidates at uniform
-    // density), then the whole sphere when it holds few enough points
-    for (int pass = 0; pass < 4; ++pass) {
-        if (pass == 3 && n - 1 > SVM_DL_MAXC) { msg = "neighbour search did not converge"; return SVM_ERR_TOPOLOGY; }
-        const double r = pass < 3 ? r0 * (1 + pass) : 2.0 * Rs * (1.0 + 1e-9);
+    hipLaunchKernelGGL(svm::k_fill_double, blocks(n), dim3(tb), 0, st, rad, n, r0);
+    hipLaunchKernelGGL(svm::k_fill_double, blocks(n), dim3(tb), 0, st, rlo, n, 0.0);
+    hipLaunchKernelGGL(svm::k_fill_double, blocks(n), dim3(tb), 0, st, rhi, n, 0.0);
+    // stars, each with its own search radius: r0 (about 40 candidates at
+    // uniform density) first, then doubled for the stars that need a wider
+    // search and bisected down for the crowded ones (k_dl_radius)
+    const double whole = 2.0 * Rs * (1.0 + 1e-9);
+    const int max_passes = 40;
+    for (int pass = 0; pass < max_passes; ++pass) {
         hipLaunchKernelGGL(svm::k_dl_star, dim3((n + SVM_DL_WAVES - 1) / SVM_DL_WAVES), dim3(64 * SVM_DL_WAVES), 0, st,
-                           (const double*)o.P, n, g, (const int*)start, (const int*)list, r, Rs, cap, o.nbr, o.deg,
-                           status);
+                           (const double*)o.P, n, g, (const int*)start, (const int*)list, (const double*)rad, Rs,
+                           cap, o.nbr, o.deg, status);
+        hipLaunchKernelGGL(svm::k_dl_radius, blocks(n), dim3(tb), 0, st, status, rad, rlo, rhi, n, whole);
         // the pass's outcome, reduced on the device: {stars to redo, first failed point}
         const int init[2] = {0, n};
         int ho[2] = {0, n};
@@ -769,7 +776,7 @@ int dl_triangulate_dev(hipStream_t st, DevScratch& m, double* P, int n, double R
             return worst == svm::DL_OPEN ? SVM_ERR_RING_OVERFLOW : SVM_ERR_TOPOLOGY;
         }
         if (!retry) break;
-        if (pass == 3) { msg = "neighbour search did not converge"; return SVM_ERR_TOPOLOGY; }
+        if (pass == max_passes - 1) { msg = "neighbour search did not converge"; return SVM_ERR_TOPOLOGY; }
     }
     // canonical triangles
     hipLaunchKernelGGL(svm::k_dl_tri_count, blocks(n), dim3(tb), 0, st, n, cap, (const int*)o.nbr,

@@ -42,8 +42,8 @@ namespace svm {
 
 enum {
     DL_OK = 0,
-    DL_RETRY = 1,        // circumcircle not inside the search ball: redo with a larger radius
-    DL_CROWDED = 2,      // more than SVM_DL_MAXC candidates
+    DL_RETRY = 1,        // circumcircle not inside the search ball: redo with another radius
+    DL_CROWDED = 2,      // more than SVM_DL_MAXC candidates at every radius that could prove the star
     DL_OPEN = 3,         // the star did not close within the ring capacity
     DL_INVALID = 4,      // a facet failed the exact check
     DL_DEGENERATE = 5,   // four co-circular points
@@ -144,8 +144,8 @@ __device__ inline bool dl_beats(const double* pi, const double* pj, const double
 // point k.  At least 4 waves per SIMD (128 VGPRs): measured faster than the
 // compiler's own choice of 132 registers and 3 waves.
 __global__ __launch_bounds__(64 * SVM_DL_WAVES, 4) void k_dl_star(const double* P, int n, DlGrid g, const int* start,
-                                                               const int* list, double r, double R, int cap,
-                                                               int* nbr, int* deg, int* status) {
+                                                               const int* list, const double* rad, double R,
+                                                               int cap, int* nbr, int* deg, int* status) {
     __shared__ int s_idx[SVM_DL_WAVES][SVM_DL_MAXC];
     __shared__ int s_ring[SVM_DL_WAVES][64];
     __shared__ int s_bstart[SVM_DL_WAVES][64];   // first entry of each bin of a chunk
@@ -155,6 +155,7 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES, 4) void k_dl_star(const double* 
     if (i >= n) return;
     const int st0 = status[i];
     if (st0 != DL_PENDING && st0 != DL_RETRY) return;
+    const double r = rad[i];   // this star's search radius
     double pi[3];
     dl_load(P, i, pi);
     // ---- gather the candidates within chord distance r.  Binning is monotone
@@ -437,6 +438,35 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES, 4) void k_dl_star(const double* 
     for (int s = lane; s < d; s += 64) nbr[(size_t)i * cap + s] = ring[s];
 }
 
+// Search radius of every star for the next pass, from this pass's status:
+// a star that must search wider (DL_RETRY) doubles its radius, or bisects
+// between the largest radius that was too small and the smallest one that
+// held too many candidates; a crowded star (DL_CROWDED) bisects down the same
+// way and is redone, until the two bounds are within a thousandth of each
+// other (then it stays DL_CROWDED: no radius both holds SVM_DL_MAXC
+// candidates and proves the star).  Radii never exceed `whole` (a ball that
+// covers the sphere).
+__global__ __launch_bounds__(256) void k_dl_radius(int* status, double* rad, double* rlo, double* rhi, int n,
+                                                   double whole) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int st = status[i];
+    const double r = rad[i];
+    if (st == DL_RETRY) {
+        rlo[i] = r;
+        const double hi = rhi[i];
+        const double nr = hi > 0.0 ? 0.5 * (r + hi) : 2.0 * r;
+        rad[i] = nr < whole ? nr : whole;
+    } else if (st == DL_CROWDED) {
+        rhi[i] = r;
+        const double lo = rlo[i];
+        if (r - lo > 1e-3 * r) {
+            rad[i] = lo > 0.0 ? 0.5 * (lo + r) : 0.5 * r;
+            status[i] = DL_RETRY;
+        }
+    }
+}
+
 // Outcome of a star pass for the host: out[0] = stars to redo wider,
 // out[1] = smallest point whose star failed (n if none).  out = {0, n} on entry.
 __global__ __launch_bounds__(256) void k_dl_status(const int* status, int n, int* out) {
@@ -448,6 +478,11 @@ __global__ __launch_bounds__(256) void k_dl_status(const int* status, int n, int
 }
 
 __global__ __launch_bounds__(256) void k_fill_int(int* a, int n, int value) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = value;
+}
+
+__global__ __launch_bounds__(256) void k_fill_double(double* a, int n, double value) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = value;
 }

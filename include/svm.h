@@ -119,8 +119,14 @@ int svm_build_from_cell_points(svm_context* ctx, int32_t sys, const double* cell
  * predicates.  Input whose triangulation is not unique (four co-circular
  * points on an empty circle) is rejected (SVM_ERR_TOPOLOGY); so may be input
  * with other exactly co-circular quadruples near a point, when the exact
- * tournament compares them.  triangles: int32[3*(2N-4)], counter-clockwise seen from outside, smallest
- * index first, sorted lexicographically. */
+ * tournament compares them.  Each point searches its neighbours within a
+ * radius of its own: 3.5 mean spacings first, doubled where the star cannot
+ * be proven within it (voids), bisected down where it holds more than 512
+ * candidates (dense clusters); input for which no radius does both (a point
+ * with more than 512 others inside the smallest ball that proves its star)
+ * is rejected with SVM_ERR_TOPOLOGY.  triangles: int32[3*(2N-4)],
+ * counter-clockwise seen from outside, smallest index first, sorted
+ * lexicographically. */
 int svm_delaunay_triangles(int32_t n_points, const double* points, int32_t* triangles);
 /* svm_delaunay_triangles without any device allocation.  No counterpart in
  * the reference: it is for callers that cannot allocate device memory (code

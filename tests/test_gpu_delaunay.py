@@ -141,3 +141,50 @@ def test_nearly_cocircular_points_are_triangulated_exactly(seed, eps):
     tris = svm.delaunay_triangles(pts)
     assert tris.shape == (2 * len(pts) - 4, 3)
     assert svmtest.locally_delaunay_violations(pts, tris, exact=True) == []
+
+
+def clustered_points(n, crowd, spread, seed=4):
+    rng = np.random.default_rng(seed)
+    R = np.sqrt(n / (4 * np.pi))
+    v = rng.normal(size=(n, 3))
+    v[:crowd] = np.array([0.0, 0.0, 1.0]) + spread * v[:crowd]     # a dense cap near the pole
+    return R * v / np.linalg.norm(v, axis=1)[:, None]
+
+
+@pytest.mark.parametrize("n,crowd,spread", [(2000, 1000, 0.15), (4000, 2000, 0.12)])
+def test_dense_cap_beside_a_sparse_rest(n, crowd, spread):
+    """Half the points in a dense polar cap: stars there hold 850 to 1,500
+    candidates within the first search radius, beyond the 512-candidate
+    capacity, so their radius is bisected down until the star fits and is
+    proven (at most about 300 candidates lie inside each star's proving ball,
+    checked on the CPU).  The triangulation equals Qhull's."""
+    P = clustered_points(n, crowd, spread)
+    np.testing.assert_array_equal(svm.delaunay_triangles(P), svmtest.qhull_triangles(P))
+
+
+def test_cluster_beyond_the_search_capacity_is_rejected():
+    """500 of 700 points in a tiny cap (about 450 times the mean density): the
+    smallest ball that proves a star on the cap's rim holds more than 512
+    points, so no search radius works; the input is rejected cleanly."""
+    with pytest.raises(svm.SVMError) as e:
+        svm.delaunay_triangles(clustered_points(700, 500, 0.08))
+    assert e.value.code == 5
+
+
+def test_large_void_needs_a_wide_search():
+    """2,000 points with an empty polar cap of half-angle 0.42 rad: stars on its
+    rim need search radii up to 3.3 r0 (at most 358 candidates inside the
+    proving ball, checked on the CPU), on a sphere with far more than 512
+    points, so no whole-sphere search can stand in.  The triangulation equals
+    Qhull's."""
+    rng = np.random.default_rng(11)
+    n = 2000
+    R = np.sqrt(n / (4 * np.pi))
+    pts = []
+    while len(pts) < n:
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        if v[2] < np.cos(0.42):
+            pts.append(R * v)
+    P = np.array(pts)
+    np.testing.assert_array_equal(svm.delaunay_triangles(P), svmtest.qhull_triangles(P))
