@@ -11,13 +11,16 @@
 // forward pass; activations feeding each layer are rounded to bf16, products
 // accumulate in FP32.
 //
-// MFMA mapping (v_mfma_f32_16x16x16_bf16, one wavefront per 16 cells): the
-// weights are the A operand (16 output features x 16 inputs) and the
-// activations the B operand (16 inputs x 16 cells).  Lane l holds
-// A[l%16][4(l/16)+j], B[4(l/16)+j][l%16] and D[4(l/16)+i][l%16], so the four
-// outputs a lane gets for an output block are exactly the four inputs it
-// supplies for the matching input block of the next layer: layers chain in
-// registers with no data exchange.
+// MFMA mapping (one wavefront per 16 cells): the weights are the A operand
+// (16 output features x K inputs) and the activations the B operand (K
+// inputs x 16 cells), with D[4(l/16)+i][l%16] in lane l.  The first layer
+// (16 padded inputs) is one v_mfma_f32_16x16x16_bf16 per output block, with
+// lane l holding A[l%16][4(l/16)+j] and B[4(l/16)+j][l%16].  The 64-wide
+// layers use gfx950's v_mfma_f32_16x16x32_bf16 (K = 32, eight values per lane
+// and operand): a lane supplies the four outputs it holds of each of two
+// consecutive 16-row blocks of the previous layer, and the weights are loaded
+// in that order (mlp_ldw32), so the layers chain in registers with no data
+// exchange and half as many instructions as with K = 16.
 #pragma once
 #include "svm_device.h"
 
@@ -85,6 +88,28 @@ __device__ inline mlp_v4s mlp_ldw(const unsigned short* W, int in_dim, int ob, i
     r[0] = (short)p[0]; r[1] = (short)p[1]; r[2] = (short)p[2]; r[3] = (short)p[3];
     return r;
 }
+// Operands of gfx950's v_mfma_f32_16x16x32_bf16 for a 64-wide layer: K = 32
+// inputs per instruction.  A lane of group q holds, from the previous layer,
+// outputs q*4 .. q*4+3 of every 16-row block; it gives the instruction those
+// of blocks 2*kb2 and 2*kb2+1 (eight values), and the weights are loaded in
+// the same order, so the instruction's K slots are a fixed permutation of the
+// inputs, the same for both operands.
+typedef short mlp_v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 mlp_v8bf __attribute__((ext_vector_type(8)));
+__device__ inline mlp_v8s mlp_ldw32(const unsigned short* W, int in_dim, int ob, int kb2, int lane) {
+    const int row = ob * 16 + (lane & 15), q = lane >> 4;
+    const unsigned short* p = W + (size_t)row * in_dim + kb2 * 32 + q * 4;
+    mlp_v8s r;
+    r[0] = (short)p[0]; r[1] = (short)p[1]; r[2] = (short)p[2]; r[3] = (short)p[3];
+    r[4] = (short)p[16]; r[5] = (short)p[17]; r[6] = (short)p[18]; r[7] = (short)p[19];
+    return r;
+}
+__device__ inline mlp_v4f mlp_mfma32(const mlp_v8s& w, const mlp_v4s& lo, const mlp_v4s& hi, mlp_v4f acc) {
+    const mlp_v8s b = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mlp_v8bf, w), __builtin_bit_cast(mlp_v8bf, b),
+                                                   acc, 0, 0, 0);
+}
+
 // Four floats to the bf16 B operand of the MFMA, round to nearest even (as
 // f2bf for finite values), two per v_cvt_pk_bf16_f32.
 typedef float mlp_v2f __attribute__((ext_vector_type(2)));
@@ -105,17 +130,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                      float* H) {
     const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6), nwaves = gridDim.x * 4;
-    mlp_v4s w1[4], w2[4][4], w3[4][4], w4[4];
+    mlp_v4s w1[4];
+    mlp_v8s w2[4][2], w3[4][2], w4[2];
 #pragma unroll
     for (int ob = 0; ob < 4; ++ob) {
         w1[ob] = mlp_ldw(m.w1, 16, ob, 0, lane);
-        w4[ob] = mlp_ldw(m.w4, 64, 0, ob, lane);
 #pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-            w2[ob][kb] = mlp_ldw(m.w2, 64, ob, kb, lane);
-            w3[ob][kb] = mlp_ldw(m.w3, 64, ob, kb, lane);
+        for (int kb = 0; kb < 2; ++kb) {
+            w2[ob][kb] = mlp_ldw32(m.w2, 64, ob, kb, lane);
+            w3[ob][kb] = mlp_ldw32(m.w3, 64, ob, kb, lane);
         }
     }
+    w4[0] = mlp_ldw32(m.w4, 64, 0, 0, lane);
+    w4[1] = mlp_ldw32(m.w4, 64, 0, 1, lane);
     float b1[4][4], b2[4][4], b3[4][4], b4[4], mu[4], is[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -158,8 +185,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
         for (int ob = 0; ob < 4; ++ob) {
             mlp_v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int kb = 0; kb < 4; ++kb) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w2[ob][kb], h1[kb], acc, 0, 0, 0);
+            acc = mlp_mfma32(w2[ob][0], h1[0], h1[1], acc);
+            acc = mlp_mfma32(w2[ob][1], h1[2], h1[3], acc);
             float z[4], a[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) { z[i] = acc[i] + b2[ob][i]; a[i] = silu(z[i]); }
@@ -174,8 +201,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
         for (int ob = 0; ob < 4; ++ob) {
             mlp_v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int kb = 0; kb < 4; ++kb) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w3[ob][kb], h2[kb], acc, 0, 0, 0);
+            acc = mlp_mfma32(w3[ob][0], h2[0], h2[1], acc);
+            acc = mlp_mfma32(w3[ob][1], h2[2], h2[3], acc);
             float z[4], a[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) { z[i] = acc[i] + b3[ob][i]; a[i] = silu(z[i]); }
@@ -188,8 +215,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 }
         }
         mlp_v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kb = 0; kb < 4; ++kb) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w4[kb], h3[kb], acc, 0, 0, 0);
+        acc = mlp_mfma32(w4[0], h3[0], h3[1], acc);
+        acc = mlp_mfma32(w4[1], h3[2], h3[3], acc);
         if (q == 0 && valid) {
             Y[(size_t)cell * 2] = m.out_scale * (acc[0] + b4[0]);
             Y[(size_t)cell * 2 + 1] = m.out_scale * (acc[1] + b4[1]);

@@ -121,14 +121,12 @@ __device__ inline void dl_load(const double* P, int i, double* p) {
 
 // Does candidate x beat candidate k as the wrap point after edge (pi, pj)?
 // Exactly: x lies beyond the plane (pi, pj, pk).  A tie (co-planar) is
-// broken by index and reported.
-__device__ inline bool dl_beats(const double* pi, const double* pj, const double* pk, int k, const double* px, int x,
-                                bool& degenerate) {
+// broken by index: if both lose to a third candidate the tie does not matter,
+// and if they win, their facet's circle holds four points, which the exact
+// check of every facet against every candidate reports as degenerate input.
+__device__ inline bool dl_beats(const double* pi, const double* pj, const double* pk, int k, const double* px, int x) {
     int o = orient3d_sign(pi, pj, pk, px);
-    if (o == 0) {
-        degenerate = true;
-        return x < k;
-    }
+    if (o == 0) return x < k;
     return o > 0;
 }
 
@@ -344,7 +342,7 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES, 4) void k_dl_star(const double* 
             if (q == cur) continue;
             double pq[3];
             cand_point(c, pq);
-            if (bk < 0 || dl_beats(pi, pc, pk, bk, pq, q, degenerate)) {
+            if (bk < 0 || dl_beats(pi, pc, pk, bk, pq, q)) {
                 bk = q;
                 pk[0] = pq[0]; pk[1] = pq[1]; pk[2] = pq[2];
             }
@@ -359,8 +357,8 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES, 4) void k_dl_star(const double* 
             else if (ok == bk) take = false;
             else {
                 // both lanes of the pair decide the same way: compare with the smaller index first
-                if (bk < ok) take = dl_beats(pi, pc, pk, bk, po, ok, degenerate);
-                else take = !dl_beats(pi, pc, po, ok, pk, bk, degenerate);
+                if (bk < ok) take = dl_beats(pi, pc, pk, bk, po, ok);
+                else take = !dl_beats(pi, pc, po, ok, pk, bk);
             }
             if (take) { bk = ok; pk[0] = o0; pk[1] = o1; pk[2] = o2; }
         }

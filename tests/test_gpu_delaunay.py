@@ -188,3 +188,23 @@ def test_large_void_needs_a_wide_search():
             pts.append(R * v)
     P = np.array(pts)
     np.testing.assert_array_equal(svm.delaunay_triangles(P), svmtest.qhull_triangles(P))
+
+
+def test_cocircular_quadruple_on_a_nonempty_circle_is_triangulated():
+    """Four exactly co-planar (so co-circular) points (+-s, +-s, h) whose
+    circle holds another point, the pole (0, 0, R), inside its cap: the
+    triangulation is unique (the four never share a Delaunay facet), so exact
+    ties among them in the wrap's comparisons must not reject the input.  The
+    triangulation equals Qhull's and passes the exact empty-circumcircle test."""
+    rng = np.random.default_rng(21)
+    n = 400
+    R = np.sqrt(n / (4 * np.pi))
+    s, h = 1.5, np.sqrt(R * R - 2 * 1.5 * 1.5)
+    v = rng.normal(size=(n - 5, 3))
+    v = R * v / np.linalg.norm(v, axis=1)[:, None]
+    v = v[v[:, 2] < h - 1.0][: n - 5]        # keep the cap above the square's plane empty but for the pole
+    square = np.array([[s, s, h], [-s, s, h], [-s, -s, h], [s, -s, h]])
+    P = np.concatenate([square, [[0.0, 0.0, R]], v])
+    tris = svm.delaunay_triangles(P)
+    assert svmtest.locally_delaunay_violations(P, tris, exact=True) == []
+    np.testing.assert_array_equal(tris, svmtest.qhull_triangles(P))
