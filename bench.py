@@ -49,7 +49,7 @@ def batch_params(n_sys, seed=13377, nmin=200, nmax=20000):
     return out
 
 
-def stabilise(ctx, params, steps):
+def stabilise(ctx, params, steps, eager=False):
     """The reference driver's stabilisation (sphericalVertex.cpp:100-110):
     Brownian dynamics at T=0 with (A0, P0) = (1, 1), then the target (1, p0).
     From raw random Voronoi starts the dynamics otherwise squeeze small cells
@@ -57,7 +57,10 @@ def stabilise(ctx, params, steps):
     for i in range(len(params)):
         ctx.set_preferred_parameters(i, 1.0, 1.0)
         ctx.set_brownian(i, dt=0.01, mu=1.0, T=0.0)
-    ctx.run_graph(0, 10, steps // 10)
+    if eager:
+        ctx.brownian_steps(steps)
+    else:
+        ctx.run_graph(0, 10, steps // 10)
     ctx.synchronize()
     for i, (n, p0, v0, dr) in enumerate(params):
         ctx.set_preferred_parameters(i, 1.0, p0)
@@ -396,6 +399,8 @@ def main():
     ap.add_argument("--autotune", type=int, default=1,
                     help="0: default schedule; 1: the schedule stored for this batch in --schedules, else "
                          "svm_autotune (and store its choice); 2: always svm_autotune, then store")
+    ap.add_argument("--eager", action="store_true",
+                    help="enqueue every step's kernels directly (no HIP graphs), e.g. under a kernel tracer")
     ap.add_argument("--init-steps", type=int, default=1000,
                     help="untimed active steps after the stabilisation and the schedule choice (the reference "
                          "driver's initialisation phase)")
@@ -426,12 +431,12 @@ def main():
     t_setup = time.perf_counter() - t_setup
     K, W = max(1, args.steps), max(0, args.warmup)
     print(f"setup {t_setup:.1f} s: {len(params)} spheres, {total_cells} cells", flush=True)
-    stabilise(ctx, params, 200)
+    stabilise(ctx, params, 200, eager=args.eager)
     print("stabilised (200 Brownian steps)", flush=True)
     tune_table, schedule_source, tune_steps = None, "default", 0
     if args.autotune == 1 and ctx.load_tuning(args.schedules, updater=1):
         schedule_source = "table"
-    elif args.autotune:
+    elif args.autotune and not args.eager:   # the tuner's trials replay graphs
         tune_table = ctx.autotune(updater=1, steps_per_trial=200)
         ctx.save_tuning(args.schedules, updater=1)
         schedule_source = "autotuned"
@@ -451,24 +456,32 @@ def main():
     # the timed graph in both ping-pong parities (one eager step between), so
     # that the timed replays are never a graph's first launch, whatever W is.
     half = max(1, args.init_steps // (2 * G))
-    ctx.run_graph(1, G, half)
-    ctx.active_steps(1)
-    ctx.run_graph(1, G, half)
+    if args.eager:
+        ctx.active_steps(2 * half * G + 1)
+    else:
+        ctx.run_graph(1, G, half)
+        ctx.active_steps(1)
+        ctx.run_graph(1, G, half)
     ctx.synchronize()
     init_steps = 2 * half * G + 1
     print(f"initialised ({init_steps} active steps)", flush=True)
     # exactly W warm-up steps: the remainder eagerly, then replays of the timed graph
-    if W % G:
-        ctx.active_steps(W % G)
-    if W >= G:
-        ctx.run_graph(1, G, W // G)
+    if args.eager:
+        ctx.active_steps(W)
+    else:
+        if W % G:
+            ctx.active_steps(W % G)
+        if W >= G:
+            ctx.run_graph(1, G, W // G)
     ctx.synchronize()
     print(f"warmup done ({W} steps)", flush=True)
     base = (ctx.counters(), ctx.t1_diagnostics())
     t0 = time.perf_counter()
-    if Kg:
+    if args.eager:
+        ctx.active_steps(K)
+    elif Kg:
         ctx.run_graph(1, G, Kg // G)
-    if K % 2:
+    if K % 2 and not args.eager:
         ctx.active_steps(1)
     ctx.synchronize()
     t1 = time.perf_counter()
@@ -495,7 +508,8 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "batched-active-256", "spheres": len(params), "total_cells": total_cells,
                    "N_range": [200, 20000], "p0_range": [3.5, 4.0], "v0_range": [0.01, 0.5],
-                   "Dr": [0.1, 1, 10], "dt": 0.01, "steps_per_graph": G, "schedule": ctx.tuning(),
+                   "Dr": [0.1, 1, 10], "dt": 0.01, "steps_per_graph": None if args.eager else G,
+                   "schedule": ctx.tuning(),
                    "schedule_source": schedule_source, "schedule_table": schedule_table(args.schedules),
                    "untimed_steps_before_warmup": {"stabilisation_brownian": 200, "autotune_active": tune_steps,
                                                    "initialisation_active": init_steps},

@@ -583,13 +583,11 @@ int require_topology(svm_context* ctx) {
 // ======================================================================== API
 namespace {
 
-// Device scratch of one triangulation, freed when the holder goes.
-// Device buffers handed out in call order.  rewind() starts the order again
-// and later gets reuse the buffers of the same position when they are large
-// enough, so a loop that repeats the same sequence of gets allocates only on
-// its first pass (or when a size grows).
 // Device work buffers of one call.  By default each get() is a hipMalloc,
-// kept (and reused by a rewound cursor) until the object ends.  In arena mode
+// kept until the object ends; rewind() starts the order again, and later gets
+// reuse the buffers of the same position when they are large enough, so a
+// loop that repeats the same sequence of gets allocates only on its first
+// pass (or when a size grows).  In arena mode
 // (constructed over caller-provided device memory) get() hands out 256-byte
 // aligned ranges of that memory and never allocates; a request beyond its end
 // returns nullptr and sets `overflow`.
