@@ -382,6 +382,156 @@ def fire_comparison(svm, n_sys=1024, cap=10000, lanes=1):
                             "peak_tflops": 2500.0, "frac": 18176.0 * total / (ms * 1e-3) / 1e12 / 2500.0}}
 
 
+def _oracle_state(ctx, i):
+    """The topology, positions and directors of GPU sphere i, in the layout the oracle takes."""
+    vn, vcn, ring, cnn = ctx.get_topology(i, 32)
+    return dict(vpos=ctx.positions(i), vn=vn, vcn=vcn, ring=ring, cnn=cnn, directors=ctx.directors(i))
+
+
+def cpu_baseline_configs(svm, budget_s=10.0, threads=None):
+    """The CPU oracle (the reference's CPU path restated; single-threaded like
+    the reference, or one thread per independent sphere) on the host cores, on
+    bounded samples of configurations 1, 2, 4 and 5, from states built on the
+    GPU by the same recipes as the GPU figures; and the libm build of the
+    oracle beside the default one (fdlibm-style acos/sin/cos/log shared with
+    the kernels) on a sample of configuration 3."""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import svmtest
+    from curvedspacevertexmodel_amd import warmstart as ws
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
+    out = {"host_cpu": host_cpu()}
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
+    # configuration 1: N = 512, active steps, then FIRE (single thread)
+    ctx = svm.Context([dict(n_cells=512, a0=1.0, p0=1.0, kr=1.0, t1_threshold=0.1, seed=1137)])
+    pts = svm.generate_cell_points(512, 1137)
+    ctx.build_from_cell_points(0, pts[svm.spatial_order(pts)])
+    ctx.set_directors(0, svm.generate_directors(ctx.positions(0), 1137))
+    om = svmtest.OracleModel(_oracle_state(ctx, 0), p0=1.0)
+    om.brownian(200, dt=0.01, mu=1.0, T=0.0, seed=1137, step0=0)
+    om.set_preferences(np.tile([1.0, 3.8], 512))
+    t = timed(lambda: om.active(1000, dt=0.01, mu=1.0, v0=0.1, tau=1.0, seed=1137, step0=200))
+    res = {}
+    t_f = timed(lambda: res.update(om.fire(**ws.FIRE, force_cutoff=1e-12, max_iterations=2000, stop_rms=1)))
+    out["config1_single_thread"] = {"active_cell_steps_per_s": 512 * 1000 / t,
+                                    "fire_cell_iterations_per_s": 512 * res["iterations"] / t_f,
+                                    "sample": "1000 active steps after 200 Brownian; FIRE to 2000 iterations"}
+    del ctx
+    # configuration 2: N = 262,144, Brownian T = 0.001 (single thread, steady-state window)
+    n = 262144
+    ctx = svm.Context([dict(n_cells=n, a0=1.0, p0=1.0, kr=1.0, t1_threshold=0.1, seed=1137)])
+    pts = svm.generate_cell_points(n, 1137)
+    ctx.build_from_cell_points(0, pts[svm.spatial_order(pts)])
+    ctx.set_directors(0, svm.generate_directors(ctx.positions(0), 1137))
+    ctx.set_brownian(0, dt=0.01, mu=1.0, T=0.0)
+    ctx.run_graph(0, 10, 10)
+    om = svmtest.OracleModel(_oracle_state(ctx, 0), p0=3.8)
+    om.brownian(5, dt=0.01, mu=1.0, T=0.001, seed=1137, step0=100)
+    t = timed(lambda: om.brownian(20, dt=0.01, mu=1.0, T=0.001, seed=1137, step0=105))
+    out["config2_single_thread"] = {"cell_steps_per_s": n * 20 / t,
+                                    "sample": "20 Brownian steps (T 0.001) after 5, from the GPU-stabilised state"}
+    del ctx, om
+    # configuration 4: FIRE on the 1,024-sphere batch, one thread per group of spheres
+    params = ws.batch_params(1024)
+    ctx = ws.fire_batch(params[:4 * threads], seed0=1137, cutoff=1e-10, max_iterations=10000)
+    models = [svmtest.OracleModel(_oracle_state(ctx, i), p0=p0) for i, (n_, p0) in enumerate(params[:4 * threads])]
+    del ctx
+
+    def fire_group(k):
+        done = 0
+        t0 = time.perf_counter()
+        for i in range(k, len(models), threads):
+            if time.perf_counter() - t0 > budget_s:
+                break
+            r = models[i].fire(**ws.FIRE, force_cutoff=1e-10, max_iterations=1500, stop_rms=1)
+            done += params[i][0] * r["iterations"]
+        return done
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        work = sum(ex.map(fire_group, range(threads)))
+    out["config4_all_core"] = {"cell_iterations_per_s": work / (time.perf_counter() - t0), "threads": threads,
+                               "sample": f"plain FIRE (rms stop 1e-10, the batch's parameters) on up to "
+                                         f"{4 * threads} spheres of the batch, 1500 iterations each at most"}
+    del models
+    # configuration 5: one sphere of the 50k-500k batch, active steps (single thread, FP64 forces)
+    lp = ws.batch_params(32, seed=13377, nmin=50000, nmax=500000, p0_range=(3.7, 3.85))
+    i5 = int(np.argmin([n_ for n_, _ in lp]))
+    n5, p5 = lp[i5]
+    ctx = svm.Context([dict(n_cells=n5, a0=1.0, p0=1.0, kr=1.0, t1_threshold=0.1, seed=1137 + i5)])
+    pts = svm.generate_cell_points(n5, 1137 + i5)
+    ctx.build_from_cell_points(0, pts[svm.spatial_order(pts)])
+    ctx.set_directors(0, svm.generate_directors(ctx.positions(0), 1137 + i5))
+    ctx.set_brownian(0, dt=0.01, mu=1.0, T=0.0)
+    ctx.run_graph(0, 20, 5)
+    om = svmtest.OracleModel(_oracle_state(ctx, 0), p0=p5)
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        om.active(5, dt=0.01, mu=1.0, v0=0.05, tau=1.0, seed=1137 + i5, step0=100 + steps)
+        steps += 5
+    out["config5_single_thread"] = {"cell_steps_per_s": n5 * steps / (time.perf_counter() - t0),
+                                    "sample": f"sphere {i5} of the batch (N {n5}), {steps} active steps, "
+                                              f"FP64 forces (the reference's CPU arithmetic)"}
+    del ctx, om
+    # the oracle built against glibc's acos/sin/cos/log beside the default build (configuration 3 sample)
+    bp = batch_params(256)
+    ctx, _ = build_batch(svm, bp[:8])
+    stabilise(ctx, bp[:8], 200)
+    rates = {}
+    for libm in (False, True):
+        ms = [svmtest.OracleModel(_oracle_state(ctx, i), p0=bp[i][1], libm=libm) for i in range(8)]
+        cells = sum(p[0] for p in bp[:8])
+        t = timed(lambda: [m.active(20, dt=0.01, mu=1.0, v0=bp[i][2], tau=1.0 / bp[i][3], seed=1137 + i,
+                                    step0=200) for i, m in enumerate(ms)])
+        rates["libm" if libm else "own_math"] = cells * 20 / t
+    out["config3_libm_vs_own_math_single_thread"] = {
+        **rates, "sample": "spheres 0-7 of the batch, 20 active steps each after the stabilisation"}
+    return out
+
+
+def gpu_configs(svm, fire_cap=10000):
+    """The GPU figures of configurations 1, 4 (cell-iterations/s of plain FIRE)
+    and 5 to put beside cpu_baseline_configs (configuration 2 is `single`)."""
+    from curvedspacevertexmodel_amd import warmstart as ws
+    out = {}
+    ctx = svm.Context([dict(n_cells=512, a0=1.0, p0=1.0, kr=1.0, t1_threshold=0.1, seed=1137)])
+    pts = svm.generate_cell_points(512, 1137)
+    ctx.build_from_cell_points(0, pts[svm.spatial_order(pts)])
+    ctx.set_directors(0, svm.generate_directors(ctx.positions(0), 1137))
+    ctx.set_brownian(0, dt=0.01, mu=1.0, T=0.0)
+    ctx.run_graph(0, 10, 20)
+    ctx.set_preferred_parameters(0, 1.0, 3.8)
+    ctx.set_active(0, dt=0.01, mu=1.0, v0=0.1, tau=1.0)
+    ctx.run_graph(1, 10, 1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.run_graph(1, 10, 99)
+    ctx.synchronize()
+    t_a = time.perf_counter() - t0
+    ctx.fire_set_parameters(0, max_iterations=20000, force_cutoff=1e-12, stop_rms=1, **ws.FIRE)
+    t0 = time.perf_counter()
+    ctx.fire_minimize(100)
+    ctx.synchronize()
+    t_f = time.perf_counter() - t0
+    st = ctx.fire_state(0)
+    out["config1"] = {"active_cell_steps_per_s": 512 * 1000 / t_a, "fire_cell_iterations_per_s":
+                      512 * st["iterations"] / t_f, "fire_iterations": st["iterations"],
+                      "fire_rms_force": st["force_max"], "fire_wall_s": t_f}
+    del ctx
+    params = ws.batch_params(1024)
+    ctx = ws.fire_batch(params, seed0=1137, cutoff=1e-10, max_iterations=fire_cap)
+    ctx.set_lanes(3)
+    wall, its, conv, _ = ws.run_fire(ctx, len(params))
+    out["config4"] = {"fire_cell_iterations_per_s": float(sum(n * i for (n, _), i in zip(params, its)) / wall),
+                      "wall_s": wall, "iteration_cap": fire_cap, "converged_fraction": float(conv.mean())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -408,6 +558,9 @@ def main():
                     help="persisted schedule table (svm_tuning_save / svm_tuning_load)")
     ap.add_argument("--fire-spheres", type=int, default=1024)
     ap.add_argument("--fire-cap", type=int, default=10000)
+    ap.add_argument("--all-configs", action="store_true",
+                    help="only the per-configuration GPU figures and CPU baselines (configurations 1, 2, 4, 5 "
+                         "and the libm oracle build): one JSON line, then exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, 3 launches of a plain 256 MiB -> 256 MiB FP64 copy kernel "
                          "(calibrates the HBM byte counters of a profiler)")
@@ -418,6 +571,11 @@ def main():
         sys.exit(2)
     import curvedspacevertexmodel_amd as svm
 
+    if args.all_configs:
+        out = {"gpu": gpu_configs(svm), "gpu_config2": single_sphere(svm),
+               "gpu_config5": large_batch(svm, schedules=args.schedules), "cpu": cpu_baseline_configs(svm)}
+        print(json.dumps(out), flush=True)
+        return
     if args.voronoi:
         n = 262144
         pts = svm.generate_cell_points(n, 1137)

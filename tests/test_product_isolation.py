@@ -30,11 +30,12 @@ def _package_py():
 
 def _offences(tree, allowed_functions=()):
     """Imports of checker modules and string constants that name a checker
-    path component, outside the functions named in allowed_functions."""
+    path component, outside the functions whose names start with one of
+    allowed_functions."""
     bad = []
 
     def visit(node, allowed):
-        if isinstance(node, (ast.FunctionDef, ast.AsyncFunctionDef)) and node.name in allowed_functions:
+        if isinstance(node, (ast.FunctionDef, ast.AsyncFunctionDef)) and node.name.startswith(allowed_functions):
             allowed = True
         if not allowed:
             if isinstance(node, ast.Import):
@@ -65,10 +66,10 @@ def test_package_sources_do_not_reach_the_checker():
     assert not bad, f"product modules reach oracle/ or tests/: {bad}"
 
 
-def test_bench_uses_the_checker_only_in_its_cpu_baseline():
+def test_bench_uses_the_checker_only_in_its_cpu_baselines():
     p = os.path.join(ROOT, "bench.py")
     bad = _offences(ast.parse(open(p).read(), p), allowed_functions=("cpu_baseline",))
-    assert not bad, f"bench.py reaches oracle/ or tests/ outside cpu_baseline: {bad}"
+    assert not bad, f"bench.py reaches oracle/ or tests/ outside its cpu_baseline functions: {bad}"
 
 
 def test_native_sources_do_not_include_the_checker():
