@@ -13,7 +13,7 @@ namespace svm {
 // put on the unit sphere, then rescaled onto radius R.  Draw i is Philox
 // stream CELL_POINTS, counter i.
 inline void generate_cell_points(int n, uint64_t seed, double* out) {
-    const double R = __builtin_sqrt(n / (4.0 * SVM_PI));
+    const double R = svm::sqrt_rn(n / (4.0 * SVM_PI));
     for (int i = 0; i < n; ++i) {
         double u, v;
         uniform2(seed, STREAM_CELL_POINTS, 0, (uint32_t)i, &u, &v);
@@ -25,7 +25,7 @@ inline void generate_cell_points(int n, uint64_t seed, double* out) {
             double radius = pass == 0 ? 1.0 : R;
             double nn = 0.0;
             nn += p[0] * p[0]; nn += p[1] * p[1]; nn += p[2] * p[2];
-            double inv = 1.0 / __builtin_sqrt(nn);
+            double inv = 1.0 / svm::sqrt_rn(nn);
             for (int k = 0; k < 3; ++k) p[k] = radius * (inv * p[k]);
         }
         for (int k = 0; k < 3; ++k) out[3 * (size_t)i + k] = p[k];
@@ -49,7 +49,7 @@ inline void generate_directors(int nv, const double* pos, uint64_t seed, double*
         double s = dvn / dnn;
         double a[3] = {n[0] - s * p[0], n[1] - s * p[1], n[2] - s * p[2]};
         double nn = 0.0; nn += a[0] * a[0]; nn += a[1] * a[1]; nn += a[2] * a[2];
-        double inv = 1.0 / __builtin_sqrt(nn);
+        double inv = 1.0 / svm::sqrt_rn(nn);
         for (int k = 0; k < 3; ++k) out[3 * (size_t)i + k] = inv * a[k];
     }
 }

@@ -56,6 +56,34 @@ SVM_HD double withwords(int32_t hi, uint32_t lo) {
     return bitsd(((uint64_t)(uint32_t)hi << 32) | (uint64_t)lo);
 }
 
+// ---------------------------------------------------------------- sqrt ----
+// Correctly rounded square root, the same bits as __builtin_sqrt.  On the
+// device, __builtin_sqrt expands to a Newton refinement of v_rsq_f64 wrapped
+// in an exponent scaling (for arguments below 2^-767) and a class check
+// (+-0 and +inf return themselves).  For arguments in [2^-767, DBL_MAX] the
+// scaling is by 2^0 and the class check passes the refined value through, so
+// there the refinement alone is the same computation; every other argument
+// (tiny, zero, negative, infinite, NaN) takes __builtin_sqrt itself.  This
+// saves the wrapper's eight instructions where they do nothing.
+SVM_HD double sqrt_rn(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (__builtin_expect(!(x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023), 0)) return __builtin_sqrt(x);
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return g;
+#else
+    return __builtin_sqrt(x);
+#endif
+}
+
 // ---------------------------------------------------------------- acos ----
 // The three argument ranges of the method share one rational approximation
 // R(z) = p(z)/q(z); it is evaluated once for the range's z and the three
@@ -83,7 +111,7 @@ SVM_HD double acos(double x) {
     double p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
     double q = 1.0 + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
     double r = p / q;
-    double s = __builtin_sqrt(z);
+    double s = sqrt_rn(z);
     double df = bitsd(dbits(s) & 0xffffffff00000000ull);
     double c = (z - df * df) / (s + df);
     double res_small = pio2_hi - (x - (pio2_lo - x * r));   // |x| < 0.5
@@ -96,6 +124,32 @@ SVM_HD double acos(double x) {
         else res = (x - x) / (x - x);                       // |x| > 1 or NaN
     }
     return res;
+}
+
+// acos for arguments that are mostly in [0.5, 1): the cosines of short arcs
+// (cell perimeters, the T1 edge test).  There acos takes its x >= 0.5 range
+// and nothing else, so evaluating that range alone is the same arithmetic on
+// the same operands; any other argument goes to acos itself.
+SVM_HD double acos_hi(double x) {
+    if (__builtin_expect(!(x >= 0.5 && x < 1.0), 0)) return svm::acos(x);
+    const double pS0 = 1.66666666666666657415e-01;
+    const double pS1 = -3.25565818622400915405e-01;
+    const double pS2 = 2.01212532134862925881e-01;
+    const double pS3 = -4.00555345006794114027e-02;
+    const double pS4 = 7.91534994289814532176e-04;
+    const double pS5 = 3.47933107596021167570e-05;
+    const double qS1 = -2.40339491173441421878e+00;
+    const double qS2 = 2.02094576023350569471e+00;
+    const double qS3 = -6.88283971605453293030e-01;
+    const double qS4 = 7.70381505559019352791e-02;
+    double z = (1.0 - x) * 0.5;
+    double p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
+    double q = 1.0 + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
+    double r = p / q;
+    double s = sqrt_rn(z);
+    double df = bitsd(dbits(s) & 0xffffffff00000000ull);
+    double c = (z - df * df) / (s + df);
+    return 2.0 * (df + (r * s + c));
 }
 
 // ------------------------------------------------------ sin/cos kernels ----
@@ -322,7 +376,7 @@ SVM_HD void normal2(uint64_t seed, uint32_t stream, uint64_t step, uint32_t idx,
                     double* z0, double* z1) {
     double a, b;
     uniform2(seed, stream, step, idx, &a, &b);
-    double rad = __builtin_sqrt(-2.0 * svm::log(1.0 - a));
+    double rad = sqrt_rn(-2.0 * svm::log(1.0 - a));
     double th = (2.0 * SVM_PI) * b;
     *z0 = rad * svm::cos(th);
     *z1 = rad * svm::sin(th);

@@ -304,7 +304,7 @@ SVM_HD void rodrigues(double* v, const double* k, double theta) {
 SVM_HD double geodesic(const double* p1, const double* p2, double n1, double n2, double radius) {
     double numerator = p1[0] * p2[0] + p1[1] * p2[1] + p1[2] * p2[2];
     double denominator = n1 * n2;
-    return radius * svm::acos(numerator / denominator);
+    return radius * svm::acos_hi(numerator / denominator);
 }
 
 // The T1 edge test `geodesic(p1, p2, n1, n2, R) < thr`
@@ -320,29 +320,29 @@ SVM_HD bool short_edge(const double* p1, const double* p2, double n1, double n2,
     double denominator = n1 * n2;
     double x = numerator / denominator;
     if (x <= cos_cut) return false;
-    return radius * svm::acos(x) < thr;
+    return radius * svm::acos_hi(x) < thr;
 }
 
-SVM_HD double pnorm(const double* p) { return __builtin_sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]); }
+SVM_HD double pnorm(const double* p) { return svm::sqrt_rn(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]); }
 
 // sphericalTriangleArea (sphericalDomain.h:135-166).
 SVM_HD double triangle_area(const double* p1, const double* p2, const double* p3, double radius) {
     double a[3], b[3], c[3];
     double dn;
     dn = 0.0; dn += p1[0] * p1[0]; dn += p1[1] * p1[1]; dn += p1[2] * p1[2];
-    double ia = 1.0 / __builtin_sqrt(dn);
+    double ia = 1.0 / svm::sqrt_rn(dn);
     dn = 0.0; dn += p2[0] * p2[0]; dn += p2[1] * p2[1]; dn += p2[2] * p2[2];
-    double ib = 1.0 / __builtin_sqrt(dn);
+    double ib = 1.0 / svm::sqrt_rn(dn);
     dn = 0.0; dn += p3[0] * p3[0]; dn += p3[1] * p3[1]; dn += p3[2] * p3[2];
-    double ic = 1.0 / __builtin_sqrt(dn);
+    double ic = 1.0 / svm::sqrt_rn(dn);
     for (int k = 0; k < 3; ++k) { a[k] = ia * p1[k]; b[k] = ib * p2[k]; c[k] = ic * p3[k]; }
     double p1Dotp2 = 0.0; p1Dotp2 += a[0] * b[0]; p1Dotp2 += a[1] * b[1]; p1Dotp2 += a[2] * b[2];
     double p1Dotp3 = 0.0; p1Dotp3 += a[0] * c[0]; p1Dotp3 += a[1] * c[1]; p1Dotp3 += a[2] * c[2];
     double p2Dotp3 = 0.0; p2Dotp3 += b[0] * c[0]; p2Dotp3 += b[1] * c[1]; p2Dotp3 += b[2] * c[2];
     double area = -SVM_PI;
-    area += svm::acos((p2Dotp3 - p1Dotp2 * p1Dotp3) / (__builtin_sqrt(1 - p1Dotp2 * p1Dotp2) * __builtin_sqrt(1 - p1Dotp3 * p1Dotp3)));
-    area += svm::acos((p1Dotp2 - p1Dotp3 * p2Dotp3) / (__builtin_sqrt(1 - p1Dotp3 * p1Dotp3) * __builtin_sqrt(1 - p2Dotp3 * p2Dotp3)));
-    area += svm::acos((p1Dotp3 - p1Dotp2 * p2Dotp3) / (__builtin_sqrt(1 - p1Dotp2 * p1Dotp2) * __builtin_sqrt(1 - p2Dotp3 * p2Dotp3)));
+    area += svm::acos((p2Dotp3 - p1Dotp2 * p1Dotp3) / (svm::sqrt_rn(1 - p1Dotp2 * p1Dotp2) * svm::sqrt_rn(1 - p1Dotp3 * p1Dotp3)));
+    area += svm::acos((p1Dotp2 - p1Dotp3 * p2Dotp3) / (svm::sqrt_rn(1 - p1Dotp3 * p1Dotp3) * svm::sqrt_rn(1 - p2Dotp3 * p2Dotp3)));
+    area += svm::acos((p1Dotp3 - p1Dotp2 * p2Dotp3) / (svm::sqrt_rn(1 - p1Dotp2 * p1Dotp2) * svm::sqrt_rn(1 - p2Dotp3 * p2Dotp3)));
     area *= (radius * radius);
     return area;
 }

@@ -137,7 +137,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_geometry(Dev d, int pin, int skip_
             sz = sz + Z[v];
         }
         double cs[3] = {sx, sy, sz};
-        double inv = 1.0 / __builtin_sqrt(dot0(cs, cs));
+        double inv = 1.0 / svm::sqrt_rn(dot0(cs, cs));
         double cxv = R * (inv * sx), cyv = R * (inv * sy), czv = R * (inv * sz);
         d.cx[c] = cxv;
         d.cy[c] = cyv;
@@ -156,14 +156,14 @@ __global__ __launch_bounds__(SVM_TILE) void k_geometry(Dev d, int pin, int skip_
         double pn[3] = {X[in], Y[in], Z[in]};
         double nl = pnorm(pl), nc = pnorm(pc), nnx = pnorm(pn);
         double cm[3] = {pl[1] * pc[2] - pl[2] * pc[1], pl[2] * pc[0] - pl[0] * pc[2], pl[0] * pc[1] - pl[1] * pc[0]};
-        double ncm = __builtin_sqrt(dot0(cm, cm));
+        double ncm = svm::sqrt_rn(dot0(cm, cm));
         double perimeter = 0.0, angles = 0.0;
         for (int k = 0; k < n; ++k) {
             int ip = rc[(size_t)k * Cs];
             double pp[3] = {X[ip], Y[ip], Z[ip]};
             perimeter += geodesic(pl, pc, nl, nc, R);
             double ci[3] = {pc[1] * pn[2] - pc[2] * pn[1], pc[2] * pn[0] - pc[0] * pn[2], pc[0] * pn[1] - pc[1] * pn[0]};
-            double nci = __builtin_sqrt(dot0(ci, ci));
+            double nci = svm::sqrt_rn(dot0(ci, ci));
             double determinant = pl[0] * ci[0] + pl[1] * ci[1] + pl[2] * ci[2];
             double angle = svm::acos(-dot0(ci, cm) / (nci * ncm));
             if (determinant > 0) angle *= -1;
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_fan_fix(Dev d, int pin, int skip_f
     const double cp[3] = {d.cx[c], d.cy[c], d.cz[c]};
     auto unit = [](const double* p, double* u) {
         double dn = 0.0; dn += p[0] * p[0]; dn += p[1] * p[1]; dn += p[2] * p[2];
-        const double inv = 1.0 / __builtin_sqrt(dn);
+        const double inv = 1.0 / svm::sqrt_rn(dn);
         for (int k = 0; k < 3; ++k) u[k] = inv * p[k];
     };
     double ua[3], ub[3], uc[3];
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_fan_fix(Dev d, int pin, int skip_f
         unit(pl, ub);
     }
     double d12 = 0.0; d12 += ua[0] * ub[0]; d12 += ua[1] * ub[1]; d12 += ua[2] * ub[2];
-    double s12 = __builtin_sqrt(1 - d12 * d12);
+    double s12 = svm::sqrt_rn(1 - d12 * d12);
     double pc[3] = {X[ic], Y[ic], Z[ic]};
     double a = 0.0;
     for (int k = 0; k < n; ++k) {
@@ -237,8 +237,8 @@ __global__ __launch_bounds__(SVM_TILE) void k_fan_fix(Dev d, int pin, int skip_f
         unit(pc, uc);
         double d13 = 0.0; d13 += ua[0] * uc[0]; d13 += ua[1] * uc[1]; d13 += ua[2] * uc[2];
         double d23 = 0.0; d23 += ub[0] * uc[0]; d23 += ub[1] * uc[1]; d23 += ub[2] * uc[2];
-        const double s13 = __builtin_sqrt(1 - d13 * d13);
-        const double s23 = __builtin_sqrt(1 - d23 * d23);
+        const double s13 = svm::sqrt_rn(1 - d13 * d13);
+        const double s23 = svm::sqrt_rn(1 - d23 * d23);
         double area = -SVM_PI;
         area += svm::acos((d23 - d12 * d13) / (s12 * s13));
         area += svm::acos((d12 - d13 * d23) / (s13 * s23));
@@ -727,7 +727,7 @@ __global__ __launch_bounds__(3 * SVM_CORNER_V) __attribute__((amdgpu_waves_per_e
     double disp[3], nI[3];
     if (MODE == VM_BROWNIAN) {
         double forcePrefactor = S->dt * S->mu;
-        double noisePrefactor = __builtin_sqrt(2.0 * forcePrefactor * S->T);
+        double noisePrefactor = svm::sqrt_rn(2.0 * forcePrefactor * S->T);
         double xi[3] = {0.0, 0.0, 0.0};
         if (S->T != 0.0) normal3(S->seed, S->step, (uint32_t)local, xi);
         for (int k = 0; k < 3; ++k) disp[k] = xi[k] * noisePrefactor + f[k] * forcePrefactor;
@@ -765,9 +765,9 @@ __global__ __launch_bounds__(SVM_TILE) void k_fire_control(Dev d) {
     double velocityNorm = fold_lds(d.vpart + 3 * S->vtile0 + 2, 3, S->nvtiles, buf);
     if (threadIdx.x == 0) {
         double Ndof = (double)S->nv;
-        double forceMax = S->f_stop_rms ? __builtin_sqrt(forceNorm / Ndof) : __builtin_sqrt(forceNorm) / Ndof;
+        double forceMax = S->f_stop_rms ? svm::sqrt_rn(forceNorm / Ndof) : svm::sqrt_rn(forceNorm) / Ndof;
         double scaling = 0.0;
-        if (forceNorm > 0.) scaling = __builtin_sqrt(velocityNorm / forceNorm);
+        if (forceNorm > 0.) scaling = svm::sqrt_rn(velocityNorm / forceNorm);
         double alpha = S->f_alpha, dtf = S->f_dt;
         S->mix_alpha = alpha;
         S->mix_scaling = scaling;
@@ -807,7 +807,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_adam_control(Dev d) {
     if (S->f_frozen) return;
     double forceNorm = fold_lds(d.vpart + 3 * S->vtile0 + 1, 3, S->nvtiles, buf);
     if (threadIdx.x == 0) {
-        double forceMax = __builtin_sqrt(forceNorm) / (double)S->nv;
+        double forceMax = svm::sqrt_rn(forceNorm) / (double)S->nv;
         S->a_iter += 1;
         S->a_force_max = forceMax;
         S->a_beta1t *= S->a_beta1;
@@ -966,7 +966,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_vicsek_dir(Dev d, int pin, int nin
         const double q = dvn / dnn;
         sp[0] = sp[0] - q * p[0]; sp[1] = sp[1] - q * p[1]; sp[2] = sp[2] - q * p[2];
         double ss = 0.0; ss += sp[0] * sp[0]; ss += sp[1] * sp[1]; ss += sp[2] * sp[2];
-        const double inv = 1.0 / __builtin_sqrt(ss);
+        const double inv = 1.0 / svm::sqrt_rn(ss);
         sp[0] = inv * sp[0]; sp[1] = inv * sp[1]; sp[2] = inv * sp[2];
     }
     const double* NX = d.nx[nin];
@@ -992,7 +992,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_vicsek_dir(Dev d, int pin, int nin
         const double q = dvn / dnn;
         n[0] = n[0] - q * p[0]; n[1] = n[1] - q * p[1]; n[2] = n[2] - q * p[2];
         double ss = 0.0; ss += n[0] * n[0]; ss += n[1] * n[1]; ss += n[2] * n[2];
-        const double inv = 1.0 / __builtin_sqrt(ss);
+        const double inv = 1.0 / svm::sqrt_rn(ss);
         n[0] = inv * n[0]; n[1] = inv * n[1]; n[2] = inv * n[2];
     }
     d.nx[nout][v] = n[0]; d.ny[nout][v] = n[1]; d.nz[nout][v] = n[2];

@@ -383,30 +383,30 @@ __global__ __launch_bounds__(SVM_TILE) void k_cell_descriptors(Dev d, int pin, f
     const int n = d.cnn[c];
     const double A = d.A[c], P = d.P[c];
     const double C[3] = {d.cx[c], d.cy[c], d.cz[c]};
-    const double cn = __builtin_sqrt(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]);
+    const double cn = svm::sqrt_rn(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]);
     const double nh[3] = {C[0] / cn, C[1] / cn, C[2] / cn};
     double F[3] = {0.0, 0.0, 0.0}, fabs_sum = 0.0;
     for (int k = 0; k < n; ++k) {
         int v = d.ring[(size_t)k * d.Cs + c];
         double f0 = d.fx[v], f1 = d.fy[v], f2 = d.fz[v];
         F[0] += f0; F[1] += f1; F[2] += f2;
-        fabs_sum += __builtin_sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+        fabs_sum += svm::sqrt_rn(f0 * f0 + f1 * f1 + f2 * f2);
     }
     for (int k = 0; k < 3; ++k) F[k] /= n;
     double fn = F[0] * nh[0] + F[1] * nh[1] + F[2] * nh[2];
     double Ft[3] = {F[0] - fn * nh[0], F[1] - fn * nh[1], F[2] - fn * nh[2]};
-    double fmag = __builtin_sqrt(Ft[0] * Ft[0] + Ft[1] * Ft[1] + Ft[2] * Ft[2]);
+    double fmag = svm::sqrt_rn(Ft[0] * Ft[0] + Ft[1] * Ft[1] + Ft[2] * Ft[2]);
     double e1[3];
     if (fmag > 1e-300) {
         for (int k = 0; k < 3; ++k) e1[k] = Ft[k] / fmag;
     } else {   // no force: any tangent direction
         double a[3] = {__builtin_fabs(nh[0]) < 0.9 ? 1.0 : 0.0, __builtin_fabs(nh[0]) < 0.9 ? 0.0 : 1.0, 0.0};
         double t[3] = {a[1] * nh[2] - a[2] * nh[1], a[2] * nh[0] - a[0] * nh[2], a[0] * nh[1] - a[1] * nh[0]};
-        double tn = __builtin_sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        double tn = svm::sqrt_rn(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
         for (int k = 0; k < 3; ++k) e1[k] = t[k] / tn;
     }
     const double e2[3] = {nh[1] * e1[2] - nh[2] * e1[1], nh[2] * e1[0] - nh[0] * e1[2], nh[0] * e1[1] - nh[1] * e1[0]};
-    const double q = P / __builtin_sqrt(A > 1e-12 ? A : 1e-12);
+    const double q = P / svm::sqrt_rn(A > 1e-12 ? A : 1e-12);
     double sA = 0.0, sA2 = 0.0, dip1 = 0.0, dip2 = 0.0, sq = 0.0, sdp = 0.0;
     for (int k = 0; k < n; ++k) {
         int v = d.ring[(size_t)k * d.Cs + c];
@@ -420,12 +420,12 @@ __global__ __launch_bounds__(SVM_TILE) void k_cell_descriptors(Dev d, int pin, f
         const double An = d.A[nb], Pn = d.P[nb];
         sA += An;
         sA2 += An * An;
-        sq += Pn / __builtin_sqrt(An > 1e-12 ? An : 1e-12);
+        sq += Pn / svm::sqrt_rn(An > 1e-12 ? An : 1e-12);
         sdp += Pn - d.p0[nb];
         double dv[3] = {d.cx[nb] - C[0], d.cy[nb] - C[1], d.cz[nb] - C[2]};
         double dn = dv[0] * nh[0] + dv[1] * nh[1] + dv[2] * nh[2];
         for (int k2 = 0; k2 < 3; ++k2) dv[k2] -= dn * nh[k2];
-        double dl = __builtin_sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
+        double dl = svm::sqrt_rn(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
         if (dl > 0.0) {
             double u1 = (dv[0] * e1[0] + dv[1] * e1[1] + dv[2] * e1[2]) / dl;
             double u2 = (dv[0] * e2[0] + dv[1] * e2[1] + dv[2] * e2[2]) / dl;
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_cell_descriptors(Dev d, int pin, f
     x[4] = (float)fmag;
     x[5] = (float)(fabs_sum / n);
     x[6] = (float)(mA - A);
-    x[7] = (float)__builtin_sqrt(var > 0.0 ? var : 0.0);
+    x[7] = (float)svm::sqrt_rn(var > 0.0 ? var : 0.0);
     x[8] = (float)(dip1 / n);
     x[9] = (float)(dip2 / n);
     x[10] = (float)(sq / n - q);
@@ -480,9 +480,9 @@ __global__ __launch_bounds__(SVM_TILE) void k_warm_disp(Dev d, int pin, const fl
         double emin = 1e300;
         for (int k = 0; k < 3; ++k) {
             const double dx = X[n3[k]] - X[v], dy = Yp[n3[k]] - Yp[v], dz = Zp[n3[k]] - Zp[v];
-            emin = fmin(emin, __builtin_sqrt(dx * dx + dy * dy + dz * dz));
+            emin = fmin(emin, svm::sqrt_rn(dx * dx + dy * dy + dz * dz));
         }
-        const double len = __builtin_sqrt(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
+        const double len = svm::sqrt_rn(out[0] * out[0] + out[1] * out[1] + out[2] * out[2]);
         const double lim = cap * emin;             // cap 0: no limit
         if (cap > 0.0 && len > lim && len > 0.0)
             for (int j = 0; j < 3; ++j) out[j] *= lim / len;

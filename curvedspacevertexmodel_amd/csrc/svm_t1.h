@@ -513,7 +513,7 @@ __device__ __attribute__((always_inline)) inline void flip_positions(double* v1,
                                                                      double R) {
     double mid[3] = {0.5 * (v1[0] + v2[0]), 0.5 * (v1[1] + v2[1]), 0.5 * (v1[2] + v2[2])};
     double mm = 0.0; mm += mid[0] * mid[0]; mm += mid[1] * mid[1]; mm += mid[2] * mid[2];
-    double im = 1.0 / __builtin_sqrt(mm);
+    double im = 1.0 / svm::sqrt_rn(mm);
     mid[0] = im * mid[0]; mid[1] = im * mid[1]; mid[2] = im * mid[2];
     double determinant = vC[0] * (v1[1] * v2[2] - v1[2] * v2[1])
                        + vC[1] * (v1[2] * v2[0] - v1[0] * v2[2])
@@ -525,7 +525,7 @@ __device__ __attribute__((always_inline)) inline void flip_positions(double* v1,
     for (int k = 0; k < 3; ++k) { v1[k] = v1[k] + diff[k]; v2[k] = v2[k] - diff[k]; }
     double n1 = 0.0; n1 += v1[0] * v1[0]; n1 += v1[1] * v1[1]; n1 += v1[2] * v1[2];
     double n2 = 0.0; n2 += v2[0] * v2[0]; n2 += v2[1] * v2[1]; n2 += v2[2] * v2[2];
-    double i1 = 1.0 / __builtin_sqrt(n1), i2 = 1.0 / __builtin_sqrt(n2);
+    double i1 = 1.0 / svm::sqrt_rn(n1), i2 = 1.0 / svm::sqrt_rn(n2);
     for (int k = 0; k < 3; ++k) { v1[k] = R * (i1 * v1[k]); v2[k] = R * (i2 * v2[k]); }
 }
 

@@ -387,10 +387,10 @@ __global__ __launch_bounds__(64 * SVM_DL_WAVES, 4) void k_dl_star(const double* 
                 double v[3] = {pb[0] - pi[0], pb[1] - pi[1], pb[2] - pi[2]};
                 double wv[3] = {pb[0] - rp[0], pb[1] - rp[1], pb[2] - rp[2]};
                 double cx = u[1] * v[2] - u[2] * v[1], cy = u[2] * v[0] - u[0] * v[2], cz = u[0] * v[1] - u[1] * v[0];
-                double area2 = __builtin_sqrt(cx * cx + cy * cy + cz * cz);
-                double rho = __builtin_sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) *
-                             __builtin_sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) *
-                             __builtin_sqrt(wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2]) / (2.0 * area2);
+                double area2 = svm::sqrt_rn(cx * cx + cy * cy + cz * cz);
+                double rho = svm::sqrt_rn(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) *
+                             svm::sqrt_rn(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) *
+                             svm::sqrt_rn(wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2]) / (2.0 * area2);
                 if (!(2.25 * rho * (1.0 + 1e-6) < r) && r < 2.0 * R) far = true;
             }
         }
@@ -581,7 +581,7 @@ __global__ __launch_bounds__(256) void k_dl_vertices(Dev d, int sys, int pout, c
         }
         for (int k = 0; k < 3; ++k) c[k] = (1.0 / 3) * c[k];
     }
-    double nrm = __builtin_sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    double nrm = svm::sqrt_rn(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
     const int v = S.vo + t;
     d.px[pout][v] = (radius / nrm) * c[0];
     d.py[pout][v] = (radius / nrm) * c[1];
@@ -661,7 +661,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_voronoi_move(Dev d) {
     if (local >= S->nc) return;
     const size_t Cs = (size_t)d.Cs;
     const double forcePrefactor = S->dt * S->mu;
-    const double noisePrefactor = __builtin_sqrt(2.0 * forcePrefactor * S->T);
+    const double noisePrefactor = svm::sqrt_rn(2.0 * forcePrefactor * S->T);
     double xi[3] = {0.0, 0.0, 0.0};
     if (S->T != 0.0) normal3(S->seed, S->step, (uint32_t)local, xi);
     double disp[3];
@@ -705,7 +705,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_voronoi_chain(Dev d) {
         const double u[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
         const double w[3] = {pc[0] - pa[0], pc[1] - pa[1], pc[2] - pa[2]};
         const double nv[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
-        const double nn = __builtin_sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+        const double nn = svm::sqrt_rn(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
         const double nh[3] = {nv[0] / nn, nv[1] / nn, nv[2] / nn};
         const double out = nv[0] * (pa[0] + pb[0] + pc[0]) + nv[1] * (pa[1] + pb[1] + pc[1]) +
                            nv[2] * (pa[2] + pb[2] + pc[2]);

@@ -1055,6 +1055,9 @@ double or_tree_sum(const double* x, int n) { return tree_sum(std::vector<double>
 
 // math entry points for the tests
 double or_acos(double x) { return O_ACOS(x); }
+// the library's short-arc form of acos, compared with svm::acos by the tests
+double or_acos_hi(double x) { return svm::acos_hi(x); }
+double or_own_acos(double x) { return svm::acos(x); }
 double or_sin(double x) { return O_SIN(x); }
 double or_cos(double x) { return O_COS(x); }
 double or_log(double x) { return svm::log(x); }
