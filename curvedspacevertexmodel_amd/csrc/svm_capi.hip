@@ -304,12 +304,6 @@ int launch_t1(svm_context* ctx, const Target& w, int pcur, int nin, int nout, in
         if (r) return r;
     }
     if (!ctx->t1_sequential) {
-        Timer t(ctx, "k_t1_check");
-        hipLaunchKernelGGL(svm::k_t1_check, dim3(64), dim3(SVM_TILE), 0, w.st, w.d, pcur);
-        int r = last_launch(ctx);
-        if (r) return r;
-    }
-    if (!ctx->t1_sequential) {
         Timer t(ctx, "k_t1_group");
         hipLaunchKernelGGL(svm::k_t1_group, dim3(w.d.ns), dim3(SVM_T1_GT), 0, w.st, w.d, pcur);
         int r = last_launch(ctx);
@@ -385,6 +379,7 @@ Target lane_target(svm_context* ctx, int k) {
     d.t1_list = L.t1_list;
     d.t1_flagged = L.t1_flagged;
     d.t1_logged = L.t1_logged;
+    d.t1_lead = L.t1_lead;
     d.t1_total = L.t1_total;
     d.t1_bcount = L.t1_bcount;
     d.t1_blist = L.t1_blist;
@@ -957,6 +952,8 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
     r |= dalloc(ctx, &d.vn, Vs);
     r |= dalloc(ctx, &d.vln, Vs);
     r |= dalloc(ctx, &d.t1mask, Vs / 64);
+    r |= dalloc(ctx, &d.t1fmask, Vs / 64);
+    r |= dalloc(ctx, &d.t1_vidx, Vs);
     r |= dalloc(ctx, &d.cand, Vs);
     r |= dalloc(ctx, &d.cpart, ctx->nct);
     r |= dalloc(ctx, &d.vpart, 3 * (size_t)ctx->nvt);
@@ -965,6 +962,7 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
     r |= dalloc(ctx, &d.t1_list, SVM_T1_MAXCL);
     r |= dalloc(ctx, &d.t1_flagged, SVM_T1_MAXCL);
     r |= dalloc(ctx, &d.t1_logged, SVM_T1_MAXCL);
+    r |= dalloc(ctx, &d.t1_lead, SVM_T1_MAXCL);
     r |= dalloc(ctx, &d.t1_total, 1);
     r |= dalloc(ctx, &d.t1_bcount, 1);
     r |= dalloc(ctx, &d.t1_blist, SVM_T1_MAXBL);
@@ -2200,6 +2198,7 @@ int svm_set_lanes(svm_context* ctx, int32_t n_lanes) {
         r |= dalloc(ctx, &L.t1_list, SVM_T1_MAXCL);
         r |= dalloc(ctx, &L.t1_flagged, SVM_T1_MAXCL);
         r |= dalloc(ctx, &L.t1_logged, SVM_T1_MAXCL);
+        r |= dalloc(ctx, &L.t1_lead, SVM_T1_MAXCL);
         r |= dalloc(ctx, &L.t1_total, 1);
         r |= dalloc(ctx, &L.t1_bcount, 1);
         r |= dalloc(ctx, &L.t1_blist, SVM_T1_MAXBL);

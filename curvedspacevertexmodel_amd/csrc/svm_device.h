@@ -94,6 +94,8 @@ struct Dev {
     int32_t* vln;
     // work buffers
     unsigned long long* t1mask;   // one bit per vertex row
+    unsigned long long* t1fmask;  // one bit per vertex row: the flippable candidates
+    int32_t* t1_vidx;             // per vertex row: its slot in its lane's flippable list (this step's candidates)
     int32_t* cand;                // ordered T1 candidates, per-sphere ranges [vo, vo+nv)
     double* cpart;                // per cell tile partial sums
     double* vpart;                // 3 per vertex tile partial sums
@@ -103,6 +105,7 @@ struct Dev {
     int32_t* t1_list;             // candidate vertex rows
     int32_t* t1_flagged;          // per candidate: generation in which its run overlapped
     int32_t* t1_logged;           // per candidate: generation in which its run wrote a log
+    int32_t* t1_lead;             // per candidate: slot of the candidate whose dry run covers it
     int32_t* t1_total;            // flippable-list length of the current step (copied before the reset)
     int32_t* t1_bcount;           // blocked candidates appended this step
     int32_t* t1_blist;            // blocked candidate vertex rows

@@ -259,6 +259,11 @@ SVM_HD double cos(double x) {
     }
 }
 
+// cos(+-pi/2) as cos() above returns it for the double nearest pi/2 (the
+// residual 6.123e-17 of that double); sin() returns exactly +-1 there.  A
+// quarter turn (the T1 flip) uses these instead of evaluating both functions.
+constexpr double kCosQuarterTurn = 0x1.1a62633145c07p-54;
+
 // ----------------------------------------------------------------- log ----
 SVM_HD double log(double x) {
     const double ln2_hi = 6.93147180369123816490e-01;

@@ -285,18 +285,21 @@ SVM_HD void project_normalize(double* v, const double* n) {
     v[0] = inv * a0; v[1] = inv * a1; v[2] = inv * a2;
 }
 
-// rodriguesRotation(v, k, theta) (inc/functions.h:175-181).
-SVM_HD void rodrigues(double* v, const double* k, double theta) {
+// rodriguesRotation(v, k, theta) (inc/functions.h:175-181), with c = cos(theta)
+// and s = sin(theta) given.
+SVM_HD void rodrigues_cs(double* v, const double* k, double c, double s) {
     double cp0 = k[1] * v[2] - k[2] * v[1];
     double cp1 = k[2] * v[0] - k[0] * v[2];
     double cp2 = k[0] * v[1] - k[1] * v[0];
-    double c = svm::cos(theta), s = svm::sin(theta);
     double kv = 0.0; kv += k[0] * v[0]; kv += k[1] * v[1]; kv += k[2] * v[2];
     double w = (1.0 - c) * kv;
     double a0 = c * v[0] + s * cp0 + w * k[0];
     double a1 = c * v[1] + s * cp1 + w * k[1];
     double a2 = c * v[2] + s * cp2 + w * k[2];
     v[0] = a0; v[1] = a1; v[2] = a2;
+}
+SVM_HD void rodrigues(double* v, const double* k, double theta) {
+    rodrigues_cs(v, k, svm::cos(theta), svm::sin(theta));
 }
 
 // geodesicDistance (sphericalDomain.h:94-112) with the point norms supplied:

@@ -16,11 +16,12 @@
 //                 would write.  When no two candidates of a sphere touched a
 //                 common item, their visits commute and any order gives the
 //                 sequential result.
-//   k_t1_check    blocked candidates (every short edge triangle-protected or
-//                 without a fourth cell) do nothing unless a run above
-//                 touched what they read; those that were are flagged.
-//   k_t1_group    block per sphere with flagged candidates: one dry run over
-//                 all of them together, in index order.
+//   k_t1_group    block per sphere.  First the sphere's blocked candidates
+//                 (every short edge triangle-protected or without a fourth
+//                 cell), which do nothing unless a run above touched what
+//                 they read; those that were are flagged.  Then, where
+//                 candidates are flagged, one dry run over all of them
+//                 together, in index order.
 //   k_t1_resolve  block per sphere: applies the logs of a sphere whose
 //                 candidates did not overlap; otherwise (or when a buffer was
 //                 too small) runs the sequential loop itself over the
@@ -518,9 +519,10 @@ __device__ __attribute__((always_inline)) inline void flip_positions(double* v1,
     double determinant = vC[0] * (v1[1] * v2[2] - v1[2] * v2[1])
                        + vC[1] * (v1[2] * v2[0] - v1[0] * v2[2])
                        + vC[2] * (v1[0] * v2[1] - v1[1] * v2[0]);
-    determinant = determinant > 0 ? 1. : -1.;
-    rodrigues(v1, mid, -0.5 * determinant * SVM_PI);
-    rodrigues(v2, mid, -0.5 * determinant * SVM_PI);
+    // the turn by -0.5 * (determinant > 0 ? 1 : -1) * pi: cos and sin of -+pi/2
+    const double s = determinant > 0 ? -1.0 : 1.0;
+    rodrigues_cs(v1, mid, kCosQuarterTurn, s);
+    rodrigues_cs(v2, mid, kCosQuarterTurn, s);
     double diff[3] = {0.5 * (v1[0] - v2[0]), 0.5 * (v1[1] - v2[1]), 0.5 * (v1[2] - v2[2])};
     for (int k = 0; k < 3; ++k) { v1[k] = v1[k] + diff[k]; v2[k] = v2[k] - diff[k]; }
     double n1 = 0.0; n1 += v1[0] * v1[0]; n1 += v1[1] * v1[1]; n1 += v1[2] * v1[2];
@@ -921,6 +923,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int ski
     int basef = 0, baseb = 0;
     if (lane == 0) {
         d.t1mask[v >> 6] = m;
+        d.t1fmask[v >> 6] = mf;
         if (m) atomicOr(&S->t1_flag, 1);
         if (mf) {
             int cnt = __popcll(mf);
@@ -945,6 +948,8 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int ski
             d.t1_list[pos] = v;
             d.t1_flagged[pos] = 0;
             d.t1_logged[pos] = 0;
+            d.t1_lead[pos] = pos;
+            d.t1_vidx[v] = pos;
         }
     } else if (flag) {
         int pos = baseb + __popcll(mb & below);
@@ -959,16 +964,19 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int ski
 // A blocked candidate (no short edge of it can flip in the current state)
 // has a visit without writes, whose outcome stands unless a flippable
 // candidate's run touched what the visit reads: the vertex, its later
-// neighbours, its cells and the fourth cell of each short edge.  Thread per
-// blocked candidate; an entangled one is flagged, with the runs it met, for
-// the group run.  Otherwise its no-fourth-cell skips are tallied for the
-// commit.
-__global__ __launch_bounds__(SVM_TILE) void k_t1_check(Dev d, int pcur) {
+// neighbours, its cells and the fourth cell of each short edge.  Run by the
+// group kernel's block of the candidate's sphere, a thread per blocked
+// candidate, before that block decides whether the sphere needs a group run:
+// an entangled candidate is flagged, with the runs it met, for the group run;
+// otherwise its no-fourth-cell skips are tallied for the commit.  The runs a
+// candidate can meet are its own sphere's, so one block sees all it needs.
+__device__ __attribute__((always_inline)) inline void t1_check_sphere(const Dev& d, SysParams* S, int vo, int nv,
+                                                                     int t, int nt, int* hit_any) {
     const int total = min(*d.t1_bcount, SVM_T1_MAXBL);
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const unsigned gen = (unsigned)S->t1_gen;
+    for (int i = t; i < total; i += nt) {
         const int v = d.t1_blist[i];
-        SysParams* S = d.sys + d.vtile_sys[v / SVM_TILE];
-        const unsigned gen = (unsigned)S->t1_gen;
+        if (v < vo || v >= vo + nv) continue;
         bool hit = false;
         auto look = [&](const unsigned long long* own) {
             unsigned long long st = *own;
@@ -996,7 +1004,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_check(Dev d, int pcur) {
         }
         if (hit) {
             d.t1_bflag[i] = (int)gen;
-            atomicOr(&S->t1_overlap, 1);
+            *hit_any = 1;
         } else if (d.t1_bskip[i]) {
             atomicAdd(&S->t1_bskips, d.t1_bskip[i]);   // counted if the dry runs are committed
         }
@@ -1250,6 +1258,114 @@ __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int n
     __syncthreads();
 }
 
+// Predicted components.  Two flippable candidates of a sphere closer than
+// SVM_T1_NEAR cell sizes (sqrt of the mean cell area) are taken to interact,
+// and the transitive closure of that relation splits the sphere's candidates
+// into components.  One dry run covers a whole component: its members and
+// the vertices their flips disturbed, in index order -- the sequential loop
+// restricted to the component, as a group run is.  A prediction is not
+// trusted for exactness: components whose runs still touch a common item are
+// flagged by the stamps and redone by the group run, as single runs were.
+// Every wavefront of a component computes the same closure from the same
+// inputs (the flippable mask and the moved positions, which no dry run
+// writes; the distance test is symmetric bit for bit), and the member with
+// the lowest list slot is its leader: it runs, stamps and logs under its own
+// slot, and every member records that slot in t1_lead.  Spheres with more
+// than 64 flippable candidates, or more than 64 * 64 * SVM_T1_NEAR_MAXW
+// vertex rows, keep one run per candidate.
+#ifndef SVM_T1_NEAR
+#define SVM_T1_NEAR 2.5
+#endif
+#define SVM_T1_NEAR_MAXW 16
+
+__device__ __attribute__((always_inline)) inline unsigned long long wave_or64(unsigned long long x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o, 64);
+    return x;
+}
+__device__ __attribute__((always_inline)) inline int wave_min32(int x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
+    return x;
+}
+
+// The component of candidate `row` (slot idx) in its sphere: members' local
+// indices in ascending order into seeds[0..n), and the leader's slot.  All
+// 64 lanes of the wavefront call it with the same arguments.
+template <class Sync>
+__device__ __attribute__((always_inline)) inline int t1_component(const Dev& d, const SysParams* S, int pcur, int row,
+                                                                  int idx, int lane, int* seeds, int* crow,
+                                                                  double* cpos, int* lead, Sync sync) {
+    const int vo = S->vo, nv = S->nv;
+    if (lane == 0) seeds[0] = row - vo;
+    *lead = idx;
+    if (!(SVM_T1_NEAR > 0) || nv > 64 * 64 * SVM_T1_NEAR_MAXW) { sync(); return 1; }
+    // the sphere's flippable candidates, in row order: lane L reads mask words
+    // [L * MAXW, (L + 1) * MAXW) of the sphere (vo is a multiple of 64)
+    const int w0 = vo >> 6, nw = (nv + 63) >> 6;
+    unsigned long long words[SVM_T1_NEAR_MAXW];
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < SVM_T1_NEAR_MAXW; ++k) {
+        const int wi = lane * SVM_T1_NEAR_MAXW + k;
+        words[k] = wi < nw ? d.t1fmask[w0 + wi] : 0ull;
+        cnt += __popcll(words[k]);
+    }
+    int inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    const int ncand = __shfl(inc, 63, 64);
+    if (ncand < 2 || ncand > 64) { sync(); return 1; }
+    int at = inc - cnt;
+#pragma unroll
+    for (int k = 0; k < SVM_T1_NEAR_MAXW; ++k) {
+        unsigned long long m = words[k];
+        while (m) {
+            const int b = __builtin_ctzll(m);
+            m &= m - 1ull;
+            crow[at++] = ((w0 + lane * SVM_T1_NEAR_MAXW + k) << 6) + b;
+        }
+    }
+    sync();
+    const bool act = lane < ncand;
+    const int myrow = act ? crow[lane] : -1;
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+    int myidx = 0x7fffffff;
+    if (act) {
+        p0 = d.px[pcur][myrow];
+        p1 = d.py[pcur][myrow];
+        p2 = d.pz[pcur][myrow];
+        myidx = d.t1_vidx[myrow];
+        cpos[3 * lane] = p0;
+        cpos[3 * lane + 1] = p1;
+        cpos[3 * lane + 2] = p2;
+    }
+    sync();
+    const double cell2 = 4.0 * SVM_PI * S->R * S->R / (double)S->nc;
+    const double near2 = (SVM_T1_NEAR * SVM_T1_NEAR) * cell2;
+    unsigned long long adj = 0ull;
+    if (act)
+        for (int k = 0; k < ncand; ++k) {
+            const double e0 = cpos[3 * k] - p0, e1 = cpos[3 * k + 1] - p1, e2 = cpos[3 * k + 2] - p2;
+            if (e0 * e0 + e1 * e1 + e2 * e2 < near2) adj |= 1ull << k;
+        }
+    unsigned long long comp = __ballot(myrow == row);
+    if (!comp) { sync(); return 1; }    // not in the mask: a run of its own
+    while (true) {
+        const unsigned long long nx = wave_or64(((comp >> lane) & 1ull) ? adj : 0ull) | comp;
+        if (nx == comp) break;
+        comp = nx;
+    }
+    const bool in = act && ((comp >> lane) & 1ull);
+    *lead = wave_min32(in ? myidx : 0x7fffffff);
+    if (in) seeds[__popcll(comp & ((1ull << lane) - 1ull))] = myrow - vo;
+    sync();
+    return __popcll(comp);
+}
+
 __global__ __launch_bounds__(64 * SVM_T1_WAVES) void k_t1_cluster(Dev d, int pcur) {
     __shared__ int s_vkey[SVM_T1_WAVES][SVM_T1_WV], s_vflag[SVM_T1_WAVES][SVM_T1_WV], s_vuse[SVM_T1_WAVES][SVM_T1_WV];
     __shared__ double s_vp[SVM_T1_WAVES][SVM_T1_WV * 3];
@@ -1259,7 +1375,10 @@ __global__ __launch_bounds__(64 * SVM_T1_WAVES) void k_t1_cluster(Dev d, int pcu
     __shared__ int s_cring[SVM_T1_WAVES][SVM_T1_WC * SVM_T1_RC];
     __shared__ int s_heap[SVM_T1_WAVES][64];
     __shared__ int s_tmp[SVM_T1_WAVES][SVM_T1_RC + 8];
-    __shared__ int s_misc[SVM_T1_WAVES][8];   // vfill, cfill, seed, outcome, nvert, ncell
+    __shared__ int s_misc[SVM_T1_WAVES][8];   // vfill, cfill, -, outcome, nvert, ncell
+    __shared__ int s_seed[SVM_T1_WAVES][64];
+    __shared__ int s_crow[SVM_T1_WAVES][64];
+    __shared__ double s_cpos[SVM_T1_WAVES][64 * 3];
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const WaveSync sync;
@@ -1273,21 +1392,32 @@ __global__ __launch_bounds__(64 * SVM_T1_WAVES) void k_t1_cluster(Dev d, int pcu
         if (S->t1_overflow) continue;   // the sphere takes the sequential pass
         const unsigned gen = (unsigned)S->t1_gen;
         const int vo = S->vo, nv = S->nv;
+        int lead = idx;
+        const int nmem = t1_component(d, S, pcur, row, idx, lane, s_seed[w], s_crow[w], s_cpos[w], &lead, sync);
+        if (lane == 0) d.t1_lead[idx] = lead;
+        if (lead != idx) continue;      // the leader's run covers this candidate
         CacheAcc<SVM_T1_WV, SVM_T1_WC> a{&d, pcur, false, s_vkey[w], s_vflag[w], s_vuse[w], s_vp[w], s_vvn[w],
                                          s_vvcn[w], s_ckey[w], s_cflag[w], s_cuse[w], s_ccnt[w], s_cring[w],
                                          misc + 0, misc + 1};
         a.reset(lane, 64);
-        if (lane == 0) misc[2] = row - vo;
         sync();
-        a.prefetch(misc + 2, 1, vo, lane, 64, SVM_T1_WV * 5 / 8, SVM_T1_WC * 5 / 8, sync);
+        const int* seeds = s_seed[w];
+        a.prefetch(seeds, nmem, vo, lane, 64, SVM_T1_WV * 5 / 8, SVM_T1_WC * 5 / 8, sync);
         T1Log& L = d.t1_log[idx];
         if (lane == 0) {
             Heap heap{s_heap[w], 0};
             bool heap_full = false;
             VisitStats st{0, 0};
-            int err = 0, last = -1, nvis = 0;
-            int u = row - vo;
+            int err = 0, last = -1, nvis = 0, mi = 0;
+            // the members and the later vertices their flips disturbed, in index order
             while (true) {
+                const int cu = mi < nmem ? seeds[mi] : 0x7fffffff;
+                const int hu = heap.n > 0 ? heap.a[0] : 0x7fffffff;
+                if (cu == 0x7fffffff && hu == 0x7fffffff) break;
+                int u;
+                if (hu < cu) { u = hu; heap.pop(); }
+                else { u = cu; ++mi; }
+                if (u <= last) continue;
                 last = u;
                 ++nvis;
                 err = visit_vertex(a, vo + u, u, vo, nv, S->co, S->nc, S->R, S->thr, S->thr_cos, d.cap, s_tmp[w], st,
@@ -1297,13 +1427,6 @@ __global__ __launch_bounds__(64 * SVM_T1_WAVES) void k_t1_cluster(Dev d, int pcu
                                    },
                                    [&](int, int, const int*) {});
                 if (err || a.failed() || heap_full) break;
-                int nxt = -1;
-                while (heap.n > 0) {
-                    int x = heap.pop();
-                    if (x > last) { nxt = x; break; }
-                }
-                if (nxt < 0) break;
-                u = nxt;
             }
             const bool ok = !(err || a.failed() || heap_full);
             misc[3] = !ok ? 0 : ((st.flips || st.skipped) ? 2 : 1);
@@ -1322,7 +1445,7 @@ __global__ __launch_bounds__(64 * SVM_T1_WAVES) void k_t1_cluster(Dev d, int pcu
                                                             L.cring, misc + 4, misc + 5, lane, 64, sync);
         if (lane == 0) {
             if (!ok) {
-                // this candidate's run could not be completed here: the group run redoes it
+                // this component's run could not be completed here: the group run redoes it
                 d.t1_flagged[idx] = (int)gen;
                 S->t1_overlap = 1;
             } else if (outcome == 2) {
@@ -1360,11 +1483,19 @@ __global__ __launch_bounds__(SVM_T1_GT) void k_t1_group(Dev d, int pcur) {
     __shared__ int s_absorb[SVM_T1_GABS];
     __shared__ int s_nmem, s_nabs, s_vfill, s_cfill, s_status, s_nlv, s_nlc;
     __shared__ long long s_flips, s_skipped;
+    __shared__ int s_hit;
     const int s = d.s0 + (int)blockIdx.x;
     SysParams* S = d.sys + s;
     const int lane = threadIdx.x;     // the walk runs on thread 0; the rest is shared by the block
     const BlockSync sync;
     T1GroupLog& G = d.t1_group[s];
+    const int vo = S->vo, nv = S->nv;
+    if (lane == 0) s_hit = 0;
+    __syncthreads();
+    t1_check_sphere(d, S, vo, nv, lane, SVM_T1_GT, &s_hit);
+    __syncthreads();
+    if (s_hit && lane == 0) S->t1_overlap = 1;
+    __syncthreads();
     if (!S->t1_overlap || S->t1_overflow) {
         if (lane == 0) {
             G.ok = 0;
@@ -1374,13 +1505,12 @@ __global__ __launch_bounds__(SVM_T1_GT) void k_t1_group(Dev d, int pcur) {
         return;
     }
     const int gen = S->t1_gen;
-    const int vo = S->vo, nv = S->nv;
     const int total = min(*d.t1_count, SVM_T1_MAXCL);
     if (lane == 0) s_nmem = 0;
     __syncthreads();
     for (int i = lane; i < total; i += SVM_T1_GT) {
         int row = d.t1_list[i];
-        if (row >= vo && row < vo + nv && d.t1_flagged[i] == gen) {
+        if (row >= vo && row < vo + nv && d.t1_flagged[d.t1_lead[i]] == gen) {
             int k = atomicAdd(&s_nmem, 1);
             if (k < SVM_T1_GMAX) s_mem[k] = row - vo;
         }
@@ -1473,19 +1603,32 @@ __global__ __launch_bounds__(SVM_T1_GT) void k_t1_group(Dev d, int pcur) {
             if (lane == 0) S->n_seq_reason[2] += 1;
             return;
         }
+        // absorb the runs met: flag them, then add every candidate each covers
         if (lane == 0) {
-            s_status = 0;
+            int nnew = 0;
             for (int k = 0; k < nabs; ++k) {
-                int o = s_absorb[k];
+                const int o = s_absorb[k];
                 if (d.t1_flagged[o] == gen) continue;
-                if (nmem == SVM_T1_GMAX) { S->n_seq_reason[1] += 1; s_status = 1; break; }
                 d.t1_flagged[o] = gen;
-                s_mem[nmem++] = d.t1_list[o] - vo;
+                s_absorb[nnew++] = o;
             }
-            s_nmem = nmem;
+            s_nabs = nnew;
         }
         __syncthreads();
-        if (s_status) return;
+        const int nnew = s_nabs;
+        for (int i = lane; i < total; i += SVM_T1_GT) {
+            const int ld = d.t1_lead[i];
+            bool hit = false;
+            for (int k = 0; k < nnew; ++k) hit |= s_absorb[k] == ld;
+            if (!hit) continue;
+            const int m = atomicAdd(&s_nmem, 1);
+            if (m < SVM_T1_GMAX) s_mem[m] = d.t1_list[i] - vo;
+        }
+        __syncthreads();
+        if (s_nmem > SVM_T1_GMAX) {
+            if (lane == 0) S->n_seq_reason[1] += 1;
+            return;
+        }
         nmem = s_nmem;
         a.restore(lane, SVM_T1_GT);
         __syncthreads();

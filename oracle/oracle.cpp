@@ -1058,6 +1058,8 @@ double or_acos(double x) { return O_ACOS(x); }
 // the library's short-arc form of acos, compared with svm::acos by the tests
 double or_acos_hi(double x) { return svm::acos_hi(x); }
 double or_own_acos(double x) { return svm::acos(x); }
+// the library's constant for cos(+-pi/2) in the T1 quarter turn
+double or_cos_quarter_turn() { return svm::kCosQuarterTurn; }
 double or_sin(double x) { return O_SIN(x); }
 double or_cos(double x) { return O_COS(x); }
 double or_log(double x) { return svm::log(x); }

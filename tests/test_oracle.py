@@ -65,6 +65,16 @@ def test_short_arc_acos_is_acos(oracle):
     assert math.isnan(oracle.or_acos_hi(math.nan))
 
 
+def test_quarter_turn_constants_are_the_trig_values(oracle):
+    """The T1 flip's quarter turn takes cos and sin of -+pi/2 as constants;
+    they are the bits that cos() and sin() return there."""
+    c = oracle.or_cos_quarter_turn()
+    for th in (-0.5 * math.pi, 0.5 * math.pi):
+        assert np.float64(oracle.or_cos(th)).view(np.uint64) == np.float64(c).view(np.uint64)
+        assert oracle.or_sin(th) == math.copysign(1.0, th)
+    assert c == math.cos(0.5 * math.pi)
+
+
 def test_philox_known_answers(oracle):
     """Philox4x32-10 known-answer vectors of the Random123 distribution."""
     cases = [
