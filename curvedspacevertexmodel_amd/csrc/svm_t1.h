@@ -639,9 +639,19 @@ __device__ __attribute__((always_inline)) inline int ring_only(const int (&e)[SV
     return cnt == 1 ? p : -1;
 }
 
+// What the cached cell-set search has read, handed to the cached flip that
+// follows it (nothing is written in between): the cell set's slots, ring
+// lengths and first ring entries, in the order cs[0..3], and the slots of
+// vertex1 and vertex2.
+struct FlipRegs {
+    int hc[4], n[4];
+    int e[4][SVM_T1_RR];
+    int h1, h2;
+};
+
 template <class A>
 __device__ __attribute__((always_inline)) int cell_vertex_set_cached(A& a, int vertex1, int vertex2, int* cs,
-                                                                     int* vs) {
+                                                                     int* vs, FlipRegs& fr) {
     const int h1 = a.vh(vertex1), h2 = a.vh(vertex2);
     const int* c1 = a.vcn_ptr(h1);
     const int* c2 = a.vcn_ptr(h2);
@@ -659,12 +669,12 @@ __device__ __attribute__((always_inline)) int cell_vertex_set_cached(A& a, int v
     }
     if (n[0] < 2 || n[0] > SVM_T1_RR || n[1] < 2 || n[1] > SVM_T1_RR || n[2] < 2 || n[2] > SVM_T1_RR) return -1;
     int nl[3], nn[3];
+    int et[3][SVM_T1_RR];
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
-        int e[SVM_T1_RR];
         const int* r = a.ring_ptr(hc[t]);
-        ring_regs(r, e);
-        ring_nbrs(r, e, n[t], vertex1, nl[t], nn[t]);
+        ring_regs(r, et[t]);
+        ring_nbrs(r, et[t], n[t], vertex1, nl[t], nn[t]);
         if (nl[t] == vertex2) cs[0] = cells[t];
         else if (nn[t] == vertex2) cs[2] = cells[t];
         else cs[1] = cells[t];
@@ -675,32 +685,49 @@ __device__ __attribute__((always_inline)) int cell_vertex_set_cached(A& a, int v
     const int h3 = a.ch(cs[3]);
     const int n3 = a.cnn_h(h3);
     if (n3 < 2 || n3 > SVM_T1_RR) return -1;
-    int e3[SVM_T1_RR];
-    ring_regs(a.ring_ptr(h3), e3);
+    ring_regs(a.ring_ptr(h3), fr.e[3]);
     vs[0] = t1 == 2 ? nl[2] : (t1 == 1 ? nl[1] : nl[0]);
     vs[1] = t1 == 2 ? nn[2] : (t1 == 1 ? nn[1] : nn[0]);
-    ring_nbrs(a.ring_ptr(h3), e3, n3, vertex2, vs[3], vs[2]);
+    ring_nbrs(a.ring_ptr(h3), fr.e[3], n3, vertex2, vs[3], vs[2]);
+    // the hand-off, in cell-set order (a repeated cell has the same slot and ring)
+    const int t0 = cells[0] == cs[0] ? 0 : (cells[1] == cs[0] ? 1 : 2);
+    const int t2 = cells[0] == cs[2] ? 0 : (cells[1] == cs[2] ? 1 : 2);
+    const int tt[3] = {t0, t1, t2};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const int t = tt[q];
+        fr.hc[q] = t == 0 ? hc[0] : (t == 1 ? hc[1] : hc[2]);
+        fr.n[q] = t == 0 ? n[0] : (t == 1 ? n[1] : n[2]);
+#pragma unroll
+        for (int k = 0; k < SVM_T1_RR; ++k) fr.e[q][k] = t == 0 ? et[0][k] : (t == 1 ? et[1][k] : et[2][k]);
+    }
+    fr.hc[3] = h3;
+    fr.n[3] = n3;
+    fr.h1 = h1;
+    fr.h2 = h2;
     return 1;
 }
 
+// The flip with what cell_vertex_set_cached read (fr): only vertices b and c
+// (vs[1], vs[2]) are looked up here.
 template <class A>
 __device__ __attribute__((always_inline)) int t1_flip_cached(A& a, int vertex1, int vertex2, const int* cs,
-                                                             const int* vs, double* v1, double R, int cap) {
-    const int hc1 = a.ch(cs[1]), hc3 = a.ch(cs[3]);
-    const int n1 = a.cnn_h(hc1), n3 = a.cnn_h(hc3);
+                                                             const int* vs, double* v1, double R, int cap,
+                                                             const FlipRegs& fr) {
+    const int hc1 = fr.hc[1], hc3 = fr.hc[3];
+    const int n1 = fr.n[1], n3 = fr.n[3];
     if (n1 + 1 > cap || n3 + 1 > cap) return 4;
-    const int h2 = a.vh(vertex2), hC = a.vh(vs[2]), h1 = a.vh(vertex1), hb = a.vh(vs[1]);
-    const int hc0 = a.ch(cs[0]), hc2 = a.ch(cs[2]);
-    const int n0 = a.cnn_h(hc0), n2 = a.cnn_h(hc2);
+    const int h2 = fr.h2, hC = a.vh(vs[2]), h1 = fr.h1, hb = a.vh(vs[1]);
+    const int hc0 = fr.hc[0], hc2 = fr.hc[2];
+    const int n0 = fr.n[0], n2 = fr.n[2];
     if (h1 == h2 || h1 == hb || h1 == hC || h2 == hb || h2 == hC || hb == hC || hc0 == hc1 || hc0 == hc2 ||
         hc0 == hc3 || hc1 == hc2 || hc1 == hc3 || hc2 == hc3)
         return -1;
     if (n0 < 1 || n0 > SVM_T1_RR || n2 < 1 || n2 > SVM_T1_RR || n1 + 1 > SVM_T1_RR || n3 + 1 > SVM_T1_RR) return -1;
-    int e0[SVM_T1_RR], e1[SVM_T1_RR], e2[SVM_T1_RR], e3[SVM_T1_RR];
-    ring_regs(a.ring_ptr(hc0), e0);
-    ring_regs(a.ring_ptr(hc1), e1);
-    ring_regs(a.ring_ptr(hc2), e2);
-    ring_regs(a.ring_ptr(hc3), e3);
+    const int(&e0)[SVM_T1_RR] = fr.e[0];
+    const int(&e1)[SVM_T1_RR] = fr.e[1];
+    const int(&e2)[SVM_T1_RR] = fr.e[2];
+    const int(&e3)[SVM_T1_RR] = fr.e[3];
     const int p0 = ring_only(e0, n0, vertex2), p1 = ring_only(e1, n1, vertex1);
     const int p2 = ring_only(e2, n2, vertex1), p3 = ring_only(e3, n3, vertex2);
     if (p0 < 0 || p1 < 0 || p2 < 0 || p3 < 0) return -1;
@@ -786,7 +813,12 @@ __device__ __attribute__((always_inline)) int visit_vertex(A& a, int gu, int u, 
             if (short_edge(v1, q, n1, pnorm(q), R, thr, cos_cut)) {
                 int cs[4], vs[4] = {0, 0, 0, 0};
                 int got = -1;
-                if constexpr (A::kCached) got = cell_vertex_set_cached(a, gu, vertex2, cs, vs);
+                FlipRegs fr;
+                bool have = false;
+                if constexpr (A::kCached) {
+                    got = cell_vertex_set_cached(a, gu, vertex2, cs, vs, fr);
+                    have = got > 0;
+                }
                 if (got < 0) got = cell_vertex_set(a, gu, vertex2, cs, vs) ? 1 : 0;
                 if (!got) {
                     ++st.skipped;
@@ -798,9 +830,11 @@ __device__ __attribute__((always_inline)) int visit_vertex(A& a, int gu, int u, 
                     inside &= vs[k] >= vo && vs[k] < vo + nv;
                 }
                 if (!inside) return 5;
-                if (a.cnn(cs[0]) == 3 || a.cnn(cs[2]) == 3) continue;
+                if (have ? (fr.n[0] == 3 || fr.n[2] == 3) : (a.cnn(cs[0]) == 3 || a.cnn(cs[2]) == 3)) continue;
                 int r = -1;
-                if constexpr (A::kCached) r = t1_flip_cached(a, gu, vertex2, cs, vs, v1, R, cap);
+                if constexpr (A::kCached) {
+                    if (have) r = t1_flip_cached(a, gu, vertex2, cs, vs, v1, R, cap, fr);
+                }
                 if (r < 0) r = t1_flip(a, gu, vertex2, cs, vs, v1, R, cap, tmp);
                 if (r) return r;
                 n1 = pnorm(v1);
@@ -1274,7 +1308,7 @@ __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int n
 // than 64 flippable candidates, or more than 64 * 64 * SVM_T1_NEAR_MAXW
 // vertex rows, keep one run per candidate.
 #ifndef SVM_T1_NEAR
-#define SVM_T1_NEAR 2.5
+#define SVM_T1_NEAR 3.0
 #endif
 #define SVM_T1_NEAR_MAXW 16
 
