@@ -260,8 +260,74 @@ struct CacheAcc {
     // forget the memo (after restore(), the slots are untouched again)
     __device__ __attribute__((always_inline)) void forget() { mvk0 = mvk1 = mck0 = mck1 = -1; }
     __device__ __attribute__((always_inline)) void pos(int v, double* p) {
+        if (v != mvk0 && v != mvk1) {
+            // the first probe's key and position in one round of loads
+            const unsigned h = vhash(v);
+            const int k = vkey[h];
+            const double a0 = vp[3 * h], a1 = vp[3 * h + 1], a2 = vp[3 * h + 2];
+            if (k == v) {
+                vuse[h] = 3;
+                mvk1 = mvk0; mvh1 = mvh0;
+                mvk0 = v; mvh0 = (int)h;
+                p[0] = a0; p[1] = a1; p[2] = a2;
+                return;
+            }
+        }
         int h = vslot(v);
         p[0] = vp[3 * h]; p[1] = vp[3 * h + 1]; p[2] = vp[3 * h + 2];
+    }
+    // Slots of N vertices, and of N cells with their counts and first ring
+    // entries, each in one round of LDS loads: the first probe of every key is
+    // read together; a key found there is marked touched (as vslot / cslot
+    // would), any other goes through vslot / cslot, in argument order.  A
+    // lookup only ever fills an empty slot, so a key found in the round is
+    // where the sequential lookups would have found it.
+    template <int N>
+    __device__ __attribute__((always_inline)) void vh_n(const int (&keys)[N], int (&hs)[N]) {
+        unsigned h[N];
+        int k[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            h[i] = vhash(keys[i]);
+            k[i] = vkey[h[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            if (k[i] == keys[i]) {
+                vuse[h[i]] = 3;
+                hs[i] = (int)h[i];
+            } else {
+                hs[i] = vslot(keys[i]);
+            }
+        }
+    }
+    template <int N, int RR>
+    __device__ __attribute__((always_inline)) void ch_ring_n(const int (&keys)[N], int (&hs)[N], int (&ns)[N],
+                                                             int (&e)[N][RR]) {
+        unsigned h[N];
+        int k[N], u[N], c[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            h[i] = chash(keys[i]);
+            k[i] = ckey[h[i]];
+            u[i] = cuse[h[i]];
+            c[i] = ccnt[h[i]];
+#pragma unroll
+            for (int j = 0; j < RR; ++j) e[i][j] = cring[h[i] * SVM_T1_RC + j];
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            if (k[i] == keys[i] && !(u[i] & 4)) {
+                cuse[h[i]] = 3;
+                hs[i] = (int)h[i];
+                ns[i] = c[i];
+            } else {
+                hs[i] = cslot(keys[i]);
+                ns[i] = ccnt[hs[i]];
+#pragma unroll
+                for (int j = 0; j < RR; ++j) e[i][j] = cring[hs[i] * SVM_T1_RC + j];
+            }
+        }
     }
     __device__ __attribute__((always_inline)) void set_pos(int v, const double* p) {
         int h = vslot(v);
@@ -652,7 +718,9 @@ struct FlipRegs {
 template <class A>
 __device__ __attribute__((always_inline)) int cell_vertex_set_cached(A& a, int vertex1, int vertex2, int* cs,
                                                                      int* vs, FlipRegs& fr) {
-    const int h1 = a.vh(vertex1), h2 = a.vh(vertex2);
+    int vh2[2];
+    a.template vh_n<2>({vertex1, vertex2}, vh2);
+    const int h1 = vh2[0], h2 = vh2[1];
     const int* c1 = a.vcn_ptr(h1);
     const int* c2 = a.vcn_ptr(h2);
     const int cells[3] = {c1[0], c1[1], c1[2]};
@@ -661,19 +729,15 @@ __device__ __attribute__((always_inline)) int cell_vertex_set_cached(A& a, int v
 #pragma unroll
     for (int ff = 0; ff < 3; ++ff)
         if (ct[ff] != cells[0] && ct[ff] != cells[1] && ct[ff] != cells[2]) cs[3] = ct[ff];
-    int hc[3], n[3];
-#pragma unroll
-    for (int t = 0; t < 3; ++t) {
-        hc[t] = a.ch(cells[t]);
-        n[t] = a.cnn_h(hc[t]);
-    }
+    // the three cells of vertex1 and the fourth cell, with their rings, in one round
+    int hc[4], n[4];
+    int et[4][SVM_T1_RR];
+    a.template ch_ring_n<4, SVM_T1_RR>({cells[0], cells[1], cells[2], cs[3] >= 0 ? cs[3] : cells[0]}, hc, n, et);
     if (n[0] < 2 || n[0] > SVM_T1_RR || n[1] < 2 || n[1] > SVM_T1_RR || n[2] < 2 || n[2] > SVM_T1_RR) return -1;
     int nl[3], nn[3];
-    int et[3][SVM_T1_RR];
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
         const int* r = a.ring_ptr(hc[t]);
-        ring_regs(r, et[t]);
         ring_nbrs(r, et[t], n[t], vertex1, nl[t], nn[t]);
         if (nl[t] == vertex2) cs[0] = cells[t];
         else if (nn[t] == vertex2) cs[2] = cells[t];
@@ -682,10 +746,11 @@ __device__ __attribute__((always_inline)) int cell_vertex_set_cached(A& a, int v
     if (cs[0] < 0 || cs[1] < 0 || cs[2] < 0 || cs[3] < 0) return 0;
     // cs[1] is one of the three cells: its walk around vertex1 is done
     const int t1 = cells[2] == cs[1] ? 2 : (cells[1] == cs[1] ? 1 : 0);
-    const int h3 = a.ch(cs[3]);
-    const int n3 = a.cnn_h(h3);
+    const int h3 = hc[3];
+    const int n3 = n[3];
     if (n3 < 2 || n3 > SVM_T1_RR) return -1;
-    ring_regs(a.ring_ptr(h3), fr.e[3]);
+#pragma unroll
+    for (int k = 0; k < SVM_T1_RR; ++k) fr.e[3][k] = et[3][k];
     vs[0] = t1 == 2 ? nl[2] : (t1 == 1 ? nl[1] : nl[0]);
     vs[1] = t1 == 2 ? nn[2] : (t1 == 1 ? nn[1] : nn[0]);
     ring_nbrs(a.ring_ptr(h3), fr.e[3], n3, vertex2, vs[3], vs[2]);
@@ -717,7 +782,9 @@ __device__ __attribute__((always_inline)) int t1_flip_cached(A& a, int vertex1, 
     const int hc1 = fr.hc[1], hc3 = fr.hc[3];
     const int n1 = fr.n[1], n3 = fr.n[3];
     if (n1 + 1 > cap || n3 + 1 > cap) return 4;
-    const int h2 = fr.h2, hC = a.vh(vs[2]), h1 = fr.h1, hb = a.vh(vs[1]);
+    int vbc[2];
+    a.template vh_n<2>({vs[2], vs[1]}, vbc);
+    const int h2 = fr.h2, hC = vbc[0], h1 = fr.h1, hb = vbc[1];
     const int hc0 = fr.hc[0], hc2 = fr.hc[2];
     const int n0 = fr.n[0], n2 = fr.n[2];
     if (h1 == h2 || h1 == hb || h1 == hC || h2 == hb || h2 == hC || hb == hC || hc0 == hc1 || hc0 == hc2 ||
@@ -805,8 +872,10 @@ __device__ __attribute__((always_inline)) int visit_vertex(A& a, int gu, int u, 
     double v1[3];
     a.pos(gu, v1);
     double n1 = pnorm(v1);       // changes only with v1, in a flip
+    // u's neighbour list, read again after each flip (a flip rewrites it)
+    int nbl[3] = {a.vn(gu, 0), a.vn(gu, 1), a.vn(gu, 2)};
     for (int vv = 0; vv < 3; ++vv) {
-        int vertex2 = a.vn(gu, vv);
+        int vertex2 = nbl[vv];
         if (gu < vertex2) {
             double q[3];
             a.pos(vertex2, q);
@@ -838,6 +907,7 @@ __device__ __attribute__((always_inline)) int visit_vertex(A& a, int gu, int u, 
                 if (r < 0) r = t1_flip(a, gu, vertex2, cs, vs, v1, R, cap, tmp);
                 if (r) return r;
                 n1 = pnorm(v1);
+                nbl[0] = a.vn(gu, 0); nbl[1] = a.vn(gu, 1); nbl[2] = a.vn(gu, 2);
                 ++st.flips;
                 on_flip(gu, vertex2, cs);
                 int dirty[6] = {u, vertex2 - vo, vs[0] - vo, vs[1] - vo, vs[2] - vo, vs[3] - vo};
