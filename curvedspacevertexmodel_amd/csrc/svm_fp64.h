@@ -62,12 +62,11 @@ SVM_HD double withwords(int32_t hi, uint32_t lo) {
 // in an exponent scaling (for arguments below 2^-767) and a class check
 // (+-0 and +inf return themselves).  For arguments in [2^-767, DBL_MAX] the
 // scaling is by 2^0 and the class check passes the refined value through, so
-// there the refinement alone is the same computation; every other argument
-// (tiny, zero, negative, infinite, NaN) takes __builtin_sqrt itself.  This
-// saves the wrapper's eight instructions where they do nothing.
-SVM_HD double sqrt_rn(double x) {
+// there the refinement alone is the same computation.
+// sqrt_in_range: that refinement alone; the caller guarantees the argument is
+// in [2^-767, DBL_MAX].
+SVM_HD double sqrt_in_range(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    if (__builtin_expect(!(x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023), 0)) return __builtin_sqrt(x);
     const double y = __builtin_amdgcn_rsq(x);
     double g = x * y;
     double h = y * 0.5;
@@ -82,6 +81,13 @@ SVM_HD double sqrt_rn(double x) {
 #else
     return __builtin_sqrt(x);
 #endif
+}
+// sqrt_rn: the refinement where it is exact; every other argument (tiny,
+// zero, negative, infinite, NaN) takes __builtin_sqrt itself.  This saves the
+// wrapper's eight instructions where they do nothing, at the cost of a branch.
+SVM_HD double sqrt_rn(double x) {
+    if (__builtin_expect(!(x >= 0x1p-767 && x <= 0x1.fffffffffffffp+1023), 0)) return __builtin_sqrt(x);
+    return sqrt_in_range(x);
 }
 
 // ---------------------------------------------------------------- acos ----
@@ -146,10 +152,42 @@ SVM_HD double acos_hi(double x) {
     double p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
     double q = 1.0 + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
     double r = p / q;
-    double s = sqrt_rn(z);
+    double s = sqrt_in_range(z);     // z = (1 - x) / 2 is in [2^-54, 0.25]
     double df = bitsd(dbits(s) & 0xffffffff00000000ull);
     double c = (z - df * df) / (s + df);
     return 2.0 * (df + (r * s + c));
+}
+
+// acos for arguments that are mostly in (-1, 0.5): the cosines of a cell
+// polygon's interior angles (about 120 degrees).  There acos takes its
+// |x| < 0.5 and x <= -0.5 ranges; these two are evaluated together and
+// selected, without the third range's division.  Any other argument goes to
+// acos itself.
+SVM_HD double acos_lo(double x) {
+    if (__builtin_expect(!(x < 0.5 && x > -1.0), 0)) return svm::acos(x);
+    const double pi = 3.14159265358979311600e+00;
+    const double pio2_hi = 1.57079632679489655800e+00;
+    const double pio2_lo = 6.12323399573676603587e-17;
+    const double pS0 = 1.66666666666666657415e-01;
+    const double pS1 = -3.25565818622400915405e-01;
+    const double pS2 = 2.01212532134862925881e-01;
+    const double pS3 = -4.00555345006794114027e-02;
+    const double pS4 = 7.91534994289814532176e-04;
+    const double pS5 = 3.47933107596021167570e-05;
+    const double qS1 = -2.40339491173441421878e+00;
+    const double qS2 = 2.02094576023350569471e+00;
+    const double qS3 = -6.88283971605453293030e-01;
+    const double qS4 = 7.70381505559019352791e-02;
+    const int32_t ix = hiword(x) & 0x7fffffff;
+    const bool small = ix < 0x3fe00000;                     // |x| < 0.5; otherwise x <= -0.5
+    const double z = small ? x * x : (1.0 + x) * 0.5;
+    const double p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
+    const double q = 1.0 + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
+    const double r = p / q;
+    const double s = sqrt_in_range(small ? 0.25 : z);       // used by the x <= -0.5 range, where z >= 2^-54
+    double res = small ? pio2_hi - (x - (pio2_lo - x * r)) : pi - 2.0 * (s + (r * s - pio2_lo));
+    if (ix <= 0x3c600000) res = pio2_hi + pio2_lo;          // |x| < 2^-57
+    return res;
 }
 
 // ------------------------------------------------------ sin/cos kernels ----

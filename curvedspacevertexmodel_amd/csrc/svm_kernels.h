@@ -165,7 +165,7 @@ __global__ __launch_bounds__(SVM_TILE) void k_geometry(Dev d, int pin, int skip_
             double ci[3] = {pc[1] * pn[2] - pc[2] * pn[1], pc[2] * pn[0] - pc[0] * pn[2], pc[0] * pn[1] - pc[1] * pn[0]};
             double nci = svm::sqrt_rn(dot0(ci, ci));
             double determinant = pl[0] * ci[0] + pl[1] * ci[1] + pl[2] * ci[2];
-            double angle = svm::acos(-dot0(ci, cm) / (nci * ncm));
+            double angle = svm::acos_lo(-dot0(ci, cm) / (nci * ncm));
             if (determinant > 0) angle *= -1;
             if (angle < 0) angle += 2. * SVM_PI;
             angles += angle;

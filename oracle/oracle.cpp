@@ -1057,6 +1057,7 @@ double or_tree_sum(const double* x, int n) { return tree_sum(std::vector<double>
 double or_acos(double x) { return O_ACOS(x); }
 // the library's short-arc form of acos, compared with svm::acos by the tests
 double or_acos_hi(double x) { return svm::acos_hi(x); }
+double or_acos_lo(double x) { return svm::acos_lo(x); }
 double or_own_acos(double x) { return svm::acos(x); }
 // the library's constant for cos(+-pi/2) in the T1 quarter turn
 double or_cos_quarter_turn() { return svm::kCosQuarterTurn; }

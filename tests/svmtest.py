@@ -28,9 +28,9 @@ def oracle_lib(libm=False):
     L = C.CDLL(path)
     L.or_create.restype = C.c_void_p
     for fn in ("or_energy", "or_kinetic_energy", "or_tree_sum", "or_acos", "or_sin", "or_cos", "or_log",
-               "or_acos_hi", "or_own_acos", "or_cos_quarter_turn"):
+               "or_acos_hi", "or_acos_lo", "or_own_acos", "or_cos_quarter_turn"):
         getattr(L, fn).restype = C.c_double
-    for fn in ("or_acos", "or_sin", "or_cos", "or_log", "or_acos_hi", "or_own_acos"):
+    for fn in ("or_acos", "or_sin", "or_cos", "or_log", "or_acos_hi", "or_acos_lo", "or_own_acos"):
         getattr(L, fn).argtypes = [C.c_double]
     L.or_energy.argtypes = [C.c_void_p, C.c_int]
     L.or_kinetic_energy.argtypes = [C.c_void_p, C.c_int]

@@ -48,21 +48,29 @@ def test_own_math_within_one_ulp_of_libm(oracle, fn, ref, lo, hi):
         assert f(-0.5 * math.pi) == math.cos(-0.5 * math.pi)
 
 
-def test_short_arc_acos_is_acos(oracle):
-    """acos_hi (the library's form for arc cosines, which evaluates only the
-    x >= 0.5 range) returns acos's bits for every argument: inside [0.5, 1),
-    at the range's ends, outside it and for non-finite input."""
+@pytest.mark.parametrize("fn", ["or_acos_hi", "or_acos_lo"])
+def test_range_forms_of_acos_are_acos(oracle, fn):
+    """acos_hi (arc cosines: evaluates only the x >= 0.5 range) and acos_lo
+    (polygon angles: the |x| < 0.5 and x <= -0.5 ranges) return acos's bits
+    for every argument: inside their ranges, at the ranges' ends, outside them
+    and for non-finite input."""
     rng = np.random.default_rng(5)
     xs = np.concatenate([rng.uniform(-1.0, 1.0, 20000), rng.uniform(0.5, 1.0, 20000),
-                         1.0 - np.exp(rng.uniform(-37.0, -0.7, 5000))])
-    edges = [0.5, np.nextafter(0.5, 0.0), np.nextafter(0.5, 1.0), 1.0, np.nextafter(1.0, 0.0),
-             np.nextafter(1.0, 2.0), -1.0, 0.0, -0.0, 2.0, 1e-300, np.inf, -np.inf]
+                         rng.uniform(-1.0, -0.5, 20000), rng.uniform(-0.5, 0.5, 20000),
+                         1.0 - np.exp(rng.uniform(-37.0, -0.7, 5000)),
+                         -1.0 + np.exp(rng.uniform(-37.0, -0.7, 5000)),
+                         np.exp(rng.uniform(-700.0, -0.7, 5000)) * rng.choice([-1.0, 1.0], 5000)])
+    edges = [0.5, np.nextafter(0.5, 0.0), np.nextafter(0.5, 1.0), -0.5, np.nextafter(-0.5, 0.0),
+             np.nextafter(-0.5, -1.0), 1.0, np.nextafter(1.0, 0.0), np.nextafter(1.0, 2.0), -1.0,
+             np.nextafter(-1.0, 0.0), np.nextafter(-1.0, -2.0), 0.0, -0.0, 2.0, -2.0, 1e-300, -1e-300,
+             2.0 ** -57, -(2.0 ** -57), 2.0 ** -56, np.inf, -np.inf]
+    f = getattr(oracle, fn)
     for x in list(xs) + edges:
-        a, b = oracle.or_acos_hi(float(x)), oracle.or_own_acos(float(x))
+        a, b = f(float(x)), oracle.or_own_acos(float(x))
         assert a == b or (math.isnan(a) and math.isnan(b)), x
         if not math.isnan(a):
             assert np.float64(a).view(np.uint64) == np.float64(b).view(np.uint64), x
-    assert math.isnan(oracle.or_acos_hi(math.nan))
+    assert math.isnan(f(math.nan))
 
 
 def test_quarter_turn_constants_are_the_trig_values(oracle):
