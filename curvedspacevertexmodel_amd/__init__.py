@@ -31,7 +31,7 @@ EXPORTS = [
     "svm_compute_forces", "svm_move_particles", "svm_enforce_topology", "svm_get_energies",
     "svm_get_counters", "svm_set_brownian", "svm_brownian_steps", "svm_set_active",
     "svm_active_steps", "svm_run_graph", "svm_fire_set_parameters", "svm_fire_minimize",
-    "svm_fire_get_state", "svm_set_t1_mode", "svm_get_t1_diagnostics", "svm_set_profiling", "svm_get_timing",
+    "svm_fire_get_state", "svm_set_t1_mode", "svm_set_t1_component_distance", "svm_get_t1_component_distance", "svm_get_t1_diagnostics", "svm_set_profiling", "svm_get_timing",
     "svm_copy_probe", "svm_spatial_order", "svm_time_vertex_pass",
     "svm_set_force_precision", "svm_get_force_error", "svm_delaunay_triangles",
     "svm_cell_list", "svm_mlp_create", "svm_mlp_destroy", "svm_mlp_set_parameters", "svm_mlp_get_parameters",
@@ -644,6 +644,16 @@ class Context:
 
     def set_t1_mode(self, sequential):
         self._ck(lib().svm_set_t1_mode(self.h, C.c_int32(1 if sequential else 0)), "svm_set_t1_mode")
+
+    def set_t1_component_distance(self, cells):
+        """Flippable candidates closer than `cells` cell sizes share one T1 dry run (0: one run each).
+        Same trajectory for every value."""
+        self._ck(lib().svm_set_t1_component_distance(self.h, C.c_double(cells)), "svm_set_t1_component_distance")
+
+    def t1_component_distance(self):
+        x = C.c_double(0.0)
+        self._ck(lib().svm_get_t1_component_distance(self.h, C.byref(x)), "svm_get_t1_component_distance")
+        return x.value
 
     def set_profiling(self, on):
         self._ck(lib().svm_set_profiling(self.h, C.c_int32(1 if on else 0)), "svm_set_profiling")

@@ -963,6 +963,7 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
     r |= dalloc(ctx, &d.t1_flagged, SVM_T1_MAXCL);
     r |= dalloc(ctx, &d.t1_logged, SVM_T1_MAXCL);
     r |= dalloc(ctx, &d.t1_lead, SVM_T1_MAXCL);
+    d.t1_near = SVM_T1_NEAR;
     r |= dalloc(ctx, &d.t1_total, 1);
     r |= dalloc(ctx, &d.t1_bcount, 1);
     r |= dalloc(ctx, &d.t1_blist, SVM_T1_MAXBL);
@@ -2094,6 +2095,19 @@ int svm_set_t1_mode(svm_context* ctx, int32_t mode) {
     if (!ctx || (mode != 0 && mode != 1)) return fail(ctx, SVM_ERR_ARG, "T1 mode must be 0 or 1");
     ctx->t1_sequential = mode;
     drop_graphs(ctx);
+    return SVM_OK;
+}
+
+int svm_set_t1_component_distance(svm_context* ctx, double cells) {
+    if (!ctx || !(cells >= 0.0 && cells <= 64.0))
+        return fail(ctx, SVM_ERR_ARG, "component distance must be in [0, 64] cell sizes");
+    ctx->d.t1_near = cells;
+    drop_graphs(ctx);
+    return SVM_OK;
+}
+int svm_get_t1_component_distance(svm_context* ctx, double* cells) {
+    if (!ctx || !cells) return fail(ctx, SVM_ERR_ARG, "null argument");
+    *cells = ctx->d.t1_near;
     return SVM_OK;
 }
 

@@ -106,6 +106,7 @@ struct Dev {
     int32_t* t1_flagged;          // per candidate: generation in which its run overlapped
     int32_t* t1_logged;           // per candidate: generation in which its run wrote a log
     int32_t* t1_lead;             // per candidate: slot of the candidate whose dry run covers it
+    double t1_near;               // component distance of the dry runs, in cell sizes (0: a run per candidate)
     int32_t* t1_total;            // flippable-list length of the current step (copied before the reset)
     int32_t* t1_bcount;           // blocked candidates appended this step
     int32_t* t1_blist;            // blocked candidate vertex rows

@@ -1363,7 +1363,8 @@ __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int n
 }
 
 // Predicted components.  Two flippable candidates of a sphere closer than
-// SVM_T1_NEAR cell sizes (sqrt of the mean cell area) are taken to interact,
+// d.t1_near cell sizes (sqrt of the mean cell area; SVM_T1_NEAR unless set
+// otherwise) are taken to interact,
 // and the transitive closure of that relation splits the sphere's candidates
 // into components.  One dry run covers a whole component: its members and
 // the vertices their flips disturbed, in index order -- the sequential loop
@@ -1376,7 +1377,7 @@ __device__ inline void commit_sphere(const Dev& d, SysParams* S, int pcur, int n
 // the lowest list slot is its leader: it runs, stamps and logs under its own
 // slot, and every member records that slot in t1_lead.  Spheres with more
 // than 64 flippable candidates, or more than 64 * 64 * SVM_T1_NEAR_MAXW
-// vertex rows, keep one run per candidate.
+// vertex rows, keep one run per candidate, as does a distance of 0.
 #ifndef SVM_T1_NEAR
 #define SVM_T1_NEAR 3.0
 #endif
@@ -1403,7 +1404,7 @@ __device__ __attribute__((always_inline)) inline int t1_component(const Dev& d, 
     const int vo = S->vo, nv = S->nv;
     if (lane == 0) seeds[0] = row - vo;
     *lead = idx;
-    if (!(SVM_T1_NEAR > 0) || nv > 64 * 64 * SVM_T1_NEAR_MAXW) { sync(); return 1; }
+    if (!(d.t1_near > 0.0) || nv > 64 * 64 * SVM_T1_NEAR_MAXW) { sync(); return 1; }
     // the sphere's flippable candidates, in row order: lane L reads mask words
     // [L * MAXW, (L + 1) * MAXW) of the sphere (vo is a multiple of 64)
     const int w0 = vo >> 6, nw = (nv + 63) >> 6;
@@ -1449,7 +1450,7 @@ __device__ __attribute__((always_inline)) inline int t1_component(const Dev& d, 
     }
     sync();
     const double cell2 = 4.0 * SVM_PI * S->R * S->R / (double)S->nc;
-    const double near2 = (SVM_T1_NEAR * SVM_T1_NEAR) * cell2;
+    const double near2 = (d.t1_near * d.t1_near) * cell2;
     unsigned long long adj = 0ull;
     if (act)
         for (int k = 0; k < ncand; ++k) {

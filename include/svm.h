@@ -323,6 +323,16 @@ int svm_adam_get_state(svm_context* ctx, int32_t sys, svm_adam_state* state);
  * those cannot complete; 1 always uses the sequential pass.  Both give the
  * reference's sequential order exactly; 1 exists to show that. */
 int svm_set_t1_mode(svm_context* ctx, int32_t mode);
+/* Dry runs over predicted components (mode 0): flippable candidates of a
+ * sphere closer than `cells` cell sizes (sqrt of the mean cell area) to each
+ * other, transitively, share one dry run; 0 gives one run per candidate.
+ * Default 3.  Candidates that still interact are caught and redone by the
+ * group run, so every value gives the same trajectory; the distance only
+ * moves work between the dry-run and group kernels.  No counterpart in the
+ * reference (whose T1 loop is sequential): a tuning control of this
+ * library's parallel T1 path. */
+int svm_set_t1_component_distance(svm_context* ctx, double cells);
+int svm_get_t1_component_distance(svm_context* ctx, double* cells);
 /* T1 diagnostics, int64[6*n_systems]: steps whose sequential pass ran
  * because the flippable-candidate list overflowed, the overlapping candidates
  * were too many for one group, the group run ran out of room, the

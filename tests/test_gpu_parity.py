@@ -122,40 +122,59 @@ def test_t1_paths_agree(state512):
     assert runs[1][2][4] > runs[0][2][4]
 
 
-def test_t1_paths_agree_on_a_batch():
-    """The same at batch scale: 48 spheres of 300-6000 cells near the fluid
-    transition, 200 active steps.  The dry runs, group runs and their LDS
-    fast paths (mode 0) and the always-sequential pass on global memory
-    (mode 1) give bit-identical positions and topology; the run must include
-    thousands of flips and group runs."""
+def _t1_batch_run(sequential, distance=None):
+    """48 spheres of 300-6000 cells near the fluid transition: 50 Brownian
+    stabilisation steps, then 200 active steps, on one T1 path."""
     g = np.random.Generator(np.random.Philox(4242))
     params = [(int(round(np.exp(np.log(300) + u[0] * np.log(20)))), 3.75 + 0.2 * u[1]) for u in g.random((48, 2))]
-    runs = []
-    for sequential in (False, True):
-        ctx = svm.Context([dict(n_cells=n, a0=1.0, p0=p0, kr=1.0, t1_threshold=0.1, seed=2000 + i)
-                           for i, (n, p0) in enumerate(params)])
-        for i, (n, _) in enumerate(params):
-            pts = svm.generate_cell_points(n, 2000 + i)
-            ctx.build_from_cell_points(i, pts[svm.spatial_order(pts)])
-            ctx.set_directors(i, svm.generate_directors(ctx.positions(i), 2000 + i))
-            ctx.set_preferred_parameters(i, 1.0, 1.0)
-            ctx.set_brownian(i, dt=0.01, mu=1.0, T=0.0)
-        ctx.set_t1_mode(sequential)
-        ctx.brownian_steps(50)
-        for i, (n, p0) in enumerate(params):
-            ctx.set_preferred_parameters(i, 1.0, p0)
-            ctx.set_active(i, dt=0.01, mu=1.0, v0=0.3, tau=1.0)
-        ctx.active_steps(200)
-        runs.append(([ctx.positions(i) for i in range(len(params))],
-                     [ctx.get_topology(i, svmtest.RING_CAP) for i in range(len(params))], ctx.counters()))
-    for i in range(len(params)):
-        np.testing.assert_array_equal(runs[0][0][i], runs[1][0][i])
-        for a, b in zip(runs[0][1][i], runs[1][1][i]):
+    ctx = svm.Context([dict(n_cells=n, a0=1.0, p0=p0, kr=1.0, t1_threshold=0.1, seed=2000 + i)
+                       for i, (n, p0) in enumerate(params)])
+    for i, (n, _) in enumerate(params):
+        pts = svm.generate_cell_points(n, 2000 + i)
+        ctx.build_from_cell_points(i, pts[svm.spatial_order(pts)])
+        ctx.set_directors(i, svm.generate_directors(ctx.positions(i), 2000 + i))
+        ctx.set_preferred_parameters(i, 1.0, 1.0)
+        ctx.set_brownian(i, dt=0.01, mu=1.0, T=0.0)
+    ctx.set_t1_mode(sequential)
+    if distance is not None:
+        ctx.set_t1_component_distance(distance)
+        assert ctx.t1_component_distance() == distance
+    ctx.brownian_steps(50)
+    for i, (n, p0) in enumerate(params):
+        ctx.set_preferred_parameters(i, 1.0, p0)
+        ctx.set_active(i, dt=0.01, mu=1.0, v0=0.3, tau=1.0)
+    ctx.active_steps(200)
+    return ([ctx.positions(i) for i in range(len(params))],
+            [ctx.get_topology(i, svmtest.RING_CAP) for i in range(len(params))], ctx.counters(),
+            ctx.t1_diagnostics())
+
+
+@pytest.fixture(scope="module")
+def t1_batch_sequential():
+    return _t1_batch_run(True)
+
+
+@pytest.mark.parametrize("distance", [0.0, 3.0, 12.0])
+def test_t1_paths_agree_on_a_batch(t1_batch_sequential, distance):
+    """The same at batch scale: the dry runs over predicted components, group
+    runs and their LDS fast paths (mode 0) and the always-sequential pass on
+    global memory (mode 1) give bit-identical positions and topology.  The
+    component distance moves work between the kinds of run -- 0: one run per
+    candidate, overlaps resolved by group runs; 3 (the default): few group
+    runs; 12: components too large for a run's cache, redone by group runs --
+    and must change nothing.  Each run includes thousands of flips."""
+    seq = t1_batch_sequential
+    run = _t1_batch_run(False, distance)
+    for i in range(len(seq[0])):
+        np.testing.assert_array_equal(run[0][i], seq[0][i])
+        for a, b in zip(run[1][i], seq[1][i]):
             np.testing.assert_array_equal(a, b)
-    c0, c1 = runs[0][2], runs[1][2]
+    c0, c1 = run[2], seq[2]
     np.testing.assert_array_equal(c0[:, 0], c1[:, 0])
-    print(f"flips {int(c0[:, 0].sum())}, group sphere-steps {int(c0[:, 5].sum())}, "
-          f"sequential sphere-steps {int(c0[:, 4].sum())} (mode 0) vs {int(c1[:, 4].sum())} (mode 1)")
+    dg = run[3].sum(axis=0)
+    print(f"distance {distance}: flips {int(c0[:, 0].sum())}, group sphere-steps {int(c0[:, 5].sum())}, "
+          f"group members {int(dg[6])}, sequential sphere-steps {int(c0[:, 4].sum())} (mode 0) vs "
+          f"{int(c1[:, 4].sum())} (mode 1)")
     assert c0[:, 0].sum() > 2000
     assert c0[:, 5].sum() > 10
 
