@@ -100,6 +100,36 @@ __device__ inline double fold_lds(const double* part, int stride, int n, double*
     return r;
 }
 
+// The same fixed tree by one wavefront in registers, for m = n padded to a
+// power of two at most 64 * SVM_FOLD_K: lane l holds entries l + 64 k.  The
+// tree's levels with h >= 64 add a lane's own entries k and k + h / 64, the
+// last six take entry i + h from lane i + h by a shuffle; every addition is
+// buf[i] + buf[i + h], as in fold_lds.  Each wavefront folds for itself (no
+// LDS, no barrier); all lanes return the sum.
+#define SVM_FOLD_K 4
+__device__ inline double fold_wave(const double* part, int stride, int n, int m) {
+    const int lane = threadIdx.x & 63;
+    double v[SVM_FOLD_K];
+#pragma unroll
+    for (int k = 0; k < SVM_FOLD_K; ++k) {
+        const int i = lane + 64 * k;
+        v[k] = i < n ? part[(size_t)i * stride] : 0.0;
+    }
+#pragma unroll
+    for (int q = SVM_FOLD_K / 2; q >= 1; q >>= 1)      // h = 64 q
+        if (64 * q < m) {
+#pragma unroll
+            for (int k = 0; k < q; ++k) v[k] = v[k] + v[k + q];
+        }
+    double x = v[0];
+#pragma unroll
+    for (int h = 32; h >= 1; h >>= 1) {
+        const double y = __shfl_down(x, h, 64);
+        if (h < m && lane < h) x = x + y;
+    }
+    return __shfl(x, 0, 64);
+}
+
 __device__ inline double dot0(const double* a, const double* b) {
     double s = 0.0;
     s += a[0] * b[0];
@@ -190,14 +220,19 @@ __global__ __launch_bounds__(SVM_TILE) void k_geometry(Dev d, int pin, int skip_
 // every cell of that sphere.  Each block folds its sphere's tile partials
 // from k_geometry with the same fixed tree, so every block reaches the same
 // decision without any cross-block synchronisation.
+// Spheres of up to 64 * SVM_FOLD_K tiles fold in registers; larger ones in
+// the launch's dynamic LDS (sized by the host for the largest such sphere).
 __global__ __launch_bounds__(SVM_TILE) void k_fan_fix(Dev d, int pin, int skip_frozen) {
-    __shared__ double buf[SVM_MAX_TILES];
+    extern __shared__ double fold_buf[];
     const int tile = lane_ctile(d);
     if (tile < 0) return;
     SysParams* S = d.sys + d.ctile_sys[tile];
     if (skip_frozen && S->f_frozen) return;
     const double R = S->R;
-    double total = fold_lds(d.cpart + S->ctile0, 1, S->nctiles, buf);
+    int m = 1;
+    while (m < S->nctiles) m <<= 1;
+    const double total = m <= 64 * SVM_FOLD_K ? fold_wave(d.cpart + S->ctile0, 1, S->nctiles, m)
+                                              : fold_lds(d.cpart + S->ctile0, 1, S->nctiles, fold_buf);
     double excessArea = total - 4.0 * SVM_PI * R * R;
     if (!(__builtin_fabs(excessArea) > 1e-6)) return;
     if (tile == S->ctile0 && threadIdx.x == 0) S->n_fan += 1;
