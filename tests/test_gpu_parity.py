@@ -58,6 +58,41 @@ def test_geometry_forces_n512(state512):
     assert abs(ctx.area_perimeter(0)[:, 0].sum() - 512.0) < 1e-9
 
 
+def test_fan_fallback_matches_oracle(state512):
+    """A vertex pushed far across its neighbours folds its cells: the
+    angle-sum areas no longer tile the sphere, and the oracle and the device
+    both recompute every cell of that sphere as a triangle fan
+    (recomputeAreasCPU), bit for bit, counting one fallback.  In a batch of
+    three spheres where only the middle one is folded, the other two keep
+    their angle-sum areas."""
+    base = svmtest.OracleModel(state512).positions()
+    v = 17
+    nb = state512["vn"][v]
+    folded = None
+    for f in (3.0, 4.0, 5.0, 6.0):          # the first overshoot that breaks the tiling (on the CPU)
+        p = base.copy()
+        q = p[v] + f * (p[nb].mean(axis=0) - p[v])
+        p[v] = np.linalg.norm(p[v]) * q / np.linalg.norm(q)
+        om = svmtest.OracleModel(state512)
+        om.set_positions(p)
+        n0 = om.n_fan
+        om.compute_geometry()
+        if om.n_fan > n0:
+            folded = p
+            break
+    assert folded is not None
+    ctx = make_ctx([state512, state512, state512])
+    ctx.set_positions(1, folded)
+    c0 = ctx.counters()[:, 1].copy()
+    ctx.compute_geometry()
+    np.testing.assert_array_equal(ctx.counters()[:, 1] - c0, [0, 1, 0])
+    np.testing.assert_array_equal(ctx.area_perimeter(1), om.area_perimeter())
+    plain = svmtest.OracleModel(state512)
+    plain.compute_geometry()
+    for i in (0, 2):
+        np.testing.assert_array_equal(ctx.area_perimeter(i), plain.area_perimeter())
+
+
 def test_active_steps_n512(state512):
     om = svmtest.OracleModel(state512)
     ctx = make_ctx([state512])
