@@ -677,6 +677,9 @@ def main():
                      "bytes_per_launch": VERTEX_BYTES_PER_CELL * total_cells, "avg_launch_ms": dom_ms,
                      "back_to_back_ms": b2b_ms, "fp64": fp64_roofline(dom, dom_ms)},
         "kernel_ms": timing,
+        # a step's kernels run as one chain per lane (eager launches or graph nodes)
+        "launches_per_step": {"kernels_per_lane": len(timing), "lanes": ctx.tuning()["lanes"],
+                              "total": len(timing) * ctx.tuning()["lanes"]},
         "init": {"gpu_delaunay_topology_s": t_delaunay, "setup_s": t_setup,
                  "note": "256 spheres triangulated one call each (includes host<->device copies)"},
         "health": health(ctx, params, base),
