@@ -49,6 +49,7 @@ struct svm_context {
     int t1_sequential = 0;           // 1: skip the dry runs, always use the sequential pass
     double tuned_ms = 0.0;           // ms per step of the schedule the autotuner kept (0: not tuned)
     int vertex_corner = 0;           // Brownian/active vertex pass: 0 k_vertex; 1, 2 corner form at 4, 5 waves per SIMD
+    int step_form = 0;               // svm_set_step_form bits: 1 fan check in the vertex pass, 2 no group kernel
     Dev d{};
     int pcur = 0, ncur = 0;          // current position / director buffer
     std::string err;
@@ -238,7 +239,9 @@ struct Target {
 };
 Target whole(svm_context* ctx) { return Target{ctx->d, ctx->stream}; }
 
-int launch_geometry(svm_context* ctx, const Target& w, int skip_frozen) {
+// with_fan = false leaves the area check and the fan fallback to the vertex
+// pass that follows (fan_check of k_vertex).
+int launch_geometry(svm_context* ctx, const Target& w, int skip_frozen, bool with_fan = true) {
     {
         Timer t(ctx, "k_geometry");
         hipLaunchKernelGGL(svm::k_geometry, dim3(xcd_grid(w.d.ctn)), dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur,
@@ -246,6 +249,7 @@ int launch_geometry(svm_context* ctx, const Target& w, int skip_frozen) {
         int r = last_launch(ctx);
         if (r) return r;
     }
+    if (!with_fan) return SVM_OK;
     // dynamic LDS for the fold of spheres with more tiles than a wavefront folds in registers
     int mmax = 1;
     for (int s = w.d.s0; s < w.d.s0 + w.d.ns; ++s)
@@ -261,17 +265,19 @@ int launch_geometry(svm_context* ctx, int skip_frozen) { return launch_geometry(
 // The active updater also writes the rotated directors, into the other
 // director buffer (1 - ncur); no other mode touches directors.
 template <int MODE>
-int launch_vertex(svm_context* ctx, const Target& w, int pout, int store_forces, const char* name) {
+int launch_vertex(svm_context* ctx, const Target& w, int pout, int store_forces, const char* name,
+                  int fan_check = 0) {
     Timer t(ctx, name);
     const dim3 grid(xcd_grid(w.d.vtn));
     const int nout = 1 - ctx->ncur;
     if (ctx->force_fp32 == 1)
         hipLaunchKernelGGL((svm::k_vertex<MODE, float>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
-                           ctx->ncur, nout, store_forces);
+                           ctx->ncur, nout, store_forces, fan_check);
     else if (ctx->force_fp32 == 2)
         hipLaunchKernelGGL((svm::k_vertex<MODE, svm::VecF64>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
-                           ctx->ncur, nout, store_forces);
+                           ctx->ncur, nout, store_forces, fan_check);
     else if ((MODE == svm::VM_BROWNIAN || MODE == svm::VM_ACTIVE) && ctx->vertex_corner && !store_forces) {
+        if (fan_check) return fail(ctx, SVM_ERR_ARG, "the corner vertex form has no fan check");
         constexpr int M = MODE == svm::VM_ACTIVE ? svm::VM_ACTIVE : svm::VM_BROWNIAN;
         const dim3 cgrid(xcd_grid(w.d.vtn * (SVM_TILE / SVM_CORNER_V)));
         if (ctx->vertex_corner == 2)
@@ -283,7 +289,7 @@ int launch_vertex(svm_context* ctx, const Target& w, int pout, int store_forces,
     }
     else
         hipLaunchKernelGGL((svm::k_vertex<MODE, double>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
-                           ctx->ncur, nout, store_forces);
+                           ctx->ncur, nout, store_forces, fan_check);
     return last_launch(ctx);
 }
 template <int MODE>
@@ -309,14 +315,18 @@ int launch_t1(svm_context* ctx, const Target& w, int pcur, int nin, int nout, in
         int r = last_launch(ctx);
         if (r) return r;
     }
-    if (!ctx->t1_sequential) {
+    bool large = false;
+    for (int s = w.d.s0; s < w.d.s0 + w.d.ns; ++s) large |= ctx->layout[s].nv >= SVM_COMMIT_LARGE;
+    // step form flag 2: the blocked-candidate check in k_t1_resolve instead of
+    // a k_t1_group launch (not with the large-sphere commit, which decides
+    // before k_t1_resolve)
+    const int check_blocked = (ctx->step_form & 2) && !large && !ctx->t1_sequential;
+    if (!ctx->t1_sequential && !check_blocked) {
         Timer t(ctx, "k_t1_group");
         hipLaunchKernelGGL(svm::k_t1_group, dim3(w.d.ns), dim3(SVM_T1_GT), 0, w.st, w.d, pcur);
         int r = last_launch(ctx);
         if (r) return r;
     }
-    bool large = false;
-    for (int s = w.d.s0; s < w.d.s0 + w.d.ns; ++s) large |= ctx->layout[s].nv >= SVM_COMMIT_LARGE;
     if (large && !ctx->t1_sequential) {
         Timer t(ctx, "k_t1_commit_large");
         hipLaunchKernelGGL(svm::k_t1_commit_large, dim3(w.d.ns), dim3(1024), 0, w.st, w.d, pcur, nin, nout, dir,
@@ -326,7 +336,7 @@ int launch_t1(svm_context* ctx, const Target& w, int pcur, int nin, int nout, in
     }
     Timer t(ctx, "k_t1_resolve");
     hipLaunchKernelGGL(svm::k_t1_resolve, dim3(w.d.ns), dim3(SVM_TILE), 0, w.st, w.d, pcur, nin, nout, dir,
-                       advance_step, fire_mode, ctx->t1_sequential);
+                       advance_step, fire_mode, ctx->t1_sequential, check_blocked);
     return last_launch(ctx);
 }
 int launch_t1(svm_context* ctx, int pcur, int nin, int nout, int dir, int advance_step, int fire_mode,
@@ -338,11 +348,15 @@ int launch_t1(svm_context* ctx, int pcur, int nin, int nout, int dir, int advanc
 // T1.  updater 0 = Brownian, 1 = active, 2 = Vicsek.  The ping-pong buffer
 // indices advance in ctx->pcur / ctx->ncur.
 int enqueue_step(svm_context* ctx, const Target& w, int updater) {
-    int r = launch_geometry(ctx, w, 0);
+    // step form flag 1: the area check and fan fallback in the vertex pass (the
+    // thread-per-vertex form, spheres whose tile partials one wavefront folds)
+    bool fold = (ctx->step_form & 1) && (ctx->vertex_corner == 0 || ctx->force_fp32 != 0);
+    for (int s = w.d.s0; s < w.d.s0 + w.d.ns && fold; ++s) fold = ctx->layout[s].nctiles <= 64 * SVM_FOLD_K;
+    int r = launch_geometry(ctx, w, 0, !fold);
     if (r) return r;
     int pout = 1 - ctx->pcur;
-    if (updater == 0) r = launch_vertex<svm::VM_BROWNIAN>(ctx, w, pout, 0, "k_vertex_brownian");
-    else r = launch_vertex<svm::VM_ACTIVE>(ctx, w, pout, 0, "k_vertex_active");
+    if (updater == 0) r = launch_vertex<svm::VM_BROWNIAN>(ctx, w, pout, 0, "k_vertex_brownian", fold);
+    else r = launch_vertex<svm::VM_ACTIVE>(ctx, w, pout, 0, "k_vertex_active", fold);
     if (r) return r;
     if (updater == 2) {
         // Vicsek: the same move as the active updater, T1, then the alignment
@@ -387,6 +401,7 @@ Target lane_target(svm_context* ctx, int k) {
     d.t1_logged = L.t1_logged;
     d.t1_lead = L.t1_lead;
     d.t1_total = L.t1_total;
+    d.t1_btotal = L.t1_btotal;
     d.t1_bcount = L.t1_bcount;
     d.t1_blist = L.t1_blist;
     d.t1_bflag = L.t1_bflag;
@@ -971,6 +986,7 @@ int svm_create(const svm_system_desc* systems, int32_t n_systems, svm_context** 
     r |= dalloc(ctx, &d.t1_lead, SVM_T1_MAXCL);
     d.t1_near = SVM_T1_NEAR;
     r |= dalloc(ctx, &d.t1_total, 1);
+    r |= dalloc(ctx, &d.t1_btotal, 1);
     r |= dalloc(ctx, &d.t1_bcount, 1);
     r |= dalloc(ctx, &d.t1_blist, SVM_T1_MAXBL);
     r |= dalloc(ctx, &d.t1_bflag, SVM_T1_MAXBL);
@@ -2198,6 +2214,19 @@ int svm_get_vertex_form(svm_context* ctx, int32_t* form) {
     return SVM_OK;
 }
 
+int svm_set_step_form(svm_context* ctx, int32_t form) {
+    if (!ctx || form < 0 || form > 3) return fail(ctx, SVM_ERR_ARG, "step form must be in [0, 3]");
+    ctx->step_form = form;
+    drop_graphs(ctx);
+    return SVM_OK;
+}
+
+int svm_get_step_form(svm_context* ctx, int32_t* form) {
+    if (!ctx || !form) return SVM_ERR_ARG;
+    *form = ctx->step_form;
+    return SVM_OK;
+}
+
 int svm_set_lanes(svm_context* ctx, int32_t n_lanes) {
     if (!ctx || n_lanes < 1 || n_lanes > SVM_MAX_LANES)
         return fail(ctx, SVM_ERR_ARG, "n_lanes must be in [1, " + std::to_string(SVM_MAX_LANES) + "]");
@@ -2220,6 +2249,7 @@ int svm_set_lanes(svm_context* ctx, int32_t n_lanes) {
         r |= dalloc(ctx, &L.t1_logged, SVM_T1_MAXCL);
         r |= dalloc(ctx, &L.t1_lead, SVM_T1_MAXCL);
         r |= dalloc(ctx, &L.t1_total, 1);
+        r |= dalloc(ctx, &L.t1_btotal, 1);
         r |= dalloc(ctx, &L.t1_bcount, 1);
         r |= dalloc(ctx, &L.t1_blist, SVM_T1_MAXBL);
         r |= dalloc(ctx, &L.t1_bflag, SVM_T1_MAXBL);
@@ -2291,7 +2321,7 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
         *ms = samples[1];
         return SVM_OK;
     };
-    auto record = [&](int lanes, int cb, int G, int xcd, int form, double ms) {
+    auto record = [&](int lanes, int cb, int G, int xcd, int form, int sform, double ms) {
         if (table && nrows < table_rows) {
             double* row = table + (size_t)SVM_TUNE_COLS * nrows;
             row[0] = lanes;
@@ -2299,14 +2329,16 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
             row[2] = G;
             row[3] = xcd;
             row[4] = form;
-            row[5] = ms;
+            row[5] = sform;
+            row[6] = ms;
         }
         ++nrows;
     };
     int best_cb = ctx->cluster_blocks, best_G = ctx->graph_steps, best_x = ctx->d.xcd, best_l = ctx->nlanes;
-    int best_f = 0;
+    int best_f = 0, best_s = 0;
     double best = 1e300;
     r = svm_set_vertex_form(ctx, 0);
+    if (!r) r = svm_set_step_form(ctx, 0);
     // lanes x two dry-run grids, at 10 steps per graph with the XCD mapping
     const int cb_pair[] = {256, 1024};
     for (int l = 1; l <= std::min(ctx->nsys, (int)SVM_MAX_LANES) && !r; ++l)
@@ -2316,10 +2348,21 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
             double ms = 0.0;
             if (!r) r = trial(10, &ms);
             if (r) break;
-            record(l, cb, 10, 1, 0, ms);
+            record(l, cb, 10, 1, 0, 0, ms);
             if (ms < best) { best = ms; best_l = l; best_cb = cb; best_x = 1; best_G = 10; }
         }
     if (!r) r = svm_set_lanes(ctx, best_l);
+    // the launch structures of the step (svm_set_step_form)
+    for (int sform = 1; sform <= 3 && !r; ++sform) {
+        r = svm_set_tuning(ctx, best_cb, 10, 1);
+        if (!r) r = svm_set_step_form(ctx, sform);
+        double ms = 0.0;
+        if (!r) r = trial(10, &ms);
+        if (r) break;
+        record(best_l, best_cb, 10, 1, 0, sform, ms);
+        if (ms < best) { best = ms; best_s = sform; best_x = 1; best_G = 10; }
+    }
+    if (!r) r = svm_set_step_form(ctx, best_s);
     // the corner forms of the vertex pass (the FP64 parity arithmetic only)
     for (int form = 1; form <= 2 && !r && ctx->force_fp32 == 0; ++form) {
         r = svm_set_tuning(ctx, best_cb, 10, 1);
@@ -2327,7 +2370,7 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
         double ms = 0.0;
         if (!r) r = trial(10, &ms);
         if (r) break;
-        record(best_l, best_cb, 10, 1, form, ms);
+        record(best_l, best_cb, 10, 1, form, best_s, ms);
         if (ms < best) { best = ms; best_f = form; best_x = 1; best_G = 10; }
     }
     if (!r) r = svm_set_vertex_form(ctx, best_f);
@@ -2340,7 +2383,7 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
             double ms = 0.0;
             if (!r) r = trial(10, &ms);
             if (r) break;
-            record(best_l, cb, 10, xcd, best_f, ms);
+            record(best_l, cb, 10, xcd, best_f, best_s, ms);
             if (ms < best) { best = ms; best_cb = cb; best_x = xcd; best_G = 10; }
         }
     // then the graph length
@@ -2351,7 +2394,7 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
         double ms = 0.0;
         if (!r) r = trial(G, &ms);
         if (r) break;
-        record(best_l, best_cb, G, best_x, best_f, ms);
+        record(best_l, best_cb, G, best_x, best_f, best_s, ms);
         if (ms < best) { best = ms; best_G = G; }
     }
     (void)hipEventDestroy(e0);
@@ -2360,6 +2403,7 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
     if (rows_written) *rows_written = nrows;
     r = svm_set_lanes(ctx, best_l);
     r = r ? r : svm_set_vertex_form(ctx, best_f);
+    r = r ? r : svm_set_step_form(ctx, best_s);
     r = r ? r : svm_set_tuning(ctx, best_cb, best_G, best_x);
     if (!r) ctx->tuned_ms = best;
     return r;
@@ -2379,7 +2423,9 @@ std::array<int, 4> schedule_key(svm_context* ctx, int updater) {
     for (int s = 0; s < ctx->nsys; ++s) cells += ctx->layout[s].nc;
     return {updater, ctx->force_fp32, floor_log2(ctx->nsys), floor_log2(cells / std::max(1, ctx->nsys))};
 }
-const char* kScheduleTag = "svm-schedule-v1";
+// v2 lines carry the step form; v1 lines (without it) are read as step form 0
+const char* kScheduleTag = "svm-schedule-v2";
+const char* kScheduleTagV1 = "svm-schedule-v1";
 }  // namespace
 
 int svm_tuning_save(svm_context* ctx, int32_t updater, const char* path) {
@@ -2392,17 +2438,17 @@ int svm_tuning_save(svm_context* ctx, int32_t updater, const char* path) {
             char tag[64];
             int k[4];
             if (std::sscanf(line, "%63s %d %d %d %d", tag, &k[0], &k[1], &k[2], &k[3]) == 5 &&
-                std::strcmp(tag, kScheduleTag) == 0 && k[0] == key[0] && k[1] == key[1] && k[2] == key[2] &&
-                k[3] == key[3])
+                (std::strcmp(tag, kScheduleTag) == 0 || std::strcmp(tag, kScheduleTagV1) == 0) && k[0] == key[0] &&
+                k[1] == key[1] && k[2] == key[2] && k[3] == key[3])
                 continue;   // replaced below
             keep.push_back(line);
         }
         std::fclose(f);
     }
     char row[512];
-    std::snprintf(row, sizeof row, "%s %d %d %d %d %d %d %d %d %d %.6f\n", kScheduleTag, key[0], key[1], key[2],
+    std::snprintf(row, sizeof row, "%s %d %d %d %d %d %d %d %d %d %d %.6f\n", kScheduleTag, key[0], key[1], key[2],
                   key[3], ctx->nlanes, ctx->cluster_blocks, ctx->graph_steps, ctx->d.xcd, ctx->vertex_corner,
-                  ctx->tuned_ms);
+                  ctx->step_form, ctx->tuned_ms);
     keep.push_back(row);
     const std::string tmp = std::string(path) + ".tmp";
     FILE* f = std::fopen(tmp.c_str(), "w");
@@ -2421,16 +2467,20 @@ int svm_tuning_load(svm_context* ctx, int32_t updater, const char* path, int32_t
     FILE* f = std::fopen(path, "r");
     if (!f) return SVM_OK;
     char line[512];
-    int sched[5];
+    int sched[6];
     double ms = 0.0;
     while (std::fgets(line, sizeof line, f)) {
-        char tag[64];
-        int k[4], v[5];
+        char tag[64] = {0};
+        int k[4], v[6] = {0, 0, 0, 0, 0, 0};
         double t = 0.0;
-        if (std::sscanf(line, "%63s %d %d %d %d %d %d %d %d %d %lf", tag, &k[0], &k[1], &k[2], &k[3], &v[0], &v[1],
-                        &v[2], &v[3], &v[4], &t) == 11 &&
-            std::strcmp(tag, kScheduleTag) == 0 && k[0] == key[0] && k[1] == key[1] && k[2] == key[2] &&
-            k[3] == key[3]) {
+        bool ok = false;
+        if (std::sscanf(line, "%63s", tag) == 1 && std::strcmp(tag, kScheduleTag) == 0)
+            ok = std::sscanf(line, "%63s %d %d %d %d %d %d %d %d %d %d %lf", tag, &k[0], &k[1], &k[2], &k[3], &v[0],
+                             &v[1], &v[2], &v[3], &v[4], &v[5], &t) == 12;
+        else if (std::strcmp(tag, kScheduleTagV1) == 0)
+            ok = std::sscanf(line, "%63s %d %d %d %d %d %d %d %d %d %lf", tag, &k[0], &k[1], &k[2], &k[3], &v[0],
+                             &v[1], &v[2], &v[3], &v[4], &t) == 11;
+        if (ok && k[0] == key[0] && k[1] == key[1] && k[2] == key[2] && k[3] == key[3]) {
             std::memcpy(sched, v, sizeof sched);
             ms = t;
             *found = 1;
@@ -2441,6 +2491,7 @@ int svm_tuning_load(svm_context* ctx, int32_t updater, const char* path, int32_t
     int r = svm_set_lanes(ctx, sched[0]);
     r = r ? r : svm_set_tuning(ctx, sched[1], sched[2], sched[3]);
     r = r ? r : svm_set_vertex_form(ctx, sched[4]);
+    r = r ? r : svm_set_step_form(ctx, sched[5]);
     if (r) {
         *found = 0;
         return r;

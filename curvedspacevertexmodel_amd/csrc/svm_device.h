@@ -108,6 +108,7 @@ struct Dev {
     int32_t* t1_lead;             // per candidate: slot of the candidate whose dry run covers it
     double t1_near;               // component distance of the dry runs, in cell sizes (0: a run per candidate)
     int32_t* t1_total;            // flippable-list length of the current step (copied before the reset)
+    int32_t* t1_btotal;           // blocked-list length of the current step (copied before the reset)
     int32_t* t1_bcount;           // blocked candidates appended this step
     int32_t* t1_blist;            // blocked candidate vertex rows
     int32_t* t1_bflag;            // per blocked candidate: generation in which it was entangled

@@ -215,40 +215,34 @@ __global__ __launch_bounds__(SVM_TILE) void k_geometry(Dev d, int pin, int skip_
     if (threadIdx.x == 0) d.cpart[tile] = part;
 }
 
-// The total-area check of computeGeometryCPU (src/model/sphericalVertexModel.cpp:304-310)
-// and, where it fails, triangle-fan areas (recomputeAreasCPU, :162-207) for
-// every cell of that sphere.  Each block folds its sphere's tile partials
-// from k_geometry with the same fixed tree, so every block reaches the same
-// decision without any cross-block synchronisation.
-// Spheres of up to 64 * SVM_FOLD_K tiles fold in registers; larger ones in
-// the launch's dynamic LDS (sized by the host for the largest such sphere).
-__global__ __launch_bounds__(SVM_TILE) void k_fan_fix(Dev d, int pin, int skip_frozen) {
-    extern __shared__ double fold_buf[];
-    const int tile = lane_ctile(d);
-    if (tile < 0) return;
-    SysParams* S = d.sys + d.ctile_sys[tile];
-    if (skip_frozen && S->f_frozen) return;
+// The total-area check of computeGeometryCPU (src/model/sphericalVertexModel.cpp:304-310):
+// true when the angle-sum areas of sphere S, summed to `total` (the fixed-tree
+// fold of its tile partials from k_geometry), miss 4 pi R^2 by more than
+// 1e-6, so that its cells take their triangle-fan areas.
+__device__ inline bool fan_needed(const SysParams* S, double total) {
     const double R = S->R;
-    int m = 1;
-    while (m < S->nctiles) m <<= 1;
-    const double total = m <= 64 * SVM_FOLD_K ? fold_wave(d.cpart + S->ctile0, 1, S->nctiles, m)
-                                              : fold_lds(d.cpart + S->ctile0, 1, S->nctiles, fold_buf);
     double excessArea = total - 4.0 * SVM_PI * R * R;
-    if (!(__builtin_fabs(excessArea) > 1e-6)) return;
-    if (tile == S->ctile0 && threadIdx.x == 0) S->n_fan += 1;
-    const int c = tile * SVM_TILE + threadIdx.x;
-    if (c - S->co >= S->nc) return;
-    const double* X = d.px[pin];
-    const double* Y = d.py[pin];
-    const double* Z = d.pz[pin];
+    return __builtin_fabs(excessArea) > 1e-6;
+}
+// the power of two the tile partials of n tiles are padded to
+__device__ inline int fold_width(int n) {
+    int m = 1;
+    while (m < n) m <<= 1;
+    return m;
+}
+
+// Triangle-fan area of cell c (recomputeAreasCPU, :162-207): the fan's
+// triangles (centre, ring[k-2], ring[k-1]) in ring order, each
+// triangle_area(cp, pl, pc, R) term for term; the centre's unit vector, and
+// the previous triangle's unit pc, its dot with the centre and
+// sqrt(1 - dot^2), are the same expressions on the same inputs, carried
+// instead of recomputed.  A pure function of the ring, the positions and the
+// cell centre from k_geometry.
+__device__ inline double fan_area(const Dev& d, const double* X, const double* Y, const double* Z, int c,
+                                  double R) {
     const int Cs = d.Cs;
     const int n = d.cnn[c];
     const int32_t* rc = d.ring + c;
-    // the fan's triangles (centre, ring[k-2], ring[k-1]) in ring order, each
-    // triangle_area(cp, pl, pc, R) term for term; the centre's unit vector,
-    // and the previous triangle's unit pc, its dot with the centre and
-    // sqrt(1 - dot^2), are the same expressions on the same inputs, carried
-    // instead of recomputed
     const double cp[3] = {d.cx[c], d.cy[c], d.cz[c]};
     auto unit = [](const double* p, double* u) {
         double dn = 0.0; dn += p[0] * p[0]; dn += p[1] * p[1]; dn += p[2] * p[2];
@@ -285,7 +279,30 @@ __global__ __launch_bounds__(SVM_TILE) void k_fan_fix(Dev d, int pin, int skip_f
         s12 = s13;
         pc[0] = pn[0]; pc[1] = pn[1]; pc[2] = pn[2];
     }
-    d.A[c] = a;
+    return a;
+}
+
+// The total-area check of computeGeometryCPU (src/model/sphericalVertexModel.cpp:304-310)
+// and, where it fails, triangle-fan areas (recomputeAreasCPU, :162-207) for
+// every cell of that sphere.  Each block folds its sphere's tile partials
+// from k_geometry with the same fixed tree, so every block reaches the same
+// decision without any cross-block synchronisation.
+// Spheres of up to 64 * SVM_FOLD_K tiles fold in registers; larger ones in
+// the launch's dynamic LDS (sized by the host for the largest such sphere).
+__global__ __launch_bounds__(SVM_TILE) void k_fan_fix(Dev d, int pin, int skip_frozen) {
+    extern __shared__ double fold_buf[];
+    const int tile = lane_ctile(d);
+    if (tile < 0) return;
+    SysParams* S = d.sys + d.ctile_sys[tile];
+    if (skip_frozen && S->f_frozen) return;
+    const int m = fold_width(S->nctiles);
+    const double total = m <= 64 * SVM_FOLD_K ? fold_wave(d.cpart + S->ctile0, 1, S->nctiles, m)
+                                              : fold_lds(d.cpart + S->ctile0, 1, S->nctiles, fold_buf);
+    if (!fan_needed(S, total)) return;
+    if (tile == S->ctile0 && threadIdx.x == 0) S->n_fan += 1;
+    const int c = tile * SVM_TILE + threadIdx.x;
+    if (c - S->co >= S->nc) return;
+    d.A[c] = fan_area(d, d.px[pin], d.py[pin], d.pz[pin], c, S->R);
 }
 
 // Active-director update of sphericalSelfPropelledParticle::integrateEOMCPU
@@ -509,8 +526,17 @@ __device__ inline int vertex_force<VecF64>(const Dev& d, const SysParams* S, con
     return vertex_force_vec<double>(d, S, X, Y, Z, v, f);
 }
 
+// fan_check = 1: the total-area check and the fan fallback of the geometry
+// (k_fan_fix) are done here, for spheres of at most 64 * SVM_FOLD_K cell
+// tiles.  Every wavefront folds its sphere's tile partials itself; where the
+// check fails, each thread gives the three cells of its vertex their fan
+// areas (fan_area) before the force reads them.  Every thread whose vertex
+// touches a cell stores the same value there before it loads it, so each
+// load returns that value; every cell has vertices, so every cell's area is
+// replaced, as k_fan_fix replaces it.
 template <int MODE, class FT>
-__global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, int nin, int nout, int store_forces) {
+__global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, int nin, int nout, int store_forces,
+                                                     int fan_check) {
     __shared__ double red[SVM_TILE + 1];
     const int tile = lane_vtile(d);
     if (tile < 0) return;
@@ -527,10 +553,18 @@ __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, i
         if (valid) { d.px[pout][v] = X[v]; d.py[pout][v] = Y[v]; d.pz[pout][v] = Z[v]; }
         return;
     }
+    const bool fan = fan_check && fan_needed(S, fold_wave(d.cpart + S->ctile0, 1, S->nctiles, fold_width(S->nctiles)));
     double f[3] = {0.0, 0.0, 0.0};
     double tp = 0.0, tf = 0.0, tv = 0.0;
     int nan = 0;
     if (valid) {
+        if (fan) {
+            if (local == 0) S->n_fan += 1;
+            const int4 cl = d.vcn[v];
+            d.A[cl.x] = fan_area(d, X, Y, Z, cl.x, S->R);
+            d.A[cl.y] = fan_area(d, X, Y, Z, cl.y, S->R);
+            d.A[cl.z] = fan_area(d, X, Y, Z, cl.z, S->R);
+        }
         nan = vertex_force<FT>(d, S, X, Y, Z, v, f);
         if (MODE == VM_FORCE || store_forces) {
             d.fx[v] = f[0]; d.fy[v] = f[1]; d.fz[v] = f[2];

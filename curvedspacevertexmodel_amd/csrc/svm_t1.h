@@ -1075,8 +1075,8 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int ski
 // otherwise its no-fourth-cell skips are tallied for the commit.  The runs a
 // candidate can meet are its own sphere's, so one block sees all it needs.
 __device__ __attribute__((always_inline)) inline void t1_check_sphere(const Dev& d, SysParams* S, int vo, int nv,
-                                                                     int t, int nt, int* hit_any) {
-    const int total = min(*d.t1_bcount, SVM_T1_MAXBL);
+                                                                     int t, int nt, int* hit_any, int blocked) {
+    const int total = min(blocked, SVM_T1_MAXBL);
     const unsigned gen = (unsigned)S->t1_gen;
     for (int i = t; i < total; i += nt) {
         const int v = d.t1_blist[i];
@@ -1489,7 +1489,10 @@ __global__ __launch_bounds__(64 * SVM_T1_WAVES) void k_t1_cluster(Dev d, int pcu
     const WaveSync sync;
     const int nwaves = gridDim.x * SVM_T1_WAVES;
     const int total = min(*d.t1_count, SVM_T1_MAXCL);
-    if (blockIdx.x == 0 && threadIdx.x == 0) *d.t1_total = *d.t1_count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *d.t1_total = *d.t1_count;
+        *d.t1_btotal = *d.t1_bcount;
+    }
     int* misc = s_misc[w];
     for (int idx = blockIdx.x * SVM_T1_WAVES + w; idx < total; idx += nwaves) {
         const int row = d.t1_list[idx];
@@ -1597,7 +1600,7 @@ __global__ __launch_bounds__(SVM_T1_GT) void k_t1_group(Dev d, int pcur) {
     const int vo = S->vo, nv = S->nv;
     if (lane == 0) s_hit = 0;
     __syncthreads();
-    t1_check_sphere(d, S, vo, nv, lane, SVM_T1_GT, &s_hit);
+    t1_check_sphere(d, S, vo, nv, lane, SVM_T1_GT, &s_hit, *d.t1_bcount);
     __syncthreads();
     if (s_hit && lane == 0) S->t1_overlap = 1;
     __syncthreads();
@@ -1755,10 +1758,11 @@ __global__ __launch_bounds__(SVM_T1_GT) void k_t1_group(Dev d, int pcur) {
 struct T1Ending {
     int run, commit, group;
 };
-__device__ inline T1Ending t1_ending(const Dev& d, const SysParams* S, int s, int fire_mode, int force_sequential) {
+__device__ inline T1Ending t1_ending(const Dev& d, const SysParams* S, int s, int fire_mode, int force_sequential,
+                                    int no_group) {
     const bool frozen = fire_mode && S->f_frozen;
     const int any = S->t1_flag;
-    const int group_ok = !force_sequential && S->t1_overlap && !S->t1_overflow && d.t1_group[s].ok;
+    const int group_ok = !force_sequential && !no_group && S->t1_overlap && !S->t1_overflow && d.t1_group[s].ok;
     T1Ending e;
     e.run = !frozen && any && (force_sequential || S->t1_overflow || (S->t1_overlap && !group_ok));
     e.commit = !frozen && any && !e.run;
@@ -1777,7 +1781,7 @@ __global__ __launch_bounds__(1024) void k_t1_commit_large(Dev d, int pcur, int n
     SysParams* S = d.sys + s;
     if (S->nv < SVM_COMMIT_LARGE) return;
     if (threadIdx.x == 0) {
-        const T1Ending e = t1_ending(d, S, s, fire_mode, force_sequential);
+        const T1Ending e = t1_ending(d, S, s, fire_mode, force_sequential, 0);
         sh_commit = e.commit;
         sh_group = e.group;
     }
@@ -1787,20 +1791,40 @@ __global__ __launch_bounds__(1024) void k_t1_commit_large(Dev d, int pcur, int n
     if (threadIdx.x == 0 && sh_group) S->n_t1_group += 1;
 }
 
+// check_blocked = 1 closes the step without k_t1_group (which then is not
+// launched; spheres of SVM_COMMIT_LARGE vertices or more need it): the block
+// of each sphere first checks the sphere's blocked candidates against the
+// dry runs' stamps, as k_t1_group does, over the blocked list's length that
+// k_t1_cluster copied (the live counter is reset below), and a sphere whose
+// runs overlap takes the sequential pass instead of a group run.  Both give
+// the sequential order, so the results are the same.
 __global__ __launch_bounds__(SVM_TILE) void k_t1_resolve(Dev d, int pcur, int nin, int nout, int dir_mode,
-                                                         int advance_step, int fire_mode, int force_sequential) {
+                                                         int advance_step, int fire_mode, int force_sequential,
+                                                         int check_blocked) {
     __shared__ int heapbuf[SVM_HEAP_CAP];
     __shared__ int moved[SVM_LIST_CAP];
     __shared__ int touched[SVM_LIST_CAP];
     __shared__ int scan[SVM_TILE];
     __shared__ int tmp[72];
-    __shared__ int sh_run, sh_commit, sh_group, sh_nmoved, sh_ntouched;
+    __shared__ int sh_run, sh_commit, sh_group, sh_nmoved, sh_ntouched, sh_hit;
     const int s = d.s0 + (int)blockIdx.x;
     SysParams* S = d.sys + s;
     const int t = threadIdx.x;
+    if (check_blocked && !force_sequential) {
+        if (t == 0) sh_hit = 0;
+        __syncthreads();
+        t1_check_sphere(d, S, S->vo, S->nv, t, SVM_TILE, &sh_hit, *d.t1_btotal);
+        __syncthreads();
+        if (t == 0) {
+            if (sh_hit) S->t1_overlap = 1;
+            if (S->t1_overflow & 1) S->n_seq_reason[0] += 1;
+            if (S->t1_overflow & 2) S->n_seq_reason[3] += 1;
+        }
+        __syncthreads();
+    }
     if (t == 0) {
         const bool frozen = fire_mode && S->f_frozen;
-        const T1Ending e = t1_ending(d, S, s, fire_mode, force_sequential);
+        const T1Ending e = t1_ending(d, S, s, fire_mode, force_sequential, check_blocked);
         sh_run = e.run;
         sh_commit = e.commit && S->nv < SVM_COMMIT_LARGE;   // large spheres committed by k_t1_commit_large
         sh_group = e.group;

@@ -469,6 +469,26 @@ int svm_get_tuning(svm_context* ctx, int32_t* cluster_blocks, int32_t* graph_ste
  * keeps in flight to hide memory latency. */
 int svm_set_vertex_form(svm_context* ctx, int32_t form);
 int svm_get_vertex_form(svm_context* ctx, int32_t* form);
+/* Launch structure of the Brownian, active and Vicsek steps (no counterpart
+ * in the reference, whose step is a fixed sequence of kernels and host
+ * synchronisations, src/model/sphericalVertexModel.cpp:795-892): a sum of
+ * flags.
+ *   1: the total-area check of the geometry and its triangle-fan fallback
+ *      (computeGeometryCPU :304-310, recomputeAreasCPU :162-207) run inside
+ *      the vertex pass, which folds each sphere's area partials itself, instead
+ *      of a launch of their own.  Applies with the thread-per-vertex form
+ *      (vertex form 0, or either FP32/vector precision) to lanes whose spheres
+ *      have at most 65,536 cells.
+ *   2: the T1 step closes without the group kernel: the check of blocked
+ *      candidates against the dry runs runs in the commit kernel, and a sphere
+ *      whose dry runs overlap takes the sequential pass instead of a group run
+ *      (counted in the sequential column of svm_get_counters, with no reason
+ *      column).  Applies to lanes whose spheres have fewer than 65,536
+ *      vertices; also to FIRE, Adam and warm-start T1 passes.
+ * Every form gives identical results, bit for bit; form 3 is 5 launches per
+ * lane per step against 7 for form 0. */
+int svm_set_step_form(svm_context* ctx, int32_t form);
+int svm_get_step_form(svm_context* ctx, int32_t* form);
 /* Lanes of the dynamics step (no counterpart in the reference, which steps
  * one sphere at a time): the batch is cut into n_lanes contiguous runs of
  * spheres with about equal numbers of vertices (at most SVM_MAX_LANES, at
@@ -489,21 +509,23 @@ int svm_get_lanes(svm_context* ctx, int32_t* n_lanes, int32_t* first_sphere);
  * (src/utilities/kernelTuner.cpp:35-126) but over the whole step: times
  * real steps of `updater` (0 Brownian, 1 active) under each candidate
  * schedule -- first lanes in {1 .. min(SVM_MAX_LANES, spheres)} x
- * cluster_blocks in {256, 1024} with the thread-per-vertex form; at the best
- * lane count the corner vertex forms 1 and 2 (FP64 parity forces only); then
- * the rest of cluster_blocks in {128, 256, 512, 1024} x xcd_mapping in
- * {0, 1}, all at 10 steps per graph; then graph_steps in {4, 20, 40} -- keeps
- * the fastest and reports every trial in table (rows of SVM_TUNE_COLS
- * doubles: lanes, cluster_blocks, graph_steps, xcd_mapping, vertex_form, ms
- * per step).  The steps advance the model (about steps_per_trial per
- * candidate), exactly as untuned steps would. */
-#define SVM_TUNE_COLS 6
+ * cluster_blocks in {256, 1024} with the thread-per-vertex form and step
+ * form 0; at the best lane count the step forms 1, 2 and 3; then the corner
+ * vertex forms 1 and 2 (FP64 parity forces only); then the rest of
+ * cluster_blocks in {128, 256, 512, 1024} x xcd_mapping in {0, 1}, all at 10
+ * steps per graph; then graph_steps in {4, 20, 40} -- keeps the fastest and
+ * reports every trial in table (rows of SVM_TUNE_COLS doubles: lanes,
+ * cluster_blocks, graph_steps, xcd_mapping, vertex_form, step_form, ms per
+ * step).  The steps advance the model (about steps_per_trial per candidate),
+ * exactly as untuned steps would. */
+#define SVM_TUNE_COLS 7
 int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, double* table, int32_t table_rows,
                  int32_t* rows_written);
 /* Persisted schedules (no counterpart in the reference, whose kernelTuner
  * keeps its choice in memory for one run): a text file with one line per
- * key, "svm-schedule-v1 updater precision sphere_bucket size_bucket lanes
- * cluster_blocks graph_steps xcd_mapping vertex_form ms_per_step", where the
+ * key, "svm-schedule-v2 updater precision sphere_bucket size_bucket lanes
+ * cluster_blocks graph_steps xcd_mapping vertex_form step_form ms_per_step"
+ * (a "svm-schedule-v1" line, without step_form, reads as step form 0), where the
  * key is the updater, the force precision, floor(log2(number of spheres)) and
  * floor(log2(mean cells per sphere)) of the context's batch.
  * svm_tuning_save writes the context's current schedule (and the ms per step

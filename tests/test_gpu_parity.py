@@ -58,6 +58,26 @@ def test_geometry_forces_n512(state512):
     assert abs(ctx.area_perimeter(0)[:, 0].sum() - 512.0) < 1e-9
 
 
+def folded_positions(state):
+    """Positions of `state` with vertex 17 pushed across its neighbours by the
+    first overshoot that breaks the angle-sum tiling (found on the CPU), and
+    the oracle model at those positions after its geometry pass."""
+    base = svmtest.OracleModel(state).positions()
+    v = 17
+    nb = state["vn"][v]
+    for f in (3.0, 4.0, 5.0, 6.0):
+        p = base.copy()
+        q = p[v] + f * (p[nb].mean(axis=0) - p[v])
+        p[v] = np.linalg.norm(p[v]) * q / np.linalg.norm(q)
+        om = svmtest.OracleModel(state)
+        om.set_positions(p)
+        n0 = om.n_fan
+        om.compute_geometry()
+        if om.n_fan > n0:
+            return p, om
+    raise AssertionError("no overshoot folded a cell")
+
+
 def test_fan_fallback_matches_oracle(state512):
     """A vertex pushed far across its neighbours folds its cells: the
     angle-sum areas no longer tile the sphere, and the oracle and the device
@@ -65,22 +85,7 @@ def test_fan_fallback_matches_oracle(state512):
     (recomputeAreasCPU), bit for bit, counting one fallback.  In a batch of
     three spheres where only the middle one is folded, the other two keep
     their angle-sum areas."""
-    base = svmtest.OracleModel(state512).positions()
-    v = 17
-    nb = state512["vn"][v]
-    folded = None
-    for f in (3.0, 4.0, 5.0, 6.0):          # the first overshoot that breaks the tiling (on the CPU)
-        p = base.copy()
-        q = p[v] + f * (p[nb].mean(axis=0) - p[v])
-        p[v] = np.linalg.norm(p[v]) * q / np.linalg.norm(q)
-        om = svmtest.OracleModel(state512)
-        om.set_positions(p)
-        n0 = om.n_fan
-        om.compute_geometry()
-        if om.n_fan > n0:
-            folded = p
-            break
-    assert folded is not None
+    folded, om = folded_positions(state512)
     ctx = make_ctx([state512, state512, state512])
     ctx.set_positions(1, folded)
     c0 = ctx.counters()[:, 1].copy()
@@ -157,7 +162,7 @@ def test_t1_paths_agree(state512):
     assert runs[1][2][4] > runs[0][2][4]
 
 
-def _t1_batch_run(sequential, distance=None):
+def _t1_batch_run(sequential, distance=None, step_form=0):
     """48 spheres of 300-6000 cells near the fluid transition: 50 Brownian
     stabilisation steps, then 200 active steps, on one T1 path."""
     g = np.random.Generator(np.random.Philox(4242))
@@ -171,6 +176,7 @@ def _t1_batch_run(sequential, distance=None):
         ctx.set_preferred_parameters(i, 1.0, 1.0)
         ctx.set_brownian(i, dt=0.01, mu=1.0, T=0.0)
     ctx.set_t1_mode(sequential)
+    ctx.set_step_form(step_form)
     if distance is not None:
         ctx.set_t1_component_distance(distance)
         assert ctx.t1_component_distance() == distance
