@@ -242,6 +242,23 @@ def copy_probe(svm, n_bytes=1 << 28):
           f"{2 * n_bytes / ms / 1e6:.0f} GB/s", flush=True)
 
 
+def phase_probe(svm):
+    """What a dependency between two phases of a step costs: a kernel boundary
+    in a HIP graph (the library's form) against a grid barrier inside one
+    persistent cooperative launch, 200 phases each, with 2 and 4 resident
+    workgroups per compute unit (the vertex pass's and the geometry pass's
+    residency) and 1 MiB or 64 MiB written per phase."""
+    out = []
+    for wg in (512, 1024):
+        for mb in (1, 64):
+            launch_us, barrier_us, errors = svm.phase_boundary_probe(wg, 200, mb << 20)
+            row = {"workgroups": wg, "mib_per_phase": mb, "us_per_kernel_boundary": launch_us,
+                   "us_per_grid_barrier": barrier_us, "wrong_values": errors}
+            print(f"phase probe: {row}", flush=True)
+            out.append(row)
+    return out
+
+
 def single_sphere(svm, n=262144, steps=400):
     """The single-system workload: one sphere of n seeded cells (p0 3.8),
     stabilised, then Brownian dynamics (dt 0.01, mu 1, T 0.001, the reference
@@ -561,6 +578,8 @@ def main():
     ap.add_argument("--all-configs", action="store_true",
                     help="only the per-configuration GPU figures and CPU baselines (configurations 1, 2, 4, 5 "
                          "and the libm oracle build): one JSON line, then exit")
+    ap.add_argument("--phase-probe", action="store_true",
+                    help="time a kernel boundary against a grid barrier in a persistent launch, then exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, 3 launches of a plain 256 MiB -> 256 MiB FP64 copy kernel "
                          "(calibrates the HBM byte counters of a profiler)")
@@ -571,6 +590,9 @@ def main():
         sys.exit(2)
     import curvedspacevertexmodel_amd as svm
 
+    if args.phase_probe:
+        print(json.dumps({"phase_probe": phase_probe(svm)}), flush=True)
+        return
     if args.all_configs:
         out = {"gpu": gpu_configs(svm), "gpu_config2": single_sphere(svm),
                "gpu_config5": large_batch(svm, schedules=args.schedules), "cpu": cpu_baseline_configs(svm)}

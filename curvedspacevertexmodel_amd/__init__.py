@@ -45,7 +45,7 @@ EXPORTS = [
     "svm_set_positions_device", "svm_get_positions_device", "svm_set_directors_device", "svm_get_directors_device",
     "svm_set_velocities_device", "svm_get_velocities_device", "svm_get_forces_device",
     "svm_delaunay_triangles_noalloc", "svm_set_vertex_form", "svm_get_vertex_form",
-    "svm_tuning_save", "svm_tuning_load", "svm_set_step_form", "svm_get_step_form",
+    "svm_tuning_save", "svm_tuning_load", "svm_set_step_form", "svm_get_step_form", "svm_phase_boundary_probe",
 ]
 
 NOALLOC_MAX_POINTS = 65536
@@ -267,6 +267,18 @@ def copy_probe(n_bytes, reps=3):
     if rc != 0:
         raise SVMError(rc, "svm_copy_probe")
     return ms.value
+
+
+def phase_boundary_probe(workgroups, phases=200, bytes_per_phase=64 << 20):
+    """Microseconds per phase with a kernel boundary (one launch per phase, in a HIP graph) and with a
+    grid barrier inside one persistent cooperative launch, and the values either read wrong
+    (svm_phase_boundary_probe)."""
+    a, b, e = C.c_double(0.0), C.c_double(0.0), C.c_int64(0)
+    rc = lib().svm_phase_boundary_probe(C.c_int32(workgroups), C.c_int32(phases), C.c_int64(bytes_per_phase),
+                                        C.byref(a), C.byref(b), C.byref(e))
+    if rc != 0:
+        raise SVMError(rc, "svm_phase_boundary_probe")
+    return a.value, b.value, e.value
 
 
 class Context:

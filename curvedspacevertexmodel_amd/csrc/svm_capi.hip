@@ -868,6 +868,73 @@ __global__ __launch_bounds__(256) void k_copy_probe(const double* __restrict__ a
         b[i] = a[i];
 }
 
+// ---- phase-boundary probe ----
+// One phase of the probe's work: workgroup wg writes its row of `per` doubles
+// into row buffer ph & 1, after reading (and checking) what the next
+// workgroup wrote in phase ph - 1.  Rows of one buffer are read only after
+// the boundary that follows their writes, and overwritten only after the
+// boundary after that, so every read must see the previous phase's values.
+__device__ inline void probe_phase(double* rows, int nwg, int wg, int ph, int per, unsigned* errors) {
+    if (ph > 0) {
+        const int src = wg + 1 < nwg ? wg + 1 : 0;
+        const double* r = rows + ((size_t)((ph - 1) & 1) * nwg + src) * per;
+        const double want = (double)((ph - 1) * nwg + src);
+        unsigned bad = 0;
+        for (int i = threadIdx.x; i < per; i += blockDim.x) bad += r[i] != want + (double)i;
+        if (bad) atomicAdd(errors, bad);
+    }
+    double* w = rows + ((size_t)(ph & 1) * nwg + wg) * per;
+    for (int i = threadIdx.x; i < per; i += blockDim.x) w[i] = (double)(ph * nwg + wg) + (double)i;
+}
+
+// The phases as one kernel launch per phase (the library's form).
+__global__ __launch_bounds__(256) void k_probe_phase(double* rows, int per, int ph, unsigned* errors) {
+    probe_phase(rows, (int)gridDim.x, (int)blockIdx.x, ph, per, errors);
+}
+
+// The same phases in one persistent launch, separated by a grid barrier:
+// each workgroup publishes its writes (device-scope release), arrives on a
+// counter, waits until the phase's nwg arrivals are in, then acquires.  The
+// wait is bounded: past spin_limit polls (or once any workgroup gave up) the
+// workgroup sets *gave_up and leaves, so every wavefront exits.  Launched
+// cooperatively, so all workgroups are resident at once.
+__global__ __launch_bounds__(256) void k_probe_persistent(double* rows, int per, int phases, unsigned* arrivals,
+                                                          unsigned* errors, int* gave_up, int spin_limit) {
+    __shared__ int s_quit;
+    const int nwg = (int)gridDim.x, wg = (int)blockIdx.x;
+    for (int ph = 0; ph < phases; ++ph) {
+        probe_phase(rows, nwg, wg, ph, per, errors);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_quit = 0;
+            __threadfence();
+            atomicAdd(arrivals, 1u);
+            const unsigned target = (unsigned)(ph + 1) * (unsigned)nwg;
+            int polls = 0;
+            while (__hip_atomic_load(arrivals, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (++polls > spin_limit || __hip_atomic_load(gave_up, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    __hip_atomic_store(gave_up, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    s_quit = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            __threadfence();
+        }
+        __syncthreads();
+        if (s_quit) return;
+    }
+    // the last phase's rows, checked as the next launch of k_probe_phase would
+    if (phases > 0) {
+        const int src = wg + 1 < nwg ? wg + 1 : 0;
+        const double* r = rows + ((size_t)((phases - 1) & 1) * nwg + src) * per;
+        const double want = (double)((phases - 1) * nwg + src);
+        unsigned bad = 0;
+        for (int i = threadIdx.x; i < per; i += blockDim.x) bad += r[i] != want + (double)i;
+        if (bad) atomicAdd(errors, bad);
+    }
+}
+
 extern "C" {
 
 const char* svm_last_error(svm_context* ctx) { return ctx ? ctx->err.c_str() : k_ok; }
@@ -2498,6 +2565,82 @@ int svm_tuning_load(svm_context* ctx, int32_t updater, const char* path, int32_t
     }
     ctx->tuned_ms = ms;
     return SVM_OK;
+}
+
+int svm_phase_boundary_probe(int32_t workgroups, int32_t phases, int64_t bytes_per_phase, double* us_per_launch,
+                             double* us_per_barrier, int64_t* errors) {
+    if (workgroups < 1 || phases < 2 || bytes_per_phase < 8 * 256 || !us_per_launch || !us_per_barrier || !errors)
+        return SVM_ERR_ARG;
+    int ndev = 0, dev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1 || hipGetDevice(&dev) != hipSuccess)
+        return SVM_ERR_NO_DEVICE;
+    // co-residency of the persistent grid, checked before it is launched
+    int per_cu = 0, cus = 0, coop = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_probe_persistent, 256, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess)
+        return SVM_ERR_HIP;
+    if (!coop || workgroups > per_cu * cus) return SVM_ERR_ARG;
+    const int per = (int)std::max<int64_t>(256, bytes_per_phase / 8 / workgroups);
+    double* rows = nullptr;
+    unsigned* counters = nullptr;   // arrivals, launch-form errors, persistent-form errors
+    int* gave_up = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int rc = SVM_OK;
+    auto check = [&](hipError_t e) { if (e != hipSuccess && rc == SVM_OK) rc = SVM_ERR_HIP; return rc == SVM_OK; };
+    if (check(hipMalloc(&rows, sizeof(double) * 2 * (size_t)workgroups * per)) &&
+        check(hipMalloc(&counters, 3 * sizeof(unsigned))) && check(hipMalloc(&gave_up, sizeof(int))) &&
+        check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) && check(hipEventCreate(&e0)) &&
+        check(hipEventCreate(&e1)) && check(hipMemsetAsync(rows, 0, sizeof(double) * 2 * (size_t)workgroups * per, st)) &&
+        check(hipMemsetAsync(counters, 0, 3 * sizeof(unsigned), st)) && check(hipMemsetAsync(gave_up, 0, sizeof(int), st))) {
+        // kernel boundaries: `phases` launches captured in one graph
+        check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int ph = 0; ph < phases; ++ph)
+            hipLaunchKernelGGL(k_probe_phase, dim3(workgroups), dim3(256), 0, st, rows, per, ph, counters + 1);
+        check(hipStreamEndCapture(st, &graph));
+        if (rc == SVM_OK && check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) &&
+            check(hipGraphLaunch(exec, st)) && check(hipEventRecord(e0, st)) && check(hipGraphLaunch(exec, st)) &&
+            check(hipEventRecord(e1, st)) && check(hipEventSynchronize(e1))) {
+            float ms = 0.f;
+            check(hipEventElapsedTime(&ms, e0, e1));
+            *us_per_launch = 1000.0 * ms / phases;
+        }
+        // grid barriers: one cooperative launch (a warm one, then the timed one)
+        int nph = phases, lim = 1 << 22;
+        unsigned* arrivals = counters;
+        unsigned* perr = counters + 2;
+        void* args[] = {&rows, (void*)&per, &nph, &arrivals, &perr, &gave_up, &lim};
+        for (int pass = 0; pass < 2 && rc == SVM_OK; ++pass) {
+            check(hipMemsetAsync(arrivals, 0, sizeof(unsigned), st));
+            if (pass) check(hipEventRecord(e0, st));
+            check(hipLaunchCooperativeKernel((const void*)k_probe_persistent, dim3(workgroups), dim3(256), args, 0, st));
+            if (pass) check(hipEventRecord(e1, st));
+            check(hipStreamSynchronize(st));
+        }
+        if (rc == SVM_OK) {
+            float ms = 0.f;
+            check(hipEventElapsedTime(&ms, e0, e1));
+            *us_per_barrier = 1000.0 * ms / phases;
+            unsigned h[3] = {0, 0, 0};
+            int g = 0;
+            check(hipMemcpy(h, counters, sizeof h, hipMemcpyDeviceToHost));
+            check(hipMemcpy(&g, gave_up, sizeof g, hipMemcpyDeviceToHost));
+            *errors = (int64_t)h[1] + h[2];
+            if (g) rc = SVM_ERR_HIP;   // a barrier wait ran out: no timing
+        }
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (rows) (void)hipFree(rows);
+    if (counters) (void)hipFree(counters);
+    if (gave_up) (void)hipFree(gave_up);
+    return rc;
 }
 
 int svm_copy_probe(int64_t n_bytes, int32_t reps, double* ms_per_copy) {

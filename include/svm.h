@@ -550,6 +550,20 @@ int svm_time_vertex_pass(svm_context* ctx, int32_t updater, int32_t reps, double
  * copy.  Used to calibrate profiler byte counters (FETCH_SIZE / WRITE_SIZE)
  * and as the achievable-bandwidth reference. */
 int svm_copy_probe(int64_t n_bytes, int32_t reps, double* ms_per_copy);
+/* Phase-boundary yardstick (no counterpart in the reference): what a
+ * dependency between the phases of a step costs as a kernel boundary in a
+ * HIP graph (the library's form) and as a grid barrier inside one persistent
+ * launch.  `workgroups` workgroups of 256 threads (at most what the device
+ * keeps resident at once) run `phases` phases; in each, every workgroup
+ * writes bytes_per_phase / workgroups bytes and reads what the next
+ * workgroup wrote in the phase before.  Form 1: one launch per phase,
+ * captured in one graph.  Form 2: one cooperative launch whose phases are
+ * separated by a device-scope release, an arrival counter, a bounded wait
+ * and an acquire.  Returns the microseconds per phase of each and the
+ * number of values either form read wrong (0 when both boundaries order the
+ * data).  A wait that runs out returns SVM_ERR_HIP. */
+int svm_phase_boundary_probe(int32_t workgroups, int32_t phases, int64_t bytes_per_phase, double* us_per_launch,
+                             double* us_per_barrier, int64_t* errors);
 
 #ifdef __cplusplus
 }
