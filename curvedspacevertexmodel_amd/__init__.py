@@ -46,6 +46,7 @@ EXPORTS = [
     "svm_set_velocities_device", "svm_get_velocities_device", "svm_get_forces_device",
     "svm_delaunay_triangles_noalloc", "svm_set_vertex_form", "svm_get_vertex_form",
     "svm_tuning_save", "svm_tuning_load", "svm_set_step_form", "svm_get_step_form", "svm_phase_boundary_probe",
+    "svm_set_geometry_form", "svm_get_geometry_form",
 ]
 
 NOALLOC_MAX_POINTS = 65536
@@ -100,7 +101,7 @@ MAX_LANES = 4   # SVM_MAX_LANES of include/svm.h
 COUNTER_COLS = 6      # SVM_COUNTER_COLS: per-sphere counters
 FORCE_ERR_COLS = 7    # SVM_FORCE_ERR_COLS: per-sphere force-error statistics
 T1_DIAG_COLS = 12     # SVM_T1_DIAG_COLS: per-sphere T1 diagnostics
-TUNE_COLS = 7         # SVM_TUNE_COLS: autotuner trial-table width
+TUNE_COLS = 8         # SVM_TUNE_COLS: autotuner trial-table width
 
 
 class SVMError(RuntimeError):
@@ -573,16 +574,26 @@ class Context:
         a, b, c = C.c_int32(0), C.c_int32(0), C.c_int32(0)
         self._ck(lib().svm_get_tuning(self.h, C.byref(a), C.byref(b), C.byref(c)), "svm_get_tuning")
         return dict(cluster_blocks=a.value, graph_steps=b.value, xcd_mapping=c.value, lanes=self.lanes()[0],
-                    vertex_form=self.vertex_form(), step_form=self.step_form())
+                    vertex_form=self.vertex_form(), step_form=self.step_form(), geometry_form=self.geometry_form())
 
     def set_vertex_form(self, form):
         """0: a thread per vertex; 1, 2: corner form (three wavefronts per 64 vertices) compiled for
-        4 or 5 wavefronts per SIMD.  Same results, bit for bit."""
+        4 or 5 wavefronts per SIMD; 3: form 0 compiled for 3 wavefronts per SIMD.  Same results, bit for bit."""
         self._ck(lib().svm_set_vertex_form(self.h, C.c_int32(form)), "svm_set_vertex_form")
 
     def vertex_form(self):
         f = C.c_int32(0)
         self._ck(lib().svm_get_vertex_form(self.h, C.byref(f)), "svm_get_vertex_form")
+        return f.value
+
+    def set_geometry_form(self, form):
+        """Cells per thread of the geometry pass: 1 (form 0), 2 (form 1) or 4 (form 2) per 256-row tile's
+        workgroup.  Same results, bit for bit."""
+        self._ck(lib().svm_set_geometry_form(self.h, C.c_int32(form)), "svm_set_geometry_form")
+
+    def geometry_form(self):
+        f = C.c_int32(0)
+        self._ck(lib().svm_get_geometry_form(self.h, C.byref(f)), "svm_get_geometry_form")
         return f.value
 
     def set_step_form(self, form):
@@ -609,7 +620,7 @@ class Context:
 
     def autotune(self, updater=1, steps_per_trial=60):
         """Returns the trial table [(lanes, cluster_blocks, graph_steps, xcd_mapping, vertex_form, step_form,
-        ms/step)]."""
+        geometry_form, ms/step)]."""
         tab = np.zeros((32, TUNE_COLS), np.float64)
         n = C.c_int32(0)
         self._ck(lib().svm_autotune(self.h, C.c_int32(updater), C.c_int32(steps_per_trial), _dp(tab),

@@ -50,6 +50,7 @@ struct svm_context {
     double tuned_ms = 0.0;           // ms per step of the schedule the autotuner kept (0: not tuned)
     int vertex_corner = 0;           // Brownian/active vertex pass: 0 k_vertex; 1, 2 corner form at 4, 5 waves per SIMD
     int step_form = 0;               // svm_set_step_form bits: 1 fan check in the vertex pass, 2 no group kernel
+    int geometry_form = 0;           // k_geometry: 1, 2 or 4 cells per thread (forms 0, 1, 2)
     Dev d{};
     int pcur = 0, ncur = 0;          // current position / director buffer
     std::string err;
@@ -244,8 +245,13 @@ Target whole(svm_context* ctx) { return Target{ctx->d, ctx->stream}; }
 int launch_geometry(svm_context* ctx, const Target& w, int skip_frozen, bool with_fan = true) {
     {
         Timer t(ctx, "k_geometry");
-        hipLaunchKernelGGL(svm::k_geometry, dim3(xcd_grid(w.d.ctn)), dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur,
-                           skip_frozen);
+        const dim3 grid(xcd_grid(w.d.ctn));
+        if (ctx->geometry_form == 2)
+            hipLaunchKernelGGL(svm::k_geometry<4>, grid, dim3(SVM_TILE / 4), 0, w.st, w.d, ctx->pcur, skip_frozen);
+        else if (ctx->geometry_form == 1)
+            hipLaunchKernelGGL(svm::k_geometry<2>, grid, dim3(SVM_TILE / 2), 0, w.st, w.d, ctx->pcur, skip_frozen);
+        else
+            hipLaunchKernelGGL(svm::k_geometry<1>, grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, skip_frozen);
         int r = last_launch(ctx);
         if (r) return r;
     }
@@ -276,6 +282,11 @@ int launch_vertex(svm_context* ctx, const Target& w, int pout, int store_forces,
     else if (ctx->force_fp32 == 2)
         hipLaunchKernelGGL((svm::k_vertex<MODE, svm::VecF64>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
                            ctx->ncur, nout, store_forces, fan_check);
+    else if ((MODE == svm::VM_BROWNIAN || MODE == svm::VM_ACTIVE) && ctx->vertex_corner == 3 && !store_forces) {
+        constexpr int M = MODE == svm::VM_ACTIVE ? svm::VM_ACTIVE : svm::VM_BROWNIAN;
+        hipLaunchKernelGGL((svm::k_vertex_w3<M>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout, ctx->ncur,
+                           nout, fan_check);
+    }
     else if ((MODE == svm::VM_BROWNIAN || MODE == svm::VM_ACTIVE) && ctx->vertex_corner && !store_forces) {
         if (fan_check) return fail(ctx, SVM_ERR_ARG, "the corner vertex form has no fan check");
         constexpr int M = MODE == svm::VM_ACTIVE ? svm::VM_ACTIVE : svm::VM_BROWNIAN;
@@ -350,7 +361,7 @@ int launch_t1(svm_context* ctx, int pcur, int nin, int nout, int dir, int advanc
 int enqueue_step(svm_context* ctx, const Target& w, int updater) {
     // step form flag 1: the area check and fan fallback in the vertex pass (the
     // thread-per-vertex form, spheres whose tile partials one wavefront folds)
-    bool fold = (ctx->step_form & 1) && (ctx->vertex_corner == 0 || ctx->force_fp32 != 0);
+    bool fold = (ctx->step_form & 1) && (ctx->vertex_corner == 0 || ctx->vertex_corner == 3 || ctx->force_fp32 != 0);
     for (int s = w.d.s0; s < w.d.s0 + w.d.ns && fold; ++s) fold = ctx->layout[s].nctiles <= 64 * SVM_FOLD_K;
     int r = launch_geometry(ctx, w, 0, !fold);
     if (r) return r;
@@ -2269,7 +2280,7 @@ int svm_set_tuning(svm_context* ctx, int32_t cluster_blocks, int32_t graph_steps
 }
 
 int svm_set_vertex_form(svm_context* ctx, int32_t form) {
-    if (!ctx || form < 0 || form > 2) return fail(ctx, SVM_ERR_ARG, "vertex form must be 0, 1 or 2");
+    if (!ctx || form < 0 || form > 3) return fail(ctx, SVM_ERR_ARG, "vertex form must be 0, 1, 2 or 3");
     ctx->vertex_corner = form;
     drop_graphs(ctx);
     return SVM_OK;
@@ -2278,6 +2289,19 @@ int svm_set_vertex_form(svm_context* ctx, int32_t form) {
 int svm_get_vertex_form(svm_context* ctx, int32_t* form) {
     if (!ctx || !form) return SVM_ERR_ARG;
     *form = ctx->vertex_corner;
+    return SVM_OK;
+}
+
+int svm_set_geometry_form(svm_context* ctx, int32_t form) {
+    if (!ctx || form < 0 || form > 2) return fail(ctx, SVM_ERR_ARG, "geometry form must be 0, 1 or 2");
+    ctx->geometry_form = form;
+    drop_graphs(ctx);
+    return SVM_OK;
+}
+
+int svm_get_geometry_form(svm_context* ctx, int32_t* form) {
+    if (!ctx || !form) return SVM_ERR_ARG;
+    *form = ctx->geometry_form;
     return SVM_OK;
 }
 
@@ -2388,7 +2412,9 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
         *ms = samples[1];
         return SVM_OK;
     };
-    auto record = [&](int lanes, int cb, int G, int xcd, int form, int sform, double ms) {
+    int best_cb = ctx->cluster_blocks, best_G = ctx->graph_steps, best_x = ctx->d.xcd, best_l = ctx->nlanes;
+    int best_f = 0, best_s = 0, best_g = 0;
+    auto record = [&](int lanes, int cb, int G, int xcd, int form, int sform, int gform, double ms) {
         if (table && nrows < table_rows) {
             double* row = table + (size_t)SVM_TUNE_COLS * nrows;
             row[0] = lanes;
@@ -2397,15 +2423,15 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
             row[3] = xcd;
             row[4] = form;
             row[5] = sform;
-            row[6] = ms;
+            row[6] = gform;
+            row[7] = ms;
         }
         ++nrows;
     };
-    int best_cb = ctx->cluster_blocks, best_G = ctx->graph_steps, best_x = ctx->d.xcd, best_l = ctx->nlanes;
-    int best_f = 0, best_s = 0;
     double best = 1e300;
     r = svm_set_vertex_form(ctx, 0);
     if (!r) r = svm_set_step_form(ctx, 0);
+    if (!r) r = svm_set_geometry_form(ctx, 0);
     // lanes x two dry-run grids, at 10 steps per graph with the XCD mapping
     const int cb_pair[] = {256, 1024};
     for (int l = 1; l <= std::min(ctx->nsys, (int)SVM_MAX_LANES) && !r; ++l)
@@ -2415,7 +2441,7 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
             double ms = 0.0;
             if (!r) r = trial(10, &ms);
             if (r) break;
-            record(l, cb, 10, 1, 0, 0, ms);
+            record(l, cb, 10, 1, 0, 0, 0, ms);
             if (ms < best) { best = ms; best_l = l; best_cb = cb; best_x = 1; best_G = 10; }
         }
     if (!r) r = svm_set_lanes(ctx, best_l);
@@ -2426,18 +2452,30 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
         double ms = 0.0;
         if (!r) r = trial(10, &ms);
         if (r) break;
-        record(best_l, best_cb, 10, 1, 0, sform, ms);
+        record(best_l, best_cb, 10, 1, 0, sform, 0, ms);
         if (ms < best) { best = ms; best_s = sform; best_x = 1; best_G = 10; }
     }
     if (!r) r = svm_set_step_form(ctx, best_s);
-    // the corner forms of the vertex pass (the FP64 parity arithmetic only)
-    for (int form = 1; form <= 2 && !r && ctx->force_fp32 == 0; ++form) {
+    // the geometry pass with 2 and 4 cells per thread
+    for (int gform = 1; gform <= 2 && !r; ++gform) {
+        r = svm_set_tuning(ctx, best_cb, 10, 1);
+        if (!r) r = svm_set_geometry_form(ctx, gform);
+        double ms = 0.0;
+        if (!r) r = trial(10, &ms);
+        if (r) break;
+        record(best_l, best_cb, 10, 1, 0, best_s, gform, ms);
+        if (ms < best) { best = ms; best_g = gform; best_x = 1; best_G = 10; }
+    }
+    if (!r) r = svm_set_geometry_form(ctx, best_g);
+    // the corner forms and the three-wavefront form of the vertex pass (the
+    // FP64 parity arithmetic only)
+    for (int form = 1; form <= 3 && !r && ctx->force_fp32 == 0; ++form) {
         r = svm_set_tuning(ctx, best_cb, 10, 1);
         if (!r) r = svm_set_vertex_form(ctx, form);
         double ms = 0.0;
         if (!r) r = trial(10, &ms);
         if (r) break;
-        record(best_l, best_cb, 10, 1, form, best_s, ms);
+        record(best_l, best_cb, 10, 1, form, best_s, best_g, ms);
         if (ms < best) { best = ms; best_f = form; best_x = 1; best_G = 10; }
     }
     if (!r) r = svm_set_vertex_form(ctx, best_f);
@@ -2450,7 +2488,7 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
             double ms = 0.0;
             if (!r) r = trial(10, &ms);
             if (r) break;
-            record(best_l, cb, 10, xcd, best_f, best_s, ms);
+            record(best_l, cb, 10, xcd, best_f, best_s, best_g, ms);
             if (ms < best) { best = ms; best_cb = cb; best_x = xcd; best_G = 10; }
         }
     // then the graph length
@@ -2461,7 +2499,7 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
         double ms = 0.0;
         if (!r) r = trial(G, &ms);
         if (r) break;
-        record(best_l, best_cb, G, best_x, best_f, best_s, ms);
+        record(best_l, best_cb, G, best_x, best_f, best_s, best_g, ms);
         if (ms < best) { best = ms; best_G = G; }
     }
     (void)hipEventDestroy(e0);
@@ -2471,6 +2509,7 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
     r = svm_set_lanes(ctx, best_l);
     r = r ? r : svm_set_vertex_form(ctx, best_f);
     r = r ? r : svm_set_step_form(ctx, best_s);
+    r = r ? r : svm_set_geometry_form(ctx, best_g);
     r = r ? r : svm_set_tuning(ctx, best_cb, best_G, best_x);
     if (!r) ctx->tuned_ms = best;
     return r;
@@ -2490,7 +2529,8 @@ std::array<int, 4> schedule_key(svm_context* ctx, int updater) {
     for (int s = 0; s < ctx->nsys; ++s) cells += ctx->layout[s].nc;
     return {updater, ctx->force_fp32, floor_log2(ctx->nsys), floor_log2(cells / std::max(1, ctx->nsys))};
 }
-// v2 lines carry the step form; v1 lines (without it) are read as step form 0
+// v2 lines carry the step and geometry forms; v1 lines (without them) are
+// read as forms 0
 const char* kScheduleTag = "svm-schedule-v2";
 const char* kScheduleTagV1 = "svm-schedule-v1";
 }  // namespace
@@ -2513,9 +2553,9 @@ int svm_tuning_save(svm_context* ctx, int32_t updater, const char* path) {
         std::fclose(f);
     }
     char row[512];
-    std::snprintf(row, sizeof row, "%s %d %d %d %d %d %d %d %d %d %d %.6f\n", kScheduleTag, key[0], key[1], key[2],
-                  key[3], ctx->nlanes, ctx->cluster_blocks, ctx->graph_steps, ctx->d.xcd, ctx->vertex_corner,
-                  ctx->step_form, ctx->tuned_ms);
+    std::snprintf(row, sizeof row, "%s %d %d %d %d %d %d %d %d %d %d %d %.6f\n", kScheduleTag, key[0], key[1],
+                  key[2], key[3], ctx->nlanes, ctx->cluster_blocks, ctx->graph_steps, ctx->d.xcd, ctx->vertex_corner,
+                  ctx->step_form, ctx->geometry_form, ctx->tuned_ms);
     keep.push_back(row);
     const std::string tmp = std::string(path) + ".tmp";
     FILE* f = std::fopen(tmp.c_str(), "w");
@@ -2534,16 +2574,16 @@ int svm_tuning_load(svm_context* ctx, int32_t updater, const char* path, int32_t
     FILE* f = std::fopen(path, "r");
     if (!f) return SVM_OK;
     char line[512];
-    int sched[6];
+    int sched[7];
     double ms = 0.0;
     while (std::fgets(line, sizeof line, f)) {
         char tag[64] = {0};
-        int k[4], v[6] = {0, 0, 0, 0, 0, 0};
+        int k[4], v[7] = {0, 0, 0, 0, 0, 0, 0};
         double t = 0.0;
         bool ok = false;
         if (std::sscanf(line, "%63s", tag) == 1 && std::strcmp(tag, kScheduleTag) == 0)
-            ok = std::sscanf(line, "%63s %d %d %d %d %d %d %d %d %d %d %lf", tag, &k[0], &k[1], &k[2], &k[3], &v[0],
-                             &v[1], &v[2], &v[3], &v[4], &v[5], &t) == 12;
+            ok = std::sscanf(line, "%63s %d %d %d %d %d %d %d %d %d %d %d %lf", tag, &k[0], &k[1], &k[2], &k[3],
+                             &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &t) == 13;
         else if (std::strcmp(tag, kScheduleTagV1) == 0)
             ok = std::sscanf(line, "%63s %d %d %d %d %d %d %d %d %d %lf", tag, &k[0], &k[1], &k[2], &k[3], &v[0],
                              &v[1], &v[2], &v[3], &v[4], &t) == 11;
@@ -2559,6 +2599,7 @@ int svm_tuning_load(svm_context* ctx, int32_t updater, const char* path, int32_t
     r = r ? r : svm_set_tuning(ctx, sched[1], sched[2], sched[3]);
     r = r ? r : svm_set_vertex_form(ctx, sched[4]);
     r = r ? r : svm_set_step_form(ctx, sched[5]);
+    r = r ? r : svm_set_geometry_form(ctx, sched[6]);
     if (r) {
         *found = 0;
         return r;

@@ -140,78 +140,120 @@ __device__ inline double dot0(const double* a, const double* b) {
 
 // --------------------------------------------------------------- geometry --
 // sphericalVertexModel::computeGeometryCPU (src/model/sphericalVertexModel.cpp:209-311)
-// and recomputeAreasCPU (:162-207): one thread per cell.
-__global__ __launch_bounds__(SVM_TILE) void k_geometry(Dev d, int pin, int skip_frozen) {
+// for cell c of sphere S: centre (and its spherical terms for the force
+// pass), perimeter and angle-sum area, stored; returns the area.
+__device__ __attribute__((always_inline)) inline double geometry_cell(const Dev& d, const SysParams* S,
+                                                                     const double* X, const double* Y,
+                                                                     const double* Z, int c) {
+    const double R = S->R;
+    const int Cs = d.Cs;
+    double area = 0.0;
+    const int n = d.cnn[c];
+    const int32_t* rc = d.ring + c;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int k = 0; k < n; ++k) {
+        int v = rc[(size_t)k * Cs];
+        sx = sx + X[v];
+        sy = sy + Y[v];
+        sz = sz + Z[v];
+    }
+    double cs[3] = {sx, sy, sz};
+    double inv = 1.0 / svm::sqrt_rn(dot0(cs, cs));
+    double cxv = R * (inv * sx), cyv = R * (inv * sy), czv = R * (inv * sz);
+    d.cx[c] = cxv;
+    d.cy[c] = cyv;
+    d.cz[c] = czv;
+    {   // the centre's spherical-coordinate terms, for every vertex of the cell
+        const PQ q = make_pq<double>(cxv, cyv, czv);
+        d.ctrig[c] = q.cosT;
+        d.ctrig[(size_t)Cs + c] = q.sinT;
+        d.ctrig[2 * (size_t)Cs + c] = q.cosP;
+        d.ctrig[3 * (size_t)Cs + c] = q.sinP;
+    }
+
+    int il = rc[(size_t)(n - 3) * Cs], ic = rc[(size_t)(n - 2) * Cs], in = rc[(size_t)(n - 1) * Cs];
+    double pl[3] = {X[il], Y[il], Z[il]};
+    double pc[3] = {X[ic], Y[ic], Z[ic]};
+    double pn[3] = {X[in], Y[in], Z[in]};
+    double nl = pnorm(pl), nc = pnorm(pc), nnx = pnorm(pn);
+    double cm[3] = {pl[1] * pc[2] - pl[2] * pc[1], pl[2] * pc[0] - pl[0] * pc[2], pl[0] * pc[1] - pl[1] * pc[0]};
+    double ncm = svm::sqrt_rn(dot0(cm, cm));
+    double perimeter = 0.0, angles = 0.0;
+    for (int k = 0; k < n; ++k) {
+        int ip = rc[(size_t)k * Cs];
+        double pp[3] = {X[ip], Y[ip], Z[ip]};
+        perimeter += geodesic(pl, pc, nl, nc, R);
+        double ci[3] = {pc[1] * pn[2] - pc[2] * pn[1], pc[2] * pn[0] - pc[0] * pn[2], pc[0] * pn[1] - pc[1] * pn[0]};
+        double nci = svm::sqrt_rn(dot0(ci, ci));
+        double determinant = pl[0] * ci[0] + pl[1] * ci[1] + pl[2] * ci[2];
+        double angle = svm::acos_lo(-dot0(ci, cm) / (nci * ncm));
+        if (determinant > 0) angle *= -1;
+        if (angle < 0) angle += 2. * SVM_PI;
+        angles += angle;
+        pl[0] = pc[0]; pl[1] = pc[1]; pl[2] = pc[2]; nl = nc;
+        pc[0] = pn[0]; pc[1] = pn[1]; pc[2] = pn[2]; nc = nnx;
+        pn[0] = pp[0]; pn[1] = pp[1]; pn[2] = pp[2]; nnx = pnorm(pp);
+        cm[0] = ci[0]; cm[1] = ci[1]; cm[2] = ci[2]; ncm = nci;
+    }
+    area = (angles - (n - 2) * SVM_PI);
+    int extraAngularArea = (int)__builtin_floor(area / (1.0 * SVM_PI));
+    if (extraAngularArea > 0) area -= extraAngularArea * SVM_PI;
+    area = area * R * R;
+    d.A[c] = area;
+    d.P[c] = perimeter;
+    return area;
+}
+
+// The 256-row tile's halving tree (block_tree256) over the areas of a
+// workgroup whose threads take CPT cells each: thread t holds rows
+// t + k * (256 / CPT), k < CPT.  The tree's first log2(CPT) levels pair rows
+// that one thread holds, so they are added in registers, in the tree's order;
+// the rest is block_tree256's.  Result in thread 0.
+template <int CPT>
+__device__ inline double tile_tree(const double* a, double* red) {
+    const int t = threadIdx.x;
+    if (CPT == 1) return block_tree256(a[0], red);
+    double x;
+    if (CPT == 2) {                       // rows t, t + 128
+        red[t] = a[0] + a[1];
+        __syncthreads();
+        x = t < 64 ? red[t] + red[t + 64] : 0.0;
+    } else {                              // rows t, t + 64, t + 128, t + 192
+        x = (a[0] + a[2]) + (a[1] + a[3]);
+    }
+    if (t < 64) {
+        x = x + __shfl_down(x, 32, 64);
+        x = x + __shfl_down(x, 16, 64);
+        x = x + __shfl_down(x, 8, 64);
+        x = x + __shfl_down(x, 4, 64);
+        x = x + __shfl_down(x, 2, 64);
+        x = x + __shfl_down(x, 1, 64);
+    }
+    return x;
+}
+
+// One workgroup per 256-row cell tile, CPT cells per thread (256 / CPT
+// threads); the tile's area partial for the fan check.  Every CPT gives the
+// same values, bit for bit: each cell is the same function, and the tile's
+// sum the same tree.
+template <int CPT>
+__global__ __launch_bounds__(SVM_TILE / CPT) void k_geometry(Dev d, int pin, int skip_frozen) {
     __shared__ double red[SVM_TILE + 1];
     const int tile = lane_ctile(d);
     if (tile < 0) return;
     const int s = d.ctile_sys[tile];
     SysParams* S = d.sys + s;
     if (skip_frozen && S->f_frozen) return;
-    const int c = tile * SVM_TILE + threadIdx.x;
-    const int local = c - S->co;
-    const double R = S->R;
     const double* X = d.px[pin];
     const double* Y = d.py[pin];
     const double* Z = d.pz[pin];
-    const int Cs = d.Cs;
-    double area = 0.0;
-    if (local < S->nc) {
-        const int n = d.cnn[c];
-        const int32_t* rc = d.ring + c;
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-        for (int k = 0; k < n; ++k) {
-            int v = rc[(size_t)k * Cs];
-            sx = sx + X[v];
-            sy = sy + Y[v];
-            sz = sz + Z[v];
-        }
-        double cs[3] = {sx, sy, sz};
-        double inv = 1.0 / svm::sqrt_rn(dot0(cs, cs));
-        double cxv = R * (inv * sx), cyv = R * (inv * sy), czv = R * (inv * sz);
-        d.cx[c] = cxv;
-        d.cy[c] = cyv;
-        d.cz[c] = czv;
-        {   // the centre's spherical-coordinate terms, for every vertex of the cell
-            const PQ q = make_pq<double>(cxv, cyv, czv);
-            d.ctrig[c] = q.cosT;
-            d.ctrig[(size_t)Cs + c] = q.sinT;
-            d.ctrig[2 * (size_t)Cs + c] = q.cosP;
-            d.ctrig[3 * (size_t)Cs + c] = q.sinP;
-        }
-
-        int il = rc[(size_t)(n - 3) * Cs], ic = rc[(size_t)(n - 2) * Cs], in = rc[(size_t)(n - 1) * Cs];
-        double pl[3] = {X[il], Y[il], Z[il]};
-        double pc[3] = {X[ic], Y[ic], Z[ic]};
-        double pn[3] = {X[in], Y[in], Z[in]};
-        double nl = pnorm(pl), nc = pnorm(pc), nnx = pnorm(pn);
-        double cm[3] = {pl[1] * pc[2] - pl[2] * pc[1], pl[2] * pc[0] - pl[0] * pc[2], pl[0] * pc[1] - pl[1] * pc[0]};
-        double ncm = svm::sqrt_rn(dot0(cm, cm));
-        double perimeter = 0.0, angles = 0.0;
-        for (int k = 0; k < n; ++k) {
-            int ip = rc[(size_t)k * Cs];
-            double pp[3] = {X[ip], Y[ip], Z[ip]};
-            perimeter += geodesic(pl, pc, nl, nc, R);
-            double ci[3] = {pc[1] * pn[2] - pc[2] * pn[1], pc[2] * pn[0] - pc[0] * pn[2], pc[0] * pn[1] - pc[1] * pn[0]};
-            double nci = svm::sqrt_rn(dot0(ci, ci));
-            double determinant = pl[0] * ci[0] + pl[1] * ci[1] + pl[2] * ci[2];
-            double angle = svm::acos_lo(-dot0(ci, cm) / (nci * ncm));
-            if (determinant > 0) angle *= -1;
-            if (angle < 0) angle += 2. * SVM_PI;
-            angles += angle;
-            pl[0] = pc[0]; pl[1] = pc[1]; pl[2] = pc[2]; nl = nc;
-            pc[0] = pn[0]; pc[1] = pn[1]; pc[2] = pn[2]; nc = nnx;
-            pn[0] = pp[0]; pn[1] = pp[1]; pn[2] = pp[2]; nnx = pnorm(pp);
-            cm[0] = ci[0]; cm[1] = ci[1]; cm[2] = ci[2]; ncm = nci;
-        }
-        area = (angles - (n - 2) * SVM_PI);
-        int extraAngularArea = (int)__builtin_floor(area / (1.0 * SVM_PI));
-        if (extraAngularArea > 0) area -= extraAngularArea * SVM_PI;
-        area = area * R * R;
-        d.A[c] = area;
-        d.P[c] = perimeter;
+    double area[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+        const int c = tile * SVM_TILE + threadIdx.x + k * (SVM_TILE / CPT);
+        area[k] = c - S->co < S->nc ? geometry_cell(d, S, X, Y, Z, c) : 0.0;
     }
-    double part = block_tree256(area, red);
+    const double part = tile_tree<CPT>(area, red);
     if (threadIdx.x == 0) d.cpart[tile] = part;
 }
 
@@ -535,9 +577,8 @@ __device__ inline int vertex_force<VecF64>(const Dev& d, const SysParams* S, con
 // load returns that value; every cell has vertices, so every cell's area is
 // replaced, as k_fan_fix replaces it.
 template <int MODE, class FT>
-__global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, int nin, int nout, int store_forces,
-                                                     int fan_check) {
-    __shared__ double red[SVM_TILE + 1];
+__device__ __attribute__((always_inline)) inline void vertex_pass(const Dev& d, int pin, int pout, int nin, int nout,
+                                                                 int store_forces, int fan_check, double* red) {
     const int tile = lane_vtile(d);
     if (tile < 0) return;
     const int s = d.vtile_sys[tile];
@@ -651,6 +692,23 @@ __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, i
             d.vpart[3 * tile + 2] = sv;
         }
     }
+}
+
+template <int MODE, class FT>
+__global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, int nin, int nout, int store_forces,
+                                                     int fan_check) {
+    __shared__ double red[SVM_TILE + 1];
+    vertex_pass<MODE, FT>(d, pin, pout, nin, nout, store_forces, fan_check, red);
+}
+
+// The same pass compiled for three wavefronts per SIMD (at most 168 VGPRs,
+// against the 212 the pass takes unconstrained: the compiler spills the
+// rest), an autotuner form of the Brownian and active steps.
+template <int MODE>
+__global__ __launch_bounds__(SVM_TILE) __attribute__((amdgpu_waves_per_eu(3))) void k_vertex_w3(
+    Dev d, int pin, int pout, int nin, int nout, int fan_check) {
+    __shared__ double red[SVM_TILE + 1];
+    vertex_pass<MODE, double>(d, pin, pout, nin, nout, 0, fan_check, red);
 }
 
 // Corner form of k_vertex for the Brownian and active updaters in the parity

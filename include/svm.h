@@ -460,15 +460,26 @@ int svm_set_tuning(svm_context* ctx, int32_t cluster_blocks, int32_t graph_steps
 int svm_get_tuning(svm_context* ctx, int32_t* cluster_blocks, int32_t* graph_steps, int32_t* xcd_mapping);
 /* Kernel form of the vertex pass of Brownian and active steps in the FP64
  * parity arithmetic (no counterpart in the reference): 0 = a thread per
- * vertex evaluating its three cells (k_vertex, 190 VGPRs: two wavefronts
+ * vertex evaluating its three cells (k_vertex, 212 VGPRs: two wavefronts
  * per SIMD); 1 and 2 = corner form, three wavefronts per 64 vertices,
  * wavefront k evaluating the vertex's neighbour slot k and then its cell slot
  * k, the contributions added in slot order (k_vertex_corner), compiled for
- * four (1) or five (2) wavefronts per SIMD.  All give identical results, bit
- * for bit; they differ in register use and so in how many wavefronts a SIMD
- * keeps in flight to hide memory latency. */
+ * four (1) or five (2) wavefronts per SIMD; 3 = form 0 compiled for three
+ * wavefronts per SIMD (168 VGPRs, the rest spilled).  All give identical
+ * results, bit for bit; they differ in register use and so in how many
+ * wavefronts a SIMD keeps in flight to hide memory latency. */
 int svm_set_vertex_form(svm_context* ctx, int32_t form);
 int svm_get_vertex_form(svm_context* ctx, int32_t* form);
+/* Kernel form of the geometry pass (computeGeometryCPU,
+ * src/model/sphericalVertexModel.cpp:209-311; no counterpart in the
+ * reference for the form itself): each 256-row cell tile is one workgroup
+ * whose threads take 1 (form 0), 2 (form 1) or 4 (form 2) cells each, i.e.
+ * 256, 128 or 64 threads.  A thread walks its cells' rings one after the
+ * other, so a wavefront waits on the longest sum of its lanes' ring lengths
+ * rather than on the longest single ring.  The tile's area sum keeps the
+ * 256-row halving tree.  All forms give identical results, bit for bit. */
+int svm_set_geometry_form(svm_context* ctx, int32_t form);
+int svm_get_geometry_form(svm_context* ctx, int32_t* form);
 /* Launch structure of the Brownian, active and Vicsek steps (no counterpart
  * in the reference, whose step is a fixed sequence of kernels and host
  * synchronisations, src/model/sphericalVertexModel.cpp:795-892): a sum of
@@ -509,23 +520,25 @@ int svm_get_lanes(svm_context* ctx, int32_t* n_lanes, int32_t* first_sphere);
  * (src/utilities/kernelTuner.cpp:35-126) but over the whole step: times
  * real steps of `updater` (0 Brownian, 1 active) under each candidate
  * schedule -- first lanes in {1 .. min(SVM_MAX_LANES, spheres)} x
- * cluster_blocks in {256, 1024} with the thread-per-vertex form and step
- * form 0; at the best lane count the step forms 1, 2 and 3; then the corner
- * vertex forms 1 and 2 (FP64 parity forces only); then the rest of
- * cluster_blocks in {128, 256, 512, 1024} x xcd_mapping in {0, 1}, all at 10
- * steps per graph; then graph_steps in {4, 20, 40} -- keeps the fastest and
- * reports every trial in table (rows of SVM_TUNE_COLS doubles: lanes,
- * cluster_blocks, graph_steps, xcd_mapping, vertex_form, step_form, ms per
+ * cluster_blocks in {256, 1024} with vertex, step and geometry forms 0; at
+ * the best lane count the step forms 1, 2 and 3; then the geometry forms 1
+ * and 2; then the vertex forms 1, 2 and 3 (FP64 parity forces only); then the
+ * rest of cluster_blocks in {128, 256, 512, 1024} x xcd_mapping in {0, 1},
+ * all at 10 steps per graph; then graph_steps in {4, 20, 40} -- each stage
+ * keeping the best of the stages before -- keeps the fastest and reports
+ * every trial in table (rows of SVM_TUNE_COLS doubles: lanes, cluster_blocks,
+ * graph_steps, xcd_mapping, vertex_form, step_form, geometry_form, ms per
  * step).  The steps advance the model (about steps_per_trial per candidate),
  * exactly as untuned steps would. */
-#define SVM_TUNE_COLS 7
+#define SVM_TUNE_COLS 8
 int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, double* table, int32_t table_rows,
                  int32_t* rows_written);
 /* Persisted schedules (no counterpart in the reference, whose kernelTuner
  * keeps its choice in memory for one run): a text file with one line per
  * key, "svm-schedule-v2 updater precision sphere_bucket size_bucket lanes
- * cluster_blocks graph_steps xcd_mapping vertex_form step_form ms_per_step"
- * (a "svm-schedule-v1" line, without step_form, reads as step form 0), where the
+ * cluster_blocks graph_steps xcd_mapping vertex_form step_form geometry_form
+ * ms_per_step" (a "svm-schedule-v1" line, without the step and geometry
+ * forms, reads them as 0), where the
  * key is the updater, the force precision, floor(log2(number of spheres)) and
  * floor(log2(mean cells per sphere)) of the context's batch.
  * svm_tuning_save writes the context's current schedule (and the ms per step

@@ -39,22 +39,24 @@ def test_autotune_steps_are_ordinary_steps(states):
     for i in range(len(states)):
         ctx.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
     tab = ctx.autotune(updater=1, steps_per_trial=30)
-    # lanes 1, 2, 3 (capped at the 3 spheres) x 2 grids, 3 step forms, 2 corner vertex forms, 6 more kernel
-    # schedules, 3 graph lengths
-    assert len(tab) == 20 and (tab[:, 6] > 0).all()
+    # lanes 1, 2, 3 (capped at the 3 spheres) x 2 grids, 3 step forms, 2 geometry forms, 3 vertex forms,
+    # 6 more kernel schedules, 3 graph lengths
+    assert len(tab) == 23 and (tab[:, 7] > 0).all()
     np.testing.assert_array_equal(tab[:6, 0], [1, 1, 2, 2, 3, 3])
-    np.testing.assert_array_equal(tab[:9, 4], 0)
+    np.testing.assert_array_equal(tab[:11, 4], 0)
     np.testing.assert_array_equal(tab[:6, 5], 0)
+    np.testing.assert_array_equal(tab[:9, 6], 0)
     np.testing.assert_array_equal(tab[6:9, 5], [1, 2, 3])
-    np.testing.assert_array_equal(tab[9:11, 4], [1, 2])
-    assert (tab[6:, 0] == tab[np.argmin(tab[:6, 6]), 0]).all()
+    np.testing.assert_array_equal(tab[9:11, 6], [1, 2])
+    np.testing.assert_array_equal(tab[11:14, 4], [1, 2, 3])
+    assert (tab[6:, 0] == tab[np.argmin(tab[:6, 7]), 0]).all()
     t = ctx.tuning()
-    best = tab[np.argmin(tab[:, 6])]
+    best = tab[np.argmin(tab[:, 7])]
     assert (t["lanes"], t["cluster_blocks"], t["graph_steps"], t["xcd_mapping"], t["vertex_form"],
-            t["step_form"]) == tuple(int(x) for x in best[:6])
+            t["step_form"], t["geometry_form"]) == tuple(int(x) for x in best[:7])
     # the number of steps the tuner took, from the trial table: capture warm-up + 3 samples per candidate
     steps = 0
-    for lanes, cb, G, x, form, sform, ms in tab:
+    for lanes, cb, G, x, form, sform, gform, ms in tab:
         G = int(G)
         steps += G + 3 * max(1, 30 // (3 * G)) * G
     ref = make_ctx(states, p0=3.6)
@@ -78,6 +80,7 @@ def test_persisted_schedule_round_trip(states, tmp_path):
     a.set_tuning(cluster_blocks=512, graph_steps=4, xcd_mapping=0)
     a.set_vertex_form(2)
     a.set_step_form(3)
+    a.set_geometry_form(2)
     a.save_tuning(path)
     a.save_tuning(path, updater=0)                 # a second key in the same file
     a.save_tuning(path)                            # replaces its own line
@@ -88,18 +91,19 @@ def test_persisted_schedule_round_trip(states, tmp_path):
         b.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
     assert b.load_tuning(path)
     assert b.tuning() == dict(cluster_blocks=512, graph_steps=4, xcd_mapping=0, lanes=2, vertex_form=2,
-                              step_form=3)
-    # a line of the previous format (no step form) reads as step form 0
+                              step_form=3, geometry_form=2)
+    # a line of the previous format (no step or geometry form) reads them as 0
     old = str(tmp_path / "schedules_v1.txt")
     with open(old, "w") as f:
         for l in lines:
             w = l.split()
-            f.write(" ".join(["svm-schedule-v1"] + w[1:10] + w[11:]) + "\n")
+            f.write(" ".join(["svm-schedule-v1"] + w[1:10] + w[12:]) + "\n")
     c = make_ctx(states, p0=3.6)
     c.set_step_form(1)
+    c.set_geometry_form(1)
     assert c.load_tuning(old)
     assert c.tuning() == dict(cluster_blocks=512, graph_steps=4, xcd_mapping=0, lanes=2, vertex_form=2,
-                              step_form=0)
+                              step_form=0, geometry_form=0)
     b.set_force_precision(1)                       # another key: FP32 forces
     assert not b.load_tuning(path)
     b.set_force_precision(0)
