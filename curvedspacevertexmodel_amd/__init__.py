@@ -46,7 +46,7 @@ EXPORTS = [
     "svm_set_velocities_device", "svm_get_velocities_device", "svm_get_forces_device",
     "svm_delaunay_triangles_noalloc", "svm_set_vertex_form", "svm_get_vertex_form",
     "svm_tuning_save", "svm_tuning_load", "svm_set_step_form", "svm_get_step_form", "svm_phase_boundary_probe",
-    "svm_set_geometry_form", "svm_get_geometry_form",
+    "svm_set_geometry_form", "svm_get_geometry_form", "svm_neighbor_list",
 ]
 
 NOALLOC_MAX_POINTS = 65536
@@ -167,6 +167,34 @@ def delaunay_triangles_noalloc(points, stream=None):
     if rc != 0:
         raise SVMError(rc, "svm_delaunay_triangles_noalloc")
     return out
+
+
+def neighbor_list(points, range_=1.0, box_half=None, max_neighbors=4):
+    """Metric neighbour lists (svm_neighbor_list; neighborList::computeNeighborLists): returns
+    (counts int32[N], indices int32[N, M], vectors float64[N, M, 3], distances float64[N, M]) with M the
+    largest count, each point's neighbours in index order.  box_half defaults to max|coordinate| + 2.5
+    (sphericalModel's sphereRadius R + 2.5).  Like the reference, a list that overflows max_neighbors is
+    recomputed with the needed width."""
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    n = len(pts)
+    if box_half is None:
+        box_half = float(np.abs(pts).max()) + 2.5
+    counts = np.empty(n, np.int32)
+    needed = C.c_int32(0)
+    M = max(int(max_neighbors), 1)
+    while True:
+        idx = np.empty((n, M), np.int32)
+        vec = np.empty((n, M, 3), np.float64)
+        dist = np.empty((n, M), np.float64)
+        rc = lib().svm_neighbor_list(C.c_int32(n), _dp(pts), C.c_double(range_), C.c_double(box_half),
+                                     C.c_int32(M), _ip(counts), _ip(idx), _dp(vec), _dp(dist), C.byref(needed))
+        if rc == 8 and needed.value > M:     # SVM_ERR_CAPACITY: recompute with a larger width
+            M = needed.value
+            continue
+        if rc != 0:
+            raise SVMError(rc, "svm_neighbor_list")
+        w = max(int(needed.value), 0)
+        return counts, idx[:, :w].copy(), vec[:, :w].copy(), dist[:, :w].copy()
 
 
 def cell_list(points, cell_size):

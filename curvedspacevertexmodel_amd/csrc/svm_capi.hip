@@ -27,6 +27,7 @@ static inline int xcd_grid(int ntiles) { return (ntiles + 7) / 8 * 8; }
 #include "svm_kernels.h"
 #include "svm_delaunay.h"
 #include "svm_voronoi.h"
+#include "svm_neighbors.h"
 #include "svm_mlp.h"
 #include "svm_netcdf.h"
 
@@ -1576,6 +1577,83 @@ int svm_cell_list(int32_t n_points, const double* points, double cell_size, int3
             hipMemcpyAsync(sorted_points, c.list, sizeof(int) * n_points, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
             r = SVM_ERR_HIP;
+    }
+    (void)hipStreamDestroy(st);
+    return r;
+}
+
+int svm_neighbor_list(int32_t n_points, const double* points, double range, double box_half, int32_t max_neighbors,
+                      int32_t* counts, int32_t* indices, double* vectors, double* distances, int32_t* needed) {
+    if (n_points < 1 || !points || !(range > 0.0) || !std::isfinite(range) || max_neighbors < 0 || !counts ||
+        !needed || (max_neighbors > 0 && (!indices || !vectors || !distances)))
+        return SVM_ERR_ARG;
+    double Rs = 0.0;
+    for (size_t i = 0; i < 3 * (size_t)n_points; ++i) {
+        if (!std::isfinite(points[i])) return SVM_ERR_ARG;
+        Rs = std::max(Rs, std::fabs(points[i]));
+    }
+    // no periodic image may change a difference within range or bring a far
+    // pair within range: box_half >= range and 2 box_half - 2 Rs >= range
+    if (!(box_half >= range) || !(2.0 * box_half - 2.0 * Rs >= range)) return SVM_ERR_ARG;
+    if (!(Rs > 0.0)) Rs = range;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SVM_ERR_NO_DEVICE;
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return SVM_ERR_HIP;
+    int r = SVM_OK;
+    {
+        DevScratch m;
+        DlCells c;
+        const size_t n = (size_t)n_points;
+        double* P = m.get<double>(3 * n);
+        int* cnt = m.get<int>(n);
+        int* maxc = m.get<int>(1);
+        const int tb = 256;
+        const dim3 grid((unsigned)((n + tb - 1) / tb));
+        int hmax = 0;
+        if (!P || !cnt || !maxc ||
+            hipMemcpyAsync(P, points, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+            !dl_cell_list(st, m, P, n_points, Rs, range, c)) {
+            r = SVM_ERR_HIP;
+        } else {
+            hipLaunchKernelGGL(svm::k_fill_int, dim3(1), dim3(tb), 0, st, maxc, 1, 0);
+            hipLaunchKernelGGL(svm::k_neighbors<false>, grid, dim3(tb), 0, st, (const double*)P, n_points, c.g,
+                               (const int*)c.start, (const int*)c.list, range, 0, cnt, maxc, (int*)nullptr,
+                               (double*)nullptr, (double*)nullptr);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(counts, cnt, sizeof(int) * n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(&hmax, maxc, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                r = SVM_ERR_HIP;
+        }
+        if (!r) {
+            *needed = hmax;
+            if (hmax > max_neighbors) {
+                r = SVM_ERR_CAPACITY;   // the reference's recompute with a larger Nmax (neighborList.cpp:150-153)
+            } else if (max_neighbors > 0) {
+                const size_t slots = n * (size_t)max_neighbors;
+                int* idx = m.get<int>(slots);
+                double* vec = m.get<double>(3 * slots);
+                double* dst = m.get<double>(slots);
+                if (!idx || !vec || !dst || hipMemsetAsync(idx, 0, sizeof(int) * slots, st) != hipSuccess ||
+                    hipMemsetAsync(vec, 0, sizeof(double) * 3 * slots, st) != hipSuccess ||
+                    hipMemsetAsync(dst, 0, sizeof(double) * slots, st) != hipSuccess) {
+                    r = SVM_ERR_HIP;
+                } else {
+                    hipLaunchKernelGGL(svm::k_neighbors<true>, grid, dim3(tb), 0, st, (const double*)P, n_points,
+                                       c.g, (const int*)c.start, (const int*)c.list, range, max_neighbors, cnt,
+                                       maxc, idx, vec, dst);
+                    if (hipGetLastError() != hipSuccess ||
+                        hipMemcpyAsync(indices, idx, sizeof(int) * slots, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                        hipMemcpyAsync(vectors, vec, sizeof(double) * 3 * slots, hipMemcpyDeviceToHost, st) !=
+                            hipSuccess ||
+                        hipMemcpyAsync(distances, dst, sizeof(double) * slots, hipMemcpyDeviceToHost, st) !=
+                            hipSuccess ||
+                        hipStreamSynchronize(st) != hipSuccess)
+                        r = SVM_ERR_HIP;
+                }
+            }
+        }
     }
     (void)hipStreamDestroy(st);
     return r;

@@ -150,6 +150,25 @@ int svm_delaunay_triangles_noalloc(int32_t n_points, const double* points, int32
  * With bin_of, bin_start or sorted_points NULL only grid_dim is returned. */
 int svm_cell_list(int32_t n_points, const double* points, double cell_size, int32_t* grid_dim, int32_t* bin_of,
                   int32_t* bin_start, int32_t* sorted_points);
+/* Metric neighbour lists (neighborList::computeNeighborLists,
+ * src/utilities/neighborList.cpp:102-232 and neighborList.cu:108-355; the
+ * list sphericalModel::setRadius builds with range 1 and sphereRadius R + 2.5,
+ * src/model/sphericalModel.cpp:21): for each point i the points j != i with
+ * |d| < range, d = p_i - p_j and |d| = sqrt(dx*dx + dy*dy + dz*dz), as the
+ * reference's minDist and norm form them.  box_half is the reference's
+ * sphereRadius, the half side of its periodic box; only boxes where no
+ * periodic image changes the result are accepted (box_half >= range and
+ * 2 box_half - 2 max|coordinate| >= range, which R + 2.5 with range 1 meets),
+ * otherwise SVM_ERR_ARG.  counts: int32[n]; with max_neighbors >= the largest
+ * count, indices int32[n * max_neighbors], vectors double[3 * n *
+ * max_neighbors] (d) and distances double[n * max_neighbors] (|d|), entry k of
+ * point i at i * max_neighbors + k, each point's neighbours in increasing
+ * index order (the reference's order is that of its atomics or cell scan),
+ * unused entries zero.  When some count exceeds max_neighbors only counts
+ * are written and SVM_ERR_CAPACITY is returned; *needed is the largest count
+ * either way (the reference's recompute with a larger Nmax). */
+int svm_neighbor_list(int32_t n_points, const double* points, double range, double box_half, int32_t max_neighbors,
+                      int32_t* counts, int32_t* indices, double* vectors, double* distances, int32_t* needed);
 /* Seeded cell centres: sphericalModel::setParticlePositionsRandomly + setRadius
  * (src/model/sphericalModel.cpp:11-51) with Philox draws keyed by `seed`. */
 int svm_generate_cell_points(int32_t n_cells, uint64_t seed, double* cell_points);

@@ -13,6 +13,7 @@
 //                                  src/updaters/vectorialVicsek.h:31-42
 //   energyMinimizerFIRE            src/updaters/energyMinimizerFIRE.h:21-96
 //   Simulation                     src/simulation/simulation.h:18-77
+//   neighborList                   src/utilities/neighborList.h:8-76
 //
 // Differences a caller sees: the model is built from a seed (the reference
 // takes a noiseSource) and always runs on the GPU (there is no CPU path in
@@ -45,6 +46,49 @@ inline void check(int rc, svm_context* ctx, const char* what) {
 
 // What the updaters and the Simulation see of a model (simpleModel,
 // src/model/simpleModel.h:33-145).
+// Metric neighbour lists of a set of points (neighborList,
+// src/utilities/neighborList.h:8-76): neighborList(range, sphereRadius), then
+// computeNeighborLists(points) fills neighborsPerParticle, particleIndices,
+// neighborVectors and neighborDistances, entry n of particle p at
+// p * Nmax + n (the reference's neighborIndexer(n, p)); Nmax grows and the
+// lists are recomputed when a particle has more neighbours, as in the
+// reference (:150-153).  Computed on the GPU (svm_neighbor_list); each
+// particle's neighbours in increasing index order.
+class neighborList {
+   public:
+    neighborList(double range, double sphereRadius) : maxRange(range), radius(sphereRadius) {}
+    void computeNeighborLists(const std::vector<dVec>& points) {
+        const int n = (int)points.size();
+        std::vector<double> p(3 * (size_t)n);
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) p[3 * (size_t)i + k] = points[i][k];
+        neighborsPerParticle.assign(n, 0);
+        while (true) {
+            particleIndices.assign((size_t)n * Nmax, 0);
+            std::vector<double> vec(3 * (size_t)n * Nmax), dist((size_t)n * Nmax);
+            int32_t needed = 0;
+            const int rc = svm_neighbor_list(n, p.data(), maxRange, radius, Nmax, neighborsPerParticle.data(),
+                                             particleIndices.data(), vec.data(), dist.data(), &needed);
+            if (rc == SVM_ERR_CAPACITY && needed > Nmax) {
+                Nmax = needed;
+                continue;
+            }
+            check(rc, nullptr, "svm_neighbor_list");
+            neighborVectors.assign((size_t)n * Nmax, dVec{0.0, 0.0, 0.0});
+            for (size_t e = 0; e < (size_t)n * Nmax; ++e)
+                neighborVectors[e] = dVec{vec[3 * e], vec[3 * e + 1], vec[3 * e + 2]};
+            neighborDistances = dist;
+            return;
+        }
+    }
+    double maxRange, radius;
+    int Nmax = 4;   // neighborList::setBasics (src/utilities/neighborList.cpp:18)
+    std::vector<int32_t> neighborsPerParticle;
+    std::vector<int32_t> particleIndices;
+    std::vector<dVec> neighborVectors;
+    std::vector<double> neighborDistances;
+};
+
 class simpleModel {
    public:
     virtual ~simpleModel() = default;

@@ -614,6 +614,47 @@ void or_generate_directors(int nv, const double* pos, uint64_t seed, double* out
     }
 }
 
+// neighborList::computeCPU (src/utilities/neighborList.cpp:102-166) without
+// the cell list: every pair, the difference by
+// periodicBoundaryConditions::minDist in a cube of side 2 * box_half
+// (inc/periodicBoundaryConditions.h:124-133: whole box lengths added or
+// removed until each component is within +-box_half), its norm
+// sqrt(dot(d, d)) (inc/functions.h:131-138, 170-173), kept when
+// !(dist >= range) (neighborList.cpp:139).  Neighbours in increasing index
+// order.  counts[n]; with M >= every count, idx[n*M], vec[3*n*M], dist[n*M]
+// (unused entries zero).  Returns the largest count.
+int or_neighbor_list(int n, const double* pts, double range, double box_half, int M, int* counts, int* idx,
+                     double* vec, double* dist) {
+    const double box = 2.0 * box_half;
+    int maxc = 0;
+    for (int i = 0; i < n; ++i) {
+        int c = 0;
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            double d[3];
+            for (int k = 0; k < 3; ++k) {
+                d[k] = pts[3 * i + k] - pts[3 * j + k];
+                while (d[k] < -box_half) d[k] += box;
+                while (d[k] > box_half) d[k] -= box;
+            }
+            double dd = 0.0;
+            for (int k = 0; k < 3; ++k) dd += d[k] * d[k];
+            const double r = std::sqrt(dd);
+            if (r >= range) continue;
+            if (c < M) {
+                const size_t e = (size_t)i * M + c;
+                idx[e] = j;
+                for (int k = 0; k < 3; ++k) vec[3 * e + k] = d[k];
+                dist[e] = r;
+            }
+            ++c;
+        }
+        counts[i] = c;
+        maxc = std::max(maxc, c);
+    }
+    return maxc;
+}
+
 // Topology of the vertex model from a spherical Delaunay triangulation,
 // following convexHullCGALInterface::sphericalConvexHullForVertexModel
 // (src/model/convexHullCGAL.cpp:82-186) and getFacetCenter (:67-80).

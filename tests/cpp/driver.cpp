@@ -8,6 +8,7 @@
 //
 // Prints "energy_after_stabilise", "energy_after_active", FIRE iterations,
 // force and final energy, and a checksum of the final positions.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
@@ -64,6 +65,23 @@ int main(int argc, char** argv) {
         double vcs = 0.0;
         for (const auto& p : vor->returnPositions()) vcs += p[0] + 2.0 * p[1] + 3.0 * p[2];
         std::printf("voronoi_checksum %.17g\n", vcs);
+        // metric neighbour lists of the vertex model's final positions, as
+        // sphericalModel::setRadius configures them (range 1, sphereRadius
+        // max|coordinate| + 2.5)
+        const auto pos = model->returnPositions();
+        double rmax = 0.0;
+        for (const auto& p : pos)
+            for (int k = 0; k < 3; ++k) rmax = std::fmax(rmax, std::fabs(p[k]));
+        svm::neighborList nl(1.0, rmax + 2.5);
+        nl.computeNeighborLists(pos);
+        long long pairs = 0;
+        double ncs = 0.0;
+        for (size_t i = 0; i < pos.size(); ++i) {
+            pairs += nl.neighborsPerParticle[i];
+            for (int k = 0; k < nl.neighborsPerParticle[i]; ++k)
+                ncs += nl.particleIndices[i * nl.Nmax + k] + nl.neighborDistances[i * nl.Nmax + k];
+        }
+        std::printf("neighbor_pairs %lld\nneighbor_nmax %d\nneighbor_checksum %.17g\n", pairs, nl.Nmax, ncs);
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

@@ -40,6 +40,28 @@ def oracle_lib(libm=False):
     return L
 
 
+def oracle_neighbor_list(points, range_=1.0, box_half=None, L=None):
+    """The oracle's metric neighbour lists (or_neighbor_list): (counts, idx[N, M], vec[N, M, 3],
+    dist[N, M]) with M the largest count, neighbours in index order."""
+    L = L or oracle_lib()
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    n = len(pts)
+    if box_half is None:
+        box_half = float(np.abs(pts).max()) + 2.5
+    counts = np.zeros(n, np.int32)
+    f = L.or_neighbor_list
+    f.restype = C.c_int
+    f.argtypes = [C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                  C.c_void_p]
+    M = f(n, pts.ctypes.data, range_, box_half, 0, counts.ctypes.data, None, None, None)
+    idx = np.zeros((n, M), np.int32)
+    vec = np.zeros((n, M, 3), np.float64)
+    dist = np.zeros((n, M), np.float64)
+    f(n, pts.ctypes.data, range_, box_half, M, counts.ctypes.data, idx.ctypes.data, vec.ctypes.data,
+      dist.ctypes.data)
+    return counts, idx, vec, dist
+
+
 # ------------------------------------------------------------ topology ----
 def orient_exact(a, b, c, d=None):
     """Sign of det[b-a, c-a, d-a] (or det[a, b, c] when d is None) in exact rationals."""

@@ -81,3 +81,10 @@ def test_driver_matches_python_binding(tmp_path):
     for x, y, z in vctx.generators(0).tolist():
         cs += x + 2.0 * y + 3.0 * z
     assert float(out["voronoi_checksum"]) == cs
+    counts, idx, vec, dist = svm.neighbor_list(ctx.positions(0), 1.0)
+    assert int(out["neighbor_pairs"]) == counts.sum() and int(out["neighbor_nmax"]) == max(4, idx.shape[1])
+    cs = 0.0
+    for i in range(len(counts)):                 # the driver's summation order
+        for k in range(counts[i]):
+            cs += float(idx[i, k]) + float(dist[i, k])
+    assert float(out["neighbor_checksum"]) == cs
