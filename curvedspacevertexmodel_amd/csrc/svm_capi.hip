@@ -271,25 +271,25 @@ int launch_geometry(svm_context* ctx, int skip_frozen) { return launch_geometry(
 
 // The active updater also writes the rotated directors, into the other
 // director buffer (1 - ncur); no other mode touches directors.
-template <int MODE>
-int launch_vertex(svm_context* ctx, const Target& w, int pout, int store_forces, const char* name,
-                  int fan_check = 0) {
-    Timer t(ctx, name);
+// FAN: the vertex pass also does the geometry's area check and fan fallback
+// (step form flag 1; Brownian and active passes only).
+template <int MODE, bool FAN>
+int launch_vertex_as(svm_context* ctx, const Target& w, int pout, int store_forces) {
     const dim3 grid(xcd_grid(w.d.vtn));
     const int nout = 1 - ctx->ncur;
     if (ctx->force_fp32 == 1)
-        hipLaunchKernelGGL((svm::k_vertex<MODE, float>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
-                           ctx->ncur, nout, store_forces, fan_check);
+        hipLaunchKernelGGL((svm::k_vertex<MODE, float, FAN>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
+                           ctx->ncur, nout, store_forces);
     else if (ctx->force_fp32 == 2)
-        hipLaunchKernelGGL((svm::k_vertex<MODE, svm::VecF64>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
-                           ctx->ncur, nout, store_forces, fan_check);
+        hipLaunchKernelGGL((svm::k_vertex<MODE, svm::VecF64, FAN>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur,
+                           pout, ctx->ncur, nout, store_forces);
     else if ((MODE == svm::VM_BROWNIAN || MODE == svm::VM_ACTIVE) && ctx->vertex_corner == 3 && !store_forces) {
         constexpr int M = MODE == svm::VM_ACTIVE ? svm::VM_ACTIVE : svm::VM_BROWNIAN;
-        hipLaunchKernelGGL((svm::k_vertex_w3<M>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout, ctx->ncur,
-                           nout, fan_check);
+        hipLaunchKernelGGL((svm::k_vertex_w3<M, FAN>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
+                           ctx->ncur, nout);
     }
     else if ((MODE == svm::VM_BROWNIAN || MODE == svm::VM_ACTIVE) && ctx->vertex_corner && !store_forces) {
-        if (fan_check) return fail(ctx, SVM_ERR_ARG, "the corner vertex form has no fan check");
+        if (FAN) return fail(ctx, SVM_ERR_ARG, "the corner vertex form has no fan check");
         constexpr int M = MODE == svm::VM_ACTIVE ? svm::VM_ACTIVE : svm::VM_BROWNIAN;
         const dim3 cgrid(xcd_grid(w.d.vtn * (SVM_TILE / SVM_CORNER_V)));
         if (ctx->vertex_corner == 2)
@@ -300,9 +300,19 @@ int launch_vertex(svm_context* ctx, const Target& w, int pout, int store_forces,
                                pout, ctx->ncur, nout);
     }
     else
-        hipLaunchKernelGGL((svm::k_vertex<MODE, double>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
-                           ctx->ncur, nout, store_forces, fan_check);
+        hipLaunchKernelGGL((svm::k_vertex<MODE, double, FAN>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
+                           ctx->ncur, nout, store_forces);
     return last_launch(ctx);
+}
+
+template <int MODE>
+int launch_vertex(svm_context* ctx, const Target& w, int pout, int store_forces, const char* name,
+                  bool fan_check = false) {
+    Timer t(ctx, name);
+    if constexpr (MODE == svm::VM_BROWNIAN || MODE == svm::VM_ACTIVE)
+        if (fan_check) return launch_vertex_as<MODE, true>(ctx, w, pout, store_forces);
+    if (fan_check) return fail(ctx, SVM_ERR_ARG, "fan check only in Brownian and active passes");
+    return launch_vertex_as<MODE, false>(ctx, w, pout, store_forces);
 }
 template <int MODE>
 int launch_vertex(svm_context* ctx, int pout, int store_forces, const char* name) {

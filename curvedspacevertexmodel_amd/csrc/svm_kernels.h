@@ -568,7 +568,7 @@ __device__ inline int vertex_force<VecF64>(const Dev& d, const SysParams* S, con
     return vertex_force_vec<double>(d, S, X, Y, Z, v, f);
 }
 
-// fan_check = 1: the total-area check and the fan fallback of the geometry
+// FAN = true: the total-area check and the fan fallback of the geometry
 // (k_fan_fix) are done here, for spheres of at most 64 * SVM_FOLD_K cell
 // tiles.  Every wavefront folds its sphere's tile partials itself; where the
 // check fails, each thread gives the three cells of its vertex their fan
@@ -576,9 +576,9 @@ __device__ inline int vertex_force<VecF64>(const Dev& d, const SysParams* S, con
 // touches a cell stores the same value there before it loads it, so each
 // load returns that value; every cell has vertices, so every cell's area is
 // replaced, as k_fan_fix replaces it.
-template <int MODE, class FT>
+template <int MODE, class FT, bool FAN>
 __device__ __attribute__((always_inline)) inline void vertex_pass(const Dev& d, int pin, int pout, int nin, int nout,
-                                                                 int store_forces, int fan_check, double* red) {
+                                                                 int store_forces, double* red) {
     const int tile = lane_vtile(d);
     if (tile < 0) return;
     const int s = d.vtile_sys[tile];
@@ -594,17 +594,19 @@ __device__ __attribute__((always_inline)) inline void vertex_pass(const Dev& d, 
         if (valid) { d.px[pout][v] = X[v]; d.py[pout][v] = Y[v]; d.pz[pout][v] = Z[v]; }
         return;
     }
-    const bool fan = fan_check && fan_needed(S, fold_wave(d.cpart + S->ctile0, 1, S->nctiles, fold_width(S->nctiles)));
+    const bool fan = FAN && fan_needed(S, fold_wave(d.cpart + S->ctile0, 1, S->nctiles, fold_width(S->nctiles)));
     double f[3] = {0.0, 0.0, 0.0};
     double tp = 0.0, tf = 0.0, tv = 0.0;
     int nan = 0;
     if (valid) {
-        if (fan) {
+        if (FAN && fan) {   // rare: one loop body, not three inlined copies
             if (local == 0) S->n_fan += 1;
             const int4 cl = d.vcn[v];
-            d.A[cl.x] = fan_area(d, X, Y, Z, cl.x, S->R);
-            d.A[cl.y] = fan_area(d, X, Y, Z, cl.y, S->R);
-            d.A[cl.z] = fan_area(d, X, Y, Z, cl.z, S->R);
+#pragma unroll 1
+            for (int k = 0; k < 3; ++k) {
+                const int c = k == 0 ? cl.x : (k == 1 ? cl.y : cl.z);
+                d.A[c] = fan_area(d, X, Y, Z, c, S->R);
+            }
         }
         nan = vertex_force<FT>(d, S, X, Y, Z, v, f);
         if (MODE == VM_FORCE || store_forces) {
@@ -694,21 +696,20 @@ __device__ __attribute__((always_inline)) inline void vertex_pass(const Dev& d, 
     }
 }
 
-template <int MODE, class FT>
-__global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, int nin, int nout, int store_forces,
-                                                     int fan_check) {
+template <int MODE, class FT, bool FAN = false>
+__global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, int nin, int nout, int store_forces) {
     __shared__ double red[SVM_TILE + 1];
-    vertex_pass<MODE, FT>(d, pin, pout, nin, nout, store_forces, fan_check, red);
+    vertex_pass<MODE, FT, FAN>(d, pin, pout, nin, nout, store_forces, red);
 }
 
 // The same pass compiled for three wavefronts per SIMD (at most 168 VGPRs,
 // against the 212 the pass takes unconstrained: the compiler spills the
 // rest), an autotuner form of the Brownian and active steps.
-template <int MODE>
+template <int MODE, bool FAN = false>
 __global__ __launch_bounds__(SVM_TILE) __attribute__((amdgpu_waves_per_eu(3))) void k_vertex_w3(
-    Dev d, int pin, int pout, int nin, int nout, int fan_check) {
+    Dev d, int pin, int pout, int nin, int nout) {
     __shared__ double red[SVM_TILE + 1];
-    vertex_pass<MODE, double>(d, pin, pout, nin, nout, 0, fan_check, red);
+    vertex_pass<MODE, double, FAN>(d, pin, pout, nin, nout, 0, red);
 }
 
 // Corner form of k_vertex for the Brownian and active updaters in the parity
