@@ -278,14 +278,46 @@ def load(path=WEIGHTS):
     return net, scale
 
 
+def iteration_profile(n_sys=128, cap=10000, chunk=100, lanes=3):
+    """Where plain FIRE spends its iterations: the first `n_sys` spheres of the
+    1,024-sphere batch minimised in chunks of `chunk` iterations, with every
+    sphere's rms force read after each chunk.  Returns (thresholds, rows): for
+    each sphere that converged, the first chunk end with rms below each
+    threshold."""
+    params = batch_params(1024)[:n_sys]
+    ctx = fire_batch(params, cutoff=1e-10, max_iterations=cap)
+    ctx.set_lanes(lanes)
+    hist = []
+    for done in range(chunk, cap + 1, chunk):
+        for i in range(n_sys):
+            ctx.fire_set_max_iterations(i, done)
+        ctx.fire_minimize(chunk)
+        _, conv, rms = fire_states(ctx, n_sys)
+        hist.append(rms.copy())
+        if conv.all():
+            break
+    H = np.array(hist)
+    _, conv, _ = fire_states(ctx, n_sys)
+    thr = [1e-1, 1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-8, 1e-10]
+    rows = [[chunk * (int(np.argmax(H[:, s] < t)) + 1) for t in thr] for s in np.where(conv)[0]]
+    return thr, np.array(rows, dtype=np.int64).reshape(-1, len(thr))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["train", "train-corrector"])
+    ap.add_argument("cmd", choices=["train", "train-corrector", "profile"])
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--spheres", type=int, default=48)
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--validate", type=int, default=32, help="held-out spheres used to pick the step scale")
     args = ap.parse_args()
+    if args.cmd == "profile":
+        thr, rows = iteration_profile()
+        print(f"plain FIRE, 128 spheres of the 1,024-sphere batch, {len(rows)} converged to rms < 1e-10")
+        for k, t in enumerate(thr):
+            print(f"rms < {t:g}: median iteration {np.median(rows[:, k]):.0f}, "
+                  f"median fraction of the sphere's iterations {np.median(rows[:, k] / rows[:, -1]):.2f}")
+        return
     if args.cmd == "train-corrector":
         t0 = time.perf_counter()
         X, T = snapshot_training_set(args.spheres)
