@@ -1000,13 +1000,23 @@ __global__ __launch_bounds__(SVM_TILE) void k_t1_detect(Dev d, int pcur, int ski
     int skips = 0;
     if (local < S->nv) {
         double p[3] = {X[v], Y[v], Z[v]};
-        double n1 = pnorm(p);
         int4 nb = d.vn[v];
         int w3[3] = {nb.x, nb.y, nb.z};
+        // every neighbour's position in one round of loads (rows of this
+        // sphere, in bounds whichever edges are tested), before any test
+        double q3[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            q3[k][0] = X[w3[k]];
+            q3[k][1] = Y[w3[k]];
+            q3[k][2] = Z[w3[k]];
+        }
+        double n1 = pnorm(p);
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
             int w = w3[k];
             if (v < w) {
-                double q[3] = {X[w], Y[w], Z[w]};
+                const double* q = q3[k];
                 if (short_edge(p, q, n1, pnorm(q), S->R, S->thr, S->thr_cos)) {
                     flag = true;
                     if (!flippable) {
