@@ -578,6 +578,8 @@ def main():
     ap.add_argument("--all-configs", action="store_true",
                     help="only the per-configuration GPU figures and CPU baselines (configurations 1, 2, 4, 5 "
                          "and the libm oracle build): one JSON line, then exit")
+    ap.add_argument("--lanes", type=int, default=0,
+                    help="run the timed steps on this many lanes, whatever the schedule says (0: the schedule's)")
     ap.add_argument("--phase-probe", action="store_true",
                     help="time a kernel boundary against a grid barrier in a persistent launch, then exit")
     ap.add_argument("--copy-probe", action="store_true",
@@ -623,6 +625,9 @@ def main():
         tune_steps = int(sum(r[2] + 3 * max(1, 200 // (3 * r[2])) * r[2] for r in tune_table))
     if not args.autotune:
         ctx.set_tuning(graph_steps=max(2, args.graph_steps - args.graph_steps % 2))
+    if args.lanes:
+        ctx.set_lanes(args.lanes)
+        schedule_source += f", lanes set to {args.lanes}"
     # graph length: the schedule's, cut to an even divisor of the timed steps
     # so that exactly K steps are timed (an odd K times one eager step more),
     # and to at most W so that the warm-up replays the timed graph
