@@ -316,14 +316,20 @@ def voronoi_model(svm, pts, vsteps=20):
             "note": "FP64 forces on the generators, Brownian move, GPU retriangulation every step"}
 
 
-def large_batch(svm, steps=400, schedules=None):
+def large_batch(svm, steps=2000, schedules=None, fire_iterations=200):
     """The large batched workload: 32 spheres, N log-uniform in [50,000,
     500,000] (one Philox generator, seed 13377), p0 uniform in [3.7, 3.85],
     FP32 forces on FP64 positions with FP64 predicates.  GPU Delaunay set-up,
     the reference driver's stabilisation (100 Brownian steps at (A0, P0) =
-    (1, 1)), the learned warm start and 200 FIRE iterations, then active
-    dynamics (v0 0.05) replayed from HIP graphs over autotuned lanes; the
-    rate is taken over `steps` active steps after 40 warm-up steps."""
+    (1, 1)), the learned warm start, `fire_iterations` FIRE iterations, then
+    `steps` active steps (v0 0.05) replayed from HIP graphs over the stored
+    or autotuned lanes, timed after 40 warm-up steps.
+    FIRE to rms force < 1e-10 is not reachable on these spheres: with the
+    reference driver's FIRE parameters it diverges between 500 and 1,000
+    iterations in every force precision, and on a 50,000-cell sphere the FP64
+    parity run equals the oracle's FIRE bit for bit through the divergence
+    (tests/test_gpu_large.py), so the FIRE phase stops where it is still
+    descending."""
     from curvedspacevertexmodel_amd import warmstart as ws
     params = ws.batch_params(32, seed=13377, nmin=50000, nmax=500000, p0_range=(3.7, 3.85))
     total = int(sum(n for n, _ in params))
@@ -335,6 +341,7 @@ def large_batch(svm, steps=400, schedules=None):
         ctx.build_from_cell_points(i, pts[svm.spatial_order(pts)])
         ctx.set_directors(i, svm.generate_directors(ctx.positions(i), 1137 + i))
     t_topo = time.perf_counter() - t0
+    print(f"large batch: {len(params)} spheres, {total} cells, topology {t_topo:.1f} s", flush=True)
     ctx.set_force_precision(1)
     for i in range(len(params)):
         ctx.set_preferred_parameters(i, 1.0, 1.0)
@@ -342,17 +349,25 @@ def large_batch(svm, steps=400, schedules=None):
     ctx.run_graph(0, 20, 5)
     for i, (n, p0) in enumerate(params):
         ctx.set_preferred_parameters(i, 1.0, p0)
-    net, scale = ws.load()
-    ctx.warm_start(net, scale)
-    for i in range(len(params)):
-        ctx.fire_set_parameters(i, max_iterations=200, force_cutoff=1e-10, stop_rms=1, **ws.FIRE)
-    ctx.fire_minimize(100)
-    for i in range(len(params)):
-        ctx.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
     schedules = schedules or SCHEDULES
     if not ctx.load_tuning(schedules, updater=1):
+        for i in range(len(params)):
+            ctx.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
         ctx.autotune(updater=1, steps_per_trial=40)
         ctx.save_tuning(schedules, updater=1)
+    net, scale = ws.load()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    reverted = ctx.warm_start(net, scale)
+    for i in range(len(params)):
+        ctx.fire_set_parameters(i, max_iterations=fire_iterations, force_cutoff=1e-10, stop_rms=1, **ws.FIRE)
+    ctx.fire_minimize(100)
+    ctx.synchronize()
+    t_fire = time.perf_counter() - t0
+    its, conv, rms = ws.fire_states(ctx, len(params))
+    print(f"warm start + {fire_iterations} FIRE iterations: {t_fire:.1f} s, rms median {np.median(rms):.2g}", flush=True)
+    for i in range(len(params)):
+        ctx.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
     G = ctx.tuning()["graph_steps"]
     ctx.run_graph(1, G, max(1, 40 // G))
     ctx.synchronize()
@@ -362,10 +377,18 @@ def large_batch(svm, steps=400, schedules=None):
     ctx.synchronize()
     dt = time.perf_counter() - t0
     err = ctx.force_error()
+    ap = [ctx.area_perimeter(i)[:, 0] for i in range(len(params))]
+    tangled = [i for i, a in enumerate(ap) if not (np.isfinite(a).all() and (a > 0).all() and a.max() < 100)]
     return {"workload": "batched-large-32", "spheres": len(params), "cells": total, "N_range": [50000, 500000],
             "forces": "fp32", "topology_build_s": t_topo, "steps": reps * G, "ms_per_step": 1e3 * dt / (reps * G),
             "cell_steps_per_s": total * reps * G / dt, "schedule": ctx.tuning(),
-            "fp32_force_rms_rel_error_max": float(err[:, 1].max())}
+            "fire": {"warm_start_reverted": int(reverted), "iterations": fire_iterations, "wall_s": t_fire,
+                     "rms_median": float(np.median(rms)), "rms_max": float(np.max(rms)),
+                     "to_rms_1e-10": "not reachable: FIRE diverges between 500 and 1,000 iterations on these "
+                                     "spheres in every precision, as the oracle's FIRE does bit for bit"},
+            "tangled_spheres_after_active_steps": tangled,
+            "fp32_force_rms_rel_error_max": float(err[:, 1].max()),
+            "fp32_vs_fp64_vector_form_rms_rel_error_max": float(err[:, 5].max())}
 
 
 def fire_comparison(svm, n_sys=1024, cap=10000, lanes=1):

@@ -133,10 +133,10 @@ def test_large_batch_fp32_warm_start_fire_active():
     for i, (n, p0) in enumerate(params):
         ctx.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
     t0 = time.perf_counter()
-    ctx.run_graph(1, 20, 10)
+    ctx.run_graph(1, 20, 100)
     ctx.synchronize()
     dt = time.perf_counter() - t0
-    print(f"200 active steps (FP32 forces): {dt:.2f} s, {total * 200 / dt:.3g} cell-steps/s", flush=True)
+    print(f"2000 active steps (FP32 forces): {dt:.2f} s, {total * 2000 / dt:.3g} cell-steps/s", flush=True)
     bad = check_areas(ctx, params, "after active steps")
     # small folds still unwinding leave the tiling off by ~1e-6 with every area
     # in [0.7, 1.3]; a tangle has non-finite, non-positive or huge areas
@@ -197,3 +197,31 @@ def test_t1_commit_overflow_matches_sequential():
     np.testing.assert_array_equal(runs[0][0], runs[1][0])
     for a, b in zip(runs[0][1], runs[1][1]):
         np.testing.assert_array_equal(a, b)
+
+
+def test_fire_divergence_on_a_large_sphere_is_the_reference():
+    """FIRE with the reference driver's parameters does not reach rms 1e-10 on
+    the spheres of the large configuration: from the stabilised state of a
+    50,000-cell sphere it diverges before 1,000 iterations.  The GPU's FP64
+    parity run and the oracle's FIRE (the reference's CPU minimiser) reach the
+    same rms force and the same positions, bit for bit, so the divergence is
+    the minimiser's on this model, not the library's."""
+    from curvedspacevertexmodel_amd import warmstart as ws
+    from test_gpu_configs import oracle_from_gpu
+    n, p0 = 50000, 3.75
+    ctx = svm.Context([dict(n_cells=n, a0=1.0, p0=p0, kr=1.0, t1_threshold=0.1, seed=1137)])
+    pts = svm.generate_cell_points(n, 1137)
+    ctx.build_from_cell_points(0, pts[svm.spatial_order(pts)])
+    ctx.set_preferred_parameters(0, 1.0, 1.0)
+    ctx.set_brownian(0, dt=0.01, mu=1.0, T=0.0)
+    ctx.brownian_steps(100)
+    ctx.set_preferred_parameters(0, 1.0, p0)
+    om = oracle_from_gpu(ctx, 0, p0)
+    ctx.fire_set_parameters(0, max_iterations=1000, force_cutoff=1e-10, stop_rms=1, **ws.FIRE)
+    ctx.fire_minimize(100)
+    o = om.fire(**ws.FIRE, max_iterations=1000, force_cutoff=1e-10, stop_rms=1)
+    s = ctx.fire_state(0)
+    print(f"50,000 cells, 1,000 FIRE iterations: rms {s['force_max']:.4g} (oracle {o['force_max']:.4g})", flush=True)
+    assert s["iterations"] == o["iterations"] == 1000
+    assert s["force_max"] == o["force_max"] and s["force_max"] > 1e3
+    np.testing.assert_array_equal(ctx.positions(0), om.positions())
