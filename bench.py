@@ -518,6 +518,34 @@ def cpu_baseline_configs(svm, budget_s=10.0, threads=None):
                                     "sample": f"sphere {i5} of the batch (N {n5}), {steps} active steps, "
                                               f"FP64 forces (the reference's CPU arithmetic)"}
     del ctx, om
+    # configuration 5, all cores: the `threads` smallest spheres of the batch, one per thread
+    pick = [int(i) for i in np.argsort([n_ for n_, _ in lp])[:threads]]
+    ctx = svm.Context([dict(n_cells=lp[i][0], a0=1.0, p0=1.0, kr=1.0, t1_threshold=0.1, seed=1137 + i)
+                       for i in pick])
+    for k, i in enumerate(pick):
+        pts = svm.generate_cell_points(lp[i][0], 1137 + i)
+        ctx.build_from_cell_points(k, pts[svm.spatial_order(pts)])
+        ctx.set_directors(k, svm.generate_directors(ctx.positions(k), 1137 + i))
+        ctx.set_brownian(k, dt=0.01, mu=1.0, T=0.0)
+    ctx.run_graph(0, 20, 5)
+    models = [svmtest.OracleModel(_oracle_state(ctx, k), p0=lp[i][1]) for k, i in enumerate(pick)]
+    del ctx
+
+    def step_sphere(k):
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget_s:
+            models[k].active(2, dt=0.01, mu=1.0, v0=0.05, tau=1.0, seed=1137 + pick[k], step0=100 + done)
+            done += 2
+        return lp[pick[k]][0] * done
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        work = sum(ex.map(step_sphere, range(len(pick))))
+    out["config5_all_core"] = {"cell_steps_per_s": work / (time.perf_counter() - t0), "threads": threads,
+                               "sample": f"the {len(pick)} smallest spheres of the batch "
+                                         f"({min(lp[i][0] for i in pick)}-{max(lp[i][0] for i in pick)} cells), "
+                                         f"one per thread, active steps for {budget_s:.0f} s, FP64 forces"}
+    del models
     # the oracle built against glibc's acos/sin/cos/log beside the default build (configuration 3 sample)
     bp = batch_params(256)
     ctx, _ = build_batch(svm, bp[:8])
