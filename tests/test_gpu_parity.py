@@ -252,6 +252,34 @@ def test_batch_equivalence_and_graph(state512):
         np.testing.assert_array_equal(batch.directors(i), alone.directors(0))
 
 
+def test_graph_replays_between_eager_work(state512):
+    """Replays of one captured graph, with eager steps, state reads and
+    synchronisations between them, each equal to the same steps taken
+    eagerly (a replay must not depend on what ran before it), on 3 lanes."""
+    s2 = svmtest.seeded_state(900, 1141)
+    topos = [state512, s2, state512]
+
+    def fresh():
+        c = make_ctx(topos, seeds=[1137, 1141, 1139])
+        for i in range(3):
+            c.set_active(i, dt=0.01, mu=1.0, v0=0.1 * (i + 1), tau=1.0)
+        return c
+
+    g, e = fresh(), fresh()
+    g.set_lanes(3)
+    for eager_between in (0, 3, 1, 2, 0):
+        g.run_graph(1, 8, 1)
+        e.active_steps(8)
+        for i in range(3):
+            np.testing.assert_array_equal(g.positions(i), e.positions(i))
+            np.testing.assert_array_equal(g.directors(i), e.directors(i))
+        if eager_between:
+            g.active_steps(eager_between)
+            e.active_steps(eager_between)
+        g.synchronize()
+    np.testing.assert_array_equal(g.counters(), e.counters())
+
+
 def test_topology_from_triangles_matches_oracle(state512):
     """The library's triangle -> topology builder follows the same conventions as the oracle's."""
     ctx = svm.Context([dict(n_cells=512, seed=1137)])
