@@ -283,6 +283,12 @@ int launch_vertex_as(svm_context* ctx, const Target& w, int pout, int store_forc
     else if (ctx->force_fp32 == 2)
         hipLaunchKernelGGL((svm::k_vertex<MODE, svm::VecF64, FAN>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur,
                            pout, ctx->ncur, nout, store_forces);
+    else if ((MODE == svm::VM_BROWNIAN || MODE == svm::VM_ACTIVE) && ctx->vertex_corner == 4 && !store_forces) {
+        if (FAN) return fail(ctx, SVM_ERR_ARG, "the staged vertex form has no fan check");
+        constexpr int M = MODE == svm::VM_ACTIVE ? svm::VM_ACTIVE : svm::VM_BROWNIAN;
+        hipLaunchKernelGGL((svm::k_vertex_staged<M>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
+                           ctx->ncur, nout);
+    }
     else if ((MODE == svm::VM_BROWNIAN || MODE == svm::VM_ACTIVE) && ctx->vertex_corner == 3 && !store_forces) {
         constexpr int M = MODE == svm::VM_ACTIVE ? svm::VM_ACTIVE : svm::VM_BROWNIAN;
         hipLaunchKernelGGL((svm::k_vertex_w3<M, FAN>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, pout,
@@ -2368,7 +2374,7 @@ int svm_set_tuning(svm_context* ctx, int32_t cluster_blocks, int32_t graph_steps
 }
 
 int svm_set_vertex_form(svm_context* ctx, int32_t form) {
-    if (!ctx || form < 0 || form > 3) return fail(ctx, SVM_ERR_ARG, "vertex form must be 0, 1, 2 or 3");
+    if (!ctx || form < 0 || form > 4) return fail(ctx, SVM_ERR_ARG, "vertex form must be in [0, 4]");
     ctx->vertex_corner = form;
     drop_graphs(ctx);
     return SVM_OK;
@@ -2557,7 +2563,7 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
     if (!r) r = svm_set_geometry_form(ctx, best_g);
     // the corner forms and the three-wavefront form of the vertex pass (the
     // FP64 parity arithmetic only)
-    for (int form = 1; form <= 3 && !r && ctx->force_fp32 == 0; ++form) {
+    for (int form = 1; form <= 4 && !r && ctx->force_fp32 == 0; ++form) {
         r = svm_set_tuning(ctx, best_cb, 10, 1);
         if (!r) r = svm_set_vertex_form(ctx, form);
         double ms = 0.0;

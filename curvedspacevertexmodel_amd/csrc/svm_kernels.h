@@ -426,11 +426,28 @@ __device__ inline int vertex_force(const Dev& d, const SysParams* S, const doubl
 // the one whose previous neighbour is j and next j+1 (mod 3): its operands
 // are fixed registers, not per-cell selections of every neighbour term.  The
 // contributions are then added in the reference's cell-slot order.
-template <>
-__device__ inline int vertex_force<double>(const Dev& d, const SysParams* S, const double* X, const double* Y,
-                                           const double* Z, int v, double* f) {
+// Where the parity force reads its inputs: positions of the vertex and its
+// neighbours, and each cell's (A - A0, P - P0, centre, centre's spherical
+// terms).  GlobalSrc reads the arrays; the staged vertex pass reads most of
+// them from its workgroup's LDS windows (StagedSrc).  Either way the same
+// values enter the same expressions.
+struct GlobalSrc {
+    const Dev& d;
+    const double *X, *Y, *Z;
+    __device__ PQ pq(int w) const { return pq_at<double>(X, Y, Z, w); }
+    __device__ void cell(int c, double& aD, double& pD, PQ& qc) const {
+        const size_t Cs = (size_t)d.Cs;
+        aD = d.A[c] - d.a0[c];
+        pD = d.P[c] - d.p0[c];
+        qc.x = d.cx[c]; qc.y = d.cy[c]; qc.z = d.cz[c];
+        qc.cosT = d.ctrig[c]; qc.sinT = d.ctrig[Cs + c]; qc.cosP = d.ctrig[2 * Cs + c]; qc.sinP = d.ctrig[3 * Cs + c];
+    }
+};
+
+template <class Src>
+__device__ inline int vertex_force_f64(const Src& src, const Dev& d, const SysParams* S, int v, double* f) {
     const double R = S->R, Kr = S->kr;
-    const PQ qs = pq_at<double>(X, Y, Z, v);
+    const PQ qs = src.pq(v);
     const int4 nb = d.vn[v];
     const int4 cl = d.vcn[v];
     const int ln = d.vln[v];
@@ -440,9 +457,9 @@ __device__ inline int vertex_force<double>(const Dev& d, const SysParams* S, con
     // the renamed previous-neighbour slot of cell slots 0, 1, 2
     const int L0 = rn(ls0), L1 = rn(ls1), L2 = rn(ls2);
     const int r1 = o ? nb.z : nb.y, r2 = o ? nb.y : nb.z;
-    const PQ q0 = pq_at<double>(X, Y, Z, nb.x);
-    const PQ q1 = pq_at<double>(X, Y, Z, r1);
-    const PQ q2 = pq_at<double>(X, Y, Z, r2);
+    const PQ q0 = src.pq(nb.x);
+    const PQ q1 = src.pq(r1);
+    const PQ q2 = src.pq(r2);
     double tH[3], pH[3];
     basis(qs, tH, pH);
     const PairT<double> w0 = pair_terms(qs, q0), w1 = pair_terms(qs, q1), w2 = pair_terms(qs, q2);
@@ -450,16 +467,13 @@ __device__ inline int vertex_force<double>(const Dev& d, const SysParams* S, con
     grad_geodesic_p(qs, q0, w0, tH, pH, g0);
     grad_geodesic_p(qs, q1, w1, tH, pH, g1);
     grad_geodesic_p(qs, q2, w2, tH, pH, g2);
-    const size_t Cs = (size_t)d.Cs;
     // pair_force's sequence (sphericalVertexModel.cpp:386-413) for the cell
     // whose previous and next ring neighbours are (ql, qn)
     auto cell_force = [&](int c, const PQ& ql, const PQ& qn, const PairT<double>& wl, const PairT<double>& wn,
                           const double* gl, const double* gn, double* fSet) {
-        const double areaDifference = d.A[c] - d.a0[c];
-        const double perimeterDifference = d.P[c] - d.p0[c];
+        double areaDifference, perimeterDifference;
         PQ qc;
-        qc.x = d.cx[c]; qc.y = d.cy[c]; qc.z = d.cz[c];
-        qc.cosT = d.ctrig[c]; qc.sinT = d.ctrig[Cs + c]; qc.cosP = d.ctrig[2 * Cs + c]; qc.sinP = d.ctrig[3 * Cs + c];
+        src.cell(c, areaDifference, perimeterDifference, qc);
         const PairT<double> wc = pair_terms(qs, qc);
         double t[3];
         fSet[0] = 0.0; fSet[1] = 0.0; fSet[2] = 0.0;
@@ -496,6 +510,99 @@ __device__ inline int vertex_force<double>(const Dev& d, const SysParams* S, con
     }
     f[0] = f0; f[1] = f1; f[2] = f2;
     return nan;
+}
+
+template <>
+__device__ inline int vertex_force<double>(const Dev& d, const SysParams* S, const double* X, const double* Y,
+                                           const double* Z, int v, double* f) {
+    return vertex_force_f64(GlobalSrc{d, X, Y, Z}, d, S, v, f);
+}
+
+// The staged vertex pass's windows: SVM_STAGE_C cell rows from the least cell
+// of the tile's vertices, and the tile's vertex rows with SVM_STAGE_M rows on
+// either side.  In the canonical numbering (triangles sorted by their least
+// cell, cells in spatial order) about 96% of a tile's cell references and 95%
+// of its neighbour references fall inside.
+#define SVM_STAGE_C 512
+#define SVM_STAGE_M 64
+#define SVM_STAGE_P (SVM_TILE + 2 * SVM_STAGE_M)
+#define SVM_STAGE_DOUBLES (11 * SVM_STAGE_C + 3 * SVM_STAGE_P)
+
+// Inputs from the LDS windows where they hold them, else from the arrays:
+// each value's address is one or the other (a flat load either way, no
+// branch), and the values are the same.
+struct StagedSrc {
+    const Dev& d;
+    const double *X, *Y, *Z;
+    const double* st;   // LDS: 11 planes of SVM_STAGE_C cell rows, then 3 planes of SVM_STAGE_P vertex rows
+    int c_lo, p_lo;
+    __device__ PQ pq(int w) const {
+        const unsigned k = (unsigned)(w - p_lo);
+        const bool in = k < (unsigned)SVM_STAGE_P;
+        const double* ps = st + 11 * SVM_STAGE_C;
+        const double* qx = in ? ps + k : X + w;
+        const double* qy = in ? ps + SVM_STAGE_P + k : Y + w;
+        const double* qz = in ? ps + 2 * SVM_STAGE_P + k : Z + w;
+        return make_pq<double>(*qx, *qy, *qz);
+    }
+    __device__ void cell(int c, double& aD, double& pD, PQ& qc) const {
+        const unsigned k = (unsigned)(c - c_lo);
+        const bool in = k < (unsigned)SVM_STAGE_C;
+        const size_t Cs = (size_t)d.Cs;
+        auto at = [&](int plane, const double* g) { return *(in ? st + plane * SVM_STAGE_C + k : g + c); };
+        aD = at(0, d.A) - at(1, d.a0);
+        pD = at(2, d.P) - at(3, d.p0);
+        qc.x = at(4, d.cx); qc.y = at(5, d.cy); qc.z = at(6, d.cz);
+        qc.cosT = at(7, d.ctrig); qc.sinT = at(8, d.ctrig + Cs); qc.cosP = at(9, d.ctrig + 2 * Cs);
+        qc.sinP = at(10, d.ctrig + 3 * Cs);
+    }
+};
+
+// Fills the staged windows of vertex tile `tile` (whole block; every thread
+// reaches the barriers).  Returns the window origins in c_lo, p_lo.
+__device__ inline void stage_windows(const Dev& d, const SysParams* S, int tile, bool valid, int v, const double* X,
+                                     const double* Y, const double* Z, double* st, int* s_min, int& c_lo,
+                                     int& p_lo) {
+    int cmin = 0x7fffffff;
+    if (valid) {
+        const int4 cl = d.vcn[v];
+        cmin = min(cl.x, min(cl.y, cl.z));
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) cmin = min(cmin, __shfl_xor(cmin, o, 64));
+    if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = cmin;
+    __syncthreads();
+    c_lo = min(min(s_min[0], s_min[1]), min(s_min[2], s_min[3]));
+    p_lo = max(S->vo, tile * SVM_TILE - SVM_STAGE_M);
+    const int c_end = S->co + S->nc, p_end = S->vo + S->nv;
+    const size_t Cs = (size_t)d.Cs;
+    const int c_rows = c_lo < c_end ? min(SVM_STAGE_C, c_end - c_lo) : 0;   // c_lo is INT_MAX if no vertex is valid
+    for (int r = threadIdx.x; r < c_rows; r += SVM_TILE) {
+        const int c = c_lo + r;
+        {
+            st[r] = d.A[c];
+            st[SVM_STAGE_C + r] = d.a0[c];
+            st[2 * SVM_STAGE_C + r] = d.P[c];
+            st[3 * SVM_STAGE_C + r] = d.p0[c];
+            st[4 * SVM_STAGE_C + r] = d.cx[c];
+            st[5 * SVM_STAGE_C + r] = d.cy[c];
+            st[6 * SVM_STAGE_C + r] = d.cz[c];
+            st[7 * SVM_STAGE_C + r] = d.ctrig[c];
+            st[8 * SVM_STAGE_C + r] = d.ctrig[Cs + c];
+            st[9 * SVM_STAGE_C + r] = d.ctrig[2 * Cs + c];
+            st[10 * SVM_STAGE_C + r] = d.ctrig[3 * Cs + c];
+        }
+    }
+    double* ps = st + 11 * SVM_STAGE_C;
+    for (int r = threadIdx.x; r < SVM_STAGE_P; r += SVM_TILE) {
+        const int p = p_lo + r;
+        if (p < p_end) {
+            ps[r] = X[p];
+            ps[SVM_STAGE_P + r] = Y[p];
+            ps[2 * SVM_STAGE_P + r] = Z[p];
+        }
+    }
+    __syncthreads();
 }
 
 // Force of the same energy from the vector-form gradients (grad_arc_v /
@@ -576,9 +683,10 @@ __device__ inline int vertex_force<VecF64>(const Dev& d, const SysParams* S, con
 // touches a cell stores the same value there before it loads it, so each
 // load returns that value; every cell has vertices, so every cell's area is
 // replaced, as k_fan_fix replaces it.
-template <int MODE, class FT, bool FAN>
+template <int MODE, class FT, bool FAN, bool STAGED = false>
 __device__ __attribute__((always_inline)) inline void vertex_pass(const Dev& d, int pin, int pout, int nin, int nout,
-                                                                 int store_forces, double* red) {
+                                                                 int store_forces, double* red, double* stage = nullptr,
+                                                                 int* s_min = nullptr) {
     const int tile = lane_vtile(d);
     if (tile < 0) return;
     const int s = d.vtile_sys[tile];
@@ -595,6 +703,8 @@ __device__ __attribute__((always_inline)) inline void vertex_pass(const Dev& d, 
         return;
     }
     const bool fan = FAN && fan_needed(S, fold_wave(d.cpart + S->ctile0, 1, S->nctiles, fold_width(S->nctiles)));
+    int c_lo = 0, p_lo = 0;
+    if (STAGED) stage_windows(d, S, tile, valid, v, X, Y, Z, stage, s_min, c_lo, p_lo);
     double f[3] = {0.0, 0.0, 0.0};
     double tp = 0.0, tf = 0.0, tv = 0.0;
     int nan = 0;
@@ -608,7 +718,10 @@ __device__ __attribute__((always_inline)) inline void vertex_pass(const Dev& d, 
                 d.A[c] = fan_area(d, X, Y, Z, c, S->R);
             }
         }
-        nan = vertex_force<FT>(d, S, X, Y, Z, v, f);
+        if constexpr (STAGED)
+            nan = vertex_force_f64(StagedSrc{d, X, Y, Z, stage, c_lo, p_lo}, d, S, v, f);
+        else
+            nan = vertex_force<FT>(d, S, X, Y, Z, v, f);
         if (MODE == VM_FORCE || store_forces) {
             d.fx[v] = f[0]; d.fy[v] = f[1]; d.fz[v] = f[2];
         }
@@ -700,6 +813,18 @@ template <int MODE, class FT, bool FAN = false>
 __global__ __launch_bounds__(SVM_TILE) void k_vertex(Dev d, int pin, int pout, int nin, int nout, int store_forces) {
     __shared__ double red[SVM_TILE + 1];
     vertex_pass<MODE, FT, FAN>(d, pin, pout, nin, nout, store_forces, red);
+}
+
+// The same pass with its inputs staged in LDS (the vertex form "staged";
+// Brownian and active steps in the FP64 parity arithmetic): the tile's cell
+// window and vertex window loaded once, with coalesced loads, by the whole
+// workgroup; the force then reads them from LDS (StagedSrc).
+template <int MODE>
+__global__ __launch_bounds__(SVM_TILE) void k_vertex_staged(Dev d, int pin, int pout, int nin, int nout) {
+    __shared__ double red[SVM_TILE + 1];
+    __shared__ double stage[SVM_STAGE_DOUBLES];
+    __shared__ int s_min[SVM_TILE / 64];
+    vertex_pass<MODE, double, false, true>(d, pin, pout, nin, nout, 0, red, stage, s_min);
 }
 
 // The same pass compiled for three wavefronts per SIMD (at most 168 VGPRs,

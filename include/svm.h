@@ -484,9 +484,13 @@ int svm_get_tuning(svm_context* ctx, int32_t* cluster_blocks, int32_t* graph_ste
  * wavefront k evaluating the vertex's neighbour slot k and then its cell slot
  * k, the contributions added in slot order (k_vertex_corner), compiled for
  * four (1) or five (2) wavefronts per SIMD; 3 = form 0 compiled for three
- * wavefronts per SIMD (168 VGPRs, the rest spilled).  All give identical
- * results, bit for bit; they differ in register use and so in how many
- * wavefronts a SIMD keeps in flight to hide memory latency. */
+ * wavefronts per SIMD (168 VGPRs, the rest spilled); 4 = form 0 with each
+ * workgroup's inputs staged in LDS (a window of 512 cell rows from the least
+ * cell of its 256 vertices and its vertex rows with 64 on either side, loaded
+ * once with coalesced loads; references outside the windows read the
+ * arrays).  All give identical results, bit for bit; they differ in register
+ * use, in how many wavefronts a SIMD keeps in flight to hide memory latency,
+ * and in how the inputs are gathered. */
 int svm_set_vertex_form(svm_context* ctx, int32_t form);
 int svm_get_vertex_form(svm_context* ctx, int32_t* form);
 /* Kernel form of the geometry pass (computeGeometryCPU,

@@ -39,16 +39,16 @@ def test_autotune_steps_are_ordinary_steps(states):
     for i in range(len(states)):
         ctx.set_active(i, dt=0.01, mu=1.0, v0=0.05, tau=1.0)
     tab = ctx.autotune(updater=1, steps_per_trial=30)
-    # lanes 1, 2, 3 (capped at the 3 spheres) x 2 grids, 3 step forms, 2 geometry forms, 3 vertex forms,
+    # lanes 1, 2, 3 (capped at the 3 spheres) x 2 grids, 3 step forms, 2 geometry forms, 4 vertex forms,
     # 6 more kernel schedules, 3 graph lengths
-    assert len(tab) == 23 and (tab[:, 7] > 0).all()
+    assert len(tab) == 24 and (tab[:, 7] > 0).all()
     np.testing.assert_array_equal(tab[:6, 0], [1, 1, 2, 2, 3, 3])
     np.testing.assert_array_equal(tab[:11, 4], 0)
     np.testing.assert_array_equal(tab[:6, 5], 0)
     np.testing.assert_array_equal(tab[:9, 6], 0)
     np.testing.assert_array_equal(tab[6:9, 5], [1, 2, 3])
     np.testing.assert_array_equal(tab[9:11, 6], [1, 2])
-    np.testing.assert_array_equal(tab[11:14, 4], [1, 2, 3])
+    np.testing.assert_array_equal(tab[11:15, 4], [1, 2, 3, 4])
     assert (tab[6:, 0] == tab[np.argmin(tab[:6, 7]), 0]).all()
     t = ctx.tuning()
     best = tab[np.argmin(tab[:, 7])]

@@ -1,9 +1,10 @@
 """Kernel forms of the vertex and geometry passes give identical trajectories.
 
 svm_set_vertex_form selects how the Brownian and active steps evaluate the
-chain-rule forces: a thread per vertex (form 0, or form 3 compiled for three
-wavefronts per SIMD) or the corner form, one wavefront per cell slot (forms 1
-and 2, compiled for four and five wavefronts per SIMD).
+chain-rule forces: a thread per vertex (form 0, form 3 compiled for three
+wavefronts per SIMD, form 4 with the tile's cell and vertex windows staged in
+LDS) or the corner form, one wavefront per cell slot (forms 1 and 2, compiled
+for four and five wavefronts per SIMD).
 svm_set_geometry_form gives the geometry pass's threads 1, 2 or 4 cells each.
 Every form must give the oracle's trajectory bit for bit, over steps with T1
 flips, NaN-free and with noise.
@@ -36,7 +37,8 @@ def test_forms_agree_on_a_batch():
     params = [(int(n), float(p)) for n, p in zip(np.exp(rng.uniform(np.log(200), np.log(6000), 24)).round(),
                                                   rng.uniform(3.5, 4.0, 24))]
     out = []
-    for form, lanes, gform in ((0, 1, 0), (1, 1, 0), (2, 3, 0), (1, 3, 0), (3, 1, 0), (0, 1, 1), (3, 3, 2), (0, 2, 2)):
+    for form, lanes, gform in ((0, 1, 0), (1, 1, 0), (2, 3, 0), (1, 3, 0), (3, 1, 0), (0, 1, 1), (3, 3, 2), (0, 2, 2),
+                               (4, 1, 0), (4, 3, 1)):
         ctx = batch(params, form, lanes, gform)
         for i in range(len(params)):
             ctx.set_preferred_parameters(i, 1.0, 1.0)
@@ -57,7 +59,7 @@ def test_forms_agree_on_a_batch():
         np.testing.assert_array_equal(cnt, out[0][2])
 
 
-@pytest.mark.parametrize("form", [1, 2, 3])
+@pytest.mark.parametrize("form", [1, 2, 3, 4])
 def test_vertex_forms_match_the_oracle(form):
     st = svmtest.seeded_state(512, 1137)
     om = svmtest.OracleModel(st)
