@@ -545,7 +545,7 @@ int svm_get_lanes(svm_context* ctx, int32_t* n_lanes, int32_t* first_sphere);
  * schedule -- first lanes in {1 .. min(SVM_MAX_LANES, spheres)} x
  * cluster_blocks in {256, 1024} with vertex, step and geometry forms 0; at
  * the best lane count the step forms 1, 2 and 3; then the geometry forms 1
- * and 2; then the vertex forms 1, 2 and 3 (FP64 parity forces only); then the
+ * and 2; then the vertex forms 1 to 4 (FP64 parity forces only); then the
  * rest of cluster_blocks in {128, 256, 512, 1024} x xcd_mapping in {0, 1},
  * all at 10 steps per graph; then graph_steps in {4, 20, 40} -- each stage
  * keeping the best of the stages before -- keeps the fastest and reports
