@@ -46,7 +46,7 @@ EXPORTS = [
     "svm_set_velocities_device", "svm_get_velocities_device", "svm_get_forces_device",
     "svm_delaunay_triangles_noalloc", "svm_set_vertex_form", "svm_get_vertex_form",
     "svm_tuning_save", "svm_tuning_load", "svm_set_step_form", "svm_get_step_form", "svm_phase_boundary_probe",
-    "svm_set_geometry_form", "svm_get_geometry_form", "svm_neighbor_list",
+    "svm_set_geometry_form", "svm_get_geometry_form", "svm_neighbor_list", "svm_fast_ops_check",
 ]
 
 NOALLOC_MAX_POINTS = 65536
@@ -308,6 +308,16 @@ def phase_boundary_probe(workgroups, phases=200, bytes_per_phase=64 << 20):
     if rc != 0:
         raise SVMError(rc, "svm_phase_boundary_probe")
     return a.value, b.value, e.value
+
+
+def fast_ops_check(n=1 << 24, seed=1):
+    """Counts of svm_fast_ops_check: (quotient samples in the precondition, accepted, differing from a / b,
+    root samples, accepted, differing from sqrt)."""
+    out = np.zeros(6, np.int64)
+    rc = lib().svm_fast_ops_check(C.c_int64(n), C.c_uint64(seed), out.ctypes.data_as(C.POINTER(C.c_int64)))
+    if rc != 0:
+        raise SVMError(rc, "svm_fast_ops_check")
+    return tuple(int(x) for x in out)
 
 
 class Context:

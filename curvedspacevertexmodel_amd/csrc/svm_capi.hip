@@ -51,7 +51,7 @@ struct svm_context {
     double tuned_ms = 0.0;           // ms per step of the schedule the autotuner kept (0: not tuned)
     int vertex_corner = 0;           // Brownian/active vertex pass: 0 k_vertex; 1, 2 corner form at 4, 5 waves per SIMD
     int step_form = 0;               // svm_set_step_form bits: 1 fan check in the vertex pass, 2 no group kernel
-    int geometry_form = 0;           // k_geometry: 1, 2 or 4 cells per thread (forms 0, 1, 2)
+    int geometry_form = 0;           // k_geometry: 1 (ring prefetched), 2 or 4 cells per thread (forms 0, 1, 2)
     Dev d{};
     int pcur = 0, ncur = 0;          // current position / director buffer
     std::string err;
@@ -252,7 +252,8 @@ int launch_geometry(svm_context* ctx, const Target& w, int skip_frozen, bool wit
         else if (ctx->geometry_form == 1)
             hipLaunchKernelGGL(svm::k_geometry<2>, grid, dim3(SVM_TILE / 2), 0, w.st, w.d, ctx->pcur, skip_frozen);
         else
-            hipLaunchKernelGGL(svm::k_geometry<1>, grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur, skip_frozen);
+            hipLaunchKernelGGL((svm::k_geometry<1, SVM_GEO_PF>), grid, dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur,
+                               skip_frozen);
         int r = last_launch(ctx);
         if (r) return r;
     }
@@ -2775,6 +2776,96 @@ int svm_phase_boundary_probe(int32_t workgroups, int32_t phases, int64_t bytes_p
     if (rows) (void)hipFree(rows);
     if (counters) (void)hipFree(counters);
     if (gave_up) (void)hipFree(gave_up);
+    return rc;
+}
+
+// The short root and quotient sequences of the vertex pass (svm::FastOps)
+// against the operators: sample i draws a divisor b with a random mantissa
+// and an exponent in [-420, 520), a numerator a = (random mantissa, random
+// sign) * 2^(e_b + d) with d in [-430, 430), a root argument x with an
+// exponent in [-800, 1024) (random sign one time in 8), and, one sample in 16,
+// the mantissas all ones or all zeros, zero numerators, zero, infinite, NaN
+// and subnormal root arguments.  Counts per kind: samples in the sequence's
+// precondition (divisor in [2^-384, 2^500)), accepted results (not replaced
+// by a NaN) and accepted results whose bits differ from the operator's.
+__device__ inline double bits_to_d(uint32_t hi, uint32_t lo) {
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__global__ __launch_bounds__(256) void k_fast_ops_check(uint64_t seed, int64_t n, unsigned long long* counts) {
+    unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
+    const svm::FastOps fo{};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const svm::u32x4 r = svm::philox4x32_10((uint32_t)i, (uint32_t)(i >> 32), 0u, 0x46534f31u, (uint32_t)seed,
+                                                (uint32_t)(seed >> 32));
+        const svm::u32x4 t = svm::philox4x32_10((uint32_t)i, (uint32_t)(i >> 32), 1u, 0x46534f31u, (uint32_t)seed,
+                                                (uint32_t)(seed >> 32));
+        const int special = (t.v[3] & 15u) == 0u;
+        const int eb = (int)(t.v[0] % 940u) - 420;
+        const int d = (int)(t.v[1] % 861u) - 430;
+        uint32_t mb_hi = r.v[0] & 0xfffffu, mb_lo = r.v[1];
+        uint32_t ma_hi = r.v[2] & 0xfffffu, ma_lo = r.v[3];
+        if (special) {
+            const uint32_t k = (t.v[3] >> 4) & 3u;
+            if (k == 0) { mb_hi = 0xfffffu; mb_lo = 0xffffffffu; }
+            if (k == 1) { mb_hi = 0u; mb_lo = 0u; }
+            if (k == 2) { ma_hi = 0xfffffu; ma_lo = 0xffffffffu; }
+        }
+        const double b = bits_to_d((uint32_t)(eb + 1023) << 20 | mb_hi, mb_lo);
+        int ea = eb + d;
+        ea = ea < -1022 ? -1022 : (ea > 1023 ? 1023 : ea);
+        double a = bits_to_d((uint32_t)(ea + 1023) << 20 | ma_hi | (r.v[0] & 0x80000000u), ma_lo);
+        if (special && ((t.v[3] >> 4) & 3u) == 3u) a = (t.v[2] & 1u) ? -0.0 : 0.0;
+        if (b >= 0x1p-384 && b < 0x1p500) {
+            ++c[0];
+            const double q = fo.div(a, b, fo.rcp(b));
+            if (!(q != q)) {
+                ++c[1];
+                c[2] += __builtin_bit_cast(uint64_t, q) != __builtin_bit_cast(uint64_t, a / b);
+            }
+        }
+        const int ex = (int)(t.v[2] % 1824u) - 800;
+        double x = bits_to_d((uint32_t)(ex + 1023) << 20 | (r.v[1] & 0xfffffu), r.v[2] ^ r.v[3]);
+        if ((t.v[1] & 7u) == 0u) x = -x;
+        if (special) {
+            const uint32_t k = (t.v[2] >> 8) & 7u;
+            if (k == 0) x = 0.0;
+            if (k == 1) x = __builtin_inf();
+            if (k == 2) x = __builtin_nan("");
+            if (k == 3) x = bits_to_d(r.v[0] & 0xfffffu, r.v[1]);   // subnormal
+        }
+        ++c[3];
+        const double y = fo.sqrt(x);
+        if (!(y != y)) {
+            ++c[4];
+            c[5] += __builtin_bit_cast(uint64_t, y) != __builtin_bit_cast(uint64_t, __builtin_sqrt(x));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        unsigned long long v = c[k];
+        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(counts + k, v);
+    }
+}
+
+int svm_fast_ops_check(int64_t n, uint64_t seed, int64_t* counts) {
+    if (n < 1 || !counts) return SVM_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SVM_ERR_NO_DEVICE;
+    unsigned long long* dc = nullptr;
+    int rc = SVM_OK;
+    if (hipMalloc(&dc, 6 * sizeof(unsigned long long)) != hipSuccess) return SVM_ERR_HIP;
+    unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
+    if (hipMemcpy(dc, h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) rc = SVM_ERR_HIP;
+    if (!rc) {
+        const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_fast_ops_check, dim3((unsigned)blocks), dim3(256), 0, nullptr, seed, n, dc);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+            hipMemcpy(h, dc, sizeof h, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = SVM_ERR_HIP;
+    }
+    (void)hipFree(dc);
+    for (int k = 0; k < 6; ++k) counts[k] = (int64_t)h[k];
     return rc;
 }
 

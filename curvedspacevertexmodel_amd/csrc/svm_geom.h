@@ -17,6 +17,70 @@ namespace svm {
 SVM_HD double vsqrt(double x) { return __builtin_sqrt(x); }
 SVM_HD float vsqrt(float x) { return __builtin_sqrtf(x); }
 
+// How the gradient expressions below take their square roots and quotients.
+// ExactOps: the language's correctly rounded operators, for every argument.
+// A quotient is written div(a, b, rb) with rb = rcp(b) prepared once per
+// divisor; here rcp does nothing and div is a / b.
+struct ExactOps {
+    template <class T> SVM_HD T sqrt(T x) const { return vsqrt(x); }
+    template <class T> SVM_HD T rcp(T b) const { return b; }
+    template <class T> SVM_HD T div(T a, T b, T) const { return a / b; }
+};
+
+// FastOps (FP64, device): the same correctly rounded results from shorter
+// sequences that are exact only for ordinary arguments.  Where an argument
+// or a result leaves the range in which they are exact, the result is
+// replaced by a NaN, which every later operation of the force expressions
+// carries into the force contribution, so the contribution comes out NaN; a
+// caller that sees any NaN contribution evaluates the vertex again with
+// ExactOps (which also decides the reference's NaN skips).
+//  * sqrt: the Newton refinement of v_rsq_f64 that the compiler's root wraps
+//    (sqrt_in_range); exact for x in [2^-767, DBL_MAX], accepted here on
+//    [2^-767, 2^1000), so that every root lies in [2^-384, 2^500).
+//  * rcp + div: the refined reciprocal y of the divisor (v_rcp_f64 and two
+//    Newton steps), shared by the quotients with that divisor, then
+//    q0 = a y, r = fma(-b, q0, a) (exact), q = fma(r, y, q0): the quotient
+//    sequence of the compiler's division without its operand scaling
+//    (div_scale), scaled fma (div_fmas) and special-case fixup (div_fixup),
+//    which change nothing when b and q are ordinary.  Divisors here are
+//    roots accepted by sqrt, so |b| is in [2^-384, 2^500); the quotient is
+//    accepted on [2^-400, 2^400), which keeps a = q b, a y, the remainder and
+//    the reciprocal's steps normal and finite: the result is then the
+//    correctly rounded a / b (Markstein's theorem for the final correction,
+//    given a reciprocal within an ulp and q0 within an ulp), the same bits as
+//    the division operator.  Zero, infinite and NaN quotients are replaced.
+// A check's result is consumed at once by the select that replaces the
+// value, so no flag stays live across the expressions.
+struct FastOps {
+    __device__ static uint32_t hi(double x) { return (uint32_t)(__builtin_bit_cast(uint64_t, x) >> 32); }
+    // x where ok, else a quiet NaN (only the high word is selected)
+    __device__ static double keep(double x, bool ok) {
+        const uint64_t b = __builtin_bit_cast(uint64_t, x);
+        const uint32_t h = ok ? (uint32_t)(b >> 32) : 0x7ff80000u;
+        return __builtin_bit_cast(double, ((uint64_t)h << 32) | (b & 0xffffffffull));
+    }
+    __device__ double sqrt(double x) const {
+        return keep(sqrt_in_range(x), (hi(x) - 0x10000000u) < (0x7E700000u - 0x10000000u));
+    }
+    __device__ double rcp(double b) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const double y0 = __builtin_amdgcn_rcp(b);
+        const double e0 = __builtin_fma(-b, y0, 1.0);
+        const double y1 = __builtin_fma(y0, e0, y0);
+        const double e1 = __builtin_fma(-b, y1, 1.0);
+        return __builtin_fma(y1, e1, y1);
+#else
+        return 1.0 / b;   // the host pass parses device code; it never runs it
+#endif
+    }
+    __device__ double div(double a, double b, double y) const {
+        const double q0 = a * y;
+        const double r = __builtin_fma(-b, q0, a);
+        const double q = __builtin_fma(r, y, q0);
+        return keep(q, ((hi(q) & 0x7fffffffu) - 0x26F00000u) < (0x58F00000u - 0x26F00000u));
+    }
+};
+
 // Spherical coordinates of a point, exactly as each reference gradient
 // recomputes them (sphericalDomain.h:250-261, 350-365, 218-225).
 template <class T>
@@ -26,16 +90,17 @@ struct PQT {
 };
 using PQ = PQT<double>;
 
-template <class T>
-SVM_HD PQT<T> make_pq(T x, T y, T z) {
+template <class T, class Ops = ExactOps>
+SVM_HD PQT<T> make_pq(T x, T y, T z, const Ops& ops = Ops()) {
     PQT<T> q;
     q.x = x; q.y = y; q.z = z;
-    T r = vsqrt(x * x + y * y + z * z);
-    q.cosT = z / r;
-    q.sinT = vsqrt(T(1.0) - q.cosT * q.cosT);
-    T s = vsqrt(x * x + y * y);
-    q.cosP = x / s;
-    q.sinP = y / s;
+    T r = ops.sqrt(x * x + y * y + z * z);
+    q.cosT = ops.div(z, r, ops.rcp(r));
+    q.sinT = ops.sqrt(T(1.0) - q.cosT * q.cosT);
+    T s = ops.sqrt(x * x + y * y);
+    const T rs = ops.rcp(s);
+    q.cosP = ops.div(x, s, rs);
+    q.sinP = ops.div(y, s, rs);
     return q;
 }
 
@@ -128,9 +193,11 @@ SVM_HD PairT<T> pair_terms(const PQT<T>& a, const PQT<T>& b) {
 }
 
 // grad_geodesic with the pair terms supplied.
-template <class T>
-SVM_HD void grad_geodesic_p(const PQT<T>& a, const PQT<T>& b, const PairT<T>& P, const T* tH, const T* pH, T* d) {
-    T denomInverse = T(1.) / vsqrt(T(1) - P.s * P.s);
+template <class T, class Ops = ExactOps>
+SVM_HD void grad_geodesic_p(const PQT<T>& a, const PQT<T>& b, const PairT<T>& P, const T* tH, const T* pH, T* d,
+                            const Ops& ops = Ops()) {
+    const T den = ops.sqrt(T(1) - P.s * P.s);
+    T denomInverse = ops.div(T(1.), den, ops.rcp(den));
     T gradTheta = T(1.0) * (b.cosT * a.sinT - a.cosT * P.cosPd * b.sinT) * denomInverse;
     T gradPhi = b.sinT * P.sinPd * denomInverse;
     d[0] = gradTheta * tH[0] + gradPhi * pH[0];
@@ -140,9 +207,9 @@ SVM_HD void grad_geodesic_p(const PQT<T>& a, const PQT<T>& b, const PairT<T>& P,
 
 // grad_triangle with the (a, b) and (a, c) pair terms supplied (pair_terms):
 // the same expression trees, each pair evaluated once per vertex.
-template <class T>
+template <class T, class Ops = ExactOps>
 SVM_HD void grad_triangle_p(const PQT<T>& a, const PQT<T>& b, const PQT<T>& c, const PairT<T>& P12,
-                            const PairT<T>& P13, const T* tH, const T* pH, T radius, T* d) {
+                            const PairT<T>& P13, const T* tH, const T* pH, T radius, T* d, const Ops& ops = Ops()) {
     T cosT1 = a.cosT, cosT2 = b.cosT, cosT3 = c.cosT;
     T sinT1 = a.sinT, sinT2 = b.sinT, sinT3 = c.sinT;
     T cosP1 = a.cosP, cosP2 = b.cosP, cosP3 = c.cosP;
@@ -156,18 +223,21 @@ SVM_HD void grad_triangle_p(const PQT<T>& a, const PQT<T>& b, const PQT<T>& c, c
     T d12 = P12.dd;
     T d13 = P13.dd;
     T g = T(1) - s12 * s12 - s13 * s13 - s23 * s23 + T(2.0) * s12 * s13 * s23;
-    T denom1 = vsqrt((T(-1.) + s12 * s12) * (T(-1.) + s12 * s12) * g);
-    T denom2 = vsqrt((T(-1.) + s13 * s13) * (T(-1.) + s13 * s13) * g);
-    T denom3 = vsqrt((s12 * s12 - T(1.)) * (s13 * s13 - T(1.)) * (s12 * s12 - T(1.)) * (s13 * s13 - T(1.)) * g);
-    T gradTheta = (d13 * (s12 * s12 - T(1.0)) + d12 * (s23 - s12 * s13)) / denom1
-                + (d12 * (s13 * s13 - T(1.0)) + d13 * (s23 - s12 * s13)) / denom2
-                - (d12 * (s13 * s13 - T(1.)) * (s13 - s12 * s23) + d13 * (s12 * s12 - T(1.0)) * (s12 - s13 * s23)) /
-                      denom3;
+    T denom1 = ops.sqrt((T(-1.) + s12 * s12) * (T(-1.) + s12 * s12) * g);
+    T denom2 = ops.sqrt((T(-1.) + s13 * s13) * (T(-1.) + s13 * s13) * g);
+    T denom3 = ops.sqrt((s12 * s12 - T(1.)) * (s13 * s13 - T(1.)) * (s12 * s12 - T(1.)) * (s13 * s13 - T(1.)) * g);
+    const T r1 = ops.rcp(denom1), r2 = ops.rcp(denom2), r3 = ops.rcp(denom3);
+    T gradTheta = ops.div(d13 * (s12 * s12 - T(1.0)) + d12 * (s23 - s12 * s13), denom1, r1)
+                + ops.div(d12 * (s13 * s13 - T(1.0)) + d13 * (s23 - s12 * s13), denom2, r2)
+                - ops.div(d12 * (s13 * s13 - T(1.)) * (s13 - s12 * s23) + d13 * (s12 * s12 - T(1.0)) * (s12 - s13 * s23),
+                          denom3, r3);
     gradTheta *= radius;
-    T gradPhi = ((s12 * s13 - s23) * sinP1MinusP2 * sinT2 - (T(-1) + s12 * s12) * sinP1MinusP3 * sinT3) / (denom1)
-              - ((T(-1) + s13 * s13) * sinP1MinusP2 * sinT2 + (-(s12 * s13) + s23) * sinP1MinusP3 * sinT3) / (denom2)
-              + ((T(-1) + s13 * s13) * (s13 - s12 * s23) * sinP1MinusP2 * sinT2 + (T(-1) + s12 * s12) * (s12 - s13 * s23) *
-                 sinP1MinusP3 * sinT3) / (denom3);
+    T gradPhi = ops.div((s12 * s13 - s23) * sinP1MinusP2 * sinT2 - (T(-1) + s12 * s12) * sinP1MinusP3 * sinT3, denom1, r1)
+              - ops.div((T(-1) + s13 * s13) * sinP1MinusP2 * sinT2 + (-(s12 * s13) + s23) * sinP1MinusP3 * sinT3,
+                        denom2, r2)
+              + ops.div((T(-1) + s13 * s13) * (s13 - s12 * s23) * sinP1MinusP2 * sinT2 +
+                            (T(-1) + s12 * s12) * (s12 - s13 * s23) * sinP1MinusP3 * sinT3,
+                        denom3, r3);
     gradPhi *= radius;
     T determinant = a.x * (b.y * c.z - b.z * c.y)
                   + a.y * (b.z * c.x - b.x * c.z)

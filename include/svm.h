@@ -601,6 +601,18 @@ int svm_copy_probe(int64_t n_bytes, int32_t reps, double* ms_per_copy);
 int svm_phase_boundary_probe(int32_t workgroups, int32_t phases, int64_t bytes_per_phase, double* us_per_launch,
                              double* us_per_barrier, int64_t* errors);
 
+/* Self-check of the vertex pass's short root and quotient sequences (no
+ * counterpart in the reference; the reference uses the operators): n random
+ * samples under `seed` of quotients a / b (divisors across and beyond
+ * [2^-384, 2^500), quotients across and beyond [2^-400, 2^400), zero
+ * numerators, all-ones and all-zero mantissas) and of roots (arguments across
+ * [2^-800, 2^1024), negative, zero, infinite, NaN and subnormal ones), each
+ * evaluated by the sequences and by the operators.  counts[6]: quotient
+ * samples in the sequences' precondition, accepted quotients, accepted
+ * quotients that differ from a / b in any bit; root samples, accepted roots,
+ * accepted roots that differ from sqrt(x). */
+int svm_fast_ops_check(int64_t n, uint64_t seed, int64_t* counts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,7 +5,8 @@ chain-rule forces: a thread per vertex (form 0, form 3 compiled for three
 wavefronts per SIMD, form 4 with the tile's cell and vertex windows staged in
 LDS) or the corner form, one wavefront per cell slot (forms 1 and 2, compiled
 for four and five wavefronts per SIMD).
-svm_set_geometry_form gives the geometry pass's threads 1, 2 or 4 cells each.
+svm_set_geometry_form gives the geometry pass's threads 1, 2 or 4 cells each
+(forms 0, 1, 2; form 0 reads the first eight ring slots of its cell at once).
 Every form must give the oracle's trajectory bit for bit, over steps with T1
 flips, NaN-free and with noise.
 """
@@ -79,10 +80,10 @@ def test_vertex_forms_match_the_oracle(form):
     np.testing.assert_array_equal(ctx.positions(0), om.positions())
 
 
-@pytest.mark.parametrize("gform", [1, 2])
+@pytest.mark.parametrize("gform", [0, 1, 2])
 def test_geometry_forms_match_the_oracle(gform):
-    """Areas, perimeters and cell centres of the geometry pass with 2 and 4
-    cells per thread, then active steps, against the oracle."""
+    """Areas, perimeters and cell centres of the geometry pass with 1, 2 and
+    4 cells per thread, then active steps, against the oracle."""
     st = svmtest.seeded_state(700, 1140)
     om = svmtest.OracleModel(st)
     ctx = svm.Context([dict(n_cells=700, a0=1.0, p0=3.8, kr=1.0, t1_threshold=0.1, seed=1140)])
