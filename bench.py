@@ -683,7 +683,8 @@ def main():
     # so that exactly K steps are timed (an odd K times one eager step more),
     # and to at most W so that the warm-up replays the timed graph
     Kg = K - K % 2
-    G = max([g for g in range(2, ctx.tuning()["graph_steps"] + 1, 2) if Kg % g == 0] or [2])
+    Gmax = min(ctx.tuning()["graph_steps"], max(2, W))
+    G = max([g for g in range(2, Gmax + 1, 2) if Kg % g == 0] or [2])
     print(f"schedule ({schedule_source}): {ctx.tuning()}; {G} steps per timed graph", flush=True)
     # the reference driver's initialisation (sphericalVertex.cpp:113-121):
     # the dynamics at the target parameters until the T1 transient that
