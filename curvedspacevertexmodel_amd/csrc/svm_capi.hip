@@ -258,13 +258,8 @@ int launch_geometry(svm_context* ctx, const Target& w, int skip_frozen, bool wit
         if (r) return r;
     }
     if (!with_fan) return SVM_OK;
-    // dynamic LDS for the fold of spheres with more tiles than a wavefront folds in registers
-    int mmax = 1;
-    for (int s = w.d.s0; s < w.d.s0 + w.d.ns; ++s)
-        while (mmax < ctx->layout[s].nctiles) mmax <<= 1;
-    const size_t lds = mmax > 64 * SVM_FOLD_K ? (size_t)mmax * sizeof(double) : 0;
     Timer t(ctx, "k_fan_fix");
-    hipLaunchKernelGGL(svm::k_fan_fix, dim3(xcd_grid(w.d.ctn)), dim3(SVM_TILE), lds, w.st, w.d, ctx->pcur,
+    hipLaunchKernelGGL(svm::k_fan_fix, dim3(xcd_grid(w.d.ctn)), dim3(SVM_TILE), 0, w.st, w.d, ctx->pcur,
                        skip_frozen);
     return last_launch(ctx);
 }

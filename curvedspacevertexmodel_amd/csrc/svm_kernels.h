@@ -130,6 +130,32 @@ __device__ inline double fold_wave(const double* part, int stride, int n, int m)
     return __shfl(x, 0, 64);
 }
 
+// The same fixed tree by a whole 256-thread block, for m = n padded to a
+// power of two in (64 * SVM_FOLD_K, 256 * SVM_FOLD_B]: thread t holds entries
+// t + 256 k in registers and adds them in pairs for the levels h >= 256;
+// block_tree256 takes the last eight levels.  The same additions of the same
+// pairs as fold_lds, with no LDS round trip per level: a tile's workgroup
+// folds a large sphere's tile partials in a few loads and three barriers.
+// Every thread returns the sum.
+#define SVM_FOLD_B 16
+static_assert(SVM_MAX_TILES <= 256 * SVM_FOLD_B, "fold_block covers every sphere's tiles");
+__device__ inline double fold_block(const double* part, int stride, int n, int m, double* red) {
+    const int t = threadIdx.x;
+    double v[SVM_FOLD_B];
+#pragma unroll
+    for (int k = 0; k < SVM_FOLD_B; ++k) {
+        const int i = t + 256 * k;
+        v[k] = i < n ? part[(size_t)i * stride] : 0.0;
+    }
+#pragma unroll
+    for (int q = SVM_FOLD_B / 2; q >= 1; q >>= 1)      // h = 256 q
+        if (256 * q < m) {
+#pragma unroll
+            for (int k = 0; k < q; ++k) v[k] = v[k] + v[k + q];
+        }
+    return block_tree256(v[0], red);
+}
+
 __device__ inline double dot0(const double* a, const double* b) {
     double s = 0.0;
     s += a[0] * b[0];
@@ -379,17 +405,17 @@ __device__ inline double fan_area(const Dev& d, const double* X, const double* Y
 // every cell of that sphere.  Each block folds its sphere's tile partials
 // from k_geometry with the same fixed tree, so every block reaches the same
 // decision without any cross-block synchronisation.
-// Spheres of up to 64 * SVM_FOLD_K tiles fold in registers; larger ones in
-// the launch's dynamic LDS (sized by the host for the largest such sphere).
+// Spheres of up to 64 * SVM_FOLD_K tiles fold in one wavefront's registers,
+// larger ones in the block's (fold_block).
 __global__ __launch_bounds__(SVM_TILE) void k_fan_fix(Dev d, int pin, int skip_frozen) {
-    extern __shared__ double fold_buf[];
+    __shared__ double red[SVM_TILE + 1];
     const int tile = lane_ctile(d);
     if (tile < 0) return;
     SysParams* S = d.sys + d.ctile_sys[tile];
     if (skip_frozen && S->f_frozen) return;
     const int m = fold_width(S->nctiles);
     const double total = m <= 64 * SVM_FOLD_K ? fold_wave(d.cpart + S->ctile0, 1, S->nctiles, m)
-                                              : fold_lds(d.cpart + S->ctile0, 1, S->nctiles, fold_buf);
+                                              : fold_block(d.cpart + S->ctile0, 1, S->nctiles, m, red);
     if (!fan_needed(S, total)) return;
     if (tile == S->ctile0 && threadIdx.x == 0) S->n_fan += 1;
     const int c = tile * SVM_TILE + threadIdx.x;
