@@ -176,8 +176,8 @@ __device__ inline double dot0(const double* a, const double* b) {
 // slots' vertex indices from LDS (`sidx`, this thread's column of a
 // [PF][SVM_TILE] array).  The arithmetic is the same in the same order.
 // With 8 slots (97% of rings) the pass takes 0.091 ms on the 256-sphere
-// batch against 0.100 ms without (98 against 76 VGPRs); 6 slots (80 VGPRs)
-// gave 0.092 ms.
+// batch against 0.100 ms without (98 against 76 VGPRs, 4 against 6 waves per
+// SIMD); 6 slots (80 VGPRs) gave 0.092 ms.
 #define SVM_GEO_PF 8
 template <int PF = 0>
 __device__ __attribute__((always_inline)) inline double geometry_cell(const Dev& d, const SysParams* S,
