@@ -312,8 +312,8 @@ def phase_boundary_probe(workgroups, phases=200, bytes_per_phase=64 << 20):
 
 def fast_ops_check(n=1 << 24, seed=1):
     """Counts of svm_fast_ops_check: (quotient samples in the precondition, accepted, differing from a / b,
-    root samples, accepted, differing from sqrt)."""
-    out = np.zeros(6, np.int64)
+    root samples, accepted, differing from sqrt, accepted quotients and roots not the nearest double)."""
+    out = np.zeros(8, np.int64)
     rc = lib().svm_fast_ops_check(C.c_int64(n), C.c_uint64(seed), out.ctypes.data_as(C.POINTER(C.c_int64)))
     if rc != 0:
         raise SVMError(rc, "svm_fast_ops_check")

@@ -2779,15 +2779,49 @@ int svm_phase_boundary_probe(int32_t workgroups, int32_t phases, int64_t bytes_p
 // and an exponent in [-420, 520), a numerator a = (random mantissa, random
 // sign) * 2^(e_b + d) with d in [-430, 430), a root argument x with an
 // exponent in [-800, 1024) (random sign one time in 8), and, one sample in 16,
-// the mantissas all ones or all zeros, zero numerators, zero, infinite, NaN
-// and subnormal root arguments.  Counts per kind: samples in the sequence's
-// precondition (divisor in [2^-384, 2^500)), accepted results (not replaced
-// by a NaN) and accepted results whose bits differ from the operator's.
+// the mantissas all ones or all zeros (all-ones divisors over powers of two
+// among them: quotients just above a rounding midpoint), zero numerators,
+// zero, infinite, NaN and subnormal root arguments and all-ones root
+// mantissas.  Counts per kind: samples in the sequence's precondition
+// (divisor in [2^-384, 2^500)), accepted results (not replaced by a NaN),
+// accepted results whose bits differ from the operator's, and accepted
+// results that are not the double nearest to the exact quotient or root.
 __device__ inline double bits_to_d(uint32_t hi, uint32_t lo) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
+// ulp of a normal double of biased exponent e > 52 (a power of two)
+__device__ inline double ulp_of(uint64_t u) { return bits_to_d((uint32_t)(((u >> 52) & 0x7ff) - 52) << 20, 0u); }
+// Is q (normal, |q| in [2^-400, 2^400), |b| in [2^-384, 2^500)) the double
+// nearest to a / b, ties to even?  r = a - b q is exact for a q within an
+// ulp of the quotient, and a / b = q + r / b lies on the side of q that
+// r / b's sign gives, where the spacing is an ulp (half an ulp below a power
+// of two); |b| times half the spacing is exact.
+__device__ inline bool nearest_quotient(double a, double b, double q) {
+    const double r = __builtin_fma(-b, q, a);
+    if (r == 0.0) return true;
+    const uint64_t u = __builtin_bit_cast(uint64_t, q);
+    double sp = ulp_of(u);
+    const bool away = ((r > 0) == (b > 0)) ? (q > 0) : (q < 0);
+    if (!away && (u & 0xfffffffffffffull) == 0) sp *= 0.5;
+    const double half = __builtin_fabs(b) * (0.5 * sp);
+    const double ar = __builtin_fabs(r);
+    return ar < half || (ar == half && (u & 1) == 0);
+}
+// Is g (positive normal, g^2 normal) the double nearest to sqrt(x)?  With
+// r = x - g^2 (exact for a faithful g), sqrt(x) <= g + s/2 iff
+// r <= g s + s^2 / 4, and >= g - s'/2 iff r >= -(g s' - s'^2 / 4), s and s'
+// the spacings above and below g; r - g s and r + g s' are exact where the
+// answer is close (Sterbenz).  Roots are never exactly at a midpoint.
+__device__ inline bool nearest_root(double x, double g) {
+    const double r = __builtin_fma(-g, g, x);
+    const uint64_t u = __builtin_bit_cast(uint64_t, g);
+    const double s = ulp_of(u);
+    const double sl = (u & 0xfffffffffffffull) == 0 ? 0.5 * s : s;
+    if (r > 0) return r - g * s <= 0.25 * s * s;
+    return r + g * sl >= 0.25 * sl * sl;
+}
 __global__ __launch_bounds__(256) void k_fast_ops_check(uint64_t seed, int64_t n, unsigned long long* counts) {
-    unsigned long long c[6] = {0, 0, 0, 0, 0, 0};
+    unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const svm::FastOps fo{};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const svm::u32x4 r = svm::philox4x32_10((uint32_t)i, (uint32_t)(i >> 32), 0u, 0x46534f31u, (uint32_t)seed,
@@ -2801,7 +2835,11 @@ __global__ __launch_bounds__(256) void k_fast_ops_check(uint64_t seed, int64_t n
         uint32_t ma_hi = r.v[2] & 0xfffffu, ma_lo = r.v[3];
         if (special) {
             const uint32_t k = (t.v[3] >> 4) & 3u;
-            if (k == 0) { mb_hi = 0xfffffu; mb_lo = 0xffffffffu; }
+            if (k == 0) {
+                mb_hi = 0xfffffu;
+                mb_lo = 0xffffffffu;
+                if (t.v[2] & 2u) { ma_hi = 0u; ma_lo = 0u; }   // a power of two
+            }
             if (k == 1) { mb_hi = 0u; mb_lo = 0u; }
             if (k == 2) { ma_hi = 0xfffffu; ma_lo = 0xffffffffu; }
         }
@@ -2816,6 +2854,7 @@ __global__ __launch_bounds__(256) void k_fast_ops_check(uint64_t seed, int64_t n
             if (!(q != q)) {
                 ++c[1];
                 c[2] += __builtin_bit_cast(uint64_t, q) != __builtin_bit_cast(uint64_t, a / b);
+                c[6] += !nearest_quotient(a, b, q);
             }
         }
         const int ex = (int)(t.v[2] % 1824u) - 800;
@@ -2827,16 +2866,18 @@ __global__ __launch_bounds__(256) void k_fast_ops_check(uint64_t seed, int64_t n
             if (k == 1) x = __builtin_inf();
             if (k == 2) x = __builtin_nan("");
             if (k == 3) x = bits_to_d(r.v[0] & 0xfffffu, r.v[1]);   // subnormal
+            if (k == 4) x = bits_to_d((uint32_t)(ex + 1023) << 20 | 0xfffffu, 0xffffffffu);
         }
         ++c[3];
         const double y = fo.sqrt(x);
         if (!(y != y)) {
             ++c[4];
             c[5] += __builtin_bit_cast(uint64_t, y) != __builtin_bit_cast(uint64_t, __builtin_sqrt(x));
+            c[7] += !nearest_root(x, y);
         }
     }
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < 8; ++k) {
         unsigned long long v = c[k];
         for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
         if ((threadIdx.x & 63) == 0 && v) atomicAdd(counts + k, v);
@@ -2849,8 +2890,8 @@ int svm_fast_ops_check(int64_t n, uint64_t seed, int64_t* counts) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SVM_ERR_NO_DEVICE;
     unsigned long long* dc = nullptr;
     int rc = SVM_OK;
-    if (hipMalloc(&dc, 6 * sizeof(unsigned long long)) != hipSuccess) return SVM_ERR_HIP;
-    unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
+    if (hipMalloc(&dc, 8 * sizeof(unsigned long long)) != hipSuccess) return SVM_ERR_HIP;
+    unsigned long long h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (hipMemcpy(dc, h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) rc = SVM_ERR_HIP;
     if (!rc) {
         const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
@@ -2860,7 +2901,7 @@ int svm_fast_ops_check(int64_t n, uint64_t seed, int64_t* counts) {
             rc = SVM_ERR_HIP;
     }
     (void)hipFree(dc);
-    for (int k = 0; k < 6; ++k) counts[k] = (int64_t)h[k];
+    for (int k = 0; k < 8; ++k) counts[k] = (int64_t)h[k];
     return rc;
 }
 

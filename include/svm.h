@@ -605,12 +605,15 @@ int svm_phase_boundary_probe(int32_t workgroups, int32_t phases, int64_t bytes_p
  * counterpart in the reference; the reference uses the operators): n random
  * samples under `seed` of quotients a / b (divisors across and beyond
  * [2^-384, 2^500), quotients across and beyond [2^-400, 2^400), zero
- * numerators, all-ones and all-zero mantissas) and of roots (arguments across
- * [2^-800, 2^1024), negative, zero, infinite, NaN and subnormal ones), each
- * evaluated by the sequences and by the operators.  counts[6]: quotient
- * samples in the sequences' precondition, accepted quotients, accepted
- * quotients that differ from a / b in any bit; root samples, accepted roots,
- * accepted roots that differ from sqrt(x). */
+ * numerators, all-ones and all-zero mantissas, all-ones divisors over powers
+ * of two) and of roots (arguments across [2^-800, 2^1024), negative, zero,
+ * infinite, NaN and subnormal ones, all-ones mantissas), each evaluated by
+ * the sequences and by the operators.  counts[8]: quotient samples in the
+ * sequences' precondition, accepted quotients, accepted quotients that
+ * differ from a / b in any bit; root samples, accepted roots, accepted roots
+ * that differ from sqrt(x); accepted quotients and accepted roots that are
+ * not the double nearest to the exact value (checked with exact fma
+ * remainders). */
 int svm_fast_ops_check(int64_t n, uint64_t seed, int64_t* counts);
 
 #ifdef __cplusplus

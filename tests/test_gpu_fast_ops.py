@@ -16,10 +16,11 @@ pytestmark = pytest.mark.gpu
 
 def test_fast_ops_equal_the_operators():
     n = 1 << 26
-    dq, aq, bad_q, dr, ar, bad_r = svm.fast_ops_check(n, seed=20261204)
-    print(f"quotients: {dq} in precondition, {aq} accepted, {bad_q} differ; "
-          f"roots: {dr} drawn, {ar} accepted, {bad_r} differ", flush=True)
+    dq, aq, bad_q, dr, ar, bad_r, far_q, far_r = svm.fast_ops_check(n, seed=20261204)
+    print(f"quotients: {dq} in precondition, {aq} accepted, {bad_q} differ, {far_q} not nearest; "
+          f"roots: {dr} drawn, {ar} accepted, {bad_r} differ, {far_r} not nearest", flush=True)
     assert bad_q == 0 and bad_r == 0
+    assert far_q == 0 and far_r == 0
     assert dr == n and 0.4 * n < dq < n
     assert aq > 0.8 * dq and 0.6 * n < ar < 0.9 * n
 
