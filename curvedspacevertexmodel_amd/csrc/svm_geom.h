@@ -41,14 +41,14 @@ struct ExactOps {
 //    Newton steps), shared by the quotients with that divisor, then
 //    q0 = a y, r = fma(-b, q0, a) (exact), q = fma(r, y, q0): the quotient
 //    sequence of the compiler's division without its operand scaling
-//    (div_scale), scaled fma (div_fmas) and special-case fixup (div_fixup),
-//    which change nothing when b and q are ordinary.  Divisors here are
-//    roots accepted by sqrt, so |b| is in [2^-384, 2^500); the quotient is
-//    accepted on [2^-400, 2^400), which keeps a = q b, a y, the remainder and
-//    the reciprocal's steps normal and finite: the result is then the
-//    correctly rounded a / b (Markstein's theorem for the final correction,
-//    given a reciprocal within an ulp and q0 within an ulp), the same bits as
-//    the division operator.  Zero, infinite and NaN quotients are replaced.
+//    (div_scale), scaled fma (div_fmas) and special-case fixup (div_fixup).
+//    Divisors here are roots accepted by sqrt, so |b| is in [2^-384, 2^500);
+//    the quotient is accepted on [2^-400, 2^400), which keeps a = q b, a y,
+//    the remainder and the reciprocal's steps normal and finite.  There a
+//    scaling by a power of two, if the operator applies one, commutes with
+//    every rounding, and the fixup passes ordinary values through, so the
+//    result is the division operator's, bit for bit: the same seed and the
+//    same refinement.  Zero, infinite and NaN quotients are replaced.
 // A check's result is consumed at once by the select that replaces the
 // value, so no flag stays live across the expressions.
 struct FastOps {
