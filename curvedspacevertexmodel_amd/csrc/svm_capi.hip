@@ -2605,6 +2605,8 @@ int svm_autotune(svm_context* ctx, int32_t updater, int32_t steps_per_trial, dou
     return r;
 }
 
+}  // extern "C"
+
 // ---- persisted schedules ----
 namespace {
 int floor_log2(long long x) {
@@ -2624,6 +2626,8 @@ std::array<int, 4> schedule_key(svm_context* ctx, int updater) {
 const char* kScheduleTag = "svm-schedule-v2";
 const char* kScheduleTagV1 = "svm-schedule-v1";
 }  // namespace
+
+extern "C" {
 
 int svm_tuning_save(svm_context* ctx, int32_t updater, const char* path) {
     if (!ctx || !path || updater < 0 || updater > 2) return fail(ctx, SVM_ERR_ARG, "bad argument");
@@ -2920,18 +2924,19 @@ int svm_copy_probe(int64_t n_bytes, int32_t reps, double* ms_per_copy) {
         const int threads = 256;
         const int blocks = (int)std::min<size_t>((n + threads - 1) / threads, 16384);
         hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(threads), 0, nullptr, (const double*)a, a, n);  // touch
-        hipEventRecord(e0, nullptr);
-        for (int r = 0; r < reps; ++r)
+        if (hipEventRecord(e0, nullptr) != hipSuccess) rc = SVM_ERR_HIP;
+        for (int r = 0; r < reps && !rc; ++r)
             hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(threads), 0, nullptr, (const double*)a, b, n);
-        hipEventRecord(e1, nullptr);
         float ms = 0.f;
-        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = SVM_ERR_HIP;
+        if (rc || hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = SVM_ERR_HIP;
         *ms_per_copy = ms / reps;
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    if (a) hipFree(a);
-    if (b) hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
     return rc;
 }
 
