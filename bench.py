@@ -680,10 +680,10 @@ def main():
         ctx.set_lanes(args.lanes)
         schedule_source += f", lanes set to {args.lanes}"
     # graph length: the schedule's, cut to an even divisor of the timed steps
-    # so that exactly K steps are timed (an odd K times one eager step more),
-    # and to at most W so that the warm-up replays the timed graph
+    # so that exactly K steps are timed (an odd K times one eager step more);
+    # the initialisation below replays it before any timed launch
     Kg = K - K % 2
-    Gmax = min(ctx.tuning()["graph_steps"], max(2, W))
+    Gmax = ctx.tuning()["graph_steps"]
     G = max([g for g in range(2, Gmax + 1, 2) if Kg % g == 0] or [2])
     print(f"schedule ({schedule_source}): {ctx.tuning()}; {G} steps per timed graph", flush=True)
     # the reference driver's initialisation (sphericalVertex.cpp:113-121):
