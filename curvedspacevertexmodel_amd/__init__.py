@@ -37,7 +37,7 @@ EXPORTS = [
     "svm_cell_list", "svm_mlp_create", "svm_mlp_destroy", "svm_mlp_set_parameters", "svm_mlp_get_parameters",
     "svm_mlp_forward", "svm_mlp_gradient", "svm_mlp_train", "svm_mlp_time_forward", "svm_cell_descriptors",
     "svm_warm_start", "svm_fire_set_max_iterations", "svm_set_tuning", "svm_get_tuning", "svm_autotune",
-    "svm_set_lanes", "svm_get_lanes",
+    "svm_set_lanes", "svm_get_lanes", "svm_set_lane_spheres",
     "svm_adam_set_parameters", "svm_adam_minimize", "svm_adam_get_state", "svm_adam_set_max_iterations",
     "svm_dyn_set_reference", "svm_dyn_measure", "svm_nc_open", "svm_nc_write", "svm_nc_num_records", "svm_nc_close",
     "svm_set_vicsek", "svm_vicsek_steps",
@@ -648,6 +648,11 @@ class Context:
     def set_lanes(self, n_lanes):
         """Cut the batch into n_lanes sphere ranges whose steps run on streams of their own."""
         self._ck(lib().svm_set_lanes(self.h, C.c_int32(n_lanes)), "svm_set_lanes")
+
+    def set_lane_spheres(self, first_sphere):
+        """Lanes over the given sphere ranges: lane k starts at sphere first_sphere[k]."""
+        first = np.ascontiguousarray(first_sphere, np.int32)
+        self._ck(lib().svm_set_lane_spheres(self.h, C.c_int32(len(first)), _ip(first)), "svm_set_lane_spheres")
 
     def lanes(self):
         """(number of lanes, first sphere of each lane)."""

@@ -2408,10 +2408,10 @@ int svm_get_step_form(svm_context* ctx, int32_t* form) {
     return SVM_OK;
 }
 
-int svm_set_lanes(svm_context* ctx, int32_t n_lanes) {
-    if (!ctx || n_lanes < 1 || n_lanes > SVM_MAX_LANES)
-        return fail(ctx, SVM_ERR_ARG, "n_lanes must be in [1, " + std::to_string(SVM_MAX_LANES) + "]");
-    const int n = std::min((int)n_lanes, ctx->nsys);
+// Lanes over the sphere ranges [s0[k], s0[k+1]): streams, events and T1
+// buffers for every lane above the first (kept until svm_destroy).
+static int apply_lanes(svm_context* ctx, const std::vector<int>& s0) {
+    const int n = (int)s0.size() - 1;
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     if (!ctx->lane_fork) HIPCHECK(ctx, hipEventCreateWithFlags(&ctx->lane_fork, hipEventDisableTiming));
     while ((int)ctx->lane_streams.size() < n - 1) {
@@ -2439,6 +2439,16 @@ int svm_set_lanes(svm_context* ctx, int32_t n_lanes) {
         if (r) return fail(ctx, SVM_ERR_HIP, "lane buffers: out of device memory");
         ctx->lane_t1.push_back(L);
     }
+    ctx->lane_s0 = s0;
+    ctx->nlanes = n;
+    drop_graphs(ctx);
+    return SVM_OK;
+}
+
+int svm_set_lanes(svm_context* ctx, int32_t n_lanes) {
+    if (!ctx || n_lanes < 1 || n_lanes > SVM_MAX_LANES)
+        return fail(ctx, SVM_ERR_ARG, "n_lanes must be in [1, " + std::to_string(SVM_MAX_LANES) + "]");
+    const int n = std::min((int)n_lanes, ctx->nsys);
     // contiguous sphere ranges with about equal numbers of vertex tiles
     std::vector<int> s0(1, 0);
     long long acc = 0;
@@ -2450,10 +2460,20 @@ int svm_set_lanes(svm_context* ctx, int32_t n_lanes) {
     }
     while ((int)s0.size() < n) s0.push_back(ctx->nsys - (n - (int)s0.size()));
     s0.push_back(ctx->nsys);
-    ctx->lane_s0 = s0;
-    ctx->nlanes = n;
-    drop_graphs(ctx);
-    return SVM_OK;
+    return apply_lanes(ctx, s0);
+}
+
+int svm_set_lane_spheres(svm_context* ctx, int32_t n_lanes, const int32_t* first_sphere) {
+    if (!ctx || n_lanes < 1 || n_lanes > SVM_MAX_LANES || n_lanes > ctx->nsys || !first_sphere)
+        return fail(ctx, SVM_ERR_ARG, "n_lanes must be in [1, " + std::to_string(SVM_MAX_LANES) +
+                                          "] and at most the number of spheres, first_sphere non-null");
+    std::vector<int> s0(first_sphere, first_sphere + n_lanes);
+    if (s0[0] != 0) return fail(ctx, SVM_ERR_ARG, "the first lane must start at sphere 0");
+    for (int k = 1; k < n_lanes; ++k)
+        if (s0[k] <= s0[k - 1] || s0[k] >= ctx->nsys)
+            return fail(ctx, SVM_ERR_ARG, "lane starts must increase strictly and stay below the sphere count");
+    s0.push_back(ctx->nsys);
+    return apply_lanes(ctx, s0);
 }
 
 int svm_get_lanes(svm_context* ctx, int32_t* n_lanes, int32_t* first_sphere) {

@@ -539,6 +539,14 @@ int svm_get_step_form(svm_context* ctx, int32_t* form);
 #define SVM_MAX_LANES 4
 int svm_set_lanes(svm_context* ctx, int32_t n_lanes);
 int svm_get_lanes(svm_context* ctx, int32_t* n_lanes, int32_t* first_sphere);
+/* svm_set_lanes with the sphere ranges given: lane k runs spheres
+ * [first_sphere[k], first_sphere[k+1]) (the last lane up to the sphere
+ * count).  first_sphere[0] must be 0 and the starts must increase strictly;
+ * SVM_ERR_ARG otherwise.  For callers that weigh spheres by more than their
+ * size (T1 activity differs by an order of magnitude between spheres of one
+ * batch).  No counterpart in the reference, which steps one sphere at a
+ * time. */
+int svm_set_lane_spheres(svm_context* ctx, int32_t n_lanes, const int32_t* first_sphere);
 /* Project autotuner, in the manner of kernelTuner's median-of-samples scan
  * (src/utilities/kernelTuner.cpp:35-126) but over the whole step: times
  * real steps of `updater` (0 Brownian, 1 active) under each candidate

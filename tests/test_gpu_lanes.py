@@ -27,7 +27,10 @@ def build(ps, lanes):
         ctx.set_directors(i, svm.generate_directors(ctx.positions(i), 3100 + i))
         ctx.set_preferred_parameters(i, 1.0, 1.0)
         ctx.set_brownian(i, dt=0.01, mu=1.0, T=0.0)
-    ctx.set_lanes(lanes)
+    if isinstance(lanes, int):
+        ctx.set_lanes(lanes)
+    else:
+        ctx.set_lane_spheres(lanes)
     return ctx
 
 
@@ -78,6 +81,23 @@ def test_lanes_give_the_one_lane_trajectory(updater):
         n, first = ctx.lanes()
         assert n == lanes and first[0] == 0 and (np.diff(first) > 0).all()
         assert_same(got, ref)
+
+
+def test_explicit_lane_ranges():
+    """Lanes over caller-chosen sphere ranges (svm_set_lane_spheres) give the
+    one-lane trajectory too; malformed ranges are refused."""
+    ps = params()
+    _, ref = trajectory(ps, 1, 1)
+    for starts in ([0, 1], [0, 5, 6, 13], [0, 12]):
+        ctx, got = trajectory(ps, starts, 1)
+        n, first = ctx.lanes()
+        assert n == len(starts) and list(first) == starts
+        assert_same(got, ref)
+    ctx = build(ps[:5], 1)
+    for bad in ([1, 3], [0, 3, 3], [0, 5], [0, 1, 2, 3, 4], []):
+        with pytest.raises(svm.SVMError):
+            ctx.set_lane_spheres(bad)
+    assert ctx.lanes()[0] == 1
 
 
 def test_lane_partition_and_limits():
