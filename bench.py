@@ -689,7 +689,7 @@ def main():
     # the reference driver's initialisation (sphericalVertex.cpp:113-121):
     # the dynamics at the target parameters until the T1 transient that
     # follows the stabilisation has passed (thousands of flips per step in
-    # the first steps, about 600 per step after a few hundred).  It replays
+    # the first steps, 400-700 per step after a thousand).  It replays
     # the timed graph in both ping-pong parities (one eager step between), so
     # that the timed replays are never a graph's first launch, whatever W is.
     half = max(1, args.init_steps // (2 * G))
