@@ -66,16 +66,17 @@ def stabilise(ctx, params, steps, eager=False):
         ctx.set_preferred_parameters(i, 1.0, p0)
 
 
-def health(ctx, params, base):
+def health(ctx, params, base, after):
     """Geometry invariants after the run (the angle-sum areas must tile each
-    sphere) and the T1 counters accumulated since `base` (a snapshot of
-    counters and diagnostics taken before the timed steps)."""
+    sphere) and the T1 counters of the timed steps: `base` and `after` are
+    snapshots of counters and diagnostics taken right before and right after
+    them."""
     ctx.compute_geometry()
     dev = 0.0
     for i, (n, *_rest) in enumerate(params):
         dev = max(dev, abs(ctx.area_perimeter(i)[:, 0].sum() / n - 1.0))
-    c = ctx.counters() - base[0]
-    dg = ctx.t1_diagnostics()
+    c = after[0] - base[0]
+    dg = after[1]
     r = (dg - base[1]).sum(axis=0)
     return {"t1_sequential_reasons": {"flippable_list_overflow": int(r[0]), "group_too_large": int(r[1]),
                                       "group_out_of_room": int(r[2]), "blocked_list_overflow": int(r[3])},
@@ -689,7 +690,7 @@ def main():
     # the reference driver's initialisation (sphericalVertex.cpp:113-121):
     # the dynamics at the target parameters until the T1 transient that
     # follows the stabilisation has passed (thousands of flips per step in
-    # the first steps, 400-700 per step after a thousand).  It replays
+    # the first steps, about 450 per step after a thousand).  It replays
     # the timed graph in both ping-pong parities (one eager step between), so
     # that the timed replays are never a graph's first launch, whatever W is.
     half = max(1, args.init_steps // (2 * G))
@@ -723,6 +724,7 @@ def main():
     ctx.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
+    after = (ctx.counters(), ctx.t1_diagnostics())
     steps, warm = K, W
     value = total_cells * steps / elapsed
     print(f"timed {steps} steps in {elapsed:.3f} s", flush=True)
@@ -761,7 +763,7 @@ def main():
                               "total": len(timing) * ctx.tuning()["lanes"]},
         "init": {"gpu_delaunay_topology_s": t_delaunay, "setup_s": t_setup,
                  "note": "256 spheres triangulated one call each (includes host<->device copies)"},
-        "health": health(ctx, params, base),
+        "health": health(ctx, params, base, after),
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(ctx, params)
